@@ -1,0 +1,71 @@
+"""Loader and prototypes of libflexrt_gpu.so (the C ABI in include/flexrt_gpu.h).
+
+The library is loaded from the package directory only.  If it is absent the
+import of any GPU entry point raises: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+from ._abi import FrtConfig, FrtRayView, FrtStepStats
+from ._build import LIB_PATH
+
+_lib = None
+
+_CTX = C.c_void_p
+_PROTOS = {
+    "frt_version": (C.c_char_p, []),
+    "frt_config_default": (C.c_int, [C.POINTER(FrtConfig), C.c_int32]),
+    "frt_create": (C.c_int, [C.POINTER(FrtConfig), C.POINTER(_CTX)]),
+    "frt_destroy": (C.c_int, [_CTX]),
+    "frt_last_error": (C.c_char_p, [_CTX]),
+    "frt_set_spectrum": (C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_float)]),
+    "frt_set_sources": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "frt_upload_field": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_float)]),
+    "frt_download_field": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_float)]),
+    "frt_set_time": (C.c_int, [_CTX, C.c_float]),
+    "frt_get_time": (C.c_int, [_CTX, C.POINTER(C.c_float)]),
+    "frt_init_rt": (C.c_int, [_CTX]),
+    "frt_get_nphot_min_max": (C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "frt_new_rays": (C.c_int, [_CTX]),
+    "frt_update_rays": (C.c_int, [_CTX]),
+    "frt_find_avg_unu": (C.c_int, [_CTX]),
+    "frt_merge_rays": (C.c_int, [_CTX]),
+    "frt_split_rays": (C.c_int, [_CTX]),
+    "frt_update_thermal": (C.c_int, [_CTX]),
+    "frt_step": (C.c_int, [_CTX, C.POINTER(FrtStepStats)]),
+    "frt_get_stats": (C.c_int, [_CTX, C.POINTER(FrtStepStats)]),
+    "frt_ray_count": (C.c_int, [_CTX, C.POINTER(C.c_int64)]),
+    "frt_download_rays": (C.c_int, [_CTX, C.POINTER(FrtRayView)]),
+    "frt_upload_rays": (C.c_int, [_CTX, C.POINTER(FrtRayView)]),
+    "frt_download_paths": (C.c_int, [_CTX, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_float)]),
+    "frt_set_profiling": (C.c_int, [_CTX, C.c_int32]),
+    "frt_kernel_time": (C.c_int, [_CTX, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "frt_reset_kernel_times": (C.c_int, [_CTX]),
+    "frt_deposit_bytes": (C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "frt_copy_kernel": (C.c_int, [_CTX, C.c_int64, C.c_int32]),
+    "frt_synchronize": (C.c_int, [_CTX]),
+}
+
+EXPORTED = sorted(_PROTOS)
+
+
+def lib_path():
+    return LIB_PATH
+
+
+def load():
+    """Load the built library (raises FileNotFoundError if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            "libflexrt_gpu.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

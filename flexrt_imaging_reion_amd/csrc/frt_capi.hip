@@ -1,0 +1,779 @@
+// frt_capi.hip -- context management and the extern "C" entry points.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "frt_internal.h"
+
+namespace frt {
+
+int set_error(frt_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_check(frt_ctx* ctx, hipError_t e, const char* what) {
+  if (e == hipSuccess) return FRT_OK;
+  return set_error(ctx, FRT_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int ensure_cub_tmp(frt_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->cub_tmp_bytes) return FRT_OK;
+  hipFree(ctx->cub_tmp);
+  ctx->cub_tmp = nullptr;
+  size_t want = std::max(bytes, ctx->cub_tmp_bytes * 2);
+  hipError_t e = hipMalloc(&ctx->cub_tmp, want);
+  if (e != hipSuccess) {
+    ctx->cub_tmp_bytes = 0;
+    return set_error(ctx, FRT_E_NOMEM, "scan/sort workspace");
+  }
+  ctx->cub_tmp_bytes = want;
+  return FRT_OK;
+}
+
+void prof_begin(frt_ctx* ctx, const char* name, ProfRec* rec) {
+  rec->name = nullptr;
+  if (!ctx->prof) return;
+  if (ctx->event_pool.size() < 2) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t ev;
+      if (hipEventCreate(&ev) == hipSuccess) ctx->event_pool.push_back(ev);
+    }
+  }
+  if (ctx->event_pool.size() < 2) return;
+  rec->a = ctx->event_pool.back();
+  ctx->event_pool.pop_back();
+  rec->b = ctx->event_pool.back();
+  ctx->event_pool.pop_back();
+  rec->name = name;
+  hipEventRecord(rec->a, ctx->stream);
+}
+
+void prof_end(frt_ctx* ctx, ProfRec* rec) {
+  if (!rec->name) return;
+  hipEventRecord(rec->b, ctx->stream);
+  ctx->prof_pending.push_back(*rec);
+}
+
+void prof_resolve(frt_ctx* ctx) {
+  for (auto& r : ctx->prof_pending) {
+    hipEventSynchronize(r.b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, r.a, r.b);
+    auto& acc = ctx->prof_acc[r.name];
+    acc.first += (double)ms;
+    acc.second += 1;
+    ctx->event_pool.push_back(r.a);
+    ctx->event_pool.push_back(r.b);
+  }
+  ctx->prof_pending.clear();
+}
+
+}  // namespace frt
+
+using namespace frt;
+
+namespace {
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  return hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+}
+
+int alloc_rays(frt_ctx* ctx, RaySoA& b, int64_t cap, int nfreq) {
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = dalloc(&b.cell, cap);
+  if (e == hipSuccess) e = dalloc(&b.pixel, cap);
+  if (e == hipSuccess) e = dalloc(&b.order, cap);
+  if (e == hipSuccess) e = dalloc(&b.itheta, cap);
+  if (e == hipSuccess) e = dalloc(&b.xhat, cap);
+  if (e == hipSuccess) e = dalloc(&b.yhat, cap);
+  if (e == hipSuccess) e = dalloc(&b.zhat, cap);
+  if (e == hipSuccess) e = dalloc(&b.nphot, cap * nfreq);
+  if (e == hipSuccess) e = dalloc(&b.r, cap);
+  if (e == hipSuccess) e = dalloc(&b.dr, cap);
+  if (e == hipSuccess) e = dalloc(&b.x, cap);
+  if (e == hipSuccess) e = dalloc(&b.y, cap);
+  if (e == hipSuccess) e = dalloc(&b.z, cap);
+  if (e == hipSuccess) e = dalloc(&b.time, cap);
+  if (e != hipSuccess) return set_error(ctx, FRT_E_NOMEM, std::string("ray buffer: ") + hipGetErrorString(e));
+  return FRT_OK;
+}
+
+void free_rays(RaySoA& b) {
+  hipFree(b.cell); hipFree(b.pixel); hipFree(b.order); hipFree(b.itheta);
+  hipFree(b.xhat); hipFree(b.yhat); hipFree(b.zhat); hipFree(b.nphot);
+  hipFree(b.r); hipFree(b.dr); hipFree(b.x); hipFree(b.y); hipFree(b.z); hipFree(b.time);
+  memset(&b, 0, sizeof(b));
+}
+
+int read_counters(frt_ctx* ctx) {
+  int rc = hip_check(ctx, hipMemcpyAsync(ctx->hcnt, ctx->dcnt, sizeof(DevCounters), hipMemcpyDeviceToHost, ctx->stream), "counters");
+  if (rc) return rc;
+  return hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+}
+
+int reset_counters(frt_ctx* ctx) {
+  return hip_check(ctx, hipMemsetAsync(ctx->dcnt, 0, sizeof(DevCounters), ctx->stream), "reset counters");
+}
+
+// Rays held after removals: drop dead rays (no splits) so the buffer holds live rays only.
+int ensure_compact(frt_ctx* ctx) {
+  if (!ctx->have_alive) return FRT_OK;
+  const int64_t n = ctx->nrays;
+  std::vector<int32_t> dummy;
+  // reuse the split path with splitting disabled: threshold +inf
+  const float saved_merge = ctx->nphot_merge;
+  const double saved_min = ctx->nphot_min;
+  ctx->nphot_merge = INFINITY;
+  int64_t after = 0;
+  int rc = launch_split_compact(ctx, &after);
+  ctx->nphot_merge = saved_merge;
+  ctx->nphot_min = saved_min;
+  ctx->have_alive = false;
+  (void)n;
+  return rc;
+}
+
+int check_ready(frt_ctx* ctx) {
+  if (!ctx) return FRT_E_ARG;
+  if (!ctx->spectrum_set) return set_error(ctx, FRT_E_STATE, "frt_set_spectrum has not been called");
+  return FRT_OK;
+}
+
+int do_update_rays(frt_ctx* ctx) {
+  int rc = ensure_compact(ctx);
+  if (rc) return rc;
+  ctx->g.t = ctx->t;
+  ctx->stats.rays_start = ctx->nrays;
+  if ((rc = launch_add_ray_dt(ctx))) return rc;
+  if ((rc = launch_march(ctx))) return rc;
+  if ((rc = launch_segments(ctx))) return rc;
+  if ((rc = read_counters(ctx))) return rc;
+  ctx->stats.rays_moved = (int64_t)ctx->hcnt->moved;
+  ctx->stats.segments = ctx->nseg;
+  if (ctx->hcnt->path_overflow)
+    return set_error(ctx, FRT_E_PATH_OVERFLOW,
+                     "a ray crossed more than path_length cells in one step; raise path_length");
+  ctx->have_segments = true;
+  ctx->have_alive = true;
+  return FRT_OK;
+}
+
+int do_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
+  if (!ctx->have_segments) return set_error(ctx, FRT_E_STATE, "find_avg_unu needs update_rays first");
+  ctx->g.t = ctx->t;
+  ctx->g.nphot_min = ctx->nphot_min;
+  int rc = launch_find_avg_unu(ctx, fuse_thermal);
+  if (rc) return rc;
+  ctx->have_segments = false;
+  ctx->have_alive = true;
+  return FRT_OK;
+}
+
+int fill_alive(frt_ctx* ctx) {
+  if (ctx->have_alive) return FRT_OK;
+  if (ctx->nrays > 0) {
+    int rc = hip_check(ctx, hipMemsetAsync(ctx->alive, 1, (size_t)ctx->nrays, ctx->stream), "alive");
+    if (rc) return rc;
+  }
+  ctx->have_alive = true;
+  return FRT_OK;
+}
+
+int do_merge(frt_ctx* ctx) {
+  int rc = fill_alive(ctx);
+  if (rc) return rc;
+  ctx->g.nphot_min = ctx->nphot_min;
+  return launch_merge(ctx);
+}
+
+int do_split(frt_ctx* ctx) {
+  int rc = fill_alive(ctx);
+  if (rc) return rc;
+  int64_t after = 0;
+  rc = launch_split_compact(ctx, &after);
+  ctx->have_alive = false;
+  return rc;
+}
+
+int finish_stats(frt_ctx* ctx) {
+  int rc = read_counters(ctx);
+  if (rc) return rc;
+  DevCounters* h = ctx->hcnt;
+  ctx->stats.rays_dumped = (int64_t)h->dumped;
+  ctx->stats.rays_terminated = (int64_t)h->terminated;
+  ctx->stats.merge_groups = (int64_t)h->groups;
+  ctx->stats.rays_merged_away = (int64_t)h->merged_away;
+  ctx->stats.rays_split = (int64_t)h->splits;
+  ctx->stats.thermal_substeps = (int64_t)h->substeps;
+  ctx->stats.rays_end = ctx->nrays;
+  ctx->stats.nphot_min = ctx->nphot_min;
+  ctx->stats.nphot_max = ctx->nphot_max;
+  ctx->stats.nphot_merge = ctx->nphot_merge;
+  ctx->stats.t = ctx->t;
+  ctx->stats.step = ctx->step;
+  if (ctx->prof) prof_resolve(ctx);
+  return FRT_OK;
+}
+
+float* field_ptr(frt_ctx* ctx, int field) {
+  if (field >= 0 && field < C_NSCALAR) return ctx->c.f[field];
+  if (field == FRT_F_GAMMA_H1_NU) return ctx->c.gamma_h1_nu;
+  if (field == FRT_F_GAMMA_HE1_NU) return ctx->c.gamma_he1_nu;
+  if (field == FRT_F_NGAM_NU) return ctx->c.ngam_nu;
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* frt_version(void) { return "flexrt-mi355x 0.1 (gfx950)"; }
+
+int frt_config_default(frt_config* cfg, int32_t n) {
+  if (!cfg || n <= 0) return FRT_E_ARG;
+  memset(cfg, 0, sizeof(*cfg));
+  cfg->n = n;
+  cfg->nfreq = 1;
+  cfg->path_length = 4;
+  cfg->itercount = 5;
+  cfg->precision = FRT_PREC_FP64;
+  cfg->hpx_lvl = 0;
+  cfg->l_max = 12;
+  cfg->ray_per_cell_max = 22;
+  cfg->rec_case_b = 1;
+  cfg->temp_ev = 1;
+  cfg->recomb_cool = 1;
+  cfg->free_free = 1;
+  cfg->compton = 1;
+  cfg->frac_limit = 1;
+  cfg->ray_capacity = 0;
+  cfg->seed = 1;
+  cfg->box_ckpc = 40 * 1e3 / 0.68;
+  cfg->zz = 12.0038;
+  cfg->a_rt = 1. / (1. + 12.0038);
+  cfg->dt = 0.;  // 0: dt = dx / (cl_factor c) as update_dt (rt_funcs.cc:9-12)
+  cfg->cl_factor = 0.3;
+  cfg->nl_const = 0.5;
+  cfg->safety_factor = 2.;
+  cfg->min_dist = 1e-4;
+  cfg->temp_min = 1e2;
+  cfg->hh = 0.68;
+  cfg->omega_m = 0.305147;
+  cfg->omega_l = 0.6948;
+  return FRT_OK;
+}
+
+int frt_create(const frt_config* cfg_in, frt_ctx** out) {
+  if (!cfg_in || !out) return FRT_E_ARG;
+  *out = nullptr;
+  const frt_config& c = *cfg_in;
+  if (c.n < 2 || c.n > 1024 || c.nfreq < 1 || c.nfreq > FRT_MAX_FREQ || c.path_length < 1 ||
+      c.itercount < 1 || c.l_max < 0 || c.l_max >= FRT_MAX_LEVEL - 1 || c.hpx_lvl != 0 ||
+      (c.precision != FRT_PREC_FP64 && c.precision != FRT_PREC_MIXED) || !(c.a_rt > 0.) ||
+      !(c.box_ckpc > 0.))
+    return FRT_E_ARG;
+  const int64_t n3 = (int64_t)c.n * c.n * c.n;
+  const int64_t cap = c.ray_capacity > 0 ? c.ray_capacity : 30 * n3;
+  if (cap * (int64_t)c.path_length >= (1ll << 31) || cap >= (1ll << 31) - 1) return FRT_E_ARG;
+
+  frt_ctx* ctx = new frt_ctx();
+  ctx->cfg = c;
+  ctx->cur = 0;
+  ctx->nrays = 0;
+  hipGetDevice(&ctx->device);
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return FRT_E_HIP;
+  }
+
+  GridConst& g = ctx->g;
+  memset(&g, 0, sizeof(g));
+  g.n = c.n;
+  g.n3 = (int)n3;
+  g.nfreq = c.nfreq;
+  g.P = c.path_length;
+  g.cap = cap;
+  // dx0 = Lx0/Nx*kpc_to_cm and dx = (Lx0*a)/Nx*kpc_to_cm with float Lx0, a
+  // (global_variables.cc:69, gas_funcs.cc:592,710, init_funcs.cc:22-24)
+  const float Lx0 = (float)c.box_ckpc;
+  const float a_rt = (float)c.a_rt;
+  g.dx0 = (float)((double)(Lx0 / (float)c.n) * FRT_KPC_TO_CM);
+  const float Lx = Lx0 * a_rt;
+  g.dx = (float)((double)(Lx / (float)c.n) * FRT_KPC_TO_CM);
+  g.sf = g.dx / g.dx0;
+  const float clf = (float)c.cl_factor;
+  g.clight = (float)((double)clf * FRT_CL);
+  g.nc = (float)c.nl_const;
+  if (c.dt > 0.)
+    g.dt = (float)c.dt;
+  else
+    g.dt = (float)((double)(1.0f * g.dx / clf) / FRT_CL);
+  g.min_dist = (float)c.min_dist;
+  g.temp_min = (float)c.temp_min;
+  g.vol = (double)g.dx * (double)g.dx * (double)g.dx;
+  g.l_max = c.l_max;
+  g.hpx_lvl = c.hpx_lvl;
+  g.frac_limit = c.frac_limit;
+  g.rec_case_b = c.rec_case_b;
+  g.temp_ev = c.temp_ev;
+  g.recomb_cool = c.recomb_cool;
+  g.free_free = c.free_free;
+  g.compton = c.compton;
+  const float zz = (float)c.zz;
+  g.zz = zz;
+  g.hubble = frt_hubble((float)(1. / (1. + (double)zz)), c.hh, c.omega_m, c.omega_l);
+  g.compton_coef = frt_compton_rate((double)zz);
+  g.t_cmb = 2.726 * (double)(1.f + zz);
+  g.nhe_to_nh = (float)((FRT_Y_HE / 4.) / (1. - FRT_Y_HE));
+  const float safety = (float)c.safety_factor;
+  for (int l = 0; l < FRT_MAX_LEVEL; ++l) {
+    // rmax_from_l / rmin_from_l (ray_tracing.cc:66-79)
+    g.rmax_tab[l] = (float)((double)g.dx0 * dm_pow(2, l) * dm_pow(3. / FRT_PI_REF / (double)safety, 0.5));
+    g.rmin_tab[l] = (l == 0) ? 0.f : (float)((double)g.dx0 * dm_pow(2, l - 1) * dm_pow(3. / FRT_PI_REF / (double)safety, 0.5));
+  }
+  ctx->key_bits = 1;
+  while ((1ll << ctx->key_bits) < n3) ctx->key_bits++;
+  ctx->t = 0.f;
+  ctx->nphot_merge = 0.f;
+  ctx->nphot_min = 0.;
+  ctx->nphot_max = 0.;
+  ctx->step = 0;
+  ctx->cast_calls = 0;
+  ctx->spectrum_set = 0;
+  memset(&ctx->stats, 0, sizeof(ctx->stats));
+  ctx->prof = 0;
+  ctx->seg_cap = 0;
+  ctx->mcap = cap;
+  ctx->have_segments = false;
+  ctx->have_alive = false;
+
+  int rc = FRT_OK;
+  for (int b = 0; b < 2 && rc == FRT_OK; ++b) rc = alloc_rays(ctx, ctx->buf[b], cap, c.nfreq);
+  hipError_t e = hipSuccess;
+  const int P = c.path_length;
+  if (rc == FRT_OK) {
+    if (e == hipSuccess) e = dalloc(&ctx->path_cell, (size_t)cap * P);
+    if (e == hipSuccess) e = dalloc(&ctx->path_dist, (size_t)cap * P);
+    if (e == hipSuccess) e = dalloc(&ctx->seg_cnt, cap + 1);
+    if (e == hipSuccess) e = dalloc(&ctx->seg_off, cap + 1);
+    if (e == hipSuccess) e = dalloc(&ctx->alive, cap + 1);
+    if (e == hipSuccess) e = dalloc(&ctx->n_out, cap + 1);
+    if (e == hipSuccess) e = dalloc(&ctx->out_off, cap + 1);
+    if (e == hipSuccess) e = dalloc(&ctx->cand_flag, cap + 1);
+    if (e == hipSuccess) e = dalloc(&ctx->cand_off, cap + 1);
+    if (e == hipSuccess) e = dalloc(&ctx->cand_key_tmp, cap + 1);
+    for (int b = 0; b < 2; ++b) {
+      if (e == hipSuccess) e = dalloc(&ctx->m_key[b], cap + 1);
+      if (e == hipSuccess) e = dalloc(&ctx->m_val[b], cap + 1);
+    }
+    if (e == hipSuccess) e = dalloc(&ctx->pdf, FRT_PDF_BINS);
+    if (e == hipSuccess) e = dalloc(&ctx->cell_start, n3);
+    if (e == hipSuccess) e = dalloc(&ctx->cell_end, n3);
+    const size_t nf = (size_t)(C_NSCALAR + 3 * c.nfreq) * (size_t)n3;
+    if (e == hipSuccess) e = dalloc(&ctx->cell_block, nf);
+    if (e == hipSuccess) e = dalloc(&ctx->rot_fwd, 9 * FRT_NTHETA);
+    if (e == hipSuccess) e = dalloc(&ctx->rot_inv, 9 * FRT_NTHETA);
+    if (e == hipSuccess) e = dalloc(&ctx->dcnt, 1);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hcnt, sizeof(DevCounters), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hscalar, 64, hipHostMallocDefault);
+    if (e != hipSuccess) rc = set_error(ctx, FRT_E_NOMEM, std::string("allocation: ") + hipGetErrorString(e));
+  }
+  if (rc == FRT_OK) {
+    for (int f = 0; f < C_NSCALAR; ++f) ctx->c.f[f] = ctx->cell_block + (size_t)f * n3;
+    ctx->c.gamma_h1_nu = ctx->cell_block + (size_t)C_NSCALAR * n3;
+    ctx->c.gamma_he1_nu = ctx->c.gamma_h1_nu + (size_t)c.nfreq * n3;
+    ctx->c.ngam_nu = ctx->c.gamma_he1_nu + (size_t)c.nfreq * n3;
+    e = hipMemsetAsync(ctx->cell_block, 0, sizeof(float) * (size_t)(C_NSCALAR + 3 * c.nfreq) * n3, ctx->stream);
+    // clumping factor defaults to const_clump = 1 (user_inputs.h:87, set_clump subgrid.cc:799)
+    if (e == hipSuccess) {
+      std::vector<float> ones((size_t)n3, 1.0f);
+      e = hipMemcpy(ctx->c.f[C_CLUMP], ones.data(), sizeof(float) * n3, hipMemcpyHostToDevice);
+    }
+    std::vector<float> fwd(9 * FRT_NTHETA), inv(9 * FRT_NTHETA);
+    build_rotation_tables(fwd.data(), inv.data());
+    if (e == hipSuccess) e = hipMemcpy(ctx->rot_fwd, fwd.data(), 4 * fwd.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->rot_inv, inv.data(), 4 * inv.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->dcnt, 0, sizeof(DevCounters), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = set_error(ctx, FRT_E_HIP, std::string("init: ") + hipGetErrorString(e));
+  }
+  if (rc != FRT_OK) {
+    std::string msg = ctx->err;
+    frt_destroy(ctx);
+    fprintf(stderr, "frt_create: %s\n", msg.c_str());
+    return rc;
+  }
+  *out = ctx;
+  return FRT_OK;
+}
+
+int frt_destroy(frt_ctx* ctx) {
+  if (!ctx) return FRT_E_ARG;
+  hipStreamSynchronize(ctx->stream);
+  for (int b = 0; b < 2; ++b) free_rays(ctx->buf[b]);
+  hipFree(ctx->path_cell); hipFree(ctx->path_dist); hipFree(ctx->seg_cnt); hipFree(ctx->seg_off);
+  hipFree(ctx->alive); hipFree(ctx->n_out); hipFree(ctx->out_off); hipFree(ctx->cand_flag);
+  hipFree(ctx->cand_off); hipFree(ctx->cand_key_tmp);
+  for (int b = 0; b < 2; ++b) {
+    hipFree(ctx->seg_key[b]); hipFree(ctx->seg_val[b]); hipFree(ctx->m_key[b]); hipFree(ctx->m_val[b]);
+  }
+  hipFree(ctx->inv_perm); hipFree(ctx->contrib.gh); hipFree(ctx->pdf);
+  hipFree(ctx->cell_start); hipFree(ctx->cell_end); hipFree(ctx->cell_block);
+  hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
+  hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt);
+  if (ctx->hcnt) hipHostFree(ctx->hcnt);
+  if (ctx->hscalar) hipHostFree(ctx->hscalar);
+  hipFree(ctx->cub_tmp); hipFree(ctx->copy_src); hipFree(ctx->copy_dst);
+  for (auto& r : ctx->prof_pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+  for (auto ev : ctx->event_pool) hipEventDestroy(ev);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return FRT_OK;
+}
+
+const char* frt_last_error(const frt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int frt_set_spectrum(frt_ctx* ctx, const double* nu_phot, const float* spectrum) {
+  if (!ctx || !nu_phot || !spectrum) return FRT_E_ARG;
+  for (int nu = 0; nu < ctx->g.nfreq; ++nu) {
+    ctx->g.nu[nu] = nu_phot[nu];
+    ctx->g.spectrum[nu] = spectrum[nu];
+    ctx->g.sigH[nu] = frt_sigmapi_H1(nu_phot[nu]);
+    ctx->g.sigHe[nu] = frt_sigmapi_He1(nu_phot[nu]);
+    ctx->g.sigHf[nu] = (float)ctx->g.sigH[nu];
+    ctx->g.sigHef[nu] = (float)ctx->g.sigHe[nu];
+  }
+  ctx->spectrum_set = 1;
+  return FRT_OK;
+}
+
+int frt_set_sources(frt_ctx* ctx, int32_t nsrc, const int32_t* cell, const double* lum) {
+  if (!ctx || nsrc < 0 || (nsrc > 0 && (!cell || !lum))) return FRT_E_ARG;
+  for (int s = 0; s < nsrc; ++s)
+    if (cell[s] < 0 || cell[s] >= ctx->g.n3) return set_error(ctx, FRT_E_ARG, "source cell out of range");
+  hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
+  ctx->src_cell = nullptr; ctx->src_lum = nullptr; ctx->src_ray_dt = nullptr;
+  ctx->nsrc = nsrc;
+  ctx->h_src_cell.assign(cell, cell + nsrc);
+  ctx->h_src_lum.assign(lum, lum + nsrc);
+  hipError_t e = dalloc(&ctx->src_cell, nsrc);
+  if (e == hipSuccess) e = dalloc(&ctx->src_lum, nsrc);
+  if (e == hipSuccess) e = dalloc(&ctx->src_ray_dt, nsrc);
+  if (e == hipSuccess && nsrc > 0) e = hipMemcpy(ctx->src_cell, cell, 4 * (size_t)nsrc, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nsrc > 0) e = hipMemcpy(ctx->src_lum, lum, 8 * (size_t)nsrc, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nsrc > 0) e = hipMemset(ctx->src_ray_dt, 0, 4 * (size_t)nsrc);
+  return hip_check(ctx, e, "sources");
+}
+
+int frt_upload_field(frt_ctx* ctx, int32_t field, const float* host) {
+  if (!ctx || !host) return FRT_E_ARG;
+  float* d = field_ptr(ctx, field);
+  if (!d) return set_error(ctx, FRT_E_ARG, "unknown field");
+  const size_t n3 = (size_t)ctx->g.n3;
+  hipError_t e;
+  if (field < C_NSCALAR) {
+    e = hipMemcpy(d, host, 4 * n3, hipMemcpyHostToDevice);
+  } else {
+    const int F = ctx->g.nfreq;
+    std::vector<float> t((size_t)F * n3);
+    for (size_t k = 0; k < n3; ++k)
+      for (int nu = 0; nu < F; ++nu) t[(size_t)nu * n3 + k] = host[k * F + nu];
+    e = hipMemcpy(d, t.data(), 4 * t.size(), hipMemcpyHostToDevice);
+  }
+  return hip_check(ctx, e, "upload field");
+}
+
+int frt_download_field(frt_ctx* ctx, int32_t field, float* host) {
+  if (!ctx || !host) return FRT_E_ARG;
+  float* d = field_ptr(ctx, field);
+  if (!d) return set_error(ctx, FRT_E_ARG, "unknown field");
+  const size_t n3 = (size_t)ctx->g.n3;
+  int rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (rc) return rc;
+  hipError_t e;
+  if (field < C_NSCALAR) {
+    e = hipMemcpy(host, d, 4 * n3, hipMemcpyDeviceToHost);
+  } else {
+    const int F = ctx->g.nfreq;
+    std::vector<float> t((size_t)F * n3);
+    e = hipMemcpy(t.data(), d, 4 * t.size(), hipMemcpyDeviceToHost);
+    for (size_t k = 0; k < n3; ++k)
+      for (int nu = 0; nu < F; ++nu) host[k * F + nu] = t[(size_t)nu * n3 + k];
+  }
+  return hip_check(ctx, e, "download field");
+}
+
+int frt_set_time(frt_ctx* ctx, float t) {
+  if (!ctx) return FRT_E_ARG;
+  ctx->t = t;
+  return FRT_OK;
+}
+
+int frt_get_time(const frt_ctx* ctx, float* t) {
+  if (!ctx || !t) return FRT_E_ARG;
+  *t = ctx->t;
+  return FRT_OK;
+}
+
+int frt_init_rt(frt_ctx* ctx) {
+  if (!ctx) return FRT_E_ARG;
+  ctx->nrays = 0;
+  ctx->cur = 0;
+  ctx->have_alive = false;
+  ctx->have_segments = false;
+  if (ctx->nsrc > 0) {
+    std::vector<float> rdt((size_t)ctx->nsrc, ctx->g.dt);
+    hipError_t e = hipMemcpy(ctx->src_ray_dt, rdt.data(), 4 * (size_t)ctx->nsrc, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_check(ctx, e, "init_rt");
+  }
+  return FRT_OK;
+}
+
+int frt_get_nphot_min_max(frt_ctx* ctx, double* nphot_min, double* nphot_max) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  int counter = 0;
+  double avg = 0.;
+  for (int s = 0; s < ctx->nsrc; ++s) {
+    const double lum = ctx->h_src_lum[s];
+    if (lum > 0.) {
+      double tmp = 0.;
+      for (int nu = 0; nu < ctx->g.nfreq; ++nu)
+        tmp += (double)ctx->g.spectrum[nu] * lum * (double)ctx->g.dt / 12. / dm_pow(2., 2. * ctx->cfg.hpx_lvl) / FRT_PHOT_NORM;
+      avg += dm_log10(tmp);
+      counter += 1;
+    }
+  }
+  avg /= counter;
+  avg = dm_pow(10., avg);
+  ctx->nphot_max = 1e3 * avg;
+  ctx->nphot_min = 1e-10 * avg;
+  ctx->g.nphot_min = ctx->nphot_min;
+  if (nphot_min) *nphot_min = ctx->nphot_min;
+  if (nphot_max) *nphot_max = ctx->nphot_max;
+  return FRT_OK;
+}
+
+int frt_new_rays(frt_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = ensure_compact(ctx))) return rc;
+  const int64_t before = ctx->nrays;
+  if ((rc = launch_cast(ctx))) return rc;
+  ctx->stats.rays_new = ctx->nrays - before;
+  return hip_check(ctx, hipStreamSynchronize(ctx->stream), "new_rays");
+}
+
+int frt_update_rays(frt_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = reset_counters(ctx))) return rc;
+  return do_update_rays(ctx);
+}
+
+int frt_find_avg_unu(frt_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = do_find_avg_unu(ctx, false))) return rc;
+  return finish_stats(ctx);
+}
+
+int frt_merge_rays(frt_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = do_merge(ctx))) return rc;
+  return finish_stats(ctx);
+}
+
+int frt_split_rays(frt_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = do_split(ctx))) return rc;
+  return finish_stats(ctx);
+}
+
+int frt_update_thermal(frt_ctx* ctx) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = launch_thermal(ctx))) return rc;
+  return finish_stats(ctx);
+}
+
+int frt_step(frt_ctx* ctx, frt_step_stats* stats) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = reset_counters(ctx))) return rc;
+  ctx->t += ctx->g.dt;  // main.cc:123
+  if ((rc = do_update_rays(ctx))) return rc;
+  if ((rc = do_find_avg_unu(ctx, true))) return rc;
+  if ((rc = do_merge(ctx))) return rc;
+  if ((rc = do_split(ctx))) return rc;
+  const int64_t before = ctx->nrays;
+  if ((rc = launch_cast(ctx))) return rc;
+  ctx->stats.rays_new = ctx->nrays - before;
+  ctx->step += 1;
+  if ((rc = finish_stats(ctx))) return rc;
+  if (stats) *stats = ctx->stats;
+  return FRT_OK;
+}
+
+int frt_get_stats(const frt_ctx* ctx, frt_step_stats* stats) {
+  if (!ctx || !stats) return FRT_E_ARG;
+  *stats = ctx->stats;
+  return FRT_OK;
+}
+
+int frt_ray_count(const frt_ctx* ctx, int64_t* count) {
+  if (!ctx || !count) return FRT_E_ARG;
+  *count = ctx->nrays;
+  return FRT_OK;
+}
+
+int frt_download_rays(frt_ctx* ctx, frt_ray_view* v) {
+  if (!ctx || !v) return FRT_E_ARG;
+  int rc = ensure_compact(ctx);
+  if (rc) return rc;
+  const int64_t n = ctx->nrays;
+  if (v->count < n) {
+    v->count = n;
+    return set_error(ctx, FRT_E_ARG, "ray view too small");
+  }
+  v->count = n;
+  if ((rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync"))) return rc;
+  if (n == 0) return FRT_OK;
+  RaySoA& b = ctx->buf[ctx->cur];
+  hipError_t e = hipSuccess;
+#define CP(dst, src, sz) if (e == hipSuccess && (dst)) e = hipMemcpy(dst, src, (size_t)(sz) * n, hipMemcpyDeviceToHost)
+  CP(v->order, b.order, 2); CP(v->pixel, b.pixel, 4); CP(v->itheta, b.itheta, 2);
+  CP(v->xhat, b.xhat, 4); CP(v->yhat, b.yhat, 4); CP(v->zhat, b.zhat, 4);
+  CP(v->r, b.r, 4); CP(v->dr, b.dr, 4); CP(v->x, b.x, 4); CP(v->y, b.y, 4); CP(v->z, b.z, 4);
+  CP(v->time, b.time, 4); CP(v->cell, b.cell, 4);
+#undef CP
+  if (e == hipSuccess && v->nphot) {
+    const int F = ctx->g.nfreq;
+    std::vector<float> t((size_t)F * n);
+    for (int nu = 0; nu < F && e == hipSuccess; ++nu)
+      e = hipMemcpy(t.data() + (size_t)nu * n, b.nphot + (size_t)nu * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
+    for (int64_t i = 0; i < n; ++i)
+      for (int nu = 0; nu < F; ++nu) v->nphot[i * F + nu] = t[(size_t)nu * n + i];
+  }
+  return hip_check(ctx, e, "download rays");
+}
+
+int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* v) {
+  if (!ctx || !v || v->count < 0) return FRT_E_ARG;
+  const int64_t n = v->count;
+  if (n > ctx->g.cap) return set_error(ctx, FRT_E_CAPACITY, "more rays than capacity");
+  if (n > 0 && (!v->order || !v->pixel || !v->itheta || !v->xhat || !v->yhat || !v->zhat || !v->nphot ||
+                !v->r || !v->dr || !v->x || !v->y || !v->z || !v->time || !v->cell))
+    return FRT_E_ARG;
+  for (int64_t i = 0; i < n; ++i)
+    if (v->cell[i] < 0 || v->cell[i] >= ctx->g.n3 || v->order[i] < 0 || v->order[i] > ctx->cfg.l_max ||
+        v->itheta[i] >= FRT_NTHETA)
+      return set_error(ctx, FRT_E_ARG, "ray field out of range");
+  int rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (rc) return rc;
+  ctx->cur = 0;
+  RaySoA& b = ctx->buf[0];
+  hipError_t e = hipSuccess;
+#define CP(dst, src, sz) if (e == hipSuccess) e = hipMemcpy(dst, src, (size_t)(sz) * n, hipMemcpyHostToDevice)
+  if (n > 0) {
+    CP(b.order, v->order, 2); CP(b.pixel, v->pixel, 4); CP(b.itheta, v->itheta, 2);
+    CP(b.xhat, v->xhat, 4); CP(b.yhat, v->yhat, 4); CP(b.zhat, v->zhat, 4);
+    CP(b.r, v->r, 4); CP(b.dr, v->dr, 4); CP(b.x, v->x, 4); CP(b.y, v->y, 4); CP(b.z, v->z, 4);
+    CP(b.time, v->time, 4); CP(b.cell, v->cell, 4);
+    const int F = ctx->g.nfreq;
+    std::vector<float> t((size_t)F * n);
+    for (int64_t i = 0; i < n; ++i)
+      for (int nu = 0; nu < F; ++nu) t[(size_t)nu * n + i] = v->nphot[i * F + nu];
+    for (int nu = 0; nu < F && e == hipSuccess; ++nu)
+      e = hipMemcpy(b.nphot + (size_t)nu * ctx->g.cap, t.data() + (size_t)nu * n, 4 * (size_t)n, hipMemcpyHostToDevice);
+  }
+#undef CP
+  ctx->nrays = n;
+  ctx->have_alive = false;
+  ctx->have_segments = false;
+  return hip_check(ctx, e, "upload rays");
+}
+
+int frt_download_paths(frt_ctx* ctx, int64_t n, int32_t* ray_end, int32_t* cells, float* dists) {
+  if (!ctx || !ray_end || !cells || !dists) return FRT_E_ARG;
+  if (!ctx->have_segments) return set_error(ctx, FRT_E_STATE, "no paths: call update_rays first");
+  if (n != ctx->nrays) return set_error(ctx, FRT_E_ARG, "n must equal the ray count");
+  int rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (rc || n == 0) return rc;
+  const int P = ctx->g.P;
+  std::vector<int32_t> c((size_t)P * n);
+  std::vector<float> d((size_t)P * n);
+  hipError_t e = hipMemcpy(ray_end, ctx->seg_cnt, 4 * (size_t)n, hipMemcpyDeviceToHost);
+  for (int p = 0; p < P && e == hipSuccess; ++p) {
+    e = hipMemcpy(c.data() + (size_t)p * n, ctx->path_cell + (size_t)p * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(d.data() + (size_t)p * n, ctx->path_dist + (size_t)p * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
+  }
+  if (e != hipSuccess) return hip_check(ctx, e, "download paths");
+  for (int64_t i = 0; i < n; ++i)
+    for (int p = 0; p < P; ++p) {
+      const bool used = p < ray_end[i];
+      cells[i * P + p] = used ? c[(size_t)p * n + i] : -1;
+      dists[i * P + p] = used ? d[(size_t)p * n + i] : 0.f;
+    }
+  return FRT_OK;
+}
+
+int frt_set_profiling(frt_ctx* ctx, int32_t enable) {
+  if (!ctx) return FRT_E_ARG;
+  ctx->prof = enable ? 1 : 0;
+  return FRT_OK;
+}
+
+int frt_kernel_time(const frt_ctx* ctx, const char* name, double* total_ms, int64_t* launches) {
+  if (!ctx || !name) return FRT_E_ARG;
+  auto it = ctx->prof_acc.find(name);
+  if (total_ms) *total_ms = (it == ctx->prof_acc.end()) ? 0. : it->second.first;
+  if (launches) *launches = (it == ctx->prof_acc.end()) ? 0 : it->second.second;
+  return FRT_OK;
+}
+
+int frt_reset_kernel_times(frt_ctx* ctx) {
+  if (!ctx) return FRT_E_ARG;
+  prof_resolve(ctx);
+  ctx->prof_acc.clear();
+  return FRT_OK;
+}
+
+int frt_deposit_bytes(const frt_ctx* ctx, double* bytes_per_launch, int64_t* launches) {
+  if (!ctx) return FRT_E_ARG;
+  const int64_t l = ctx->deposit_launches_last;
+  if (bytes_per_launch) *bytes_per_launch = l ? ctx->deposit_bytes_last / (double)l : 0.;
+  if (launches) *launches = l;
+  return FRT_OK;
+}
+
+int frt_copy_kernel(frt_ctx* ctx, int64_t nbytes, int32_t repeats) {
+  if (!ctx || nbytes <= 0 || repeats < 1) return FRT_E_ARG;
+  if (nbytes != ctx->copy_bytes) {
+    hipFree(ctx->copy_src);
+    hipFree(ctx->copy_dst);
+    ctx->copy_src = ctx->copy_dst = nullptr;
+    hipError_t e = hipMalloc(&ctx->copy_src, (size_t)nbytes);
+    if (e == hipSuccess) e = hipMalloc(&ctx->copy_dst, (size_t)nbytes);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->copy_src, 0, (size_t)nbytes, ctx->stream);
+    if (e != hipSuccess) return hip_check(ctx, e, "copy buffers");
+    ctx->copy_bytes = nbytes;
+  }
+  int rc = launch_copy(ctx, nbytes, repeats);
+  if (rc) return rc;
+  rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "copy");
+  if (ctx->prof) prof_resolve(ctx);
+  return rc;
+}
+
+int frt_synchronize(frt_ctx* ctx) {
+  if (!ctx) return FRT_E_ARG;
+  return hip_check(ctx, hipStreamSynchronize(ctx->stream), "synchronize");
+}
+
+}  // extern "C"

@@ -1,0 +1,227 @@
+// frt_internal.h -- device state and kernel launchers of the RT library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/flexrt_gpu.h"
+#include "frt_detmath.h"
+#include "frt_healpix.h"
+#include "frt_rates.h"
+
+#define FRT_NALPHA 35
+#define FRT_PHOT_NORM 1e60
+#define FRT_PDF_BINS 10000
+#define FRT_MAX_LEVEL 16
+
+// Ray buffer in structure-of-arrays form; nphot is bin-major [nfreq][cap].
+struct RaySoA {
+  int32_t* cell;
+  int32_t* pixel;
+  int16_t* order;
+  uint16_t* itheta;
+  float* xhat;
+  float* yhat;
+  float* zhat;
+  float* nphot;
+  float* r;
+  float* dr;
+  float* x;
+  float* y;
+  float* z;
+  float* time;
+};
+
+// Scalars every kernel may need, passed by value.
+struct GridConst {
+  int n;            // cells per side
+  int n3;           // n^3
+  int nfreq;
+  int P;            // path_length
+  int64_t cap;      // ray capacity (stride of per-ray arrays)
+  float dx;         // proper cell size, cm (dx[i], uniform)
+  float dx0;        // comoving cell size, cm
+  float sf;         // dx[0] / dx0
+  float clight;     // cl_factor * cl (set_clight)
+  float nc;         // nl_const
+  float dt;         // time step, s
+  float min_dist;
+  float temp_min;
+  float t;          // simulation time after t += dt
+  double vol;       // dx*dy*dz as double
+  double nphot_min;
+  float nphot_merge;
+  int l_max;
+  int hpx_lvl;
+  int frac_limit;
+  int rec_case_b;
+  int temp_ev, recomb_cool, free_free, compton;
+  float hubble;         // H(1/(1+zz)) as float
+  double compton_coef;  // compton_rate(zz)
+  double t_cmb;         // 2.726*(1 + zz)
+  float zz;
+  float nhe_to_nh;
+  double sigH[FRT_MAX_FREQ];
+  double sigHe[FRT_MAX_FREQ];
+  float sigHf[FRT_MAX_FREQ];
+  float sigHef[FRT_MAX_FREQ];
+  double nu[FRT_MAX_FREQ];
+  float spectrum[FRT_MAX_FREQ];
+  float rmax_tab[FRT_MAX_LEVEL];
+  float rmin_tab[FRT_MAX_LEVEL];
+};
+
+// Device-side counters of one step (integer reductions only).
+struct DevCounters {
+  unsigned long long moved;
+  unsigned long long dumped;
+  unsigned long long terminated;
+  unsigned long long alive;
+  unsigned long long merged_away;
+  unsigned long long groups;
+  unsigned long long splits;
+  unsigned long long substeps;
+  unsigned int path_overflow;
+  unsigned int pad;
+};
+
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+// Cell fields, each n^3 floats (per-bin fields bin-major [nfreq][n^3]).
+enum CellField {
+  C_NH = 0, C_NHE, C_NH1, C_NHE1, C_NH2, C_NHE2, C_NE, C_NTOT, C_FH1, C_FHE1, C_FH2, C_FHE2,
+  C_TEMP, C_DNE_DT, C_CLUMP, C_NH1_PREV, C_AVG_NH1, C_AVG_NHE1, C_RECOMB_H2, C_RECOMB_HE2,
+  C_GAMMA_H1_TOT, C_GAMMA_HE1_TOT, C_RECRATE_H2, C_RECRATE_HE2, C_TREION, C_TREION_FLAG,
+  C_HEAT, C_COOL, C_IF_SPEED, C_INC_FLUX, C_NSCALAR
+};
+
+struct CellPtrs {
+  float* f[C_NSCALAR];
+  float* gamma_h1_nu;   // [nfreq][n3]
+  float* gamma_he1_nu;  // [nfreq][n3]
+  float* ngam_nu;       // [nfreq][n3]
+};
+
+// Contributions of path segments to their cell, stored in cell-sorted order.
+struct ContribPtrs {
+  void* gh;    // sum over bins of the HI deposit, per segment
+  void* ghe;   // sum over bins of the HeI deposit
+  void* ghnu;  // [nfreq][seg_cap] per-bin HI deposit (last pass)
+  void* ghenu; // [nfreq][seg_cap]
+  void* ngnu;  // [nfreq][seg_cap] photons entering the segment / phot_norm
+  int64_t stride;
+};
+
+struct frt_ctx {
+  frt_config cfg;
+  GridConst g;
+  hipStream_t stream;
+  int device;
+  std::string err;
+
+  // rays
+  RaySoA buf[2];
+  int cur;
+  int64_t nrays;
+  int32_t* path_cell;  // [P][cap]
+  float* path_dist;    // [P][cap]
+  int32_t* seg_cnt;    // [cap+1] segments per ray (0 if not moved)
+  int32_t* seg_off;    // [cap+1]
+  uint8_t* alive;      // [cap]
+  int32_t* n_out;      // [cap+1]
+  int32_t* out_off;    // [cap+1]
+  int32_t* cand_flag;  // [cap+1]
+  int32_t* cand_off;   // [cap+1]
+  uint32_t* cand_key_tmp;  // [cap]
+  bool have_segments;  // update_rays ran and find_avg_unu has not yet consumed the paths
+  bool have_alive;     // find_avg_unu ran in this step (alive flags valid)
+  int64_t nseg;
+
+  // segment sort
+  int64_t seg_cap;
+  uint32_t* seg_key[2];
+  int32_t* seg_val[2];
+  int32_t* inv_perm;
+  ContribPtrs contrib;
+  int32_t* cell_start;
+  int32_t* cell_end;
+  int key_bits;
+
+  // merge
+  int64_t mcap;
+  uint32_t* m_key[2];
+  int32_t* m_val[2];
+  int32_t* pdf;  // [FRT_PDF_BINS]
+
+  // cells
+  CellPtrs c;
+  float* cell_block;
+
+  // sources
+  int32_t nsrc;
+  int32_t* src_cell;
+  double* src_lum;
+  float* src_ray_dt;
+  std::vector<int32_t> h_src_cell;
+  std::vector<double> h_src_lum;
+
+  // tables
+  float* rot_fwd;  // [NTHETA][9]
+  float* rot_inv;  // [NTHETA][9]
+
+  // scalars of the reference's global state
+  float t;
+  float nphot_merge;
+  double nphot_min, nphot_max;
+  int64_t step;
+  uint64_t cast_calls;
+  int spectrum_set;
+  frt_step_stats stats;
+
+  DevCounters* dcnt;       // device
+  DevCounters* hcnt;       // pinned host mirror
+  int64_t* hscalar;        // pinned host scratch
+  void* cub_tmp;
+  size_t cub_tmp_bytes;
+
+  // profiling
+  int prof;
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, std::pair<double, int64_t>> prof_acc;
+  double deposit_bytes_last;
+  int64_t deposit_launches_last;
+  void* copy_src;
+  void* copy_dst;
+  int64_t copy_bytes;
+};
+
+// ---- kernel launchers (frt_kernels.hip) ---------------------------------------
+namespace frt {
+void build_rotation_tables(float* fwd, float* inv);
+uint32_t itheta_hash(uint64_t seed, uint64_t stream, uint64_t counter);
+
+int launch_march(frt_ctx* ctx);
+int launch_segments(frt_ctx* ctx);
+int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal);
+int launch_thermal(frt_ctx* ctx);
+int launch_merge(frt_ctx* ctx);
+int launch_split_compact(frt_ctx* ctx, int64_t* n_after);
+int launch_cast(frt_ctx* ctx);
+int launch_add_ray_dt(frt_ctx* ctx);
+int launch_copy(frt_ctx* ctx, int64_t nbytes, int repeats);
+
+// helpers shared with the C-ABI file
+int ensure_cub_tmp(frt_ctx* ctx, size_t bytes);
+int set_error(frt_ctx* ctx, int code, const std::string& msg);
+int hip_check(frt_ctx* ctx, hipError_t e, const char* what);
+void prof_begin(frt_ctx* ctx, const char* name, ProfRec* rec);
+void prof_end(frt_ctx* ctx, ProfRec* rec);
+void prof_resolve(frt_ctx* ctx);
+}  // namespace frt

@@ -1,0 +1,1498 @@
+// frt_kernels.hip -- gfx950 kernels of the per-timestep RT core.
+//
+// Data layout in HBM
+//   rays    : structure of arrays, one ray per index, list order = the order
+//             in which the reference's serial slot scan would meet them
+//             (survivors keep their order, split children replace their
+//             parent, new casts append in source order);
+//   paths   : path_cell/path_dist [path_length][capacity], written once per
+//             step by the march and re-read by every C2Ray pass;
+//   segments: stable-sorted by cell once per step; every pass writes each
+//             segment's deposit into its cell-sorted slot and one thread per
+//             cell sums its run in ray order (no float atomics, so the sums
+//             do not depend on scheduling);
+//   cells   : one float array per reference field, [nfreq][n^3] for per-bin
+//             fields.
+// Every floating-point expression keeps the reference's float/double
+// promotions (noted per line); the library is built with -ffp-contract=off.
+#include <hipcub/hipcub.hpp>
+
+#include "frt_internal.h"
+#include "frt_rng.h"
+
+namespace frt {
+
+static constexpr int BLK = 256;
+
+static inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+__device__ __forceinline__ int wrap_mod(int a, int b) {
+  while (a < 0) a += b;
+  return a % b;
+}
+
+// Add the number of set flags of this wavefront to *ctr (one atomic per wave).
+// Must be reached by every lane of the wave.
+__device__ __forceinline__ void wave_count(unsigned long long* ctr, bool flag) {
+  unsigned long long m = __ballot(flag ? 1 : 0);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
+
+// v' = M v in float with the reference's left-to-right sums (healpix_funcs.cc:112-133).
+__device__ __forceinline__ void rotate3(const float* m, float x, float y, float z, float* ox,
+                                        float* oy, float* oz) {
+  *ox = m[0] * x + m[1] * y + m[2] * z;
+  *oy = m[3] * x + m[4] * y + m[5] * z;
+  *oz = m[6] * x + m[7] * y + m[8] * z;
+}
+
+// sum_nphot (io_funcs.cc:10-16): float sum over bins in bin order.
+__device__ __forceinline__ float sum_nphot(const float* nphot, int64_t cap, int nfreq, int64_t i) {
+  float s = 0.f;
+  for (int nu = 0; nu < nfreq; ++nu) s += nphot[nu * cap + i];
+  return s;
+}
+
+// Corner correction after a crossing (ray_tracing.cc:1400-1417).
+__device__ __forceinline__ float corner(float v, float dx, float min_dist) {
+  if (v <= 0.f) v = min_dist * dx;                                      // float*float
+  if ((double)v >= (double)dx * (1. - (double)min_dist / 2.))          // double compare
+    v = (float)((double)dx * (1. - (double)min_dist));
+  return v;
+}
+
+// Crossing time to the cell face along one axis (ray_tracing.cc:1274-1302).
+__device__ __forceinline__ float face_time(float hat, float pos, float dx, float clgt) {
+  if (hat > 0.f) return (dx - pos) / hat / clgt;                         // float
+  if (hat < 0.f) return (float)(((-1. * (double)pos) / (double)hat) / (double)clgt);  // -1.* is double
+  return INFINITY;                                                       // 9e99 stored in float
+}
+
+// ---------------------------------------------------------------------------
+// advance_ray (ray_tracing.cc:1222-1512), C2Ray mode, fion = 1.
+__global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridConst g,
+                                               int32_t* __restrict__ path_cell,
+                                               float* __restrict__ path_dist,
+                                               int32_t* __restrict__ seg_cnt,
+                                               uint8_t* __restrict__ alive,
+                                               DevCounters* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  bool moved = false;
+  bool overflow = false;
+  if (i < nrays) {
+    alive[i] = 1;
+    const int n = g.n;
+    const int ind = rb.cell[i];
+    const int ci = ind / (n * n);
+    const int rem = ind - ci * n * n;
+    const int cj = rem / n;
+    const int ck = rem - cj * n;
+    const float xhat = rb.xhat[i], yhat = rb.yhat[i], zhat = rb.zhat[i];
+    float x0 = rb.x[i], y0 = rb.y[i], z0 = rb.z[i];
+    const float deltat = rb.time[i] + g.dt;  // ray[x].time += dt
+    float dist = g.clight * deltat;
+    const float hi = g.nc * g.dx;                             // nc*dx[i] (float)
+    const double lo = (1. - (double)g.nc) * (double)g.dx;     // (1. - nc)*dx[i] (double)
+    const float ax = x0 + dist * xhat, ay = y0 + dist * yhat, az = z0 + dist * zhat;
+    moved = (ax > hi) || (ay > hi) || (az > hi) || ((double)ax < lo) || ((double)ay < lo) ||
+            ((double)az < lo);
+    if (!moved) {
+      rb.time[i] = deltat;
+      seg_cnt[i] = 0;
+    } else {
+      float r0 = rb.r[i];
+      float dr = rb.dr[i];
+      r0 += dist / g.sf;  // comoving radius
+      dr -= dist / g.sf;
+      int inew = ci, jnew = cj, knew = ck;
+      int p = 0;
+      float tlast = 0.f, tmin = 0.f;
+      int guard = 0;
+      while (tlast < deltat) {
+        const float clgt = g.clight;
+        const float tx = face_time(xhat, x0, g.dx, clgt);
+        const float ty = face_time(yhat, y0, g.dx, clgt);
+        const float tz = face_time(zhat, z0, g.dx, clgt);
+        int tag;
+        if ((tx <= ty) && (tx <= tz)) {
+          tmin = tx;
+          tag = 0;
+        } else if ((ty <= tx) && (ty <= tz)) {
+          tmin = ty;
+          tag = 1;
+        } else {
+          tmin = tz;
+          tag = 2;
+        }
+        if ((tlast + tmin) < deltat) {
+          dist = tmin * clgt;
+          if (p < g.P) {
+            path_cell[(int64_t)p * g.cap + i] = inew * n * n + jnew * n + knew;
+            path_dist[(int64_t)p * g.cap + i] = dist;
+          } else {
+            overflow = true;
+          }
+          p += 1;
+          x0 += dist * xhat;
+          y0 += dist * yhat;
+          z0 += dist * zhat;
+          const float snap_lo = g.min_dist * g.dx;
+          const float snap_hi = (float)((double)g.dx * (1. - (double)g.min_dist));
+          if (tag == 0) {
+            if (xhat > 0.f) {
+              inew = wrap_mod(inew + 1, n);
+              x0 = snap_lo;
+            } else {
+              inew = wrap_mod(inew - 1, n);
+              x0 = snap_hi;
+            }
+          } else if (tag == 1) {
+            if (yhat > 0.f) {
+              jnew = wrap_mod(jnew + 1, n);
+              y0 = snap_lo;
+            } else {
+              jnew = wrap_mod(jnew - 1, n);
+              y0 = snap_hi;
+            }
+          } else {
+            if (zhat > 0.f) {
+              knew = wrap_mod(knew + 1, n);
+              z0 = snap_lo;
+            } else {
+              knew = wrap_mod(knew - 1, n);
+              z0 = snap_hi;
+            }
+          }
+          x0 = corner(x0, g.dx, g.min_dist);
+          y0 = corner(y0, g.dx, g.min_dist);
+          z0 = corner(z0, g.dx, g.min_dist);
+        }
+        tlast += tmin;
+        if (++guard > g.P + 64) {  // cannot happen for finite input; keeps the loop bounded
+          overflow = true;
+          break;
+        }
+      }
+      // last segment: remaining time in the final cell (fion = 1 branch, :1434-1437)
+      dist = (deltat - tlast + tmin) * g.clight;
+      dist = frt_maxd(dist, g.min_dist * g.dx);
+      if (p < g.P) {
+        path_cell[(int64_t)p * g.cap + i] = inew * n * n + jnew * n + knew;
+        path_dist[(int64_t)p * g.cap + i] = dist;
+      } else {
+        overflow = true;
+      }
+      seg_cnt[i] = (p + 1 < g.P) ? p + 1 : g.P;
+      x0 += dist * xhat;
+      y0 += dist * yhat;
+      z0 += dist * zhat;
+      x0 = corner(x0, g.dx, g.min_dist);
+      y0 = corner(y0, g.dx, g.min_dist);
+      z0 = corner(z0, g.dx, g.min_dist);
+      rb.r[i] = r0;
+      rb.dr[i] = dr;
+      rb.x[i] = x0;
+      rb.y[i] = y0;
+      rb.z[i] = z0;
+      rb.time[i] = 0.f;
+    }
+  }
+  wave_count(&cnt->moved, moved);
+  if (overflow) atomicOr(&cnt->path_overflow, 1u);
+}
+
+// Segment keys for the per-step cell sort.
+__global__ void __launch_bounds__(BLK) k_seg_keys(int64_t nrays, int64_t cap,
+                                                  const int32_t* __restrict__ seg_cnt,
+                                                  const int32_t* __restrict__ seg_off,
+                                                  const int32_t* __restrict__ path_cell,
+                                                  uint32_t* __restrict__ key,
+                                                  int32_t* __restrict__ val) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= nrays) return;
+  const int c = seg_cnt[i];
+  const int base = seg_off[i];
+  for (int p = 0; p < c; ++p) {
+    key[base + p] = (uint32_t)path_cell[(int64_t)p * cap + i];
+    val[base + p] = base + p;
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_fill_i32(int32_t* a, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict__ key, int64_t nseg,
+                                                     int32_t* __restrict__ start,
+                                                     int32_t* __restrict__ end) {
+  const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (s >= nseg) return;
+  const uint32_t k = key[s];
+  if (s == 0 || key[s - 1] != k) start[k] = (int32_t)s;
+  if (s == nseg - 1 || key[s + 1] != k) end[k] = (int32_t)(s + 1);
+}
+
+__global__ void __launch_bounds__(BLK) k_inv_perm(const int32_t* __restrict__ val, int64_t nseg,
+                                                  int32_t* __restrict__ inv) {
+  const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (s < nseg) inv[val[s]] = (int32_t)s;
+}
+
+// ---------------------------------------------------------------------------
+// find_avg_unu ray loop (ray_tracing.cc:701-938), one thread per moved ray.
+// FP64 form: the reference's double locals and libm exp (through detmath),
+// with the per-cell accumulation in double.
+template <bool LAST>
+__global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, GridConst g,
+                                                      const int32_t* __restrict__ path_cell,
+                                                      const float* __restrict__ path_dist,
+                                                      const int32_t* __restrict__ seg_cnt,
+                                                      const int32_t* __restrict__ seg_off,
+                                                      const int32_t* __restrict__ inv_perm,
+                                                      const float* __restrict__ avgH,
+                                                      const float* __restrict__ avgHe,
+                                                      ContribPtrs cp, uint8_t* __restrict__ alive,
+                                                      DevCounters* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  bool dumped_flag = false, removed = false;
+  const int c = (i < nrays) ? seg_cnt[i] : 0;
+  if (c > 0) {
+    const int F = g.nfreq;
+    double* cgh = (double*)cp.gh;
+    double* cghe = (double*)cp.ghe;
+    double nph[FRT_MAX_FREQ], tauh[FRT_MAX_FREQ], tauhe[FRT_MAX_FREQ];
+    for (int nu = 0; nu < F; ++nu) {
+      nph[nu] = (double)rb.nphot[nu * g.cap + i] * FRT_PHOT_NORM;
+      tauh[nu] = 0.;
+      tauhe[nu] = 0.;
+    }
+    const int base = seg_off[i];
+    const double dtd = (double)g.dt;
+    int p = 0;
+    bool dumped = false;
+    for (; p < c; ++p) {
+      const int cell = path_cell[(int64_t)p * g.cap + i];
+      const double dist = (double)path_dist[(int64_t)p * g.cap + i];
+      const float ah1 = avgH[cell];
+      const float ahe1 = avgHe[cell];
+      double dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ], nin[FRT_MAX_FREQ];
+      double tot = 0.;
+      for (int nu = 0; nu < F; ++nu) {
+        dth[nu] = ((double)ah1 * g.sigH[nu]) * dist;    // avg_nh1*sigma*dist
+        dthe[nu] = ((double)ahe1 * g.sigHe[nu]) * dist;
+        nin[nu] = nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]);
+        tot += nin[nu] / FRT_PHOT_NORM;
+      }
+      const bool dump = tot < g.nphot_min;
+      double gh_sum = 0., ghe_sum = 0.;
+      const int s = inv_perm[base + p];
+      for (int nu = 0; nu < F; ++nu) {
+        double fh = 0., fhe = 0., att = 0.;
+        const bool any = (dth[nu] > 0.) || (dthe[nu] > 0.);
+        double eh = 1., ehe = 1.;
+        if (any || !dump) {
+          eh = dm_exp(-dth[nu]);
+          ehe = (dthe[nu] == 0.) ? 1. : dm_exp(-dthe[nu]);  // exp(-0) = 1 exactly
+        }
+        if (any) {
+          const double a = (1. - eh) * ehe;
+          const double b = (1. - ehe) * eh;
+          fh = a / (a + b);
+          fhe = b / (a + b);
+          if (g.frac_limit && (a + b) == 0.) {
+            fh = 1. / (1. + dm_exp(dthe[nu] - dth[nu]));
+            fhe = 1. / (1. + dm_exp(dth[nu] - dthe[nu]));
+          }
+        }
+        double gh, ghe;
+        if (dump) {
+          gh = fh * nin[nu] / dtd / g.vol / (double)ah1;
+          ghe = fhe * nin[nu] / dtd / g.vol / (double)ahe1;
+        } else {
+          // exp(-dth - dthe): with dthe = 0 the argument is exactly -dth
+          att = 1. - ((dthe[nu] == 0.) ? eh : dm_exp(-dth[nu] - dthe[nu]));
+          gh = fh * nin[nu] * att / dtd / g.vol / (double)ah1;
+          ghe = fhe * nin[nu] * att / dtd / g.vol / (double)ahe1;
+        }
+        gh_sum += gh;
+        ghe_sum += ghe;
+        if (LAST) {
+          ((double*)cp.ghnu)[nu * cp.stride + s] = gh;
+          ((double*)cp.ghenu)[nu * cp.stride + s] = ghe;
+          ((double*)cp.ngnu)[nu * cp.stride + s] = nin[nu] / FRT_PHOT_NORM;
+        }
+        tauh[nu] += dth[nu];
+        tauhe[nu] += dthe[nu];
+      }
+      cgh[s] = gh_sum;
+      cghe[s] = ghe_sum;
+      if (dump) {
+        dumped = true;
+        ++p;
+        break;
+      }
+    }
+    // segments behind a dump deposit nothing this pass
+    for (; p < c; ++p) {
+      const int s = inv_perm[base + p];
+      cgh[s] = 0.;
+      cghe[s] = 0.;
+      if (LAST) {
+        for (int nu = 0; nu < F; ++nu) {
+          ((double*)cp.ghnu)[nu * cp.stride + s] = 0.;
+          ((double*)cp.ghenu)[nu * cp.stride + s] = 0.;
+          ((double*)cp.ngnu)[nu * cp.stride + s] = 0.;
+        }
+      }
+    }
+    if (LAST) {
+      // photon update and removal (ray_tracing.cc:923-935)
+      float sum = 0.f;
+      for (int nu = 0; nu < F; ++nu) {
+        const float v = (float)(nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]) / FRT_PHOT_NORM);
+        rb.nphot[nu * g.cap + i] = v;
+        sum += v;
+      }
+      dumped_flag = dumped;
+      if ((double)sum > g.nphot_min) {
+        rb.cell[i] = path_cell[(int64_t)(c - 1) * g.cap + i];
+      } else {
+        alive[i] = 0;
+        removed = true;
+      }
+    }
+  }
+  if (LAST) {
+    wave_count(&cnt->dumped, dumped_flag);
+    wave_count(&cnt->terminated, removed);
+  }
+}
+
+// Mixed form: FP32 attenuation and photon bookkeeping; per-cell sums are
+// compensated (Kahan) in the cell kernel.
+template <bool LAST>
+__global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays, GridConst g,
+                                                       const int32_t* __restrict__ path_cell,
+                                                       const float* __restrict__ path_dist,
+                                                       const int32_t* __restrict__ seg_cnt,
+                                                       const int32_t* __restrict__ seg_off,
+                                                       const int32_t* __restrict__ inv_perm,
+                                                       const float* __restrict__ avgH,
+                                                       const float* __restrict__ avgHe,
+                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
+                                                       DevCounters* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  bool dumped_flag = false, removed = false;
+  const int c = (i < nrays) ? seg_cnt[i] : 0;
+  if (c > 0) {
+    const int F = g.nfreq;
+    float* cgh = (float*)cp.gh;
+    float* cghe = (float*)cp.ghe;
+    float nph[FRT_MAX_FREQ], trans[FRT_MAX_FREQ];
+    for (int nu = 0; nu < F; ++nu) {
+      nph[nu] = rb.nphot[nu * g.cap + i];  // in units of phot_norm
+      trans[nu] = 1.f;                     // exp(-tau) carried multiplicatively
+    }
+    const int base = seg_off[i];
+    // 1/(dt vol) in units of phot_norm photons
+    const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
+    const float nmin = (float)g.nphot_min;
+    int p = 0;
+    bool dumped = false;
+    for (; p < c; ++p) {
+      const int cell = path_cell[(int64_t)p * g.cap + i];
+      const float dist = path_dist[(int64_t)p * g.cap + i];
+      const float ah1 = avgH[cell];
+      const float ahe1 = avgHe[cell];
+      const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
+      // absorbed fractions 1 - exp(-dtau) through expm1: in ionized cells dtau is
+      // ~1e-5 and 1 - expf(-dtau) would keep only ~3 significant digits
+      float nin[FRT_MAX_FREQ], eh[FRT_MAX_FREQ], ehe[FRT_MAX_FREQ], omh[FRT_MAX_FREQ],
+          omhe[FRT_MAX_FREQ], dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ];
+      float tot = 0.f;
+      for (int nu = 0; nu < F; ++nu) {
+        dth[nu] = ah1 * g.sigHf[nu] * dist;
+        dthe[nu] = ahe1 * g.sigHef[nu] * dist;
+        omh[nu] = -expm1f(-dth[nu]);
+        omhe[nu] = (dthe[nu] == 0.f) ? 0.f : -expm1f(-dthe[nu]);
+        eh[nu] = expf(-dth[nu]);
+        ehe[nu] = (dthe[nu] == 0.f) ? 1.f : expf(-dthe[nu]);
+        nin[nu] = nph[nu] * trans[nu];
+        tot += nin[nu];
+      }
+      const bool dump = tot < nmin;
+      float gh_sum = 0.f, ghe_sum = 0.f;
+      const int s = inv_perm[base + p];
+      for (int nu = 0; nu < F; ++nu) {
+        float fh = 0.f, fhe = 0.f;
+        if ((dth[nu] > 0.f) || (dthe[nu] > 0.f)) {
+          const float a = omh[nu] * ehe[nu];
+          const float b = omhe[nu] * eh[nu];
+          const float den = a + b;
+          if (den > 0.f) {
+            fh = a / den;
+            fhe = b / den;
+          } else if (g.frac_limit) {
+            fh = 1.f / (1.f + expf(dthe[nu] - dth[nu]));
+            fhe = 1.f / (1.f + expf(dth[nu] - dthe[nu]));
+          } else {
+            fh = fhe = __int_as_float(0x7fc00000);
+          }
+        }
+        const float eabs = dump ? nin[nu] : nin[nu] * ((dthe[nu] == 0.f) ? omh[nu] : -expm1f(-(dth[nu] + dthe[nu])));
+        const float gh = fh * eabs * rh1;
+        const float ghe = fhe * eabs * rhe1;
+        gh_sum += gh;
+        ghe_sum += ghe;
+        if (LAST) {
+          ((float*)cp.ghnu)[nu * cp.stride + s] = gh;
+          ((float*)cp.ghenu)[nu * cp.stride + s] = ghe;
+          ((float*)cp.ngnu)[nu * cp.stride + s] = nin[nu];
+        }
+        trans[nu] *= eh[nu] * ehe[nu];
+      }
+      cgh[s] = gh_sum;
+      cghe[s] = ghe_sum;
+      if (dump) {
+        dumped = true;
+        ++p;
+        break;
+      }
+    }
+    for (; p < c; ++p) {
+      const int s = inv_perm[base + p];
+      cgh[s] = 0.f;
+      cghe[s] = 0.f;
+      if (LAST) {
+        for (int nu = 0; nu < F; ++nu) {
+          ((float*)cp.ghnu)[nu * cp.stride + s] = 0.f;
+          ((float*)cp.ghenu)[nu * cp.stride + s] = 0.f;
+          ((float*)cp.ngnu)[nu * cp.stride + s] = 0.f;
+        }
+      }
+    }
+    if (LAST) {
+      float sum = 0.f;
+      for (int nu = 0; nu < F; ++nu) {
+        const float v = nph[nu] * trans[nu];
+        rb.nphot[nu * g.cap + i] = v;
+        sum += v;
+      }
+      dumped_flag = dumped;
+      if ((double)sum > g.nphot_min) {
+        rb.cell[i] = path_cell[(int64_t)(c - 1) * g.cap + i];
+      } else {
+        alive[i] = 0;
+        removed = true;
+      }
+    }
+  }
+  if (LAST) {
+    wave_count(&cnt->dumped, dumped_flag);
+    wave_count(&cnt->terminated, removed);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// update_thermal for one cell (gas_funcs.cc:362-419 with set_Treion_default
+// :104-129, update_heat_cool :134-242, get_thermal_tstep :353-358), subgrid off.
+struct ThermalIn {
+  float nH, nHe, nH1, nH2, nHe2, ne, ntot, dne, fH2, ah1, ahe1, nH1_prev;
+};
+
+__device__ __forceinline__ int thermal_cell(const GridConst& g, const ThermalIn& in,
+                                            const float* gh1nu, const float* ghe1nu,
+                                            int64_t n3, int64_t k, float& temp, float& heat,
+                                            float& cool, float& if_speed, float& inc_flux) {
+  // set_Treion_default
+  const float fion_new = (in.nH - in.nH1) / in.nH;
+  const float fion_old = (in.nH - in.nH1_prev) / in.nH;
+  const float deltax = g.dx * (fion_new - fion_old);
+  const float vIF = deltax / g.dt;
+  const float Cn[5] = {9.5432f, -1.6441e-2f, -9.8010e-3f, 4.1664e-3f, -2.2710e-4f};
+  const double L = dm_log((double)vIF / 1e5);
+  float lnT = 0.f;
+  for (int t = 0; t < 5; ++t) lnT = (float)((double)lnT + (double)Cn[t] * dm_pow(L, (double)t));
+  if ((fion_new > fion_old) && ((double)fion_new < 0.98))
+    temp = (fion_old * temp + (fion_new - fion_old) * dm_expf(lnT)) / fion_new;
+  const float vpos = frt_maxd(0.f, vIF);
+  if_speed = vpos;
+  inc_flux = (float)((1. + (double)g.nhe_to_nh) * (double)in.nH * (double)vpos);
+
+  const double k23 = 2. / 3. / FRT_K_B / (double)in.ntot;
+  const double Hd = (double)g.hubble;
+  const float dne_over_n = in.dne / in.ntot;
+  const float min_dt = (float)(0.1 * (double)g.dt);
+  const bool photoheat = (double)in.fH2 > 0.98;
+  float t_temp = 0.f;
+  int nsub = 0;
+  while (t_temp < g.dt) {
+    // get_thermal_tstep uses the heating/cooling of the previous sub-step
+    const float dTdt = (float)(k23 * (double)(heat - cool) - (2. * Hd + (double)dne_over_n) * (double)temp);
+    const float tstep = frt_maxd(min_dt, frt_mind((float)(0.1 * (double)temp / (double)frt_absd(dTdt)), g.dt));
+    const float dt_temp = frt_mind(tstep, g.dt - t_temp);
+    // update_heat_cool
+    heat = 0.f;
+    cool = 0.f;
+    if (photoheat) {
+      for (int nu = 0; nu < g.nfreq; ++nu) {
+        heat = (float)((double)heat + (double)(in.ah1 * gh1nu[nu * n3 + k]) * FRT_H_PLANCK *
+                                          (FRT_H_EV * g.nu[nu] - FRT_ETH_H1) / FRT_H_EV);
+        heat = (float)((double)heat + (double)(in.ahe1 * ghe1nu[nu * n3 + k]) * FRT_H_PLANCK *
+                                          (FRT_H_EV * g.nu[nu] - FRT_ETH_HE1) / FRT_H_EV);
+      }
+    }
+    const double Tm = (double)frt_maxd(g.temp_min, temp);
+    if (g.recomb_cool) {
+      if (g.rec_case_b) {
+        cool = (float)((double)cool + frt_rcrB_H2(Tm) * (double)in.ne * (double)in.nH2);
+        cool = (float)((double)cool + frt_rcrB_He2(Tm) * (double)in.ne * (double)in.nHe2);
+      } else {
+        cool = (float)((double)cool + frt_rcrA_H2(Tm) * (double)in.ne * (double)in.nH2);
+        cool = (float)((double)cool + frt_rcrA_He2(Tm) * (double)in.ne * (double)in.nHe2);
+      }
+    }
+    if (g.free_free) {
+      const float nff = in.nH2 + in.nHe2;  // + 4 nHe3, nHe3 = 0 in C2Ray mode
+      cool = (float)((double)cool + frt_free_free_rate(Tm) * (double)in.ne * (double)nff);
+    }
+    if (g.compton) cool = (float)((double)cool - g.compton_coef * (double)in.ne * (g.t_cmb - Tm));
+    temp = (float)(((double)temp + k23 * (double)(heat - cool) * (double)dt_temp) /
+                   (1. + 2. * Hd * (double)dt_temp + 1. / (double)in.ntot * (double)in.dne * (double)dt_temp));
+    t_temp += dt_temp;
+    if (++nsub >= 64) break;
+  }
+  temp = frt_maxd(temp, g.temp_min);
+  return nsub;
+}
+
+// find_avg_unu cell loop (ray_tracing.cc:941-1214, C2Ray branch 1077-1136,
+// finalize 1139-1210), one thread per cell.  FIRST also performs the
+// iteration-0 reset of :655-683; LAST finalizes and, with THERMAL, runs
+// update_thermal in the same pass (merge and split touch no cell field).
+template <typename ACC, bool FIRST, bool LAST, bool THERMAL>
+__global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribPtrs cp,
+                                              const int32_t* __restrict__ cstart,
+                                              const int32_t* __restrict__ cend,
+                                              DevCounters* cnt) {
+  const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  int nsub = 0;
+  if (k < g.n3) {
+    const int64_t n3 = g.n3;
+    const int s0 = cstart[k], s1 = cend[k];
+    const ACC* cgh = (const ACC*)cp.gh;
+    const ACC* cghe = (const ACC*)cp.ghe;
+    double GH, GHe;
+    double GHnu[FRT_MAX_FREQ], GHenu[FRT_MAX_FREQ], NGnu[FRT_MAX_FREQ];
+    if (sizeof(ACC) == 8) {
+      double a = 0., b = 0.;
+      for (int s = s0; s < s1; ++s) {
+        a += (double)cgh[s];
+        b += (double)cghe[s];
+      }
+      GH = a;
+      GHe = b;
+      if (LAST) {
+        for (int nu = 0; nu < g.nfreq; ++nu) {
+          double x = 0., y = 0., z = 0.;
+          for (int s = s0; s < s1; ++s) {
+            x += (double)((const ACC*)cp.ghnu)[nu * cp.stride + s];
+            y += (double)((const ACC*)cp.ghenu)[nu * cp.stride + s];
+            z += (double)((const ACC*)cp.ngnu)[nu * cp.stride + s];
+          }
+          GHnu[nu] = x;
+          GHenu[nu] = y;
+          NGnu[nu] = z;
+        }
+      }
+    } else {
+      // compensated float sums in ray order
+      float a = 0.f, ca = 0.f, b = 0.f, cb = 0.f;
+      for (int s = s0; s < s1; ++s) {
+        float ya = (float)cgh[s] - ca;
+        float ta = a + ya;
+        ca = (ta - a) - ya;
+        a = ta;
+        float yb = (float)cghe[s] - cb;
+        float tb = b + yb;
+        cb = (tb - b) - yb;
+        b = tb;
+      }
+      GH = (double)a;
+      GHe = (double)b;
+      if (LAST) {
+        for (int nu = 0; nu < g.nfreq; ++nu) {
+          float x = 0.f, cx = 0.f, y = 0.f, cy = 0.f, z = 0.f;
+          for (int s = s0; s < s1; ++s) {
+            float yx = (float)((const ACC*)cp.ghnu)[nu * cp.stride + s] - cx;
+            float tx = x + yx;
+            cx = (tx - x) - yx;
+            x = tx;
+            float yy = (float)((const ACC*)cp.ghenu)[nu * cp.stride + s] - cy;
+            float ty = y + yy;
+            cy = (ty - y) - yy;
+            y = ty;
+            z += (float)((const ACC*)cp.ngnu)[nu * cp.stride + s];
+          }
+          GHnu[nu] = (double)x;
+          GHenu[nu] = (double)y;
+          NGnu[nu] = (double)z;
+        }
+      }
+    }
+
+    const float nH = cf.f[C_NH][k];
+    const float nHe = cf.f[C_NHE][k];
+    const float clump = cf.f[C_CLUMP][k];
+    float rH2, rHe2, ah1, ahe1, dne;
+    if (FIRST) {
+      const float T = cf.f[C_TEMP][k];
+      const double Tm = (double)frt_maxd(g.temp_min, T);
+      if (g.rec_case_b) {
+        rH2 = (float)frt_alphaB_H2(Tm);
+        rHe2 = (float)frt_alphaB_He2(Tm);
+      } else {
+        rH2 = (float)frt_alphaA_H2(Tm);
+        rHe2 = (float)frt_alphaA_He2(Tm);
+      }
+      if (((double)T >= 3e4) && ((double)T <= 1e6)) rHe2 += (float)frt_Dalpha_He2(Tm);
+      cf.f[C_RECOMB_H2][k] = rH2;
+      cf.f[C_RECOMB_HE2][k] = rHe2;
+      dne = -cf.f[C_NE][k];
+      if (!LAST) cf.f[C_DNE_DT][k] = dne;
+      ah1 = cf.f[C_NH1][k];
+      ahe1 = cf.f[C_NHE1][k];
+    } else {
+      rH2 = cf.f[C_RECOMB_H2][k];
+      rHe2 = cf.f[C_RECOMB_HE2][k];
+      ah1 = cf.f[C_AVG_NH1][k];
+      ahe1 = cf.f[C_AVG_NHE1][k];
+      dne = cf.f[C_DNE_DT][k];
+    }
+    const double dtd = (double)g.dt;
+    const double nelec = (double)(nH - ah1 + nHe - ahe1);  // float sum, then double
+    const double x0 = 1. - (double)cf.f[C_FH1][k];
+    const double pr = GH + (double)(clump * rH2) * nelec;
+    const double xeq = GH / pr;
+    const double ti = 1. / pr;
+    const double e_h = dm_exp(-dtd / ti);
+    const float avg_h1 = (float)((double)nH * (1. - (xeq + ti / dtd * (x0 - xeq) * (1. - e_h))));
+    const double x0he = 1. - (double)cf.f[C_FHE1][k];
+    const double prhe = GHe + (double)(clump * rHe2) * nelec;
+    const double xeqhe = GHe / prhe;
+    const double tihe = 1. / prhe;
+    const double e_he = dm_exp(-dtd / tihe);
+    const float avg_he1 = (float)((double)nHe * (1. - (xeqhe + tihe / dtd * (x0he - xeqhe) * (1. - e_he))));
+    cf.f[C_AVG_NH1][k] = avg_h1;
+    cf.f[C_AVG_NHE1][k] = avg_he1;
+
+    if (LAST) {
+      const float nH1n = (float)((double)nH * (1. - (xeq + (x0 - xeq) * e_h)));
+      const float nH2n = nH - nH1n;
+      const float nHe1n = (float)((double)nHe * (1. - (xeqhe + (x0he - xeqhe) * e_he)));
+      const float nHe2n = nHe - nHe1n;
+      if (((double)(avg_h1 / nH) < 0.5) && (cf.f[C_TREION_FLAG][k] == 0.f)) {
+        cf.f[C_TREION][k] = g.t;
+        cf.f[C_TREION_FLAG][k] = 1.f;
+      }
+      for (int nu = 0; nu < g.nfreq; ++nu) {
+        cf.gamma_h1_nu[nu * n3 + k] = (float)GHnu[nu];
+        cf.gamma_he1_nu[nu * n3 + k] = (float)GHenu[nu];
+        cf.ngam_nu[nu * n3 + k] = (float)NGnu[nu];
+      }
+      cf.f[C_GAMMA_H1_TOT][k] = (float)GH;
+      cf.f[C_GAMMA_HE1_TOT][k] = (float)GHe;
+      cf.f[C_RECRATE_H2][k] = (float)((double)(clump * rH2) * nelec * (double)nH2n);
+      cf.f[C_RECRATE_HE2][k] = (float)((double)(clump * rHe2) * nelec * (double)nHe2n);
+      const float ne_n = nH2n + nHe2n;  // + 2 nHe3, nHe3 = 0
+      float dne_n = dne + ne_n;
+      dne_n = dne_n / g.dt;
+      const float ntot_n = nH + nHe + ne_n;
+      const float fH2n = nH2n / nH;
+      cf.f[C_NH1][k] = nH1n;
+      cf.f[C_NH2][k] = nH2n;
+      cf.f[C_NHE1][k] = nHe1n;
+      cf.f[C_NHE2][k] = nHe2n;
+      cf.f[C_NE][k] = ne_n;
+      cf.f[C_DNE_DT][k] = dne_n;
+      cf.f[C_NTOT][k] = ntot_n;
+      cf.f[C_FH2][k] = fH2n;
+      cf.f[C_FHE2][k] = nHe2n / nHe;
+      cf.f[C_FH1][k] = nH1n / nH;
+      cf.f[C_FHE1][k] = nHe1n / nHe;
+      if (THERMAL) {
+        if (g.temp_ev) {
+          ThermalIn in;
+          in.nH = nH;
+          in.nHe = nHe;
+          in.nH1 = nH1n;
+          in.nH2 = nH2n;
+          in.nHe2 = nHe2n;
+          in.ne = ne_n;
+          in.ntot = ntot_n;
+          in.dne = dne_n;
+          in.fH2 = fH2n;
+          in.ah1 = avg_h1;
+          in.ahe1 = avg_he1;
+          in.nH1_prev = cf.f[C_NH1_PREV][k];
+          float temp = cf.f[C_TEMP][k];
+          float heat = cf.f[C_HEAT][k];
+          float cool = cf.f[C_COOL][k];
+          float ifs, inc;
+          nsub = thermal_cell(g, in, cf.gamma_h1_nu, cf.gamma_he1_nu, n3, k, temp, heat, cool, ifs, inc);
+          cf.f[C_TEMP][k] = temp;
+          cf.f[C_HEAT][k] = heat;
+          cf.f[C_COOL][k] = cool;
+          cf.f[C_IF_SPEED][k] = ifs;
+          cf.f[C_INC_FLUX][k] = inc;
+          cf.f[C_NH1_PREV][k] = nH1n;
+        } else {
+          cf.f[C_TEMP][k] = ((double)(nH1n / nH) > 0.5) ? 100.f : 1e4f;
+          cf.f[C_IF_SPEED][k] = 0.f;
+          cf.f[C_INC_FLUX][k] = 0.f;
+        }
+      }
+    }
+  }
+  if (LAST && THERMAL) {
+    // per-wave sum of sub-step counts
+    unsigned long long v = (unsigned long long)nsub;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt->substeps, v);
+  }
+}
+
+// Standalone update_thermal (used when the phases are called one by one).
+__global__ void __launch_bounds__(BLK) k_thermal(CellPtrs cf, GridConst g, DevCounters* cnt) {
+  const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  int nsub = 0;
+  if (k < g.n3) {
+    if (g.temp_ev) {
+      ThermalIn in;
+      in.nH = cf.f[C_NH][k];
+      in.nHe = cf.f[C_NHE][k];
+      in.nH1 = cf.f[C_NH1][k];
+      in.nH2 = cf.f[C_NH2][k];
+      in.nHe2 = cf.f[C_NHE2][k];
+      in.ne = cf.f[C_NE][k];
+      in.ntot = cf.f[C_NTOT][k];
+      in.dne = cf.f[C_DNE_DT][k];
+      in.fH2 = cf.f[C_FH2][k];
+      in.ah1 = cf.f[C_AVG_NH1][k];
+      in.ahe1 = cf.f[C_AVG_NHE1][k];
+      in.nH1_prev = cf.f[C_NH1_PREV][k];
+      float temp = cf.f[C_TEMP][k];
+      float heat = cf.f[C_HEAT][k];
+      float cool = cf.f[C_COOL][k];
+      float ifs, inc;
+      nsub = thermal_cell(g, in, cf.gamma_h1_nu, cf.gamma_he1_nu, g.n3, k, temp, heat, cool, ifs, inc);
+      cf.f[C_TEMP][k] = temp;
+      cf.f[C_HEAT][k] = heat;
+      cf.f[C_COOL][k] = cool;
+      cf.f[C_IF_SPEED][k] = ifs;
+      cf.f[C_INC_FLUX][k] = inc;
+      cf.f[C_NH1_PREV][k] = in.nH1;
+    } else {
+      cf.f[C_TEMP][k] = ((double)(cf.f[C_NH1][k] / cf.f[C_NH][k]) > 0.5) ? 100.f : 1e4f;
+      cf.f[C_IF_SPEED][k] = 0.f;
+      cf.f[C_INC_FLUX][k] = 0.f;
+    }
+  }
+  unsigned long long v = (unsigned long long)nsub;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt->substeps, v);
+}
+
+// ---------------------------------------------------------------------------
+// Merging (ray_tracing.cc:176-227 calc_nphot_merge, :324-477 merge).
+__global__ void __launch_bounds__(BLK) k_count_alive(const uint8_t* __restrict__ alive, int64_t n,
+                                                     DevCounters* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  wave_count(&cnt->alive, (i < n) && alive[i]);
+}
+
+__global__ void __launch_bounds__(BLK) k_pdf(RaySoA rb, int64_t nrays, GridConst g,
+                                             const uint8_t* __restrict__ alive, double lg_min,
+                                             double del, int32_t* __restrict__ pdf) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= nrays || !alive[i]) return;
+  const float s = sum_nphot(rb.nphot, g.cap, g.nfreq, i);
+  // (int) maxd((log10(sum) - lg_min)/del, 0.): maxd narrows to float
+  int ind = (int)frt_maxd((float)((dm_log10((double)s) - lg_min) / del), 0.f);
+  if (ind > FRT_PDF_BINS - 1) ind = FRT_PDF_BINS - 1;
+  if (ind < 0) ind = 0;  // only reachable for a NaN photon count
+  atomicAdd(&pdf[ind], 1);
+}
+
+struct Mat9 {
+  float m[9];
+};
+
+__global__ void __launch_bounds__(BLK) k_merge_flag(RaySoA rb, int64_t nrays, GridConst g,
+                                                    const uint8_t* __restrict__ alive, Mat9 inv,
+                                                    int32_t* __restrict__ cand,
+                                                    uint32_t* __restrict__ key) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i > nrays) return;
+  if (i == nrays) {
+    cand[i] = 0;
+    return;
+  }
+  int c = 0;
+  if (alive[i]) {
+    const float s = sum_nphot(rb.nphot, g.cap, g.nfreq, i);
+    if (s < g.nphot_merge) {
+      c = 1;
+      float vx, vy, vz;
+      rotate3(inv.m, rb.xhat[i], rb.yhat[i], rb.zhat[i], &vx, &vy, &vz);  // get_unit_vector(.., 6)
+      const int bin = hpx_vec2pix_nest(g.hpx_lvl, (double)vx, (double)vy, (double)vz);
+      const int nb = 12 << (2 * g.hpx_lvl);
+      key[i] = (uint32_t)rb.cell[i] * (uint32_t)nb + (uint32_t)bin;
+    }
+  }
+  cand[i] = c;
+}
+
+__global__ void __launch_bounds__(BLK) k_merge_scatter(int64_t nrays, const int32_t* __restrict__ cand,
+                                                       const int32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ key_tmp,
+                                                       uint32_t* __restrict__ key,
+                                                       int32_t* __restrict__ val) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= nrays || !cand[i]) return;
+  key[off[i]] = key_tmp[i];
+  val[off[i]] = (int32_t)i;
+}
+
+// l_from_r (ray_tracing.cc:87-95).
+__device__ __forceinline__ int l_from_r_dev(const GridConst& g, float r, float safety) {
+  const float q = r / g.dx0;
+  if ((double)q < 1.) return g.hpx_lvl;
+  const double arg = FRT_PI_REF * (double)safety / 3. * ((double)q * (double)q);
+  int l = (int)ceil(dm_log(arg) / dm_log(4.));
+  if (l > g.l_max) l = g.l_max;
+  if (l < g.hpx_lvl) l = g.hpx_lvl;
+  return l;
+}
+
+// One thread per occupied (cell, direction) bin: accumulates its members in
+// list order exactly as the serial phase 1, then normalizes (phase 2).
+__global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, GridConst g,
+                                                      const uint32_t* __restrict__ key,
+                                                      const int32_t* __restrict__ val,
+                                                      uint8_t* __restrict__ alive, Mat9 fwd,
+                                                      uint32_t itheta_m, float safety,
+                                                      DevCounters* cnt) {
+  const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  bool head = false;
+  int64_t merged = 0;
+  if (s < ncand) head = (s == 0) || (key[s - 1] != key[s]);
+  if (head) {
+    const int F = g.nfreq;
+    const int64_t cap = g.cap;
+    int64_t e = s + 1;
+    while (e < ncand && key[e] == key[s]) ++e;
+    const int b = val[s];
+    float NP[FRT_MAX_FREQ];
+    for (int nu = 0; nu < F; ++nu) NP[nu] = rb.nphot[nu * cap + b];
+    const float nb = sum_nphot(rb.nphot, cap, F, b);
+    const float rrb = rb.r[b] / g.dx0;
+    float R = (float)((double)rrb * (double)rrb * (double)nb);
+    float DR = rb.dr[b] * nb;
+    float X = rb.x[b] * nb;
+    float Y = rb.y[b] * nb;
+    float Z = rb.z[b] * nb;
+    float T = rb.time[b] * nb;
+    for (int64_t j = s + 1; j < e; ++j) {
+      const int m = val[j];
+      const float nm = sum_nphot(rb.nphot, cap, F, m);
+      for (int nu = 0; nu < F; ++nu) NP[nu] += rb.nphot[nu * cap + m];
+      const float rrm = rb.r[m] / g.dx0;
+      R = (float)((double)R + (double)rrm * (double)rrm * (double)nm);
+      DR += rb.dr[m] * nm;
+      X += rb.x[m] * nm;
+      Y += rb.y[m] * nm;
+      Z += rb.z[m] * nm;
+      T += rb.time[m] * nm;
+      alive[m] = 0;
+      ++merged;
+    }
+    float ntot = 0.f;
+    for (int nu = 0; nu < F; ++nu) ntot += NP[nu];
+    R = R / ntot;
+    R = (float)sqrt((double)R);
+    R = R * g.dx0;
+    DR = DR / ntot;
+    X = X / ntot;
+    Y = Y / ntot;
+    Z = Z / ntot;
+    T = T / ntot;
+    if (e - s > 1) {
+      const int nbins = 12 << (2 * g.hpx_lvl);
+      const int bin = (int)(key[s] % (uint32_t)nbins);
+      double vx, vy, vz;
+      hpx_pix2vec_nest(g.hpx_lvl, bin, &vx, &vy, &vz);
+      const int order = l_from_r_dev(g, R, safety);
+      const float rmin = g.rmin_tab[order];
+      const float rmax = g.rmax_tab[order];
+      R = frt_maxd(rmax - DR, rmin);
+      DR = rmax - R;
+      const int pixel = hpx_vec2pix_nest(order, vx, vy, vz);
+      hpx_pix2vec_nest(order, pixel, &vx, &vy, &vz);
+      float ox, oy, oz;
+      rotate3(fwd.m, (float)vx, (float)vy, (float)vz, &ox, &oy, &oz);
+      rb.order[b] = (int16_t)order;
+      rb.pixel[b] = pixel;
+      rb.itheta[b] = (uint16_t)itheta_m;
+      rb.xhat[b] = ox;
+      rb.yhat[b] = oy;
+      rb.zhat[b] = oz;
+    }
+    for (int nu = 0; nu < F; ++nu) rb.nphot[nu * cap + b] = NP[nu];
+    rb.r[b] = R;
+    rb.dr[b] = DR;
+    rb.x[b] = X;
+    rb.y[b] = Y;
+    rb.z[b] = Z;
+    rb.time[b] = T;
+  }
+  wave_count(&cnt->groups, head);
+  unsigned long long v = (unsigned long long)merged;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt->merged_away, v);
+}
+
+// ---------------------------------------------------------------------------
+// split_rays decision (rt_funcs.cc:211-237): rays out per input ray.
+__global__ void __launch_bounds__(BLK) k_split_flag(RaySoA rb, int64_t nrays, GridConst g,
+                                                    const uint8_t* __restrict__ alive,
+                                                    float split_thresh, int32_t* __restrict__ n_out,
+                                                    DevCounters* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  bool split = false;
+  if (i < nrays) {
+    int out = 0;
+    if (alive[i]) {
+      const int l = rb.order[i];
+      const float r = rb.r[i];
+      split = (r > g.rmax_tab[l]) && (l < g.l_max) &&
+              (sum_nphot(rb.nphot, g.cap, g.nfreq, i) > split_thresh);
+      out = split ? 4 : 1;
+    }
+    n_out[i] = out;
+  } else if (i == nrays) {
+    n_out[i] = 0;
+  }
+  wave_count(&cnt->splits, split);
+}
+
+// Compaction into the other buffer; split parents become 4 children
+// (split_ray, ray_tracing.cc:482-619).
+__global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, int64_t nrays, GridConst g,
+                                                 const int32_t* __restrict__ n_out,
+                                                 const int32_t* __restrict__ out_off,
+                                                 const float* __restrict__ rot_fwd) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= nrays) return;
+  const int no = n_out[i];
+  if (no == 0) return;
+  const int64_t o = out_off[i];
+  const int F = g.nfreq;
+  const int64_t cap = g.cap;
+  if (no == 1) {
+    dst.cell[o] = src.cell[i];
+    dst.pixel[o] = src.pixel[i];
+    dst.order[o] = src.order[i];
+    dst.itheta[o] = src.itheta[i];
+    dst.xhat[o] = src.xhat[i];
+    dst.yhat[o] = src.yhat[i];
+    dst.zhat[o] = src.zhat[i];
+    for (int nu = 0; nu < F; ++nu) dst.nphot[nu * cap + o] = src.nphot[nu * cap + i];
+    dst.r[o] = src.r[i];
+    dst.dr[o] = src.dr[i];
+    dst.x[o] = src.x[i];
+    dst.y[o] = src.y[i];
+    dst.z[o] = src.z[i];
+    dst.time[o] = src.time[i];
+    return;
+  }
+  const int l = src.order[i];
+  const int pn = src.pixel[i];
+  const float r0 = src.r[i];
+  const float x0o = src.x[i], y0o = src.y[i], z0o = src.z[i];
+  const float time = src.time[i];
+  const uint16_t it = src.itheta[i];
+  const float* m = rot_fwd + 9 * (int64_t)it;
+  const double vox = (double)src.xhat[i], voy = (double)src.yhat[i], voz = (double)src.zhat[i];
+  const int n = g.n;
+  const int ind = src.cell[i];
+  const int ic = ind / (n * n);
+  const int rem = ind - ic * n * n;
+  const int jc = rem / n;
+  const int kc = rem - jc * n;
+  const float rsf = r0 * g.sf;
+  const float rmax = g.rmax_tab[l + 1];
+  for (int ch = 0; ch < 4; ++ch) {
+    const int pix = 4 * pn + ch;
+    double px, py, pz;
+    hpx_pix2vec_nest(l + 1, pix, &px, &py, &pz);
+    float vx, vy, vz;
+    rotate3(m, (float)px, (float)py, (float)pz, &vx, &vy, &vz);
+    float xn = (float)((double)x0o + ((double)vx - vox) * (double)rsf);
+    float yn = (float)((double)y0o + ((double)vy - voy) * (double)rsf);
+    float zn = (float)((double)z0o + ((double)vz - voz) * (double)rsf);
+    int icn = ic, jcn = jc, kcn = kc;
+    // periodic cell walk (ray_tracing.cc:549-572); the guard only bounds non-finite input
+    int w = 0;
+    while (xn > g.dx && w < 4096) { xn -= g.dx; icn = wrap_mod(icn + 1, n); ++w; }
+    while ((double)xn < 0. && w < 4096) { xn += g.dx; icn = wrap_mod(icn - 1, n); ++w; }
+    while (yn > g.dx && w < 4096) { yn -= g.dx; jcn = wrap_mod(jcn + 1, n); ++w; }
+    while ((double)yn < 0. && w < 4096) { yn += g.dx; jcn = wrap_mod(jcn - 1, n); ++w; }
+    while (zn > g.dx && w < 4096) { zn -= g.dx; kcn = wrap_mod(kcn + 1, n); ++w; }
+    while ((double)zn < 0. && w < 4096) { zn += g.dx; kcn = wrap_mod(kcn - 1, n); ++w; }
+    const int64_t q = o + ch;
+    dst.cell[q] = icn * n * n + jcn * n + kcn;
+    dst.pixel[q] = pix;
+    dst.order[q] = (int16_t)(l + 1);
+    dst.itheta[q] = it;
+    dst.xhat[q] = vx;
+    dst.yhat[q] = vy;
+    dst.zhat[q] = vz;
+    for (int nu = 0; nu < F; ++nu) dst.nphot[nu * cap + q] = (float)((double)src.nphot[nu * cap + i] / 4.);
+    dst.r[q] = r0;
+    dst.dr[q] = rmax - r0;
+    dst.x[q] = xn;
+    dst.y[q] = yn;
+    dst.z[q] = zn;
+    dst.time[q] = time;
+  }
+}
+
+// new_rays/init_rays (rt_funcs.cc:240-261, ray_tracing.cc:230-321), hpx_lvl 0:
+// 12 rays per source cell, appended in source order.
+__global__ void __launch_bounds__(BLK) k_cast(RaySoA dst, int64_t base, int nsrc, GridConst g,
+                                              const int32_t* __restrict__ src_cell,
+                                              const double* __restrict__ src_lum,
+                                              float* __restrict__ src_ray_dt,
+                                              const float* __restrict__ rot_fwd, uint64_t seed,
+                                              uint64_t stream) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (t >= (int64_t)nsrc * 12) return;
+  const int s = (int)(t / 12);
+  const int pix = (int)(t - (int64_t)s * 12);
+  const uint32_t it = frt_itheta_hash(seed, stream, (uint64_t)s);
+  double px, py, pz;
+  hpx_pix2vec_nest(0, pix, &px, &py, &pz);
+  float vx, vy, vz;
+  rotate3(rot_fwd + 9 * (int64_t)it, (float)px, (float)py, (float)pz, &vx, &vy, &vz);
+  const int64_t q = base + t;
+  const double lum = src_lum[s];
+  const double rdt = (double)src_ray_dt[s];
+  const int64_t cap = g.cap;
+  for (int nu = 0; nu < g.nfreq; ++nu)
+    dst.nphot[nu * cap + q] = (float)((double)g.spectrum[nu] * lum * rdt / 12. / 1. / FRT_PHOT_NORM);
+  dst.cell[q] = src_cell[s];
+  dst.pixel[q] = pix;
+  dst.order[q] = 0;
+  dst.itheta[q] = (uint16_t)it;
+  dst.xhat[q] = vx;
+  dst.yhat[q] = vy;
+  dst.zhat[q] = vz;
+  dst.r[q] = 0.f;
+  dst.dr[q] = g.rmax_tab[0] - 0.f;
+  const float half = (float)((double)g.dx / 2.);
+  dst.x[q] = half;
+  dst.y[q] = half;
+  dst.z[q] = half;
+  dst.time[q] = 0.f;
+}
+
+__global__ void __launch_bounds__(BLK) k_zero_ray_dt(float* __restrict__ ray_dt, int nsrc) {
+  const int s = blockIdx.x * BLK + threadIdx.x;
+  if (s < nsrc) ray_dt[s] = 0.f;
+}
+
+__global__ void __launch_bounds__(BLK) k_add_ray_dt(float* __restrict__ ray_dt, int nsrc, float dt) {
+  const int s = blockIdx.x * BLK + threadIdx.x;
+  if (s < nsrc) ray_dt[s] += dt;
+}
+
+__global__ void __launch_bounds__(BLK) k_copy(const int4* __restrict__ a, int4* __restrict__ b,
+                                              int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n16; i += (int64_t)gridDim.x * BLK)
+    b[i] = a[i];
+}
+
+// ===========================================================================
+// Host-side launchers.
+
+#define FRT_LAUNCH(ctx, name, grid, kern, ...)                                   \
+  do {                                                                          \
+    if ((grid) > 0) {                                                           \
+      ProfRec _rec;                                                             \
+      prof_begin(ctx, name, &_rec);                                             \
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, (ctx)->stream, __VA_ARGS__); \
+      prof_end(ctx, &_rec);                                                     \
+      hipError_t _e = hipGetLastError();                                        \
+      if (_e != hipSuccess) return hip_check(ctx, _e, name);                    \
+    }                                                                           \
+  } while (0)
+
+static RaySoA& cur_buf(frt_ctx* ctx) { return ctx->buf[ctx->cur]; }
+
+int launch_add_ray_dt(frt_ctx* ctx) {
+  if (ctx->nsrc > 0)
+    FRT_LAUNCH(ctx, "add_ray_dt", grid_for(ctx->nsrc), k_add_ray_dt, ctx->src_ray_dt, ctx->nsrc, ctx->g.dt);
+  return FRT_OK;
+}
+
+int launch_march(frt_ctx* ctx) {
+  FRT_LAUNCH(ctx, "march", grid_for(ctx->nrays), k_march, cur_buf(ctx), ctx->nrays, ctx->g,
+             ctx->path_cell, ctx->path_dist, ctx->seg_cnt, ctx->alive, ctx->dcnt);
+  return FRT_OK;
+}
+
+static int exclusive_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, int64_t n, const char* name) {
+  size_t bytes = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream);
+  if (e != hipSuccess) return hip_check(ctx, e, name);
+  int rc = ensure_cub_tmp(ctx, bytes);
+  if (rc) return rc;
+  ProfRec rec;
+  prof_begin(ctx, name, &rec);
+  e = hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, in, out, (int)n, ctx->stream);
+  prof_end(ctx, &rec);
+  return hip_check(ctx, e, name);
+}
+
+static int sort_pairs(frt_ctx* ctx, const uint32_t* kin, uint32_t* kout, const int32_t* vin,
+                      int32_t* vout, int64_t n, int bits, const char* name) {
+  size_t bytes = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0,
+                                                     bits, ctx->stream);
+  if (e != hipSuccess) return hip_check(ctx, e, name);
+  int rc = ensure_cub_tmp(ctx, bytes);
+  if (rc) return rc;
+  ProfRec rec;
+  prof_begin(ctx, name, &rec);
+  e = hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, kin, kout, vin, vout, (int)n, 0, bits,
+                                         ctx->stream);
+  prof_end(ctx, &rec);
+  return hip_check(ctx, e, name);
+}
+
+static int read_i32(frt_ctx* ctx, const int32_t* dptr, int64_t* out) {
+  int rc = hip_check(ctx, hipMemcpyAsync(ctx->hscalar, dptr, 4, hipMemcpyDeviceToHost, ctx->stream), "read scalar");
+  if (rc) return rc;
+  rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (rc) return rc;
+  *out = (int64_t)(*(int32_t*)ctx->hscalar);
+  return FRT_OK;
+}
+
+static int grow_segments(frt_ctx* ctx, int64_t need) {
+  if (need <= ctx->seg_cap) return FRT_OK;
+  int64_t cap = ctx->seg_cap ? ctx->seg_cap : 1024;
+  while (cap < need) cap *= 2;
+  for (int b = 0; b < 2; ++b) {
+    hipFree(ctx->seg_key[b]);
+    hipFree(ctx->seg_val[b]);
+    ctx->seg_key[b] = nullptr;
+    ctx->seg_val[b] = nullptr;
+  }
+  hipFree(ctx->inv_perm);
+  hipFree(ctx->contrib.gh);
+  ctx->inv_perm = nullptr;
+  ctx->contrib.gh = nullptr;
+  const int F = ctx->g.nfreq;
+  const size_t acc = (ctx->cfg.precision == FRT_PREC_MIXED) ? 4 : 8;
+  const size_t per = acc * (size_t)(2 + 3 * F);
+  hipError_t e = hipSuccess;
+  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+    e = hipMalloc((void**)&ctx->seg_key[b], (size_t)cap * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&ctx->seg_val[b], (size_t)cap * 4);
+  }
+  if (e == hipSuccess) e = hipMalloc((void**)&ctx->inv_perm, (size_t)cap * 4);
+  void* blk = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&blk, per * (size_t)cap);
+  if (e != hipSuccess) {
+    ctx->seg_cap = 0;
+    return set_error(ctx, FRT_E_NOMEM, "segment buffers: " + std::string(hipGetErrorString(e)));
+  }
+  char* c = (char*)blk;
+  ctx->contrib.gh = c;
+  ctx->contrib.ghe = c + acc * cap;
+  ctx->contrib.ghnu = c + acc * 2 * cap;
+  ctx->contrib.ghenu = c + acc * (2 + F) * cap;
+  ctx->contrib.ngnu = c + acc * (2 + 2 * F) * cap;
+  ctx->contrib.stride = cap;
+  ctx->seg_cap = cap;
+  return FRT_OK;
+}
+
+// Segment offsets, cell sort and cell ranges of the current paths.
+int launch_segments(frt_ctx* ctx) {
+  const int64_t n = ctx->nrays;
+  int rc;
+  FRT_LAUNCH(ctx, "fill", grid_for(1), k_fill_i32, ctx->seg_cnt + n, (int64_t)1, 0);
+  rc = exclusive_scan(ctx, ctx->seg_cnt, ctx->seg_off, n + 1, "scan_segments");
+  if (rc) return rc;
+  int64_t nseg = 0;
+  rc = read_i32(ctx, ctx->seg_off + n, &nseg);
+  if (rc) return rc;
+  ctx->nseg = nseg;
+  rc = grow_segments(ctx, nseg > 0 ? nseg : 1);
+  if (rc) return rc;
+  FRT_LAUNCH(ctx, "fill", grid_for(ctx->g.n3), k_fill_i32, ctx->cell_start, (int64_t)ctx->g.n3, 0);
+  FRT_LAUNCH(ctx, "fill", grid_for(ctx->g.n3), k_fill_i32, ctx->cell_end, (int64_t)ctx->g.n3, 0);
+  if (nseg == 0) return FRT_OK;
+  FRT_LAUNCH(ctx, "seg_keys", grid_for(n), k_seg_keys, n, ctx->g.cap, ctx->seg_cnt, ctx->seg_off,
+             ctx->path_cell, ctx->seg_key[0], ctx->seg_val[0]);
+  rc = sort_pairs(ctx, ctx->seg_key[0], ctx->seg_key[1], ctx->seg_val[0], ctx->seg_val[1], nseg,
+                  ctx->key_bits, "sort_segments");
+  if (rc) return rc;
+  FRT_LAUNCH(ctx, "cell_ranges", grid_for(nseg), k_cell_ranges, ctx->seg_key[1], nseg,
+             ctx->cell_start, ctx->cell_end);
+  FRT_LAUNCH(ctx, "inv_perm", grid_for(nseg), k_inv_perm, ctx->seg_val[1], nseg, ctx->inv_perm);
+  return FRT_OK;
+}
+
+int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
+  const int I = ctx->cfg.itercount;
+  const int64_t n = ctx->nrays;
+  const bool mixed = ctx->cfg.precision == FRT_PREC_MIXED;
+  GridConst g = ctx->g;
+  RaySoA rb = cur_buf(ctx);
+  const unsigned gc = grid_for(g.n3);
+  const int F = g.nfreq;
+  ctx->deposit_bytes_last = 0.;
+  ctx->deposit_launches_last = 0;
+  for (int it = 0; it < I; ++it) {
+    const bool first = (it == 0), last = (it == I - 1);
+    const float* aH = first ? ctx->c.f[C_NH1] : ctx->c.f[C_AVG_NH1];
+    const float* aHe = first ? ctx->c.f[C_NHE1] : ctx->c.f[C_AVG_NHE1];
+    if (mixed) {
+      if (last)
+        FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<true>, rb, n, g, ctx->path_cell,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+                   ctx->alive, ctx->dcnt);
+      else
+        FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<false>, rb, n, g, ctx->path_cell,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+                   ctx->alive, ctx->dcnt);
+    } else {
+      if (last)
+        FRT_LAUNCH(ctx, "deposit_fp64", grid_for(n), k_deposit_fp64<true>, rb, n, g, ctx->path_cell,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+                   ctx->alive, ctx->dcnt);
+      else
+        FRT_LAUNCH(ctx, "deposit_fp64", grid_for(n), k_deposit_fp64<false>, rb, n, g, ctx->path_cell,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+                   ctx->alive, ctx->dcnt);
+    }
+    // algorithmic bytes of this deposit launch (model used for the roofline):
+    // per ray: seg_cnt + seg_off (8) + nphot read (4F); per segment: path cell +
+    // dist (8) + inv_perm (4) + avg_nh1/avg_nhe1 (8) + 2 deposits; last pass adds
+    // 3F per-bin deposits per segment and the ray write-back (4F + 4 + 1).
+    {
+      const double acc = mixed ? 4. : 8.;
+      double b = (double)n * 8. + (double)ctx->stats.rays_moved * (4. * F) +
+                 (double)ctx->nseg * (20. + 2. * acc);
+      if (last) b += (double)ctx->nseg * 3. * F * acc + (double)ctx->stats.rays_moved * (4. * F + 5.);
+      ctx->deposit_bytes_last += b;
+      ctx->deposit_launches_last += 1;
+    }
+#define FRT_CHEM(ACC, F1, L1, TH)                                                           \
+  FRT_LAUNCH(ctx, "chem", gc, (k_chem<ACC, F1, L1, TH>), ctx->c, g, ctx->contrib, ctx->cell_start, \
+             ctx->cell_end, ctx->dcnt)
+    if (mixed) {
+      if (first && last) {
+        if (fuse_thermal) FRT_CHEM(float, true, true, true); else FRT_CHEM(float, true, true, false);
+      } else if (first) {
+        FRT_CHEM(float, true, false, false);
+      } else if (last) {
+        if (fuse_thermal) FRT_CHEM(float, false, true, true); else FRT_CHEM(float, false, true, false);
+      } else {
+        FRT_CHEM(float, false, false, false);
+      }
+    } else {
+      if (first && last) {
+        if (fuse_thermal) FRT_CHEM(double, true, true, true); else FRT_CHEM(double, true, true, false);
+      } else if (first) {
+        FRT_CHEM(double, true, false, false);
+      } else if (last) {
+        if (fuse_thermal) FRT_CHEM(double, false, true, true); else FRT_CHEM(double, false, true, false);
+      } else {
+        FRT_CHEM(double, false, false, false);
+      }
+    }
+#undef FRT_CHEM
+  }
+  return FRT_OK;
+}
+
+int launch_thermal(frt_ctx* ctx) {
+  FRT_LAUNCH(ctx, "thermal", grid_for(ctx->g.n3), k_thermal, ctx->c, ctx->g, ctx->dcnt);
+  return FRT_OK;
+}
+
+// count_rays, calc_nphot_merge (if needed) and merge.
+int launch_merge(frt_ctx* ctx) {
+  const int64_t n = ctx->nrays;
+  GridConst g = ctx->g;
+  RaySoA rb = cur_buf(ctx);
+  int rc;
+  // count_rays
+  FRT_LAUNCH(ctx, "count_alive", grid_for(n), k_count_alive, ctx->alive, n, ctx->dcnt);
+  rc = hip_check(ctx, hipMemcpyAsync(ctx->hcnt, ctx->dcnt, sizeof(DevCounters), hipMemcpyDeviceToHost, ctx->stream), "counters");
+  if (rc) return rc;
+  rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (rc) return rc;
+  const int64_t count = (int64_t)ctx->hcnt->alive;
+  const int64_t nb = 12ll << (2 * ctx->cfg.hpx_lvl);
+  const int64_t num_thresh = (int64_t)g.n3 * (int64_t)(ctx->cfg.ray_per_cell_max - nb);
+  ctx->stats.rays_counted = count;
+  ctx->stats.num_thresh = num_thresh;
+  if (count > num_thresh) {
+    // calc_nphot_merge: log-spaced photon-count histogram, top-down cumulative count
+    const double lg_max = dm_log10(ctx->nphot_max);
+    const double lg_min = dm_log10(ctx->nphot_min);
+    const double del = (lg_max - lg_min) / (10000 - 1);
+    FRT_LAUNCH(ctx, "fill", grid_for(FRT_PDF_BINS), k_fill_i32, ctx->pdf, (int64_t)FRT_PDF_BINS, 0);
+    FRT_LAUNCH(ctx, "pdf", grid_for(n), k_pdf, rb, n, g, ctx->alive, lg_min, del, ctx->pdf);
+    std::vector<int32_t> pdf(FRT_PDF_BINS);
+    rc = hip_check(ctx, hipMemcpyAsync(pdf.data(), ctx->pdf, 4 * FRT_PDF_BINS, hipMemcpyDeviceToHost, ctx->stream), "pdf");
+    if (rc) return rc;
+    rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (rc) return rc;
+    ctx->nphot_merge = 0.f;
+    int64_t cum = 0;
+    for (int b = FRT_PDF_BINS - 1; b >= 0; --b) {
+      cum += pdf[b];
+      if (cum >= num_thresh) {
+        ctx->nphot_merge = (float)dm_pow(10., b * del + lg_min);
+        break;
+      }
+    }
+  }
+  ctx->g.nphot_merge = ctx->nphot_merge;
+  g.nphot_merge = ctx->nphot_merge;
+  // merge(): rotation index drawn every pass (ray_tracing.cc:327)
+  const uint32_t itm = frt_itheta_hash(ctx->cfg.seed, FRT_STREAM_MERGE + (uint64_t)ctx->step, 0);
+  if (!(ctx->nphot_merge > 0.f) || n == 0) {
+    ctx->stats.merge_candidates = 0;
+    return FRT_OK;
+  }
+  Mat9 inv, fwd;
+  rc = hip_check(ctx, hipMemcpyAsync(inv.m, ctx->rot_inv + 9 * (int64_t)itm, 36, hipMemcpyDeviceToHost, ctx->stream), "rot");
+  if (rc) return rc;
+  rc = hip_check(ctx, hipMemcpyAsync(fwd.m, ctx->rot_fwd + 9 * (int64_t)itm, 36, hipMemcpyDeviceToHost, ctx->stream), "rot");
+  if (rc) return rc;
+  rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (rc) return rc;
+  FRT_LAUNCH(ctx, "merge_flag", grid_for(n + 1), k_merge_flag, rb, n, g, ctx->alive, inv, ctx->cand_flag,
+             ctx->cand_key_tmp);
+  rc = exclusive_scan(ctx, ctx->cand_flag, ctx->cand_off, n + 1, "scan_merge");
+  if (rc) return rc;
+  int64_t ncand = 0;
+  rc = read_i32(ctx, ctx->cand_off + n, &ncand);
+  if (rc) return rc;
+  ctx->stats.merge_candidates = ncand;
+  if (ncand == 0) return FRT_OK;
+  FRT_LAUNCH(ctx, "merge_scatter", grid_for(n), k_merge_scatter, n, ctx->cand_flag, ctx->cand_off,
+             ctx->cand_key_tmp, ctx->m_key[0], ctx->m_val[0]);
+  int bits = 1;
+  while ((1ll << bits) < (int64_t)g.n3 * nb) ++bits;
+  rc = sort_pairs(ctx, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1], ncand, bits,
+                  "sort_merge");
+  if (rc) return rc;
+  FRT_LAUNCH(ctx, "merge_groups", grid_for(ncand), k_merge_groups, rb, ncand, g, ctx->m_key[1],
+             ctx->m_val[1], ctx->alive, fwd, itm, (float)ctx->cfg.safety_factor, ctx->dcnt);
+  return FRT_OK;
+}
+
+// split_rays into the other buffer; returns the ray count after the split.
+int launch_split_compact(frt_ctx* ctx, int64_t* n_after) {
+  const int64_t n = ctx->nrays;
+  GridConst g = ctx->g;
+  // maxd(4.*nphot_merge, 4.*nphot_min): both narrowed to float
+  const float split_thresh = frt_maxd((float)(4. * (double)ctx->nphot_merge), (float)(4. * ctx->nphot_min));
+  FRT_LAUNCH(ctx, "split_flag", grid_for(n + 1), k_split_flag, cur_buf(ctx), n, g, ctx->alive,
+             split_thresh, ctx->n_out, ctx->dcnt);
+  int rc = exclusive_scan(ctx, ctx->n_out, ctx->out_off, n + 1, "scan_split");
+  if (rc) return rc;
+  int64_t total = 0;
+  rc = read_i32(ctx, ctx->out_off + n, &total);
+  if (rc) return rc;
+  if (total + (int64_t)ctx->nsrc * 12 > ctx->g.cap)
+    return set_error(ctx, FRT_E_CAPACITY,
+                     "ray buffer full: " + std::to_string(total + (int64_t)ctx->nsrc * 12) +
+                         " rays needed, capacity " + std::to_string(ctx->g.cap));
+  FRT_LAUNCH(ctx, "compact", grid_for(n), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], n, g,
+             ctx->n_out, ctx->out_off, ctx->rot_fwd);
+  ctx->cur = 1 - ctx->cur;
+  ctx->nrays = total;
+  *n_after = total;
+  return FRT_OK;
+}
+
+int launch_cast(frt_ctx* ctx) {
+  if (ctx->nsrc == 0) return FRT_OK;
+  const int64_t nnew = (int64_t)ctx->nsrc * 12;
+  if (ctx->nrays + nnew > ctx->g.cap)
+    return set_error(ctx, FRT_E_CAPACITY, "ray buffer full in new_rays");
+  FRT_LAUNCH(ctx, "cast", grid_for(nnew), k_cast, cur_buf(ctx), ctx->nrays, ctx->nsrc, ctx->g,
+             ctx->src_cell, ctx->src_lum, ctx->src_ray_dt, ctx->rot_fwd, ctx->cfg.seed,
+             (uint64_t)ctx->cast_calls);
+  FRT_LAUNCH(ctx, "zero_ray_dt", grid_for(ctx->nsrc), k_zero_ray_dt, ctx->src_ray_dt, ctx->nsrc);
+  ctx->nrays += nnew;
+  ctx->cast_calls += 1;
+  return FRT_OK;
+}
+
+int launch_copy(frt_ctx* ctx, int64_t nbytes, int repeats) {
+  const int64_t n16 = nbytes / 16;
+  unsigned grid = (unsigned)std::min<int64_t>((n16 + BLK - 1) / BLK, 256 * 64);
+  for (int r = 0; r < repeats; ++r)
+    FRT_LAUNCH(ctx, "copy", grid, k_copy, (const int4*)ctx->copy_src, (int4*)ctx->copy_dst, n16);
+  return FRT_OK;
+}
+
+// Rotation matrices of get_unit_vector for every rotation index
+// (healpix_funcs.cc:21-39 angle grid, :112-133 matrices), float arithmetic.
+void build_rotation_tables(float* fwd, float* inv) {
+  for (int i = 0; i < FRT_NALPHA; ++i)
+    for (int j = 0; j < FRT_NALPHA; ++j)
+      for (int k = 0; k < FRT_NALPHA; ++k) {
+        const int ind = i * FRT_NALPHA * FRT_NALPHA + j * FRT_NALPHA + k;
+        const float a0 = (float)(2. * FRT_PI_REF / FRT_NALPHA * i);
+        const float b0 = (float)(2. * FRT_PI_REF / FRT_NALPHA * j);
+        const float g0 = (float)(2. * FRT_PI_REF / FRT_NALPHA * k);
+        const float ca = dm_cosf(a0), sa = dm_sinf(a0);
+        const float cb = dm_cosf(b0), sb = dm_sinf(b0);
+        const float cg = dm_cosf(g0), sg = dm_sinf(g0);
+        float* f = fwd + 9 * ind;
+        f[0] = cb * cg;
+        f[1] = -cb * sg;
+        f[2] = sb;
+        f[3] = sa * sb * cg + ca * sg;
+        f[4] = -sa * sb * sg + ca * cg;
+        f[5] = -sa * cb;
+        f[6] = -ca * sb * cg + sa * sg;
+        f[7] = ca * sb * sg + sa * cg;
+        f[8] = ca * cb;
+        float* v = inv + 9 * ind;
+        v[0] = cg * cb;
+        v[1] = sg * ca + cg * sb * sa;
+        v[2] = sg * sa - cg * sb * ca;
+        v[3] = -sg * cb;
+        v[4] = cg * ca - sg * sb * sa;
+        v[5] = cg * sa + sg * sb * ca;
+        v[6] = sb;
+        v[7] = -cb * sa;
+        v[8] = cb * ca;
+      }
+}
+
+}  // namespace frt

@@ -1,0 +1,205 @@
+"""Host-side mirror of the reference's per-step RT routines on the GPU.
+
+``FlexRT`` owns one ``frt_ctx`` of libflexrt_gpu.so.  Its methods carry the
+names of the reference routines they replace (rt_funcs.cc, ray_tracing.cc,
+gas_funcs.cc) and take and return numpy arrays in the reference's layouts.
+Every call goes through the C ABI; a missing library raises at construction.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._abi import (FIELD_ID, PER_BIN, PREC_FP64, PREC_MIXED, FrtConfig, FrtStepStats,
+                   config_struct, make_ray_view)
+
+
+class FlexRTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("frt error %d: %s" % (code, msg))
+        self.code = code
+
+
+def default_config(n):
+    """The reference's compile-time configuration (user_inputs.h) for an n^3 grid."""
+    lib = _lib.load()
+    s = FrtConfig()
+    rc = lib.frt_config_default(C.byref(s), n)
+    if rc:
+        raise FlexRTError(rc, "frt_config_default")
+    return {name: getattr(s, name) for name, _ in FrtConfig._fields_}
+
+
+class FlexRT:
+    """One GPU RT context (device state resident in HBM)."""
+
+    def __init__(self, cfg):
+        self._lib = _lib.load()
+        self.cfg = dict(cfg)
+        s = config_struct(self.cfg)
+        h = C.c_void_p()
+        rc = self._lib.frt_create(C.byref(s), C.byref(h))
+        if rc:
+            raise FlexRTError(rc, "frt_create failed (see stderr)")
+        self._h = h
+        self.n = int(self.cfg["n"])
+        self.nfreq = int(self.cfg["nfreq"])
+        self.path_length = int(self.cfg["path_length"])
+
+    # -- plumbing ------------------------------------------------------------
+    def _check(self, rc, what):
+        if rc:
+            msg = self._lib.frt_last_error(self._h)
+            raise FlexRTError(rc, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.frt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- inputs ----------------------------------------------------------------
+    def set_spectrum(self, nu_phot, spectrum):
+        nu = np.ascontiguousarray(nu_phot, dtype=np.float64)
+        sp = np.ascontiguousarray(spectrum, dtype=np.float32)
+        assert nu.size == self.nfreq and sp.size == self.nfreq
+        self._check(self._lib.frt_set_spectrum(
+            self._h, nu.ctypes.data_as(C.POINTER(C.c_double)),
+            sp.ctypes.data_as(C.POINTER(C.c_float))), "set_spectrum")
+
+    def set_sources(self, cells, lum):
+        c = np.ascontiguousarray(cells, dtype=np.int32)
+        lu = np.ascontiguousarray(lum, dtype=np.float64)
+        assert c.size == lu.size
+        self._check(self._lib.frt_set_sources(
+            self._h, int(c.size), c.ctypes.data_as(C.POINTER(C.c_int32)),
+            lu.ctypes.data_as(C.POINTER(C.c_double))), "set_sources")
+
+    def _field_size(self, name):
+        n3 = self.n ** 3
+        return n3 * self.nfreq if name in PER_BIN else n3
+
+    def upload_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32).reshape(-1)
+        assert a.size == self._field_size(name), name
+        self._check(self._lib.frt_upload_field(
+            self._h, FIELD_ID[name], a.ctypes.data_as(C.POINTER(C.c_float))), "upload " + name)
+
+    def download_field(self, name):
+        a = np.empty(self._field_size(name), dtype=np.float32)
+        self._check(self._lib.frt_download_field(
+            self._h, FIELD_ID[name], a.ctypes.data_as(C.POINTER(C.c_float))), "download " + name)
+        return a.reshape(-1, self.nfreq) if name in PER_BIN else a
+
+    def set_time(self, t):
+        self._check(self._lib.frt_set_time(self._h, float(t)), "set_time")
+
+    def get_time(self):
+        t = C.c_float()
+        self._check(self._lib.frt_get_time(self._h, C.byref(t)), "get_time")
+        return t.value
+
+    # -- reference routines ----------------------------------------------------------
+    def init_rt(self):
+        self._check(self._lib.frt_init_rt(self._h), "init_rt")
+
+    def get_nphot_min_max(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(self._lib.frt_get_nphot_min_max(self._h, C.byref(a), C.byref(b)), "get_nphot_min_max")
+        return a.value, b.value
+
+    def new_rays(self):
+        self._check(self._lib.frt_new_rays(self._h), "new_rays")
+
+    def update_rays(self):
+        self._check(self._lib.frt_update_rays(self._h), "update_rays")
+
+    def find_avg_unu(self):
+        self._check(self._lib.frt_find_avg_unu(self._h), "find_avg_unu")
+
+    def merge_rays(self):
+        self._check(self._lib.frt_merge_rays(self._h), "merge_rays")
+
+    def split_rays(self):
+        self._check(self._lib.frt_split_rays(self._h), "split_rays")
+
+    def update_thermal(self):
+        self._check(self._lib.frt_update_thermal(self._h), "update_thermal")
+
+    def step(self):
+        st = FrtStepStats()
+        self._check(self._lib.frt_step(self._h, C.byref(st)), "step")
+        return st.as_dict()
+
+    def stats(self):
+        st = FrtStepStats()
+        self._check(self._lib.frt_get_stats(self._h, C.byref(st)), "get_stats")
+        return st.as_dict()
+
+    # -- rays --------------------------------------------------------------------------
+    def ray_count(self):
+        n = C.c_int64()
+        self._check(self._lib.frt_ray_count(self._h, C.byref(n)), "ray_count")
+        return n.value
+
+    def rays(self):
+        n = self.ray_count()
+        v, arrays = make_ray_view(n, self.nfreq)
+        self._check(self._lib.frt_download_rays(self._h, C.byref(v)), "download_rays")
+        return {k: a[: v.count] for k, a in arrays.items()}
+
+    def upload_rays(self, rays):
+        n = len(rays["cell"])
+        v, arrays = make_ray_view(n, self.nfreq, rays)
+        self._keep = arrays
+        self._check(self._lib.frt_upload_rays(self._h, C.byref(v)), "upload_rays")
+
+    def paths(self):
+        n = self.ray_count()
+        P = self.path_length
+        end = np.zeros(n, dtype=np.int32)
+        cells = np.zeros((n, P), dtype=np.int32)
+        dists = np.zeros((n, P), dtype=np.float32)
+        self._check(self._lib.frt_download_paths(
+            self._h, n, end.ctypes.data_as(C.POINTER(C.c_int32)),
+            cells.ctypes.data_as(C.POINTER(C.c_int32)),
+            dists.ctypes.data_as(C.POINTER(C.c_float))), "download_paths")
+        return end, cells, dists
+
+    # -- measurement -------------------------------------------------------------------
+    def set_profiling(self, on):
+        self._check(self._lib.frt_set_profiling(self._h, 1 if on else 0), "set_profiling")
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(), C.c_int64()
+        self._check(self._lib.frt_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)),
+                    "kernel_time")
+        return ms.value, n.value
+
+    def reset_kernel_times(self):
+        self._check(self._lib.frt_reset_kernel_times(self._h), "reset_kernel_times")
+
+    def deposit_bytes(self):
+        b, n = C.c_double(), C.c_int64()
+        self._check(self._lib.frt_deposit_bytes(self._h, C.byref(b), C.byref(n)), "deposit_bytes")
+        return b.value, n.value
+
+    def copy_kernel(self, nbytes, repeats=1):
+        self._check(self._lib.frt_copy_kernel(self._h, int(nbytes), int(repeats)), "copy_kernel")
+
+    def synchronize(self):
+        self._check(self._lib.frt_synchronize(self._h), "synchronize")
+
+
+__all__ = ["FlexRT", "FlexRTError", "default_config", "PREC_FP64", "PREC_MIXED"]

@@ -1,0 +1,249 @@
+/* flexrt_gpu.h -- C ABI of the MI355X radiative-transfer core.
+ *
+ * The reference (bayu-wilson/FlexRT_imaging_reion) is one C++ translation unit
+ * whose hot-path functions take no arguments and talk through ~120 global
+ * arrays (global_variables.cc).  This library keeps that state resident in
+ * HBM inside an frt_ctx and exposes one entry point per reference routine on
+ * the per-timestep path, plus a fused step.  A driver that keeps main.cc
+ * unchanged binds these from same-named replacement files for
+ * ray_tracing.cc, rt_funcs.cc, healpix_funcs.cc, rates.cc and subgrid.cc
+ * (see the binding stubs shipped with the library).
+ *
+ * Conventions
+ *  - Every function returns an frt_status (0 = FRT_OK).  No C++ exception
+ *    crosses the ABI; frt_last_error() holds a message for the last failure.
+ *  - Host pointers are borrowed for the duration of the call only.
+ *  - Calls on one frt_ctx are ordered on the context's HIP stream and must
+ *    come from one host thread at a time.
+ *  - Grid index of cell (i, j, k) is i*n*n + j*n + k (ray_tracing.cc:313).
+ *  - Per-frequency fields are exchanged in the reference layout
+ *    [cell][nfreq] (global_variables.cc:136-138, 210-212).
+ */
+#ifndef FLEXRT_GPU_H
+#define FLEXRT_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRT_MAX_FREQ 8
+#define FRT_NTHETA 42875 /* Nalpha^3 rotation triples, healpix_funcs.cc:16-17 */
+
+typedef struct frt_ctx frt_ctx;
+
+typedef enum {
+  FRT_OK = 0,
+  FRT_E_ARG = 1,           /* invalid argument or configuration */
+  FRT_E_HIP = 2,           /* HIP runtime or kernel launch failure */
+  FRT_E_CAPACITY = 3,      /* ray buffer full ("processor full", ray_tracing.cc:275-287) */
+  FRT_E_PATH_OVERFLOW = 4, /* a ray crossed more than path_length cells in a step (ray_tracing.cc:1323) */
+  FRT_E_STATE = 5,         /* call out of order (e.g. find_avg_unu before update_rays) */
+  FRT_E_NOMEM = 6          /* device allocation failed */
+} frt_status;
+
+typedef enum { FRT_PREC_FP64 = 0, FRT_PREC_MIXED = 1 } frt_precision;
+
+/* Runtime form of the reference's compile-time configuration
+ * (user_inputs.h, global_constants.h, global_variables.cc). */
+typedef struct frt_config {
+  int32_t n;                /* cells per side, Nx = Ny = Nz (user_inputs.h:75-77) */
+  int32_t nfreq;            /* frequency bins (user_inputs.h:78), 1..FRT_MAX_FREQ */
+  int32_t path_length;      /* segments stored per ray per step (user_inputs.h:107) */
+  int32_t itercount;        /* C2Ray iterations per step (user_inputs.h:85) */
+  int32_t precision;        /* frt_precision */
+  int32_t hpx_lvl;          /* HEALPix level for casting and merging (user_inputs.h:81) */
+  int32_t l_max;            /* maximum ray order (global_variables.cc:55) */
+  int32_t ray_per_cell_max; /* target rays per cell (global_variables.cc:40) */
+  int32_t rec_case_b;       /* 1: case B recombination, 0: case A (user_inputs.h:4) */
+  int32_t temp_ev;          /* temperature evolution (user_inputs.h:22) */
+  int32_t recomb_cool;      /* recombination cooling (user_inputs.h:17) */
+  int32_t free_free;        /* free-free cooling (user_inputs.h:20) */
+  int32_t compton;          /* Compton heating/cooling (user_inputs.h:21) */
+  int32_t frac_limit;       /* 1: where both HI and HeI segment depths underflow exp()
+                               the HI/HeI split of ray_tracing.cc:770-771 is 0/0; use
+                               its limit 1/(1+exp(dtau_he - dtau_h)) instead of NaN */
+  int32_t reserved0;
+  int32_t reserved1;
+  int64_t ray_capacity;     /* rays held per buffer; 0 selects 30 n^3 (global_variables.cc:40-43) */
+  uint64_t seed;            /* seed of the rotation-index stream that replaces rand() */
+  double box_ckpc;          /* comoving box side in kpc, Lx0 (user_inputs.h:92) */
+  double a_rt;              /* scale factor of the grid: proper dx = a_rt * comoving dx */
+  double zz;                /* redshift for H(a) and Compton terms (global_variables.cc:312) */
+  double dt;                /* time step in s (global_variables.cc:38) */
+  double cl_factor;         /* reduced speed of light / c (user_inputs.h:84) */
+  double nl_const;          /* cell-crossing fraction that triggers a move (user_inputs.h:83) */
+  double safety_factor;     /* rays-per-cell split parameter (user_inputs.h:86) */
+  double min_dist;          /* in-cell snap distance in cell units (global_variables.cc:305) */
+  double temp_min;          /* temperature floor in K (global_variables.cc:304) */
+  double hh, omega_m, omega_l; /* cosmology used by H(a) (cosmo_funcs.cc:13-15) */
+} frt_config;
+
+/* Per-step counters (all reductions are deterministic). */
+typedef struct frt_step_stats {
+  int64_t step;             /* steps completed by this context */
+  int64_t rays_start;       /* rays in the buffer when update_rays began */
+  int64_t rays_moved;       /* rays that crossed the nc test (ray_tracing.cc:1252) */
+  int64_t segments;         /* path segments stored by the march */
+  int64_t rays_dumped;      /* rays whose last C2Ray pass ended in the dump rule (ray_tracing.cc:782) */
+  int64_t rays_terminated;  /* rays removed by the photon minimum (ray_tracing.cc:931-933) */
+  int64_t rays_counted;     /* count_rays() before merging (main.cc:158) */
+  int64_t num_thresh;       /* merge threshold (ray_tracing.cc:157-170) */
+  int64_t merge_candidates; /* rays with sum_nphot < nphot_merge (ray_tracing.cc:339) */
+  int64_t merge_groups;     /* occupied (cell, direction) bins */
+  int64_t rays_merged_away; /* rays absorbed into a base ray (ray_tracing.cc:401) */
+  int64_t rays_split;       /* parent rays split into 4 (rt_funcs.cc:221) */
+  int64_t rays_new;         /* rays cast by new_rays */
+  int64_t rays_end;         /* rays in the buffer after new_rays */
+  int64_t thermal_substeps; /* total update_thermal sub-steps over all cells */
+  double nphot_min;         /* photon floor (ray_tracing.cc:153) */
+  double nphot_max;
+  float nphot_merge;        /* merge threshold in photons/phot_norm (ray_tracing.cc:217) */
+  float t;                  /* simulation time after the step, s */
+} frt_step_stats;
+
+/* Cell fields (float32, n^3 values each, or n^3*nfreq for the _NU fields).
+ * Names follow global_variables.cc. */
+typedef enum {
+  FRT_F_NH = 0,        /* nH */
+  FRT_F_NHE,           /* nHe */
+  FRT_F_NH1,           /* nH1 */
+  FRT_F_NHE1,          /* nHe1 */
+  FRT_F_NH2,           /* nH2 */
+  FRT_F_NHE2,          /* nHe2 */
+  FRT_F_NE,            /* ne */
+  FRT_F_NTOT,          /* n_tot */
+  FRT_F_FH1,           /* f_H1 */
+  FRT_F_FHE1,          /* f_He1 */
+  FRT_F_FH2,           /* f_H2 */
+  FRT_F_FHE2,          /* f_He2 */
+  FRT_F_TEMP,          /* temp */
+  FRT_F_DNE_DT,        /* dne_dt */
+  FRT_F_CLUMP,         /* clump (set_clump, subgrid.cc:799) */
+  FRT_F_NH1_PREV,      /* nH1_prev */
+  FRT_F_AVG_NH1,       /* avg_nh1 */
+  FRT_F_AVG_NHE1,      /* avg_nhe1 */
+  FRT_F_RECOMB_H2,     /* recomb_H2 */
+  FRT_F_RECOMB_HE2,    /* recomb_He2 */
+  FRT_F_GAMMA_H1_TOT,  /* gamma_H1_tot */
+  FRT_F_GAMMA_HE1_TOT, /* gamma_He1_tot */
+  FRT_F_RECRATE_H2,    /* recrate_H2 */
+  FRT_F_RECRATE_HE2,   /* recrate_He2 */
+  FRT_F_TREION,        /* treion */
+  FRT_F_TREION_FLAG,   /* treion_flag as 0.0 / 1.0 */
+  FRT_F_HEAT_RATE,     /* heat_rate */
+  FRT_F_COOL_RATE,     /* cool_rate */
+  FRT_F_IF_SPEED,      /* IF_speed */
+  FRT_F_INC_FLUX,      /* Inc_Flux */
+  FRT_F_GAMMA_H1_NU,   /* gamma_H1_nu[cell][nfreq] */
+  FRT_F_GAMMA_HE1_NU,  /* gamma_He1_nu[cell][nfreq] */
+  FRT_F_NGAM_NU,       /* ngam_nu[cell][nfreq] */
+  FRT_F_COUNT
+} frt_field;
+
+/* Host view of the ray buffer in structure-of-arrays form (ray_type,
+ * global_variables.cc:20-36).  Arrays hold `count` entries; nphot holds
+ * count*nfreq values in [ray][nfreq] order.  cell is path[0].ind. */
+typedef struct frt_ray_view {
+  int64_t count;
+  int16_t* order;
+  int32_t* pixel;
+  uint16_t* itheta;
+  float* xhat;
+  float* yhat;
+  float* zhat;
+  float* nphot;
+  float* r;
+  float* dr;
+  float* x;
+  float* y;
+  float* z;
+  float* time;
+  int32_t* cell;
+} frt_ray_view;
+
+/* ---- context ------------------------------------------------------------ */
+const char* frt_version(void);
+/* Fill cfg with the reference's compile-time values (user_inputs.h,
+ * global_variables.cc) for an n^3 grid. */
+int frt_config_default(frt_config* cfg, int32_t n);
+int frt_create(const frt_config* cfg, frt_ctx** out);
+int frt_destroy(frt_ctx* ctx);
+const char* frt_last_error(const frt_ctx* ctx);
+
+/* ---- inputs -------------------------------------------------------------- */
+/* Spectrum weights per bin and the bin frequencies in Hz: spectrum[]
+ * (gas_funcs.cc:44-70) and nu_phot[] (user_inputs.h:95). */
+int frt_set_spectrum(frt_ctx* ctx, const double* nu_phot, const float* spectrum);
+/* Source cells and photon rates (photons/s), as source_lum after
+ * read_source_catalog (io_funcs.cc:157-233): one entry per source cell. */
+int frt_set_sources(frt_ctx* ctx, int32_t nsrc, const int32_t* cell, const double* lum);
+int frt_upload_field(frt_ctx* ctx, int32_t field, const float* host);
+int frt_download_field(frt_ctx* ctx, int32_t field, float* host);
+int frt_set_time(frt_ctx* ctx, float t);
+int frt_get_time(const frt_ctx* ctx, float* t);
+
+/* ---- reference routines on the per-step path ----------------------------- */
+/* init_rt (rt_funcs.cc:36-126), fresh start: empty ray buffer, ray_dt = dt
+ * for every source cell. */
+int frt_init_rt(frt_ctx* ctx);
+/* get_nphot_min_max (ray_tracing.cc:129-154). */
+int frt_get_nphot_min_max(frt_ctx* ctx, double* nphot_min, double* nphot_max);
+/* new_rays + init_rays (rt_funcs.cc:240-261, ray_tracing.cc:230-321). */
+int frt_new_rays(frt_ctx* ctx);
+/* update_rays + advance_ray (rt_funcs.cc:159-208, ray_tracing.cc:1222-1512),
+ * C2Ray mode: stores each moved ray's path segments. */
+int frt_update_rays(frt_ctx* ctx);
+/* find_avg_unu (ray_tracing.cc:623-1216): itercount C2Ray passes of
+ * attenuation, deposition and the time-averaged ionization update, then the
+ * per-ray photon update and removal. */
+int frt_find_avg_unu(frt_ctx* ctx);
+/* main.cc:153-165: set_lmerge, get_num_thresh, count_rays,
+ * calc_nphot_merge (when count > num_thresh) and merge. */
+int frt_merge_rays(frt_ctx* ctx);
+/* split_rays + split_ray + set_to_true (rt_funcs.cc:211-237,146-156,
+ * ray_tracing.cc:482-619). */
+int frt_split_rays(frt_ctx* ctx);
+/* update_thermal (gas_funcs.cc:362-457). */
+int frt_update_thermal(frt_ctx* ctx);
+/* One timestep in main.cc:117-198 order (set_clump and set_clight are
+ * uniform here): t += dt, update_rays, find_avg_unu, merge, split_rays,
+ * update_thermal, new_rays.  Statistics are copied to *stats if non-NULL. */
+int frt_step(frt_ctx* ctx, frt_step_stats* stats);
+int frt_get_stats(const frt_ctx* ctx, frt_step_stats* stats);
+
+/* ---- ray buffer ----------------------------------------------------------- */
+int frt_ray_count(const frt_ctx* ctx, int64_t* count);
+/* Copy the ray buffer to caller-owned arrays sized for view->count >= the
+ * current count (set on return). */
+int frt_download_rays(frt_ctx* ctx, frt_ray_view* view);
+/* Replace the ray buffer with view->count rays (restart, tests). */
+int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* view);
+/* Path segments stored by the last update_rays, before find_avg_unu
+ * consumes them: ray_end[i] segments for ray i (0 if it did not move),
+ * cells/dists laid out [ray][path_length] (ray[x].path, ray_end[x];
+ * ray_tracing.cc:1323-1327,1439-1442,1471).  n must equal the ray count. */
+int frt_download_paths(frt_ctx* ctx, int64_t n, int32_t* ray_end, int32_t* cells, float* dists);
+
+/* ---- measurement ----------------------------------------------------------- */
+/* When enabled, every kernel launch is bracketed by HIP events on the
+ * context stream and its duration accumulated per kernel name. */
+int frt_set_profiling(frt_ctx* ctx, int32_t enable);
+/* Accumulated time (ms) and launch count for the named kernel. */
+int frt_kernel_time(const frt_ctx* ctx, const char* name, double* total_ms, int64_t* launches);
+int frt_reset_kernel_times(frt_ctx* ctx);
+/* Algorithmic bytes moved by the deposit kernel in the last step (model
+ * used for the roofline figure). */
+int frt_deposit_bytes(const frt_ctx* ctx, double* bytes_per_launch, int64_t* launches);
+/* Device-to-device copy of nbytes through a plain copy kernel, used to
+ * calibrate HBM counters. */
+int frt_copy_kernel(frt_ctx* ctx, int64_t nbytes, int32_t repeats);
+/* Block until all work queued on the context stream has finished. */
+int frt_synchronize(frt_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLEXRT_GPU_H */

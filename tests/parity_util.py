@@ -1,0 +1,64 @@
+"""Shared helpers of the parity tests: run the GPU library and the CPU checker
+on the same seeded inputs and compare."""
+import numpy as np
+
+from flexrt_imaging_reion_amd import inputs
+
+COUNT_KEYS = ["rays_start", "rays_moved", "segments", "rays_dumped", "rays_terminated",
+              "rays_counted", "merge_candidates", "merge_groups", "rays_merged_away",
+              "rays_split", "rays_new", "rays_end", "thermal_substeps"]
+
+STATE_FIELDS = ["nH1", "nHe1", "nH2", "nHe2", "ne", "n_tot", "f_H1", "f_He1", "temp", "dne_dt",
+                "avg_nh1", "avg_nhe1", "gamma_H1_tot", "gamma_He1_tot", "gamma_H1_nu",
+                "gamma_He1_nu", "recrate_H2", "recrate_He2", "treion", "treion_flag",
+                "heat_rate", "cool_rate", "IF_speed", "Inc_Flux", "nH1_prev", "recomb_H2",
+                "recomb_He2", "ngam_nu"]
+
+
+def make_pair(wl, precision=0):
+    from flexrt_imaging_reion_amd.rt import FlexRT
+    from oracle.pyoracle import Oracle
+
+    cfg = dict(wl["cfg"])
+    cfg["precision"] = precision
+    g = FlexRT(cfg)
+    ocfg = dict(wl["cfg"])
+    ocfg["precision"] = 0
+    o = Oracle(ocfg)
+    inputs.setup(g, wl)
+    inputs.setup(o, wl)
+    return g, o
+
+
+def counts(st):
+    return {k: int(st[k]) for k in COUNT_KEYS}
+
+
+def fields(rt):
+    return {k: rt.download_field(k) for k in STATE_FIELDS}
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def max_rel(a, b, floor=0.0):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.maximum(np.abs(b), floor)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(den > 0, np.abs(a - b) / den, np.where(a == b, 0.0, np.inf))
+    return float(np.nanmax(r)) if r.size else 0.0
+
+
+def compare_rays(ra, rb):
+    """Return the first ray field that differs bitwise, or None."""
+    if len(ra["cell"]) != len(rb["cell"]):
+        return "count %d != %d" % (len(ra["cell"]), len(rb["cell"]))
+    for k in ra:
+        if not bits_equal(ra[k], rb[k]):
+            idx = np.nonzero((np.asarray(ra[k]) != np.asarray(rb[k])).reshape(len(ra["cell"]), -1).any(axis=1))[0]
+            return "%s differs at %d rays (first %d)" % (k, idx.size, idx[0] if idx.size else -1)
+    return None
