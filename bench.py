@@ -1,0 +1,230 @@
+"""Benchmark of the per-timestep RT core on one MI355X.
+
+Workload (the metric's configuration): 128^3 cells, 40 cMpc/h box at z=7,
+lognormal density (sigma_ln 1, P(k)~k^-2, seed 2), 4096 sources drawn with
+probability ~(1+delta)^2 and Ndot log-uniform in [1e51, 1e56] /s, 5 frequency
+bins, clumping C=3, 20 steps of 0.5 Myr.  One bench "step" is one RT timestep
+(march, 5 C2Ray passes, merge, split, thermal update, casting).
+
+The chosen (mixed) form is timed over --steps steps from the initial state
+after --warmup untimed steps; the FP64 form is timed the same way beside it
+and its final state is the reference for the mixed-form errors (the FP64 GPU
+path is bit-identical to the CPU checker, see tests/test_gpu_parity.py).
+The CPU checker runs a bounded sample of the same workload on one host core.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-RT-updates/s at 128^3, 4096 src: mixed vs FP64 GPU at stated x_HI err"
+KERNELS = ["march", "scan_segments", "seg_keys", "sort_segments", "cell_ranges", "inv_perm",
+           "deposit_mixed", "deposit_fp64", "chem", "count_alive", "pdf", "merge_flag",
+           "scan_merge", "merge_scatter", "sort_merge", "merge_groups", "split_flag",
+           "scan_split", "compact", "cast", "zero_ray_dt", "add_ray_dt", "fill"]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="batch128")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    return ap.parse_args()
+
+
+def kernel_bytes(name, wl, st, mixed):
+    """Algorithmic HBM bytes of one launch of `name` in a step with stats `st`.
+
+    deposit: per ray 4 (seg count); per moved ray 4 (offset) + 4F (photons in);
+      per segment 4 (cell) + 4 (length) + 4 (sorted slot) + 8 (avg_nh1, avg_nhe1)
+      + 2 acc (H and He deposits); the last pass adds 3F acc per segment (per-bin
+      deposits and photons) and 4F + 5 per moved ray (photons out, cell, alive).
+    chem: per cell 8 (run bounds) + 36 (nH, nHe, clump, f_H1, f_He1, recomb x2,
+      avg x2 or their iteration-0 sources) + 8 (avg out); per segment 2 acc; the
+      last pass adds 3F acc per segment, 12F per cell (per-bin fields) and 100
+      per cell (finalize + thermal: ~25 fields).
+    march: per ray 56 read (cell, direction, position, time, r, dr) + 5 (flags)
+      + 28 written back; per segment 8 (cell, length).
+    The per-pass figures are averaged over the itercount passes of the step.
+    """
+    F = int(wl["cfg"]["nfreq"])
+    I = int(wl["cfg"]["itercount"])
+    acc = 4.0 if mixed else 8.0
+    n3 = float(wl["cfg"]["n"]) ** 3
+    nr, nm, ns = float(st["rays_start"]), float(st["rays_moved"]), float(st["segments"])
+    if name.startswith("deposit"):
+        per = nr * 4 + nm * (4 + 4 * F) + ns * (20 + 2 * acc)
+        last = ns * 3 * F * acc + nm * (4 * F + 5)
+        return per + last / I
+    if name == "chem":
+        per = n3 * 52 + ns * 2 * acc
+        last = ns * 3 * F * acc + n3 * (12 * F + 100)
+        return per + last / I
+    if name == "march":
+        return nr * 89 + ns * 8
+    return None
+
+
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def run_form(wl, precision, steps, warmup, profile):
+    import torch
+
+    from flexrt_imaging_reion_amd import inputs
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    cfg = dict(wl["cfg"])
+    cfg["precision"] = precision
+    g = FlexRT(cfg)
+    inputs.setup(g, wl)
+    log("precision %d: context ready, %d warmup steps" % (precision, warmup))
+    for _ in range(warmup):
+        g.step()
+    inputs.setup(g, wl)  # back to the initial state: time steps 1..K of the workload
+    g.synchronize()
+    g.set_profiling(profile)
+    g.reset_kernel_times()
+    stats = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(g.step())
+    g.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    log("precision %d: %d steps in %.3f s, %d rays at the end" % (precision, steps, wall, stats[-1]["rays_end"]))
+    ktimes = {k: g.kernel_time(k) for k in KERNELS}
+    state = {f: g.download_field(f) for f in ("nH1", "nH", "temp", "gamma_H1_tot")}
+    g.close()
+    return wall, stats, ktimes, state
+
+
+def roofline(wl, stats, ktimes, mixed):
+    """Roofline figure of the kernel with the largest share of GPU time."""
+    best = None
+    for name, (ms, launches) in ktimes.items():
+        if launches == 0:
+            continue
+        if best is None or ms > best[1]:
+            best = (name, ms, launches)
+    name, ms, launches = best
+    total_bytes = 0.0
+    n_launch = 0
+    for st in stats:
+        b = kernel_bytes(name, wl, st, mixed)
+        if b is None:
+            break
+        per_step = wl["cfg"]["itercount"] if (name.startswith("deposit") or name == "chem") else 1
+        total_bytes += b * per_step
+        n_launch += per_step
+    if n_launch == 0 or n_launch != launches:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel": name, "kernel_ms_total": ms, "launches": launches}
+    avg_ms = ms / launches
+    avg_bytes = total_bytes / n_launch
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
+            "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": round(avg_bytes),
+            "launches": launches}
+
+
+def cpu_baseline(wl, seconds):
+    from flexrt_imaging_reion_amd import inputs
+    from oracle.pyoracle import Oracle
+
+    o = Oracle(wl["cfg"])
+    inputs.setup(o, wl)
+    t0 = time.perf_counter()
+    k = 0
+    while k < wl["steps"]:
+        o.step()
+        k += 1
+        log("cpu checker step %d done at %.1f s" % (k, time.perf_counter() - t0))
+        if time.perf_counter() - t0 > seconds:
+            break
+    wall = time.perf_counter() - t0
+    n3 = float(wl["cfg"]["n"]) ** 3
+    o.close()
+    return {"value": round(n3 * k / wall, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "first %d of %d steps of the same workload, CPU checker (oracle/frt_oracle.cc, "
+                      "g++ -O3, serial), %.1f s" % (k, wl["steps"], wall)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: the RT core is measured at one GPU's share "
+              "(--gpus 1); multi-GPU sharding is a design note, not a measured path.")
+        sys.exit(2)
+    from flexrt_imaging_reion_amd import inputs
+
+    wl = inputs.make_workload(a.workload, steps=max(a.steps, 1))
+    log("workload %s built: %d source cells" % (a.workload, len(wl["src_cells"])))
+    steps = a.steps
+    n3 = float(wl["cfg"]["n"]) ** 3
+    wall_m, st_m, kt_m, state_m = run_form(wl, 1, steps, a.warmup, True)
+    out = {
+        "metric": METRIC,
+        "value": round(n3 * steps / wall_m, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * wall_m / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 march/deposit + f64 chemistry",
+        "data": "synthetic",
+        "config": {"workload": "%s: %d^3 cells, %.0f cMpc/h, z=%.1f, %d source cells, 5 bins, "
+                               "%d steps x %.1f Myr, C=%s, mixed precision"
+                               % (a.workload, wl["cfg"]["n"], wl["params"]["box_cmpc_h"], wl["params"]["z"],
+                                  len(wl["src_cells"]), steps, wl["params"]["dt_myr"], wl["params"]["clump"]),
+                   "parallelism": "single GPU"},
+        "roofline": roofline(wl, st_m, kt_m, True),
+        "rays_end": int(st_m[-1]["rays_end"]),
+        "segments_per_step": int(np.mean([s["segments"] for s in st_m])),
+    }
+    if not a.no_fp64:
+        wall_f, st_f, kt_f, state_f = run_form(wl, 0, steps, a.warmup, True)
+        xm = state_m["nH1"] / state_m["nH"]
+        xf = state_f["nH1"] / state_f["nH"]
+        rel = lambda u, v: float(np.max(np.abs(u.astype(np.float64) - v) / np.maximum(np.abs(v), 1e-300)))
+        gmax = float(np.abs(state_f["gamma_H1_tot"]).max()) or 1.0
+        keys = ("rays_moved", "segments", "rays_dumped", "rays_terminated", "rays_merged_away",
+                "rays_split", "rays_end")
+        out["fp64"] = {"value": round(n3 * steps / wall_f, 1), "ms_per_step": round(1e3 * wall_f / steps, 3),
+                       "roofline": roofline(wl, st_f, kt_f, False)}
+        out["mixed_over_fp64"] = round(wall_f / wall_m, 3)
+        out["errors_vs_fp64"] = {
+            "x_HI_max_rel": rel(xm, xf),
+            "ionized_volume_fraction_abs": abs(float((xm < 0.5).mean()) - float((xf < 0.5).mean())),
+            "Gamma_HI_max_abs_over_max": float(np.abs(state_m["gamma_H1_tot"].astype(np.float64) -
+                                                      state_f["gamma_H1_tot"]).max()) / gmax,
+            "T_max_rel": rel(state_m["temp"], state_f["temp"]),
+            "ray_counts_exact": all(all(sm[k] == sf[k] for k in keys) for sm, sf in zip(st_m, st_f)),
+        }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

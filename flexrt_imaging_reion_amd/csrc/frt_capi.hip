@@ -519,8 +519,14 @@ int frt_get_time(const frt_ctx* ctx, float* t) {
 
 int frt_init_rt(frt_ctx* ctx) {
   if (!ctx) return FRT_E_ARG;
+  // fresh start: empty buffer, run clock and merge threshold at their initial values
   ctx->nrays = 0;
   ctx->cur = 0;
+  ctx->t = 0.f;
+  ctx->step = 0;
+  ctx->cast_calls = 0;
+  ctx->nphot_merge = 0.f;
+  ctx->g.nphot_merge = 0.f;
   ctx->have_alive = false;
   ctx->have_segments = false;
   if (ctx->nsrc > 0) {

@@ -267,9 +267,19 @@ def make_workload(name, n=None, n_src=None, steps=None, precision=0, seed_offset
 
 
 def setup(rt, wl):
-    """main.cc:29-105 for a fresh start on an RT backend (GPU or checker)."""
-    for k, v in wl["fields"].items():
-        rt.upload_field(k, v)
+    """main.cc:29-105 for a fresh start on an RT backend (GPU or checker).
+
+    Every cell field is uploaded (the ones init() does not set as zeros, as
+    the reference's zero-initialized globals), so a context can be reset."""
+    from ._abi import FIELDS, PER_BIN
+
+    n3 = int(wl["cfg"]["n"]) ** 3
+    nf = int(wl["cfg"]["nfreq"])
+    for k in FIELDS:
+        if k in wl["fields"]:
+            rt.upload_field(k, wl["fields"][k])
+        else:
+            rt.upload_field(k, np.zeros(n3 * nf if k in PER_BIN else n3, dtype=np.float32))
     rt.set_spectrum(wl["nu_phot"], wl["spectrum"])
     rt.set_sources(wl["src_cells"], wl["src_lum"])
     rt.init_rt()

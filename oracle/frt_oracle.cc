@@ -1112,6 +1112,10 @@ float orc_get_time(void* h) { return ((State*)h)->t; }
 void orc_init_rt(void* h) {
   State* s = (State*)h;
   s->ray.clear();
+  s->t = 0.f;
+  s->step = 0;
+  s->cast_calls = 0;
+  s->nphot_merge = 0.f;
   for (auto& r : s->ray_dt) r = s->dt;
 }
 

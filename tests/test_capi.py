@@ -1,0 +1,62 @@
+"""The C-ABI library loads and exports every entry point include/flexrt_gpu.h
+declares; host-only calls work without a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from flexrt_imaging_reion_amd import _lib, inputs
+from flexrt_imaging_reion_amd._abi import FIELDS, FrtConfig, FrtRayView, FrtStepStats
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "flexrt_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(frt_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_bound_functions():
+    decl = declared_functions()
+    assert decl == _lib.EXPORTED, set(decl) ^ set(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_field_enum_matches_header():
+    text = open(HEADER).read()
+    body = text[text.index("typedef enum {\n  FRT_F_NH"):text.index("} frt_field;")]
+    names = re.findall(r"(FRT_F_\w+)", body)
+    assert names[-1] == "FRT_F_COUNT"
+    assert len(names) - 1 == len(FIELDS)
+
+
+def test_struct_sizes_match_c_layout():
+    # frt_config: 16 int32, int64, uint64, 12 double
+    assert C.sizeof(FrtConfig) == 16 * 4 + 8 + 8 + 12 * 8
+    assert C.sizeof(FrtStepStats) == 15 * 8 + 2 * 8 + 2 * 4
+    assert C.sizeof(FrtRayView) == 8 + 14 * 8
+
+
+def test_config_default_without_gpu():
+    lib = _lib.load()
+    s = FrtConfig()
+    assert lib.frt_config_default(C.byref(s), 32) == 0
+    got = {name: getattr(s, name) for name, _ in FrtConfig._fields_}
+    assert got == inputs.reference_defaults(32)
+    assert lib.frt_config_default(None, 32) != 0
+    assert lib.frt_version().decode().startswith("flexrt")
+
+
+def test_oracle_library_loads():
+    from oracle import pyoracle
+
+    lib = pyoracle.load()
+    for name in pyoracle._PROTOS:
+        assert hasattr(lib, name), name
