@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the runs, copy MIB MiB through the library's plain copy kernel "
+                         "(reference dispatch for HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -213,8 +216,15 @@ def main():
         out["fp64"] = {"value": round(n3 * steps / wall_f, 1), "ms_per_step": round(1e3 * wall_f / steps, 3),
                        "roofline": roofline(wl, st_f, kt_f, False)}
         out["mixed_over_fp64"] = round(wall_f / wall_m, 3)
+        rx = np.abs(xm.astype(np.float64) - xf) / np.abs(xf.astype(np.float64))
+        first_diff = next((k for k, (sm, sf) in enumerate(zip(st_m, st_f))
+                           if any(sm[q] != sf[q] for q in keys)), None)
         out["errors_vs_fp64"] = {
             "x_HI_max_rel": rel(xm, xf),
+            "x_HI_rel_p99_9": float(np.percentile(rx, 99.9)),
+            "x_HI_rel_median": float(np.median(rx)),
+            "cells_x_HI_rel_over_1e-3": int((rx > 1e-3).sum()),
+            "first_step_with_count_difference": first_diff,
             "ionized_volume_fraction_abs": abs(float((xm < 0.5).mean()) - float((xf < 0.5).mean())),
             "Gamma_HI_max_abs_over_max": float(np.abs(state_m["gamma_H1_tot"].astype(np.float64) -
                                                       state_f["gamma_H1_tot"]).max()) / gmax,
@@ -223,6 +233,12 @@ def main():
         }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
+    if a.calibrate_copy:
+        from flexrt_imaging_reion_amd.rt import FlexRT
+
+        with FlexRT(inputs.reference_defaults(8)) as c:
+            c.copy_kernel(a.calibrate_copy << 20, 3)
+        out["copy_calibration_bytes"] = a.calibrate_copy << 20
     print(json.dumps(out), flush=True)
 
 

@@ -276,7 +276,8 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     return FRT_E_ARG;
   const int64_t n3 = (int64_t)c.n * c.n * c.n;
   const int64_t cap = c.ray_capacity > 0 ? c.ray_capacity : 30 * n3;
-  if (cap * (int64_t)c.path_length >= (1ll << 31) || cap >= (1ll << 31) - 1) return FRT_E_ARG;
+  // segment indices are int32; split bookkeeping packs (ray << 2 | child) in 32 bits
+  if (cap * (int64_t)c.path_length >= (1ll << 31) || cap >= (1ll << 30)) return FRT_E_ARG;
 
   frt_ctx* ctx = new frt_ctx();
   ctx->cfg = c;
@@ -419,7 +420,7 @@ int frt_destroy(frt_ctx* ctx) {
   for (int b = 0; b < 2; ++b) {
     hipFree(ctx->seg_key[b]); hipFree(ctx->seg_val[b]); hipFree(ctx->m_key[b]); hipFree(ctx->m_val[b]);
   }
-  hipFree(ctx->inv_perm); hipFree(ctx->contrib.gh); hipFree(ctx->pdf);
+  hipFree(ctx->contrib.pair); hipFree(ctx->pdf);
   hipFree(ctx->cell_start); hipFree(ctx->cell_end); hipFree(ctx->cell_block);
   hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
   hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt);

@@ -108,14 +108,13 @@ struct CellPtrs {
   float* ngam_nu;       // [nfreq][n3]
 };
 
-// Contributions of path segments to their cell, stored in cell-sorted order.
+// Deposits of path segments, one slot per segment in segment order; the cell
+// kernel reads a cell's slots through perm (cell-sorted position -> slot).
 struct ContribPtrs {
-  void* gh;    // sum over bins of the HI deposit, per segment
-  void* ghe;   // sum over bins of the HeI deposit
-  void* ghnu;  // [nfreq][seg_cap] per-bin HI deposit (last pass)
-  void* ghenu; // [nfreq][seg_cap]
-  void* ngnu;  // [nfreq][seg_cap] photons entering the segment / phot_norm
-  int64_t stride;
+  void* pair;          // [slot] {H, He} deposit summed over bins (2 values)
+  void* rec;           // [slot] last pass: gh[F], ghe[F], photons[F], padded to rec_len
+  const int32_t* perm; // cell-sorted position -> segment slot
+  int rec_len;         // values per record (16 or 32)
 };
 
 struct frt_ctx {
@@ -147,7 +146,6 @@ struct frt_ctx {
   int64_t seg_cap;
   uint32_t* seg_key[2];
   int32_t* seg_val[2];
-  int32_t* inv_perm;
   ContribPtrs contrib;
   int32_t* cell_start;
   int32_t* cell_end;

@@ -31,11 +31,28 @@ __device__ __forceinline__ int wrap_mod(int a, int b) {
   return a % b;
 }
 
-// Add the number of set flags of this wavefront to *ctr (one atomic per wave).
-// Must be reached by every lane of the wave.
-__device__ __forceinline__ void wave_count(unsigned long long* ctr, bool flag) {
-  unsigned long long m = __ballot(flag ? 1 : 0);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
+// Grid-stride launches are capped at this many blocks, so a per-block counter
+// flush costs at most MAX_GRID atomics per kernel on one address.
+static constexpr unsigned MAX_GRID = 4096;
+static inline unsigned sgrid(int64_t n) {
+  const int64_t g = (n + BLK - 1) / BLK;
+  return (unsigned)(g < (int64_t)MAX_GRID ? g : (int64_t)MAX_GRID);
+}
+
+// Add v summed over the block to *ctr with one atomic.  Every thread of the
+// block must reach the call.
+__device__ __forceinline__ void block_add(unsigned long long* ctr, unsigned long long v) {
+  __shared__ unsigned long long part[BLK / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) part[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < BLK / 64; ++k) t += part[k];
+    if (t) atomicAdd(ctr, t);
+  }
+  __syncthreads();
 }
 
 // v' = M v in float with the reference's left-to-right sums (healpix_funcs.cc:112-133).
@@ -76,10 +93,10 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
                                                int32_t* __restrict__ seg_cnt,
                                                uint8_t* __restrict__ alive,
                                                DevCounters* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  bool moved = false;
+  unsigned long long nmoved = 0;
   bool overflow = false;
-  if (i < nrays) {
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+    bool moved = false;
     alive[i] = 1;
     const int n = g.n;
     const int ind = rb.cell[i];
@@ -196,8 +213,9 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
       rb.z[i] = z0;
       rb.time[i] = 0.f;
     }
+    nmoved += moved ? 1 : 0;
   }
-  wave_count(&cnt->moved, moved);
+  block_add(&cnt->moved, nmoved);
   if (overflow) atomicOr(&cnt->path_overflow, 1u);
 }
 
@@ -233,14 +251,23 @@ __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict_
   if (s == nseg - 1 || key[s + 1] != k) end[k] = (int32_t)(s + 1);
 }
 
-__global__ void __launch_bounds__(BLK) k_inv_perm(const int32_t* __restrict__ val, int64_t nseg,
-                                                  int32_t* __restrict__ inv) {
-  const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (s < nseg) inv[val[s]] = (int32_t)s;
-}
 
 // ---------------------------------------------------------------------------
 // find_avg_unu ray loop (ray_tracing.cc:701-938), one thread per moved ray.
+//
+// Each segment's deposit is written to its own slot in segment order
+// (coalesced): a {H, He} pair (bin sums) in the first passes, a full record
+// {gh[F], ghe[F], photons[F]} padded to a whole number of cache lines in the
+// last pass.  The cell kernel gathers a cell's slots through the stable cell
+// sort, i.e. in ray order.
+
+// Record helpers: write REC values of type T at rec (all of them, so the
+// stores cover whole cache lines).
+template <typename T>
+__device__ __forceinline__ void store_record(T* rec, const T* v, int n, int rec_len) {
+  for (int k = 0; k < rec_len; ++k) rec[k] = (k < n) ? v[k] : (T)0;
+}
+
 // FP64 form: the reference's double locals and libm exp (through detmath),
 // with the per-cell accumulation in double.
 template <bool LAST>
@@ -249,123 +276,120 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
                                                       const float* __restrict__ path_dist,
                                                       const int32_t* __restrict__ seg_cnt,
                                                       const int32_t* __restrict__ seg_off,
-                                                      const int32_t* __restrict__ inv_perm,
                                                       const float* __restrict__ avgH,
                                                       const float* __restrict__ avgHe,
                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
                                                       DevCounters* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  bool dumped_flag = false, removed = false;
-  const int c = (i < nrays) ? seg_cnt[i] : 0;
-  if (c > 0) {
-    const int F = g.nfreq;
-    double* cgh = (double*)cp.gh;
-    double* cghe = (double*)cp.ghe;
-    double nph[FRT_MAX_FREQ], tauh[FRT_MAX_FREQ], tauhe[FRT_MAX_FREQ];
-    for (int nu = 0; nu < F; ++nu) {
-      nph[nu] = (double)rb.nphot[nu * g.cap + i] * FRT_PHOT_NORM;
-      tauh[nu] = 0.;
-      tauhe[nu] = 0.;
-    }
-    const int base = seg_off[i];
-    const double dtd = (double)g.dt;
-    int p = 0;
-    bool dumped = false;
-    for (; p < c; ++p) {
-      const int cell = path_cell[(int64_t)p * g.cap + i];
-      const double dist = (double)path_dist[(int64_t)p * g.cap + i];
-      const float ah1 = avgH[cell];
-      const float ahe1 = avgHe[cell];
-      double dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ], nin[FRT_MAX_FREQ];
-      double tot = 0.;
+  unsigned long long ndumped = 0, nremoved = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+    bool dumped_flag = false, removed = false;
+    const int c = seg_cnt[i];
+    if (c > 0) {
+      const int F = g.nfreq;
+      double nph[FRT_MAX_FREQ], tauh[FRT_MAX_FREQ], tauhe[FRT_MAX_FREQ];
       for (int nu = 0; nu < F; ++nu) {
-        dth[nu] = ((double)ah1 * g.sigH[nu]) * dist;    // avg_nh1*sigma*dist
-        dthe[nu] = ((double)ahe1 * g.sigHe[nu]) * dist;
-        nin[nu] = nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]);
-        tot += nin[nu] / FRT_PHOT_NORM;
+        nph[nu] = (double)rb.nphot[nu * g.cap + i] * FRT_PHOT_NORM;
+        tauh[nu] = 0.;
+        tauhe[nu] = 0.;
       }
-      const bool dump = tot < g.nphot_min;
-      double gh_sum = 0., ghe_sum = 0.;
-      const int s = inv_perm[base + p];
-      for (int nu = 0; nu < F; ++nu) {
-        double fh = 0., fhe = 0., att = 0.;
-        const bool any = (dth[nu] > 0.) || (dthe[nu] > 0.);
-        double eh = 1., ehe = 1.;
-        if (any || !dump) {
-          eh = dm_exp(-dth[nu]);
-          ehe = (dthe[nu] == 0.) ? 1. : dm_exp(-dthe[nu]);  // exp(-0) = 1 exactly
-        }
-        if (any) {
-          const double a = (1. - eh) * ehe;
-          const double b = (1. - ehe) * eh;
-          fh = a / (a + b);
-          fhe = b / (a + b);
-          if (g.frac_limit && (a + b) == 0.) {
-            fh = 1. / (1. + dm_exp(dthe[nu] - dth[nu]));
-            fhe = 1. / (1. + dm_exp(dth[nu] - dthe[nu]));
-          }
-        }
-        double gh, ghe;
-        if (dump) {
-          gh = fh * nin[nu] / dtd / g.vol / (double)ah1;
-          ghe = fhe * nin[nu] / dtd / g.vol / (double)ahe1;
-        } else {
-          // exp(-dth - dthe): with dthe = 0 the argument is exactly -dth
-          att = 1. - ((dthe[nu] == 0.) ? eh : dm_exp(-dth[nu] - dthe[nu]));
-          gh = fh * nin[nu] * att / dtd / g.vol / (double)ah1;
-          ghe = fhe * nin[nu] * att / dtd / g.vol / (double)ahe1;
-        }
-        gh_sum += gh;
-        ghe_sum += ghe;
-        if (LAST) {
-          ((double*)cp.ghnu)[nu * cp.stride + s] = gh;
-          ((double*)cp.ghenu)[nu * cp.stride + s] = ghe;
-          ((double*)cp.ngnu)[nu * cp.stride + s] = nin[nu] / FRT_PHOT_NORM;
-        }
-        tauh[nu] += dth[nu];
-        tauhe[nu] += dthe[nu];
-      }
-      cgh[s] = gh_sum;
-      cghe[s] = ghe_sum;
-      if (dump) {
-        dumped = true;
-        ++p;
-        break;
-      }
-    }
-    // segments behind a dump deposit nothing this pass
-    for (; p < c; ++p) {
-      const int s = inv_perm[base + p];
-      cgh[s] = 0.;
-      cghe[s] = 0.;
-      if (LAST) {
+      const int base = seg_off[i];
+      const double dtd = (double)g.dt;
+      int p = 0;
+      bool dumped = false;
+      for (; p < c; ++p) {
+        const int cell = path_cell[(int64_t)p * g.cap + i];
+        const double dist = (double)path_dist[(int64_t)p * g.cap + i];
+        const float ah1 = avgH[cell];
+        const float ahe1 = avgHe[cell];
+        double dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ], nin[FRT_MAX_FREQ];
+        double tot = 0.;
         for (int nu = 0; nu < F; ++nu) {
-          ((double*)cp.ghnu)[nu * cp.stride + s] = 0.;
-          ((double*)cp.ghenu)[nu * cp.stride + s] = 0.;
-          ((double*)cp.ngnu)[nu * cp.stride + s] = 0.;
+          dth[nu] = ((double)ah1 * g.sigH[nu]) * dist;    // avg_nh1*sigma*dist
+          dthe[nu] = ((double)ahe1 * g.sigHe[nu]) * dist;
+          nin[nu] = nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]);
+          tot += nin[nu] / FRT_PHOT_NORM;
+        }
+        const bool dump = tot < g.nphot_min;
+        double gh_sum = 0., ghe_sum = 0.;
+        double rec[3 * FRT_MAX_FREQ];
+        for (int nu = 0; nu < F; ++nu) {
+          double fh = 0., fhe = 0., att = 0.;
+          const bool any = (dth[nu] > 0.) || (dthe[nu] > 0.);
+          double eh = 1., ehe = 1.;
+          if (any || !dump) {
+            eh = dm_exp(-dth[nu]);
+            ehe = (dthe[nu] == 0.) ? 1. : dm_exp(-dthe[nu]);  // exp(-0) = 1 exactly
+          }
+          if (any) {
+            const double a = (1. - eh) * ehe;
+            const double b = (1. - ehe) * eh;
+            fh = a / (a + b);
+            fhe = b / (a + b);
+            if (g.frac_limit && (a + b) == 0.) {
+              fh = 1. / (1. + dm_exp(dthe[nu] - dth[nu]));
+              fhe = 1. / (1. + dm_exp(dth[nu] - dthe[nu]));
+            }
+          }
+          double gh, ghe;
+          if (dump) {
+            gh = fh * nin[nu] / dtd / g.vol / (double)ah1;
+            ghe = fhe * nin[nu] / dtd / g.vol / (double)ahe1;
+          } else {
+            // exp(-dth - dthe): with dthe = 0 the argument is exactly -dth
+            att = 1. - ((dthe[nu] == 0.) ? eh : dm_exp(-dth[nu] - dthe[nu]));
+            gh = fh * nin[nu] * att / dtd / g.vol / (double)ah1;
+            ghe = fhe * nin[nu] * att / dtd / g.vol / (double)ahe1;
+          }
+          gh_sum += gh;
+          ghe_sum += ghe;
+          rec[nu] = gh;
+          rec[F + nu] = ghe;
+          rec[2 * F + nu] = nin[nu] / FRT_PHOT_NORM;
+          tauh[nu] += dth[nu];
+          tauhe[nu] += dthe[nu];
+        }
+        const int64_t slot = base + p;
+        if (LAST)
+          store_record((double*)cp.rec + slot * cp.rec_len, rec, 3 * F, cp.rec_len);
+        else
+          ((double2*)cp.pair)[slot] = make_double2(gh_sum, ghe_sum);
+        if (dump) {
+          dumped = true;
+          ++p;
+          break;
+        }
+      }
+      // segments behind a dump deposit nothing this pass
+      for (; p < c; ++p) {
+        const int64_t slot = base + p;
+        if (LAST)
+          store_record((double*)cp.rec + slot * cp.rec_len, (const double*)nullptr, 0, cp.rec_len);
+        else
+          ((double2*)cp.pair)[slot] = make_double2(0., 0.);
+      }
+      if (LAST) {
+        // photon update and removal (ray_tracing.cc:923-935)
+        float sum = 0.f;
+        for (int nu = 0; nu < F; ++nu) {
+          const float v = (float)(nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]) / FRT_PHOT_NORM);
+          rb.nphot[nu * g.cap + i] = v;
+          sum += v;
+        }
+        dumped_flag = dumped;
+        if ((double)sum > g.nphot_min) {
+          rb.cell[i] = path_cell[(int64_t)(c - 1) * g.cap + i];
+        } else {
+          alive[i] = 0;
+          removed = true;
         }
       }
     }
-    if (LAST) {
-      // photon update and removal (ray_tracing.cc:923-935)
-      float sum = 0.f;
-      for (int nu = 0; nu < F; ++nu) {
-        const float v = (float)(nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]) / FRT_PHOT_NORM);
-        rb.nphot[nu * g.cap + i] = v;
-        sum += v;
-      }
-      dumped_flag = dumped;
-      if ((double)sum > g.nphot_min) {
-        rb.cell[i] = path_cell[(int64_t)(c - 1) * g.cap + i];
-      } else {
-        alive[i] = 0;
-        removed = true;
-      }
-    }
+    ndumped += dumped_flag ? 1 : 0;
+    nremoved += removed ? 1 : 0;
   }
   if (LAST) {
-    wave_count(&cnt->dumped, dumped_flag);
-    wave_count(&cnt->terminated, removed);
+    block_add(&cnt->dumped, ndumped);
+    block_add(&cnt->terminated, nremoved);
   }
 }
 
@@ -377,120 +401,118 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
                                                        const float* __restrict__ path_dist,
                                                        const int32_t* __restrict__ seg_cnt,
                                                        const int32_t* __restrict__ seg_off,
-                                                       const int32_t* __restrict__ inv_perm,
                                                        const float* __restrict__ avgH,
                                                        const float* __restrict__ avgHe,
                                                        ContribPtrs cp, uint8_t* __restrict__ alive,
                                                        DevCounters* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  bool dumped_flag = false, removed = false;
-  const int c = (i < nrays) ? seg_cnt[i] : 0;
-  if (c > 0) {
-    const int F = g.nfreq;
-    float* cgh = (float*)cp.gh;
-    float* cghe = (float*)cp.ghe;
-    float nph[FRT_MAX_FREQ], trans[FRT_MAX_FREQ];
-    for (int nu = 0; nu < F; ++nu) {
-      nph[nu] = rb.nphot[nu * g.cap + i];  // in units of phot_norm
-      trans[nu] = 1.f;                     // exp(-tau) carried multiplicatively
-    }
-    const int base = seg_off[i];
-    // 1/(dt vol) in units of phot_norm photons
-    const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
-    const float nmin = (float)g.nphot_min;
-    int p = 0;
-    bool dumped = false;
-    for (; p < c; ++p) {
-      const int cell = path_cell[(int64_t)p * g.cap + i];
-      const float dist = path_dist[(int64_t)p * g.cap + i];
-      const float ah1 = avgH[cell];
-      const float ahe1 = avgHe[cell];
-      const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
-      // absorbed fractions 1 - exp(-dtau) through expm1: in ionized cells dtau is
-      // ~1e-5 and 1 - expf(-dtau) would keep only ~3 significant digits
-      float nin[FRT_MAX_FREQ], eh[FRT_MAX_FREQ], ehe[FRT_MAX_FREQ], omh[FRT_MAX_FREQ],
-          omhe[FRT_MAX_FREQ], dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ];
-      float tot = 0.f;
+  unsigned long long ndumped = 0, nremoved = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+    bool dumped_flag = false, removed = false;
+    const int c = seg_cnt[i];
+    if (c > 0) {
+      const int F = g.nfreq;
+      float nph[FRT_MAX_FREQ], trans[FRT_MAX_FREQ];
       for (int nu = 0; nu < F; ++nu) {
-        dth[nu] = ah1 * g.sigHf[nu] * dist;
-        dthe[nu] = ahe1 * g.sigHef[nu] * dist;
-        omh[nu] = -expm1f(-dth[nu]);
-        omhe[nu] = (dthe[nu] == 0.f) ? 0.f : -expm1f(-dthe[nu]);
-        eh[nu] = expf(-dth[nu]);
-        ehe[nu] = (dthe[nu] == 0.f) ? 1.f : expf(-dthe[nu]);
-        nin[nu] = nph[nu] * trans[nu];
-        tot += nin[nu];
+        nph[nu] = rb.nphot[nu * g.cap + i];  // in units of phot_norm
+        trans[nu] = 1.f;                     // exp(-tau) carried multiplicatively
       }
-      const bool dump = tot < nmin;
-      float gh_sum = 0.f, ghe_sum = 0.f;
-      const int s = inv_perm[base + p];
-      for (int nu = 0; nu < F; ++nu) {
-        float fh = 0.f, fhe = 0.f;
-        if ((dth[nu] > 0.f) || (dthe[nu] > 0.f)) {
-          const float a = omh[nu] * ehe[nu];
-          const float b = omhe[nu] * eh[nu];
-          const float den = a + b;
-          if (den > 0.f) {
-            fh = a / den;
-            fhe = b / den;
-          } else if (g.frac_limit) {
-            fh = 1.f / (1.f + expf(dthe[nu] - dth[nu]));
-            fhe = 1.f / (1.f + expf(dth[nu] - dthe[nu]));
-          } else {
-            fh = fhe = __int_as_float(0x7fc00000);
-          }
-        }
-        const float eabs = dump ? nin[nu] : nin[nu] * ((dthe[nu] == 0.f) ? omh[nu] : -expm1f(-(dth[nu] + dthe[nu])));
-        const float gh = fh * eabs * rh1;
-        const float ghe = fhe * eabs * rhe1;
-        gh_sum += gh;
-        ghe_sum += ghe;
-        if (LAST) {
-          ((float*)cp.ghnu)[nu * cp.stride + s] = gh;
-          ((float*)cp.ghenu)[nu * cp.stride + s] = ghe;
-          ((float*)cp.ngnu)[nu * cp.stride + s] = nin[nu];
-        }
-        trans[nu] *= eh[nu] * ehe[nu];
-      }
-      cgh[s] = gh_sum;
-      cghe[s] = ghe_sum;
-      if (dump) {
-        dumped = true;
-        ++p;
-        break;
-      }
-    }
-    for (; p < c; ++p) {
-      const int s = inv_perm[base + p];
-      cgh[s] = 0.f;
-      cghe[s] = 0.f;
-      if (LAST) {
+      const int base = seg_off[i];
+      // 1/(dt vol) in units of phot_norm photons
+      const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
+      const float nmin = (float)g.nphot_min;
+      int p = 0;
+      bool dumped = false;
+      for (; p < c; ++p) {
+        const int cell = path_cell[(int64_t)p * g.cap + i];
+        const float dist = path_dist[(int64_t)p * g.cap + i];
+        const float ah1 = avgH[cell];
+        const float ahe1 = avgHe[cell];
+        const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
+        // absorbed fractions 1 - exp(-dtau) through expm1: in ionized cells dtau is
+        // ~1e-5 and 1 - expf(-dtau) would keep only ~3 significant digits
+        float nin[FRT_MAX_FREQ], eh[FRT_MAX_FREQ], ehe[FRT_MAX_FREQ], omh[FRT_MAX_FREQ],
+            omhe[FRT_MAX_FREQ], dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ];
+        float tot = 0.f;
         for (int nu = 0; nu < F; ++nu) {
-          ((float*)cp.ghnu)[nu * cp.stride + s] = 0.f;
-          ((float*)cp.ghenu)[nu * cp.stride + s] = 0.f;
-          ((float*)cp.ngnu)[nu * cp.stride + s] = 0.f;
+          dth[nu] = ah1 * g.sigHf[nu] * dist;
+          dthe[nu] = ahe1 * g.sigHef[nu] * dist;
+          omh[nu] = -expm1f(-dth[nu]);
+          omhe[nu] = (dthe[nu] == 0.f) ? 0.f : -expm1f(-dthe[nu]);
+          eh[nu] = expf(-dth[nu]);
+          ehe[nu] = (dthe[nu] == 0.f) ? 1.f : expf(-dthe[nu]);
+          nin[nu] = nph[nu] * trans[nu];
+          tot += nin[nu];
+        }
+        const bool dump = tot < nmin;
+        float gh_sum = 0.f, ghe_sum = 0.f;
+        float rec[3 * FRT_MAX_FREQ];
+        for (int nu = 0; nu < F; ++nu) {
+          float fh = 0.f, fhe = 0.f;
+          if ((dth[nu] > 0.f) || (dthe[nu] > 0.f)) {
+            const float a = omh[nu] * ehe[nu];
+            const float b = omhe[nu] * eh[nu];
+            const float den = a + b;
+            if (den > 0.f) {
+              fh = a / den;
+              fhe = b / den;
+            } else if (g.frac_limit) {
+              fh = 1.f / (1.f + expf(dthe[nu] - dth[nu]));
+              fhe = 1.f / (1.f + expf(dth[nu] - dthe[nu]));
+            } else {
+              fh = fhe = __int_as_float(0x7fc00000);
+            }
+          }
+          const float eabs =
+              dump ? nin[nu] : nin[nu] * ((dthe[nu] == 0.f) ? omh[nu] : -expm1f(-(dth[nu] + dthe[nu])));
+          const float gh = fh * eabs * rh1;
+          const float ghe = fhe * eabs * rhe1;
+          gh_sum += gh;
+          ghe_sum += ghe;
+          rec[nu] = gh;
+          rec[F + nu] = ghe;
+          rec[2 * F + nu] = nin[nu];
+          trans[nu] *= eh[nu] * ehe[nu];
+        }
+        const int64_t slot = base + p;
+        if (LAST)
+          store_record((float*)cp.rec + slot * cp.rec_len, rec, 3 * F, cp.rec_len);
+        else
+          ((float2*)cp.pair)[slot] = make_float2(gh_sum, ghe_sum);
+        if (dump) {
+          dumped = true;
+          ++p;
+          break;
+        }
+      }
+      for (; p < c; ++p) {
+        const int64_t slot = base + p;
+        if (LAST)
+          store_record((float*)cp.rec + slot * cp.rec_len, (const float*)nullptr, 0, cp.rec_len);
+        else
+          ((float2*)cp.pair)[slot] = make_float2(0.f, 0.f);
+      }
+      if (LAST) {
+        float sum = 0.f;
+        for (int nu = 0; nu < F; ++nu) {
+          const float v = nph[nu] * trans[nu];
+          rb.nphot[nu * g.cap + i] = v;
+          sum += v;
+        }
+        dumped_flag = dumped;
+        if ((double)sum > g.nphot_min) {
+          rb.cell[i] = path_cell[(int64_t)(c - 1) * g.cap + i];
+        } else {
+          alive[i] = 0;
+          removed = true;
         }
       }
     }
-    if (LAST) {
-      float sum = 0.f;
-      for (int nu = 0; nu < F; ++nu) {
-        const float v = nph[nu] * trans[nu];
-        rb.nphot[nu * g.cap + i] = v;
-        sum += v;
-      }
-      dumped_flag = dumped;
-      if ((double)sum > g.nphot_min) {
-        rb.cell[i] = path_cell[(int64_t)(c - 1) * g.cap + i];
-      } else {
-        alive[i] = 0;
-        removed = true;
-      }
-    }
+    ndumped += dumped_flag ? 1 : 0;
+    nremoved += removed ? 1 : 0;
   }
   if (LAST) {
-    wave_count(&cnt->dumped, dumped_flag);
-    wave_count(&cnt->terminated, removed);
+    block_add(&cnt->dumped, ndumped);
+    block_add(&cnt->terminated, nremoved);
   }
 }
 
@@ -581,64 +603,86 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
   if (k < g.n3) {
     const int64_t n3 = g.n3;
     const int s0 = cstart[k], s1 = cend[k];
-    const ACC* cgh = (const ACC*)cp.gh;
-    const ACC* cghe = (const ACC*)cp.ghe;
+    // sums over the cell's segments in cell-sorted (= ray) order
+    const int F = g.nfreq;
     double GH, GHe;
     double GHnu[FRT_MAX_FREQ], GHenu[FRT_MAX_FREQ], NGnu[FRT_MAX_FREQ];
     if (sizeof(ACC) == 8) {
       double a = 0., b = 0.;
+      double x[FRT_MAX_FREQ], y[FRT_MAX_FREQ], z[FRT_MAX_FREQ];
+      for (int nu = 0; nu < F; ++nu) x[nu] = y[nu] = z[nu] = 0.;
       for (int s = s0; s < s1; ++s) {
-        a += (double)cgh[s];
-        b += (double)cghe[s];
+        const int64_t c = cp.perm[s];
+        if (!LAST) {
+          const double2 v = ((const double2*)cp.pair)[c];
+          a += v.x;
+          b += v.y;
+        } else {
+          const double* r = (const double*)cp.rec + c * cp.rec_len;
+          double gs = 0., gse = 0.;
+          for (int nu = 0; nu < F; ++nu) {
+            gs += r[nu];
+            gse += r[F + nu];
+            x[nu] += r[nu];
+            y[nu] += r[F + nu];
+            z[nu] += r[2 * F + nu];
+          }
+          a += gs;
+          b += gse;
+        }
       }
       GH = a;
       GHe = b;
-      if (LAST) {
-        for (int nu = 0; nu < g.nfreq; ++nu) {
-          double x = 0., y = 0., z = 0.;
-          for (int s = s0; s < s1; ++s) {
-            x += (double)((const ACC*)cp.ghnu)[nu * cp.stride + s];
-            y += (double)((const ACC*)cp.ghenu)[nu * cp.stride + s];
-            z += (double)((const ACC*)cp.ngnu)[nu * cp.stride + s];
-          }
-          GHnu[nu] = x;
-          GHenu[nu] = y;
-          NGnu[nu] = z;
-        }
+      for (int nu = 0; nu < F; ++nu) {
+        GHnu[nu] = x[nu];
+        GHenu[nu] = y[nu];
+        NGnu[nu] = z[nu];
       }
     } else {
       // compensated float sums in ray order
       float a = 0.f, ca = 0.f, b = 0.f, cb = 0.f;
+      float x[FRT_MAX_FREQ], cx[FRT_MAX_FREQ], y[FRT_MAX_FREQ], cy[FRT_MAX_FREQ], z[FRT_MAX_FREQ];
+      for (int nu = 0; nu < F; ++nu) x[nu] = cx[nu] = y[nu] = cy[nu] = z[nu] = 0.f;
       for (int s = s0; s < s1; ++s) {
-        float ya = (float)cgh[s] - ca;
+        const int64_t c = cp.perm[s];
+        float va, vb;
+        if (!LAST) {
+          const float2 v = ((const float2*)cp.pair)[c];
+          va = v.x;
+          vb = v.y;
+        } else {
+          const float* r = (const float*)cp.rec + c * cp.rec_len;
+          va = 0.f;
+          vb = 0.f;
+          for (int nu = 0; nu < F; ++nu) {
+            va += r[nu];
+            vb += r[F + nu];
+            float yx = r[nu] - cx[nu];
+            float tx = x[nu] + yx;
+            cx[nu] = (tx - x[nu]) - yx;
+            x[nu] = tx;
+            float yy = r[F + nu] - cy[nu];
+            float ty = y[nu] + yy;
+            cy[nu] = (ty - y[nu]) - yy;
+            y[nu] = ty;
+            z[nu] += r[2 * F + nu];
+          }
+        }
+        float ya = va - ca;
         float ta = a + ya;
         ca = (ta - a) - ya;
         a = ta;
-        float yb = (float)cghe[s] - cb;
+        float yb = vb - cb;
         float tb = b + yb;
         cb = (tb - b) - yb;
         b = tb;
       }
       GH = (double)a;
       GHe = (double)b;
-      if (LAST) {
-        for (int nu = 0; nu < g.nfreq; ++nu) {
-          float x = 0.f, cx = 0.f, y = 0.f, cy = 0.f, z = 0.f;
-          for (int s = s0; s < s1; ++s) {
-            float yx = (float)((const ACC*)cp.ghnu)[nu * cp.stride + s] - cx;
-            float tx = x + yx;
-            cx = (tx - x) - yx;
-            x = tx;
-            float yy = (float)((const ACC*)cp.ghenu)[nu * cp.stride + s] - cy;
-            float ty = y + yy;
-            cy = (ty - y) - yy;
-            y = ty;
-            z += (float)((const ACC*)cp.ngnu)[nu * cp.stride + s];
-          }
-          GHnu[nu] = (double)x;
-          GHenu[nu] = (double)y;
-          NGnu[nu] = (double)z;
-        }
+      for (int nu = 0; nu < F; ++nu) {
+        GHnu[nu] = (double)x[nu];
+        GHenu[nu] = (double)y[nu];
+        NGnu[nu] = (double)z[nu];
       }
     }
 
@@ -755,12 +799,7 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       }
     }
   }
-  if (LAST && THERMAL) {
-    // per-wave sum of sub-step counts
-    unsigned long long v = (unsigned long long)nsub;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt->substeps, v);
-  }
+  if (LAST && THERMAL) block_add(&cnt->substeps, (unsigned long long)nsub);
 }
 
 // Standalone update_thermal (used when the phases are called one by one).
@@ -799,30 +838,42 @@ __global__ void __launch_bounds__(BLK) k_thermal(CellPtrs cf, GridConst g, DevCo
       cf.f[C_INC_FLUX][k] = 0.f;
     }
   }
-  unsigned long long v = (unsigned long long)nsub;
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt->substeps, v);
+  block_add(&cnt->substeps, (unsigned long long)nsub);
 }
 
 // ---------------------------------------------------------------------------
 // Merging (ray_tracing.cc:176-227 calc_nphot_merge, :324-477 merge).
 __global__ void __launch_bounds__(BLK) k_count_alive(const uint8_t* __restrict__ alive, int64_t n,
                                                      DevCounters* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  wave_count(&cnt->alive, (i < n) && alive[i]);
+  unsigned long long c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK)
+    c += alive[i] ? 1 : 0;
+  block_add(&cnt->alive, c);
 }
+
+// Photon-count histogram of calc_nphot_merge: privatized per block in LDS,
+// non-empty bins flushed with one global atomic each (integer sums: the
+// result does not depend on order).
+static constexpr unsigned PDF_GRID = 256;
 
 __global__ void __launch_bounds__(BLK) k_pdf(RaySoA rb, int64_t nrays, GridConst g,
                                              const uint8_t* __restrict__ alive, double lg_min,
                                              double del, int32_t* __restrict__ pdf) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= nrays || !alive[i]) return;
-  const float s = sum_nphot(rb.nphot, g.cap, g.nfreq, i);
-  // (int) maxd((log10(sum) - lg_min)/del, 0.): maxd narrows to float
-  int ind = (int)frt_maxd((float)((dm_log10((double)s) - lg_min) / del), 0.f);
-  if (ind > FRT_PDF_BINS - 1) ind = FRT_PDF_BINS - 1;
-  if (ind < 0) ind = 0;  // only reachable for a NaN photon count
-  atomicAdd(&pdf[ind], 1);
+  __shared__ int32_t h[FRT_PDF_BINS];
+  for (int b = threadIdx.x; b < FRT_PDF_BINS; b += BLK) h[b] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+    if (!alive[i]) continue;
+    const float s = sum_nphot(rb.nphot, g.cap, g.nfreq, i);
+    // (int) maxd((log10(sum) - lg_min)/del, 0.): maxd narrows to float
+    int ind = (int)frt_maxd((float)((dm_log10((double)s) - lg_min) / del), 0.f);
+    if (ind > FRT_PDF_BINS - 1) ind = FRT_PDF_BINS - 1;
+    if (ind < 0) ind = 0;  // only reachable for a NaN photon count
+    atomicAdd(&h[ind], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < FRT_PDF_BINS; b += BLK)
+    if (h[b]) atomicAdd(&pdf[b], h[b]);
 }
 
 struct Mat9 {
@@ -957,10 +1008,8 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, 
     rb.z[b] = Z;
     rb.time[b] = T;
   }
-  wave_count(&cnt->groups, head);
-  unsigned long long v = (unsigned long long)merged;
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&cnt->merged_away, v);
+  block_add(&cnt->groups, head ? 1ull : 0ull);
+  block_add(&cnt->merged_away, (unsigned long long)merged);
 }
 
 // ---------------------------------------------------------------------------
@@ -969,61 +1018,38 @@ __global__ void __launch_bounds__(BLK) k_split_flag(RaySoA rb, int64_t nrays, Gr
                                                     const uint8_t* __restrict__ alive,
                                                     float split_thresh, int32_t* __restrict__ n_out,
                                                     DevCounters* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  bool split = false;
-  if (i < nrays) {
+  unsigned long long nsplit = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i <= nrays; i += (int64_t)gridDim.x * BLK) {
     int out = 0;
-    if (alive[i]) {
+    if (i < nrays && alive[i]) {
       const int l = rb.order[i];
       const float r = rb.r[i];
-      split = (r > g.rmax_tab[l]) && (l < g.l_max) &&
-              (sum_nphot(rb.nphot, g.cap, g.nfreq, i) > split_thresh);
+      const bool split = (r > g.rmax_tab[l]) && (l < g.l_max) &&
+                         (sum_nphot(rb.nphot, g.cap, g.nfreq, i) > split_thresh);
       out = split ? 4 : 1;
+      nsplit += split ? 1 : 0;
     }
     n_out[i] = out;
-  } else if (i == nrays) {
-    n_out[i] = 0;
   }
-  wave_count(&cnt->splits, split);
+  block_add(&cnt->splits, nsplit);
 }
 
-// Compaction into the other buffer; split parents become 4 children
-// (split_ray, ray_tracing.cc:482-619).
-__global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, int64_t nrays, GridConst g,
-                                                 const int32_t* __restrict__ n_out,
-                                                 const int32_t* __restrict__ out_off,
-                                                 const float* __restrict__ rot_fwd) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= nrays) return;
-  const int no = n_out[i];
-  if (no == 0) return;
-  const int64_t o = out_off[i];
-  const int F = g.nfreq;
-  const int64_t cap = g.cap;
-  if (no == 1) {
-    dst.cell[o] = src.cell[i];
-    dst.pixel[o] = src.pixel[i];
-    dst.order[o] = src.order[i];
-    dst.itheta[o] = src.itheta[i];
-    dst.xhat[o] = src.xhat[i];
-    dst.yhat[o] = src.yhat[i];
-    dst.zhat[o] = src.zhat[i];
-    for (int nu = 0; nu < F; ++nu) dst.nphot[nu * cap + o] = src.nphot[nu * cap + i];
-    dst.r[o] = src.r[i];
-    dst.dr[o] = src.dr[i];
-    dst.x[o] = src.x[i];
-    dst.y[o] = src.y[i];
-    dst.z[o] = src.z[i];
-    dst.time[o] = src.time[i];
-    return;
-  }
+// Geometry of child ch (0..3) of split parent i (split_ray, ray_tracing.cc:482-619):
+// pixel 4p+ch at order l+1, direction rotated like the parent, position moved by
+// (v_child - v_parent) r and walked into the containing periodic cell.
+struct ChildGeo {
+  int cell;
+  int pix;
+  float vx, vy, vz;
+  float x, y, z;
+};
+
+__device__ __forceinline__ ChildGeo child_geometry(const RaySoA& src, int64_t i, int ch, const GridConst& g,
+                                                   const float* __restrict__ rot_fwd) {
   const int l = src.order[i];
   const int pn = src.pixel[i];
   const float r0 = src.r[i];
-  const float x0o = src.x[i], y0o = src.y[i], z0o = src.z[i];
-  const float time = src.time[i];
-  const uint16_t it = src.itheta[i];
-  const float* m = rot_fwd + 9 * (int64_t)it;
+  const float* m = rot_fwd + 9 * (int64_t)src.itheta[i];
   const double vox = (double)src.xhat[i], voy = (double)src.yhat[i], voz = (double)src.zhat[i];
   const int n = g.n;
   const int ind = src.cell[i];
@@ -1032,40 +1058,102 @@ __global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, int64_t
   const int jc = rem / n;
   const int kc = rem - jc * n;
   const float rsf = r0 * g.sf;
-  const float rmax = g.rmax_tab[l + 1];
-  for (int ch = 0; ch < 4; ++ch) {
-    const int pix = 4 * pn + ch;
-    double px, py, pz;
-    hpx_pix2vec_nest(l + 1, pix, &px, &py, &pz);
-    float vx, vy, vz;
-    rotate3(m, (float)px, (float)py, (float)pz, &vx, &vy, &vz);
-    float xn = (float)((double)x0o + ((double)vx - vox) * (double)rsf);
-    float yn = (float)((double)y0o + ((double)vy - voy) * (double)rsf);
-    float zn = (float)((double)z0o + ((double)vz - voz) * (double)rsf);
-    int icn = ic, jcn = jc, kcn = kc;
-    // periodic cell walk (ray_tracing.cc:549-572); the guard only bounds non-finite input
-    int w = 0;
-    while (xn > g.dx && w < 4096) { xn -= g.dx; icn = wrap_mod(icn + 1, n); ++w; }
-    while ((double)xn < 0. && w < 4096) { xn += g.dx; icn = wrap_mod(icn - 1, n); ++w; }
-    while (yn > g.dx && w < 4096) { yn -= g.dx; jcn = wrap_mod(jcn + 1, n); ++w; }
-    while ((double)yn < 0. && w < 4096) { yn += g.dx; jcn = wrap_mod(jcn - 1, n); ++w; }
-    while (zn > g.dx && w < 4096) { zn -= g.dx; kcn = wrap_mod(kcn + 1, n); ++w; }
-    while ((double)zn < 0. && w < 4096) { zn += g.dx; kcn = wrap_mod(kcn - 1, n); ++w; }
-    const int64_t q = o + ch;
-    dst.cell[q] = icn * n * n + jcn * n + kcn;
-    dst.pixel[q] = pix;
+  ChildGeo c;
+  c.pix = 4 * pn + ch;
+  double px, py, pz;
+  hpx_pix2vec_nest(l + 1, c.pix, &px, &py, &pz);
+  rotate3(m, (float)px, (float)py, (float)pz, &c.vx, &c.vy, &c.vz);
+  float xn = (float)((double)src.x[i] + ((double)c.vx - vox) * (double)rsf);
+  float yn = (float)((double)src.y[i] + ((double)c.vy - voy) * (double)rsf);
+  float zn = (float)((double)src.z[i] + ((double)c.vz - voz) * (double)rsf);
+  int icn = ic, jcn = jc, kcn = kc;
+  // periodic cell walk (ray_tracing.cc:549-572); the guard only bounds non-finite input
+  int w = 0;
+  while (xn > g.dx && w < 4096) { xn -= g.dx; icn = wrap_mod(icn + 1, n); ++w; }
+  while ((double)xn < 0. && w < 4096) { xn += g.dx; icn = wrap_mod(icn - 1, n); ++w; }
+  while (yn > g.dx && w < 4096) { yn -= g.dx; jcn = wrap_mod(jcn + 1, n); ++w; }
+  while ((double)yn < 0. && w < 4096) { yn += g.dx; jcn = wrap_mod(jcn - 1, n); ++w; }
+  while (zn > g.dx && w < 4096) { zn -= g.dx; kcn = wrap_mod(kcn + 1, n); ++w; }
+  while ((double)zn < 0. && w < 4096) { zn += g.dx; kcn = wrap_mod(kcn - 1, n); ++w; }
+  c.cell = icn * n * n + jcn * n + kcn;
+  c.x = xn;
+  c.y = yn;
+  c.z = zn;
+  return c;
+}
+
+// Cell of every output ray of the compaction (key of the stable cell sort);
+// values are the compacted positions, src records (input ray << 2 | child).
+__global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, int64_t nrays, GridConst g,
+                                                  const int32_t* __restrict__ n_out,
+                                                  const int32_t* __restrict__ out_off,
+                                                  const float* __restrict__ rot_fwd,
+                                                  uint32_t* __restrict__ key, int32_t* __restrict__ val,
+                                                  uint32_t* __restrict__ src_of) {
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+    const int no = n_out[i];
+    const int o = out_off[i];
+    if (no == 1) {
+      key[o] = (uint32_t)src.cell[i];
+      val[o] = o;
+      src_of[o] = (uint32_t)i << 2;
+    } else if (no == 4) {
+      for (int ch = 0; ch < 4; ++ch) {
+        key[o + ch] = (uint32_t)child_geometry(src, i, ch, g, rot_fwd).cell;
+        val[o + ch] = o + ch;
+        src_of[o + ch] = ((uint32_t)i << 2) | (uint32_t)ch;
+      }
+    }
+  }
+}
+
+// Compaction into the other buffer in stable cell order, as a gather: output q
+// takes the ray (or split child) that sorted to position q.
+__global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, int64_t total, GridConst g,
+                                                 const int32_t* __restrict__ n_out,
+                                                 const int32_t* __restrict__ sorted_pos,
+                                                 const uint32_t* __restrict__ src_of,
+                                                 const float* __restrict__ rot_fwd) {
+  const int F = g.nfreq;
+  const int64_t cap = g.cap;
+  for (int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x; q < total; q += (int64_t)gridDim.x * BLK) {
+    const uint32_t packed = src_of[sorted_pos[q]];
+    const int64_t i = (int64_t)(packed >> 2);
+    const int ch = (int)(packed & 3u);
+    if (n_out[i] == 1) {
+      dst.cell[q] = src.cell[i];
+      dst.pixel[q] = src.pixel[i];
+      dst.order[q] = src.order[i];
+      dst.itheta[q] = src.itheta[i];
+      dst.xhat[q] = src.xhat[i];
+      dst.yhat[q] = src.yhat[i];
+      dst.zhat[q] = src.zhat[i];
+      for (int nu = 0; nu < F; ++nu) dst.nphot[nu * cap + q] = src.nphot[nu * cap + i];
+      dst.r[q] = src.r[i];
+      dst.dr[q] = src.dr[i];
+      dst.x[q] = src.x[i];
+      dst.y[q] = src.y[i];
+      dst.z[q] = src.z[i];
+      dst.time[q] = src.time[i];
+      continue;
+    }
+    const int l = src.order[i];
+    const float r0 = src.r[i];
+    const ChildGeo c = child_geometry(src, i, ch, g, rot_fwd);
+    dst.cell[q] = c.cell;
+    dst.pixel[q] = c.pix;
     dst.order[q] = (int16_t)(l + 1);
-    dst.itheta[q] = it;
-    dst.xhat[q] = vx;
-    dst.yhat[q] = vy;
-    dst.zhat[q] = vz;
+    dst.itheta[q] = src.itheta[i];
+    dst.xhat[q] = c.vx;
+    dst.yhat[q] = c.vy;
+    dst.zhat[q] = c.vz;
     for (int nu = 0; nu < F; ++nu) dst.nphot[nu * cap + q] = (float)((double)src.nphot[nu * cap + i] / 4.);
     dst.r[q] = r0;
-    dst.dr[q] = rmax - r0;
-    dst.x[q] = xn;
-    dst.y[q] = yn;
-    dst.z[q] = zn;
-    dst.time[q] = time;
+    dst.dr[q] = g.rmax_tab[l + 1] - r0;
+    dst.x[q] = c.x;
+    dst.y[q] = c.y;
+    dst.z[q] = c.z;
+    dst.time[q] = src.time[i];
   }
 }
 
@@ -1148,7 +1236,7 @@ int launch_add_ray_dt(frt_ctx* ctx) {
 }
 
 int launch_march(frt_ctx* ctx) {
-  FRT_LAUNCH(ctx, "march", grid_for(ctx->nrays), k_march, cur_buf(ctx), ctx->nrays, ctx->g,
+  FRT_LAUNCH(ctx, "march", sgrid(ctx->nrays), k_march, cur_buf(ctx), ctx->nrays, ctx->g,
              ctx->path_cell, ctx->path_dist, ctx->seg_cnt, ctx->alive, ctx->dcnt);
   return FRT_OK;
 }
@@ -1201,19 +1289,19 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
     ctx->seg_key[b] = nullptr;
     ctx->seg_val[b] = nullptr;
   }
-  hipFree(ctx->inv_perm);
-  hipFree(ctx->contrib.gh);
-  ctx->inv_perm = nullptr;
-  ctx->contrib.gh = nullptr;
+  hipFree(ctx->contrib.pair);
+  ctx->contrib.pair = nullptr;
+  ctx->contrib.rec = nullptr;
   const int F = ctx->g.nfreq;
   const size_t acc = (ctx->cfg.precision == FRT_PREC_MIXED) ? 4 : 8;
-  const size_t per = acc * (size_t)(2 + 3 * F);
+  // last-pass record: 3F values rounded up to 16 or 32 (whole cache lines)
+  const int rec_len = (3 * F <= 16) ? 16 : 32;
+  const size_t per = acc * (size_t)(2 + rec_len);
   hipError_t e = hipSuccess;
   for (int b = 0; b < 2 && e == hipSuccess; ++b) {
     e = hipMalloc((void**)&ctx->seg_key[b], (size_t)cap * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&ctx->seg_val[b], (size_t)cap * 4);
   }
-  if (e == hipSuccess) e = hipMalloc((void**)&ctx->inv_perm, (size_t)cap * 4);
   void* blk = nullptr;
   if (e == hipSuccess) e = hipMalloc(&blk, per * (size_t)cap);
   if (e != hipSuccess) {
@@ -1221,12 +1309,10 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
     return set_error(ctx, FRT_E_NOMEM, "segment buffers: " + std::string(hipGetErrorString(e)));
   }
   char* c = (char*)blk;
-  ctx->contrib.gh = c;
-  ctx->contrib.ghe = c + acc * cap;
-  ctx->contrib.ghnu = c + acc * 2 * cap;
-  ctx->contrib.ghenu = c + acc * (2 + F) * cap;
-  ctx->contrib.ngnu = c + acc * (2 + 2 * F) * cap;
-  ctx->contrib.stride = cap;
+  ctx->contrib.rec = c;                        // cap x rec_len values, 64/128-byte records
+  ctx->contrib.pair = c + acc * rec_len * cap;  // cap x 2 values
+  ctx->contrib.rec_len = rec_len;
+  ctx->contrib.perm = ctx->seg_val[1];
   ctx->seg_cap = cap;
   return FRT_OK;
 }
@@ -1254,7 +1340,6 @@ int launch_segments(frt_ctx* ctx) {
   if (rc) return rc;
   FRT_LAUNCH(ctx, "cell_ranges", grid_for(nseg), k_cell_ranges, ctx->seg_key[1], nseg,
              ctx->cell_start, ctx->cell_end);
-  FRT_LAUNCH(ctx, "inv_perm", grid_for(nseg), k_inv_perm, ctx->seg_val[1], nseg, ctx->inv_perm);
   return FRT_OK;
 }
 
@@ -1274,26 +1359,26 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     const float* aHe = first ? ctx->c.f[C_NHE1] : ctx->c.f[C_AVG_NHE1];
     if (mixed) {
       if (last)
-        FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+        FRT_LAUNCH(ctx, "deposit_mixed", sgrid(n), k_deposit_mixed<true>, rb, n, g, ctx->path_cell,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
-        FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+        FRT_LAUNCH(ctx, "deposit_mixed", sgrid(n), k_deposit_mixed<false>, rb, n, g, ctx->path_cell,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     } else {
       if (last)
-        FRT_LAUNCH(ctx, "deposit_fp64", grid_for(n), k_deposit_fp64<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+        FRT_LAUNCH(ctx, "deposit_fp64", sgrid(n), k_deposit_fp64<true>, rb, n, g, ctx->path_cell,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
-        FRT_LAUNCH(ctx, "deposit_fp64", grid_for(n), k_deposit_fp64<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+        FRT_LAUNCH(ctx, "deposit_fp64", sgrid(n), k_deposit_fp64<false>, rb, n, g, ctx->path_cell,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     }
     // algorithmic bytes of this deposit launch (model used for the roofline):
     // per ray: seg_cnt + seg_off (8) + nphot read (4F); per segment: path cell +
-    // dist (8) + inv_perm (4) + avg_nh1/avg_nhe1 (8) + 2 deposits; last pass adds
+    // dist (8) + avg_nh1/avg_nhe1 (8) + 2 deposits; last pass adds
     // 3F per-bin deposits per segment and the ray write-back (4F + 4 + 1).
     {
       const double acc = mixed ? 4. : 8.;
@@ -1344,7 +1429,7 @@ int launch_merge(frt_ctx* ctx) {
   RaySoA rb = cur_buf(ctx);
   int rc;
   // count_rays
-  FRT_LAUNCH(ctx, "count_alive", grid_for(n), k_count_alive, ctx->alive, n, ctx->dcnt);
+  FRT_LAUNCH(ctx, "count_alive", sgrid(n), k_count_alive, ctx->alive, n, ctx->dcnt);
   rc = hip_check(ctx, hipMemcpyAsync(ctx->hcnt, ctx->dcnt, sizeof(DevCounters), hipMemcpyDeviceToHost, ctx->stream), "counters");
   if (rc) return rc;
   rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
@@ -1360,7 +1445,7 @@ int launch_merge(frt_ctx* ctx) {
     const double lg_min = dm_log10(ctx->nphot_min);
     const double del = (lg_max - lg_min) / (10000 - 1);
     FRT_LAUNCH(ctx, "fill", grid_for(FRT_PDF_BINS), k_fill_i32, ctx->pdf, (int64_t)FRT_PDF_BINS, 0);
-    FRT_LAUNCH(ctx, "pdf", grid_for(n), k_pdf, rb, n, g, ctx->alive, lg_min, del, ctx->pdf);
+    FRT_LAUNCH(ctx, "pdf", std::min(sgrid(n), PDF_GRID), k_pdf, rb, n, g, ctx->alive, lg_min, del, ctx->pdf);
     std::vector<int32_t> pdf(FRT_PDF_BINS);
     rc = hip_check(ctx, hipMemcpyAsync(pdf.data(), ctx->pdf, 4 * FRT_PDF_BINS, hipMemcpyDeviceToHost, ctx->stream), "pdf");
     if (rc) return rc;
@@ -1418,7 +1503,7 @@ int launch_split_compact(frt_ctx* ctx, int64_t* n_after) {
   GridConst g = ctx->g;
   // maxd(4.*nphot_merge, 4.*nphot_min): both narrowed to float
   const float split_thresh = frt_maxd((float)(4. * (double)ctx->nphot_merge), (float)(4. * ctx->nphot_min));
-  FRT_LAUNCH(ctx, "split_flag", grid_for(n + 1), k_split_flag, cur_buf(ctx), n, g, ctx->alive,
+  FRT_LAUNCH(ctx, "split_flag", sgrid(n + 1), k_split_flag, cur_buf(ctx), n, g, ctx->alive,
              split_thresh, ctx->n_out, ctx->dcnt);
   int rc = exclusive_scan(ctx, ctx->n_out, ctx->out_off, n + 1, "scan_split");
   if (rc) return rc;
@@ -1429,8 +1514,16 @@ int launch_split_compact(frt_ctx* ctx, int64_t* n_after) {
     return set_error(ctx, FRT_E_CAPACITY,
                      "ray buffer full: " + std::to_string(total + (int64_t)ctx->nsrc * 12) +
                          " rays needed, capacity " + std::to_string(ctx->g.cap));
-  FRT_LAUNCH(ctx, "compact", grid_for(n), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], n, g,
-             ctx->n_out, ctx->out_off, ctx->rot_fwd);
+  if (total > 0) {
+    // stable sort of the output rays by cell: the list order of every step
+    FRT_LAUNCH(ctx, "out_keys", sgrid(n), k_out_keys, cur_buf(ctx), n, g, ctx->n_out, ctx->out_off,
+               ctx->rot_fwd, ctx->m_key[0], ctx->m_val[0], ctx->cand_key_tmp);
+    rc = sort_pairs(ctx, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1], total, ctx->key_bits,
+                    "sort_rays");
+    if (rc) return rc;
+    FRT_LAUNCH(ctx, "compact", sgrid(total), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], total, g,
+               ctx->n_out, ctx->m_val[1], ctx->cand_key_tmp, ctx->rot_fwd);
+  }
   ctx->cur = 1 - ctx->cur;
   ctx->nrays = total;
   *n_after = total;

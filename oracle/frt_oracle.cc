@@ -46,6 +46,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <vector>
 
@@ -170,6 +171,7 @@ struct State {
   int64_t step;
   uint64_t cast_calls;
   bool have_paths;
+  bool dirty;  // rays may have been removed since the last compaction
   frt_step_stats st;
   bool path_overflow;
 };
@@ -436,6 +438,7 @@ static void update_rays(State& s) {
   s.st.rays_moved = moved;
   s.st.segments = segs;
   s.have_paths = true;
+  s.dirty = true;
 }
 
 // ---- C2Ray cell update (ray_tracing.cc:954-1136) ----------------------------------------------
@@ -631,6 +634,7 @@ static void find_avg_unu(State& s) {
   s.st.rays_dumped = dumped;
   s.st.rays_terminated = terminated;
   s.have_paths = false;
+  s.dirty = true;
 }
 
 // ---- merging -------------------------------------------------------------------------------------
@@ -841,8 +845,11 @@ static void split_rays(State& s) {
     out.push_back(ray);
   }
   for (auto& ray : out) ray.active = true;  // set_to_true
+  // list order of the next step: stable by current cell
+  std::stable_sort(out.begin(), out.end(), [](const Ray& a, const Ray& b) { return a.path_ind[0] < b.path_ind[0]; });
   s.ray.swap(out);
   s.st.rays_split = splits;
+  s.dirty = false;
 }
 
 // ---- update_thermal (gas_funcs.cc:104-242, 353-419) ------------------------------------------------
@@ -972,12 +979,16 @@ static void new_rays(State& s) {
   s.st.rays_new = (int64_t)s.ray.size() - before;
 }
 
+// Compaction outside split_rays: drop removed rays and restore the cell order.
 static void drop_dead(State& s) {
+  if (!s.dirty) return;
   std::vector<Ray> out;
   out.reserve(s.ray.size());
   for (auto& r : s.ray)
     if (r.tag) out.push_back(r);
+  std::stable_sort(out.begin(), out.end(), [](const Ray& a, const Ray& b) { return a.path_ind[0] < b.path_ind[0]; });
   s.ray.swap(out);
+  s.dirty = false;
 }
 
 static void get_nphot_min_max(State& s) {
@@ -1056,6 +1067,7 @@ void* orc_create(const frt_config* cfg) {
   s->step = 0;
   s->cast_calls = 0;
   s->have_paths = false;
+  s->dirty = false;
   s->path_overflow = false;
   std::memset(&s->st, 0, sizeof(s->st));
   const size_t n3 = (size_t)s->N3, nf = n3 * s->Nfreq;
@@ -1112,6 +1124,7 @@ float orc_get_time(void* h) { return ((State*)h)->t; }
 void orc_init_rt(void* h) {
   State* s = (State*)h;
   s->ray.clear();
+  s->dirty = false;
   s->t = 0.f;
   s->step = 0;
   s->cast_calls = 0;
@@ -1164,6 +1177,7 @@ int64_t orc_ray_count(void* h) {
 
 int orc_download_rays(void* h, frt_ray_view* v) {
   State* s = (State*)h;
+  drop_dead(*s);
   int64_t i = 0;
   const int F = s->Nfreq;
   for (auto& r : s->ray) {
@@ -1192,6 +1206,7 @@ int orc_download_rays(void* h, frt_ray_view* v) {
 int orc_upload_rays(void* h, const frt_ray_view* v) {
   State* s = (State*)h;
   s->ray.clear();
+  s->dirty = false;
   const int F = s->Nfreq;
   for (int64_t i = 0; i < v->count; i++) {
     Ray r;
