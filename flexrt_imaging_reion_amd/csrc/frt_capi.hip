@@ -17,8 +17,26 @@ int hip_check(frt_ctx* ctx, hipError_t e, const char* what) {
   return set_error(ctx, FRT_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Host <-> device copies run on the library's stream (created non-blocking,
+// so it is not ordered with the null stream) and wait for it: a copy is then
+// ordered after the kernels queued before it and before those queued after.
+hipError_t stream_copy(frt_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (bytes == 0) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e;
+}
+
+hipError_t stream_zero(frt_ctx* ctx, void* dst, size_t bytes) {
+  if (bytes == 0) return hipSuccess;
+  hipError_t e = hipMemsetAsync(dst, 0, bytes, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e;
+}
+
 int ensure_cub_tmp(frt_ctx* ctx, size_t bytes) {
   if (bytes <= ctx->cub_tmp_bytes) return FRT_OK;
+  hipStreamSynchronize(ctx->stream);  // earlier scans/sorts may still use the workspace
   hipFree(ctx->cub_tmp);
   ctx->cub_tmp = nullptr;
   size_t want = std::max(bytes, ctx->cub_tmp_bytes * 2);
@@ -390,12 +408,12 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     // clumping factor defaults to const_clump = 1 (user_inputs.h:87, set_clump subgrid.cc:799)
     if (e == hipSuccess) {
       std::vector<float> ones((size_t)n3, 1.0f);
-      e = hipMemcpy(ctx->c.f[C_CLUMP], ones.data(), sizeof(float) * n3, hipMemcpyHostToDevice);
+      e = stream_copy(ctx, ctx->c.f[C_CLUMP], ones.data(), sizeof(float) * n3, hipMemcpyHostToDevice);
     }
     std::vector<float> fwd(9 * FRT_NTHETA), inv(9 * FRT_NTHETA);
     build_rotation_tables(fwd.data(), inv.data());
-    if (e == hipSuccess) e = hipMemcpy(ctx->rot_fwd, fwd.data(), 4 * fwd.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(ctx->rot_inv, inv.data(), 4 * inv.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = stream_copy(ctx, ctx->rot_fwd, fwd.data(), 4 * fwd.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = stream_copy(ctx, ctx->rot_inv, inv.data(), 4 * inv.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemsetAsync(ctx->dcnt, 0, sizeof(DevCounters), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = set_error(ctx, FRT_E_HIP, std::string("init: ") + hipGetErrorString(e));
@@ -420,7 +438,7 @@ int frt_destroy(frt_ctx* ctx) {
   for (int b = 0; b < 2; ++b) {
     hipFree(ctx->seg_key[b]); hipFree(ctx->seg_val[b]); hipFree(ctx->m_key[b]); hipFree(ctx->m_val[b]);
   }
-  hipFree(ctx->contrib.pair); hipFree(ctx->pdf);
+  hipFree(ctx->contrib.rec); hipFree(ctx->inv_perm); hipFree(ctx->pdf);  // rec is the block base
   hipFree(ctx->cell_start); hipFree(ctx->cell_end); hipFree(ctx->cell_block);
   hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
   hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt);
@@ -454,6 +472,7 @@ int frt_set_sources(frt_ctx* ctx, int32_t nsrc, const int32_t* cell, const doubl
   if (!ctx || nsrc < 0 || (nsrc > 0 && (!cell || !lum))) return FRT_E_ARG;
   for (int s = 0; s < nsrc; ++s)
     if (cell[s] < 0 || cell[s] >= ctx->g.n3) return set_error(ctx, FRT_E_ARG, "source cell out of range");
+  hipStreamSynchronize(ctx->stream);  // the stream may still read the old arrays
   hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
   ctx->src_cell = nullptr; ctx->src_lum = nullptr; ctx->src_ray_dt = nullptr;
   ctx->nsrc = nsrc;
@@ -462,9 +481,9 @@ int frt_set_sources(frt_ctx* ctx, int32_t nsrc, const int32_t* cell, const doubl
   hipError_t e = dalloc(&ctx->src_cell, nsrc);
   if (e == hipSuccess) e = dalloc(&ctx->src_lum, nsrc);
   if (e == hipSuccess) e = dalloc(&ctx->src_ray_dt, nsrc);
-  if (e == hipSuccess && nsrc > 0) e = hipMemcpy(ctx->src_cell, cell, 4 * (size_t)nsrc, hipMemcpyHostToDevice);
-  if (e == hipSuccess && nsrc > 0) e = hipMemcpy(ctx->src_lum, lum, 8 * (size_t)nsrc, hipMemcpyHostToDevice);
-  if (e == hipSuccess && nsrc > 0) e = hipMemset(ctx->src_ray_dt, 0, 4 * (size_t)nsrc);
+  if (e == hipSuccess && nsrc > 0) e = stream_copy(ctx, ctx->src_cell, cell, 4 * (size_t)nsrc, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nsrc > 0) e = stream_copy(ctx, ctx->src_lum, lum, 8 * (size_t)nsrc, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nsrc > 0) e = stream_zero(ctx, ctx->src_ray_dt, 4 * (size_t)nsrc);
   return hip_check(ctx, e, "sources");
 }
 
@@ -475,13 +494,13 @@ int frt_upload_field(frt_ctx* ctx, int32_t field, const float* host) {
   const size_t n3 = (size_t)ctx->g.n3;
   hipError_t e;
   if (field < C_NSCALAR) {
-    e = hipMemcpy(d, host, 4 * n3, hipMemcpyHostToDevice);
+    e = stream_copy(ctx, d, host, 4 * n3, hipMemcpyHostToDevice);
   } else {
     const int F = ctx->g.nfreq;
     std::vector<float> t((size_t)F * n3);
     for (size_t k = 0; k < n3; ++k)
       for (int nu = 0; nu < F; ++nu) t[(size_t)nu * n3 + k] = host[k * F + nu];
-    e = hipMemcpy(d, t.data(), 4 * t.size(), hipMemcpyHostToDevice);
+    e = stream_copy(ctx, d, t.data(), 4 * t.size(), hipMemcpyHostToDevice);
   }
   return hip_check(ctx, e, "upload field");
 }
@@ -495,11 +514,11 @@ int frt_download_field(frt_ctx* ctx, int32_t field, float* host) {
   if (rc) return rc;
   hipError_t e;
   if (field < C_NSCALAR) {
-    e = hipMemcpy(host, d, 4 * n3, hipMemcpyDeviceToHost);
+    e = stream_copy(ctx, host, d, 4 * n3, hipMemcpyDeviceToHost);
   } else {
     const int F = ctx->g.nfreq;
     std::vector<float> t((size_t)F * n3);
-    e = hipMemcpy(t.data(), d, 4 * t.size(), hipMemcpyDeviceToHost);
+    e = stream_copy(ctx, t.data(), d, 4 * t.size(), hipMemcpyDeviceToHost);
     for (size_t k = 0; k < n3; ++k)
       for (int nu = 0; nu < F; ++nu) host[k * F + nu] = t[(size_t)nu * n3 + k];
   }
@@ -532,7 +551,7 @@ int frt_init_rt(frt_ctx* ctx) {
   ctx->have_segments = false;
   if (ctx->nsrc > 0) {
     std::vector<float> rdt((size_t)ctx->nsrc, ctx->g.dt);
-    hipError_t e = hipMemcpy(ctx->src_ray_dt, rdt.data(), 4 * (size_t)ctx->nsrc, hipMemcpyHostToDevice);
+    hipError_t e = stream_copy(ctx, ctx->src_ray_dt, rdt.data(), 4 * (size_t)ctx->nsrc, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_check(ctx, e, "init_rt");
   }
   return FRT_OK;
@@ -652,7 +671,7 @@ int frt_download_rays(frt_ctx* ctx, frt_ray_view* v) {
   if (n == 0) return FRT_OK;
   RaySoA& b = ctx->buf[ctx->cur];
   hipError_t e = hipSuccess;
-#define CP(dst, src, sz) if (e == hipSuccess && (dst)) e = hipMemcpy(dst, src, (size_t)(sz) * n, hipMemcpyDeviceToHost)
+#define CP(dst, src, sz) if (e == hipSuccess && (dst)) e = stream_copy(ctx, dst, src, (size_t)(sz) * n, hipMemcpyDeviceToHost)
   CP(v->order, b.order, 2); CP(v->pixel, b.pixel, 4); CP(v->itheta, b.itheta, 2);
   CP(v->xhat, b.xhat, 4); CP(v->yhat, b.yhat, 4); CP(v->zhat, b.zhat, 4);
   CP(v->r, b.r, 4); CP(v->dr, b.dr, 4); CP(v->x, b.x, 4); CP(v->y, b.y, 4); CP(v->z, b.z, 4);
@@ -662,7 +681,7 @@ int frt_download_rays(frt_ctx* ctx, frt_ray_view* v) {
     const int F = ctx->g.nfreq;
     std::vector<float> t((size_t)F * n);
     for (int nu = 0; nu < F && e == hipSuccess; ++nu)
-      e = hipMemcpy(t.data() + (size_t)nu * n, b.nphot + (size_t)nu * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
+      e = stream_copy(ctx, t.data() + (size_t)nu * n, b.nphot + (size_t)nu * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
     for (int64_t i = 0; i < n; ++i)
       for (int nu = 0; nu < F; ++nu) v->nphot[i * F + nu] = t[(size_t)nu * n + i];
   }
@@ -685,7 +704,7 @@ int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* v) {
   ctx->cur = 0;
   RaySoA& b = ctx->buf[0];
   hipError_t e = hipSuccess;
-#define CP(dst, src, sz) if (e == hipSuccess) e = hipMemcpy(dst, src, (size_t)(sz) * n, hipMemcpyHostToDevice)
+#define CP(dst, src, sz) if (e == hipSuccess) e = stream_copy(ctx, dst, src, (size_t)(sz) * n, hipMemcpyHostToDevice)
   if (n > 0) {
     CP(b.order, v->order, 2); CP(b.pixel, v->pixel, 4); CP(b.itheta, v->itheta, 2);
     CP(b.xhat, v->xhat, 4); CP(b.yhat, v->yhat, 4); CP(b.zhat, v->zhat, 4);
@@ -696,7 +715,7 @@ int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* v) {
     for (int64_t i = 0; i < n; ++i)
       for (int nu = 0; nu < F; ++nu) t[(size_t)nu * n + i] = v->nphot[i * F + nu];
     for (int nu = 0; nu < F && e == hipSuccess; ++nu)
-      e = hipMemcpy(b.nphot + (size_t)nu * ctx->g.cap, t.data() + (size_t)nu * n, 4 * (size_t)n, hipMemcpyHostToDevice);
+      e = stream_copy(ctx, b.nphot + (size_t)nu * ctx->g.cap, t.data() + (size_t)nu * n, 4 * (size_t)n, hipMemcpyHostToDevice);
   }
 #undef CP
   ctx->nrays = n;
@@ -714,11 +733,11 @@ int frt_download_paths(frt_ctx* ctx, int64_t n, int32_t* ray_end, int32_t* cells
   const int P = ctx->g.P;
   std::vector<int32_t> c((size_t)P * n);
   std::vector<float> d((size_t)P * n);
-  hipError_t e = hipMemcpy(ray_end, ctx->seg_cnt, 4 * (size_t)n, hipMemcpyDeviceToHost);
+  hipError_t e = stream_copy(ctx, ray_end, ctx->seg_cnt, 4 * (size_t)n, hipMemcpyDeviceToHost);
   for (int p = 0; p < P && e == hipSuccess; ++p) {
-    e = hipMemcpy(c.data() + (size_t)p * n, ctx->path_cell + (size_t)p * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
+    e = stream_copy(ctx, c.data() + (size_t)p * n, ctx->path_cell + (size_t)p * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
     if (e == hipSuccess)
-      e = hipMemcpy(d.data() + (size_t)p * n, ctx->path_dist + (size_t)p * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
+      e = stream_copy(ctx, d.data() + (size_t)p * n, ctx->path_dist + (size_t)p * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
   }
   if (e != hipSuccess) return hip_check(ctx, e, "download paths");
   for (int64_t i = 0; i < n; ++i)
@@ -762,6 +781,7 @@ int frt_deposit_bytes(const frt_ctx* ctx, double* bytes_per_launch, int64_t* lau
 int frt_copy_kernel(frt_ctx* ctx, int64_t nbytes, int32_t repeats) {
   if (!ctx || nbytes <= 0 || repeats < 1) return FRT_E_ARG;
   if (nbytes != ctx->copy_bytes) {
+    hipStreamSynchronize(ctx->stream);
     hipFree(ctx->copy_src);
     hipFree(ctx->copy_dst);
     ctx->copy_src = ctx->copy_dst = nullptr;

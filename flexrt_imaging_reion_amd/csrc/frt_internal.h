@@ -108,12 +108,12 @@ struct CellPtrs {
   float* ngam_nu;       // [nfreq][n3]
 };
 
-// Deposits of path segments, one slot per segment in segment order; the cell
-// kernel reads a cell's slots through perm (cell-sorted position -> slot).
+// Deposits of path segments, one slot per segment in cell-sorted order (a
+// cell's slots are contiguous and in ray order).
 struct ContribPtrs {
   void* pair;          // [slot] {H, He} deposit summed over bins (2 values)
-  void* rec;           // [slot] last pass: gh[F], ghe[F], photons[F], padded to rec_len
-  const int32_t* perm; // cell-sorted position -> segment slot
+  void* rec;           // [slot] last pass: {gh, ghe, photons} per bin, padded to rec_len
+  const int32_t* perm; // cell-sorted position -> segment index
   int rec_len;         // values per record (16 or 32)
 };
 
@@ -146,6 +146,7 @@ struct frt_ctx {
   int64_t seg_cap;
   uint32_t* seg_key[2];
   int32_t* seg_val[2];
+  int32_t* inv_perm;   // segment index -> cell-sorted slot
   ContribPtrs contrib;
   int32_t* cell_start;
   int32_t* cell_end;
@@ -217,6 +218,8 @@ int launch_copy(frt_ctx* ctx, int64_t nbytes, int repeats);
 
 // helpers shared with the C-ABI file
 int ensure_cub_tmp(frt_ctx* ctx, size_t bytes);
+hipError_t stream_copy(frt_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+hipError_t stream_zero(frt_ctx* ctx, void* dst, size_t bytes);
 int set_error(frt_ctx* ctx, int code, const std::string& msg);
 int hip_check(frt_ctx* ctx, hipError_t e, const char* what);
 void prof_begin(frt_ctx* ctx, const char* name, ProfRec* rec);

@@ -255,17 +255,131 @@ __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict_
 // ---------------------------------------------------------------------------
 // find_avg_unu ray loop (ray_tracing.cc:701-938), one thread per moved ray.
 //
-// Each segment's deposit is written to its own slot in segment order
-// (coalesced): a {H, He} pair (bin sums) in the first passes, a full record
-// {gh[F], ghe[F], photons[F]} padded to a whole number of cache lines in the
-// last pass.  The cell kernel gathers a cell's slots through the stable cell
-// sort, i.e. in ray order.
+// Each segment's deposit goes to its cell-sorted slot (so a cell's slots are
+// contiguous and in ray order): a {H, He} pair of bin sums in the first
+// passes, a record padded to whole cache lines in the last pass: the {gh, ghe}
+// pairs of all bins followed by the photons of all bins, stored per bin as the
+// bins are computed.
 
-// Record helpers: write REC values of type T at rec (all of them, so the
-// stores cover whole cache lines).
+// Loop over the frequency bins, unrolled to the compile-time maximum so that
+// per-bin register arrays are indexed statically (no scratch).
+#define FRT_FOR_BINS(nu) \
+  _Pragma("unroll") for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) if (nu < F)
+
+#define FRT_MAX_REC 32  // values per record at most (3 * FRT_MAX_FREQ rounded up)
+static_assert(3 * FRT_MAX_FREQ <= FRT_MAX_REC, "record too short");
+
+// Zero record of a segment behind a dump, written with 16-byte stores.
 template <typename T>
-__device__ __forceinline__ void store_record(T* rec, const T* v, int n, int rec_len) {
-  for (int k = 0; k < rec_len; ++k) rec[k] = (k < n) ? v[k] : (T)0;
+__device__ __forceinline__ void zero_record(T* rec, int rec_len) {
+  for (int k = 0; k < rec_len * (int)sizeof(T); k += 16) *(float4*)((char*)rec + k) = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Software pipeline over a ray's segments: the path entry (cell, length,
+// slot) of segment p+2 and the cell opacities of segment p+1 are requested
+// before segment p is computed, so the dependent loads overlap the math.
+struct SegPipe {
+  int64_t i, stride;
+  int c, base;
+  const int32_t* path_cell;
+  const float* path_dist;
+  const int32_t* inv_perm;
+  const float* avgH;
+  const float* avgHe;
+  // stage registers: segment p (ready), p+1 (path + opacity), p+2 (path)
+  int cell0, cell1, cell2, slot0, slot1, slot2;
+  float dist0, dist1, dist2, ah0, ahe0, ah1, ahe1;
+
+  __device__ __forceinline__ void load_path(int p, int& cell, float& dist, int& slot) const {
+    cell = path_cell[(int64_t)p * stride + i];
+    dist = path_dist[(int64_t)p * stride + i];
+    slot = inv_perm[base + p];
+  }
+  __device__ __forceinline__ void start() {
+    load_path(0, cell0, dist0, slot0);
+    if (c > 1) load_path(1, cell1, dist1, slot1);
+    ah0 = avgH[cell0];
+    ahe0 = avgHe[cell0];
+  }
+  // issue the loads of the next stages before segment p is computed
+  __device__ __forceinline__ void prefetch(int p) {
+    if (p + 2 < c) load_path(p + 2, cell2, dist2, slot2);
+    if (p + 1 < c) {
+      ah1 = avgH[cell1];
+      ahe1 = avgHe[cell1];
+    }
+  }
+  __device__ __forceinline__ void advance() {
+    cell0 = cell1; dist0 = dist1; slot0 = slot1; ah0 = ah1; ahe0 = ahe1;
+    cell1 = cell2; dist1 = dist2; slot1 = slot2;
+  }
+};
+
+// One segment of the FP64 deposit; returns true if the ray is dumped here.
+__device__ __forceinline__ bool deposit_segment_fp64(const GridConst& g, ContribPtrs& cp, bool last, int slot,
+                                                     double dist, float ah1, float ahe1, const double* nph,
+                                                     double* tauh, double* tauhe) {
+  const int F = g.nfreq;
+  const double dtd = (double)g.dt;
+  double dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ], nin[FRT_MAX_FREQ];
+  double tot = 0.;
+#pragma unroll
+  for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
+    if (nu >= F) break;
+    dth[nu] = ((double)ah1 * g.sigH[nu]) * dist;  // avg_nh1*sigma*dist
+    dthe[nu] = ((double)ahe1 * g.sigHe[nu]) * dist;
+    nin[nu] = nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]);
+    tot += nin[nu] / FRT_PHOT_NORM;
+  }
+  const bool dump = tot < g.nphot_min;
+  double gh_sum = 0., ghe_sum = 0.;
+  double* rec = (double*)cp.rec + (int64_t)slot * cp.rec_len;
+#pragma unroll
+  for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
+    if (nu >= F) break;
+    double fh = 0., fhe = 0., att = 0.;
+    const bool any = (dth[nu] > 0.) || (dthe[nu] > 0.);
+    double eh = 1., ehe = 1.;
+    if (any || !dump) {
+      eh = dm_exp(-dth[nu]);
+      ehe = (dthe[nu] == 0.) ? 1. : dm_exp(-dthe[nu]);  // exp(-0) = 1 exactly
+    }
+    if (any) {
+      const double a = (1. - eh) * ehe;
+      const double b = (1. - ehe) * eh;
+      fh = a / (a + b);
+      fhe = b / (a + b);
+      if (g.frac_limit && (a + b) == 0.) {
+        fh = 1. / (1. + dm_exp(dthe[nu] - dth[nu]));
+        fhe = 1. / (1. + dm_exp(dth[nu] - dthe[nu]));
+      }
+    }
+    double gh, ghe;
+    if (dump) {
+      gh = fh * nin[nu] / dtd / g.vol / (double)ah1;
+      ghe = fhe * nin[nu] / dtd / g.vol / (double)ahe1;
+    } else {
+      // exp(-dth - dthe): with dthe = 0 the argument is exactly -dth
+      att = 1. - ((dthe[nu] == 0.) ? eh : dm_exp(-dth[nu] - dthe[nu]));
+      gh = fh * nin[nu] * att / dtd / g.vol / (double)ah1;
+      ghe = fhe * nin[nu] * att / dtd / g.vol / (double)ahe1;
+    }
+    gh_sum += gh;
+    ghe_sum += ghe;
+    if (last) {
+      // FP64 record layout: {gh, ghe} per bin, then the photons of every bin
+      *(double2*)(rec + 2 * nu) = make_double2(gh, ghe);
+      rec[2 * F + nu] = nin[nu] / FRT_PHOT_NORM;
+    }
+    tauh[nu] += dth[nu];
+    tauhe[nu] += dthe[nu];
+  }
+  if (last) {
+    for (int k = 3 * F; k < cp.rec_len; ++k) rec[k] = 0.;
+  } else {
+    ((double2*)cp.pair)[slot] = make_double2(gh_sum, ghe_sum);
+  }
+  return dump;
 }
 
 // FP64 form: the reference's double locals and libm exp (through detmath),
@@ -276,6 +390,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
                                                       const float* __restrict__ path_dist,
                                                       const int32_t* __restrict__ seg_cnt,
                                                       const int32_t* __restrict__ seg_off,
+                                                      const int32_t* __restrict__ inv_perm,
                                                       const float* __restrict__ avgH,
                                                       const float* __restrict__ avgHe,
                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
@@ -287,97 +402,56 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
     if (c > 0) {
       const int F = g.nfreq;
       double nph[FRT_MAX_FREQ], tauh[FRT_MAX_FREQ], tauhe[FRT_MAX_FREQ];
-      for (int nu = 0; nu < F; ++nu) {
+      FRT_FOR_BINS(nu) {
         nph[nu] = (double)rb.nphot[nu * g.cap + i] * FRT_PHOT_NORM;
         tauh[nu] = 0.;
         tauhe[nu] = 0.;
       }
       const int base = seg_off[i];
-      const double dtd = (double)g.dt;
-      int p = 0;
+      SegPipe sp;
+      sp.i = i;
+      sp.stride = g.cap;
+      sp.c = c;
+      sp.base = base;
+      sp.path_cell = path_cell;
+      sp.path_dist = path_dist;
+      sp.inv_perm = inv_perm;
+      sp.avgH = avgH;
+      sp.avgHe = avgHe;
+      sp.start();
       bool dumped = false;
+      int p = 0;
+      int last_cell = sp.cell0;
       for (; p < c; ++p) {
-        const int cell = path_cell[(int64_t)p * g.cap + i];
-        const double dist = (double)path_dist[(int64_t)p * g.cap + i];
-        const float ah1 = avgH[cell];
-        const float ahe1 = avgHe[cell];
-        double dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ], nin[FRT_MAX_FREQ];
-        double tot = 0.;
-        for (int nu = 0; nu < F; ++nu) {
-          dth[nu] = ((double)ah1 * g.sigH[nu]) * dist;    // avg_nh1*sigma*dist
-          dthe[nu] = ((double)ahe1 * g.sigHe[nu]) * dist;
-          nin[nu] = nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]);
-          tot += nin[nu] / FRT_PHOT_NORM;
-        }
-        const bool dump = tot < g.nphot_min;
-        double gh_sum = 0., ghe_sum = 0.;
-        double rec[3 * FRT_MAX_FREQ];
-        for (int nu = 0; nu < F; ++nu) {
-          double fh = 0., fhe = 0., att = 0.;
-          const bool any = (dth[nu] > 0.) || (dthe[nu] > 0.);
-          double eh = 1., ehe = 1.;
-          if (any || !dump) {
-            eh = dm_exp(-dth[nu]);
-            ehe = (dthe[nu] == 0.) ? 1. : dm_exp(-dthe[nu]);  // exp(-0) = 1 exactly
-          }
-          if (any) {
-            const double a = (1. - eh) * ehe;
-            const double b = (1. - ehe) * eh;
-            fh = a / (a + b);
-            fhe = b / (a + b);
-            if (g.frac_limit && (a + b) == 0.) {
-              fh = 1. / (1. + dm_exp(dthe[nu] - dth[nu]));
-              fhe = 1. / (1. + dm_exp(dth[nu] - dthe[nu]));
-            }
-          }
-          double gh, ghe;
-          if (dump) {
-            gh = fh * nin[nu] / dtd / g.vol / (double)ah1;
-            ghe = fhe * nin[nu] / dtd / g.vol / (double)ahe1;
-          } else {
-            // exp(-dth - dthe): with dthe = 0 the argument is exactly -dth
-            att = 1. - ((dthe[nu] == 0.) ? eh : dm_exp(-dth[nu] - dthe[nu]));
-            gh = fh * nin[nu] * att / dtd / g.vol / (double)ah1;
-            ghe = fhe * nin[nu] * att / dtd / g.vol / (double)ahe1;
-          }
-          gh_sum += gh;
-          ghe_sum += ghe;
-          rec[nu] = gh;
-          rec[F + nu] = ghe;
-          rec[2 * F + nu] = nin[nu] / FRT_PHOT_NORM;
-          tauh[nu] += dth[nu];
-          tauhe[nu] += dthe[nu];
-        }
-        const int64_t slot = base + p;
-        if (LAST)
-          store_record((double*)cp.rec + slot * cp.rec_len, rec, 3 * F, cp.rec_len);
-        else
-          ((double2*)cp.pair)[slot] = make_double2(gh_sum, ghe_sum);
-        if (dump) {
+        sp.prefetch(p);
+        last_cell = sp.cell0;
+        if (deposit_segment_fp64(g, cp, LAST, sp.slot0, (double)sp.dist0, sp.ah0, sp.ahe0, nph, tauh, tauhe)) {
           dumped = true;
           ++p;
           break;
         }
+        sp.advance();
       }
       // segments behind a dump deposit nothing this pass
       for (; p < c; ++p) {
-        const int64_t slot = base + p;
+        const int slot = inv_perm[base + p];
+        last_cell = path_cell[(int64_t)p * g.cap + i];
         if (LAST)
-          store_record((double*)cp.rec + slot * cp.rec_len, (const double*)nullptr, 0, cp.rec_len);
+          zero_record((double*)cp.rec + (int64_t)slot * cp.rec_len, cp.rec_len);
         else
           ((double2*)cp.pair)[slot] = make_double2(0., 0.);
       }
       if (LAST) {
         // photon update and removal (ray_tracing.cc:923-935)
         float sum = 0.f;
-        for (int nu = 0; nu < F; ++nu) {
+        FRT_FOR_BINS(nu) {
           const float v = (float)(nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]) / FRT_PHOT_NORM);
           rb.nphot[nu * g.cap + i] = v;
           sum += v;
         }
         dumped_flag = dumped;
         if ((double)sum > g.nphot_min) {
-          rb.cell[i] = path_cell[(int64_t)(c - 1) * g.cap + i];
+          rb.cell[i] = last_cell;
         } else {
           alive[i] = 0;
           removed = true;
@@ -393,6 +467,68 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
   }
 }
 
+// One segment of the mixed deposit; returns true if the ray is dumped here.
+__device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, ContribPtrs& cp, bool last, int slot,
+                                                      float dist, float ah1, float ahe1, const float* nph,
+                                                      float* trans, float inv_dtv, float nmin) {
+  const int F = g.nfreq;
+  const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
+  float nin[FRT_MAX_FREQ];
+  float tot = 0.f;
+#pragma unroll
+  for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
+    if (nu >= F) break;
+    nin[nu] = nph[nu] * trans[nu];
+    tot += nin[nu];
+  }
+  const bool dump = tot < nmin;
+  float gh_sum = 0.f, ghe_sum = 0.f;
+  float* rec = (float*)cp.rec + (int64_t)slot * cp.rec_len;
+#pragma unroll
+  for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
+    if (nu >= F) break;
+    // absorbed fractions 1 - exp(-dtau) through expm1: in ionized cells dtau is
+    // ~1e-5 and 1 - expf(-dtau) would keep only ~3 significant digits
+    const float dth = ah1 * g.sigHf[nu] * dist;
+    const float dthe = ahe1 * g.sigHef[nu] * dist;
+    const float omh = -expm1f(-dth);
+    const float omhe = (dthe == 0.f) ? 0.f : -expm1f(-dthe);
+    const float eh = expf(-dth);
+    const float ehe = (dthe == 0.f) ? 1.f : expf(-dthe);
+    float fh = 0.f, fhe = 0.f;
+    if ((dth > 0.f) || (dthe > 0.f)) {
+      const float a = omh * ehe;
+      const float b = omhe * eh;
+      const float den = a + b;
+      if (den > 0.f) {
+        fh = a / den;
+        fhe = b / den;
+      } else if (g.frac_limit) {
+        fh = 1.f / (1.f + expf(dthe - dth));
+        fhe = 1.f / (1.f + expf(dth - dthe));
+      } else {
+        fh = fhe = __int_as_float(0x7fc00000);
+      }
+    }
+    const float eabs = dump ? nin[nu] : nin[nu] * ((dthe == 0.f) ? omh : -expm1f(-(dth + dthe)));
+    const float gh = fh * eabs * rh1;
+    const float ghe = fhe * eabs * rhe1;
+    gh_sum += gh;
+    ghe_sum += ghe;
+    if (last) {
+      *(float2*)(rec + 2 * nu) = make_float2(gh, ghe);
+      rec[2 * F + nu] = nin[nu];
+    }
+    trans[nu] *= eh * ehe;
+  }
+  if (last) {
+    for (int k = 3 * F; k < cp.rec_len; ++k) rec[k] = 0.f;
+  } else {
+    ((float2*)cp.pair)[slot] = make_float2(gh_sum, ghe_sum);
+  }
+  return dump;
+}
+
 // Mixed form: FP32 attenuation and photon bookkeeping; per-cell sums are
 // compensated (Kahan) in the cell kernel.
 template <bool LAST>
@@ -401,106 +537,69 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
                                                        const float* __restrict__ path_dist,
                                                        const int32_t* __restrict__ seg_cnt,
                                                        const int32_t* __restrict__ seg_off,
+                                                       const int32_t* __restrict__ inv_perm,
                                                        const float* __restrict__ avgH,
                                                        const float* __restrict__ avgHe,
                                                        ContribPtrs cp, uint8_t* __restrict__ alive,
                                                        DevCounters* cnt) {
   unsigned long long ndumped = 0, nremoved = 0;
+  // 1/(dt vol) in units of phot_norm photons
+  const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
+  const float nmin = (float)g.nphot_min;
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
     bool dumped_flag = false, removed = false;
     const int c = seg_cnt[i];
     if (c > 0) {
       const int F = g.nfreq;
       float nph[FRT_MAX_FREQ], trans[FRT_MAX_FREQ];
-      for (int nu = 0; nu < F; ++nu) {
+      FRT_FOR_BINS(nu) {
         nph[nu] = rb.nphot[nu * g.cap + i];  // in units of phot_norm
         trans[nu] = 1.f;                     // exp(-tau) carried multiplicatively
       }
       const int base = seg_off[i];
-      // 1/(dt vol) in units of phot_norm photons
-      const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
-      const float nmin = (float)g.nphot_min;
-      int p = 0;
+      SegPipe sp;
+      sp.i = i;
+      sp.stride = g.cap;
+      sp.c = c;
+      sp.base = base;
+      sp.path_cell = path_cell;
+      sp.path_dist = path_dist;
+      sp.inv_perm = inv_perm;
+      sp.avgH = avgH;
+      sp.avgHe = avgHe;
+      sp.start();
       bool dumped = false;
+      int p = 0;
+      int last_cell = sp.cell0;
       for (; p < c; ++p) {
-        const int cell = path_cell[(int64_t)p * g.cap + i];
-        const float dist = path_dist[(int64_t)p * g.cap + i];
-        const float ah1 = avgH[cell];
-        const float ahe1 = avgHe[cell];
-        const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
-        // absorbed fractions 1 - exp(-dtau) through expm1: in ionized cells dtau is
-        // ~1e-5 and 1 - expf(-dtau) would keep only ~3 significant digits
-        float nin[FRT_MAX_FREQ], eh[FRT_MAX_FREQ], ehe[FRT_MAX_FREQ], omh[FRT_MAX_FREQ],
-            omhe[FRT_MAX_FREQ], dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ];
-        float tot = 0.f;
-        for (int nu = 0; nu < F; ++nu) {
-          dth[nu] = ah1 * g.sigHf[nu] * dist;
-          dthe[nu] = ahe1 * g.sigHef[nu] * dist;
-          omh[nu] = -expm1f(-dth[nu]);
-          omhe[nu] = (dthe[nu] == 0.f) ? 0.f : -expm1f(-dthe[nu]);
-          eh[nu] = expf(-dth[nu]);
-          ehe[nu] = (dthe[nu] == 0.f) ? 1.f : expf(-dthe[nu]);
-          nin[nu] = nph[nu] * trans[nu];
-          tot += nin[nu];
-        }
-        const bool dump = tot < nmin;
-        float gh_sum = 0.f, ghe_sum = 0.f;
-        float rec[3 * FRT_MAX_FREQ];
-        for (int nu = 0; nu < F; ++nu) {
-          float fh = 0.f, fhe = 0.f;
-          if ((dth[nu] > 0.f) || (dthe[nu] > 0.f)) {
-            const float a = omh[nu] * ehe[nu];
-            const float b = omhe[nu] * eh[nu];
-            const float den = a + b;
-            if (den > 0.f) {
-              fh = a / den;
-              fhe = b / den;
-            } else if (g.frac_limit) {
-              fh = 1.f / (1.f + expf(dthe[nu] - dth[nu]));
-              fhe = 1.f / (1.f + expf(dth[nu] - dthe[nu]));
-            } else {
-              fh = fhe = __int_as_float(0x7fc00000);
-            }
-          }
-          const float eabs =
-              dump ? nin[nu] : nin[nu] * ((dthe[nu] == 0.f) ? omh[nu] : -expm1f(-(dth[nu] + dthe[nu])));
-          const float gh = fh * eabs * rh1;
-          const float ghe = fhe * eabs * rhe1;
-          gh_sum += gh;
-          ghe_sum += ghe;
-          rec[nu] = gh;
-          rec[F + nu] = ghe;
-          rec[2 * F + nu] = nin[nu];
-          trans[nu] *= eh[nu] * ehe[nu];
-        }
-        const int64_t slot = base + p;
-        if (LAST)
-          store_record((float*)cp.rec + slot * cp.rec_len, rec, 3 * F, cp.rec_len);
-        else
-          ((float2*)cp.pair)[slot] = make_float2(gh_sum, ghe_sum);
-        if (dump) {
+        sp.prefetch(p);
+        last_cell = sp.cell0;
+        if (deposit_segment_mixed(g, cp, LAST, sp.slot0, sp.dist0, sp.ah0, sp.ahe0, nph, trans, inv_dtv, nmin)) {
           dumped = true;
           ++p;
           break;
         }
+        sp.advance();
       }
+      // segments behind a dump deposit nothing this pass
       for (; p < c; ++p) {
-        const int64_t slot = base + p;
+        const int slot = inv_perm[base + p];
+        last_cell = path_cell[(int64_t)p * g.cap + i];
         if (LAST)
-          store_record((float*)cp.rec + slot * cp.rec_len, (const float*)nullptr, 0, cp.rec_len);
+          zero_record((float*)cp.rec + (int64_t)slot * cp.rec_len, cp.rec_len);
         else
           ((float2*)cp.pair)[slot] = make_float2(0.f, 0.f);
       }
       if (LAST) {
         float sum = 0.f;
-        for (int nu = 0; nu < F; ++nu) {
+        FRT_FOR_BINS(nu) {
           const float v = nph[nu] * trans[nu];
           rb.nphot[nu * g.cap + i] = v;
           sum += v;
         }
         dumped_flag = dumped;
         if ((double)sum > g.nphot_min) {
-          rb.cell[i] = path_cell[(int64_t)(c - 1) * g.cap + i];
+          rb.cell[i] = last_cell;
         } else {
           alive[i] = 0;
           removed = true;
@@ -514,6 +613,13 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
     block_add(&cnt->dumped, ndumped);
     block_add(&cnt->terminated, nremoved);
   }
+}
+
+// Inverse of the cell sort of segments: slot of each segment.
+__global__ void __launch_bounds__(BLK) k_inv_perm(const int32_t* __restrict__ val, int64_t nseg,
+                                                  int32_t* __restrict__ inv) {
+  for (int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * BLK)
+    inv[val[s]] = (int32_t)s;
 }
 
 // ---------------------------------------------------------------------------
@@ -610,9 +716,9 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
     if (sizeof(ACC) == 8) {
       double a = 0., b = 0.;
       double x[FRT_MAX_FREQ], y[FRT_MAX_FREQ], z[FRT_MAX_FREQ];
-      for (int nu = 0; nu < F; ++nu) x[nu] = y[nu] = z[nu] = 0.;
+      FRT_FOR_BINS(nu) x[nu] = y[nu] = z[nu] = 0.;
       for (int s = s0; s < s1; ++s) {
-        const int64_t c = cp.perm[s];
+        const int64_t c = s;
         if (!LAST) {
           const double2 v = ((const double2*)cp.pair)[c];
           a += v.x;
@@ -620,11 +726,14 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
         } else {
           const double* r = (const double*)cp.rec + c * cp.rec_len;
           double gs = 0., gse = 0.;
-          for (int nu = 0; nu < F; ++nu) {
-            gs += r[nu];
-            gse += r[F + nu];
-            x[nu] += r[nu];
-            y[nu] += r[F + nu];
+#pragma unroll
+          for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
+            if (nu >= F) break;
+            const double2 q = *(const double2*)(r + 2 * nu);
+            gs += q.x;
+            gse += q.y;
+            x[nu] += q.x;
+            y[nu] += q.y;
             z[nu] += r[2 * F + nu];
           }
           a += gs;
@@ -633,7 +742,7 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       }
       GH = a;
       GHe = b;
-      for (int nu = 0; nu < F; ++nu) {
+      FRT_FOR_BINS(nu) {
         GHnu[nu] = x[nu];
         GHenu[nu] = y[nu];
         NGnu[nu] = z[nu];
@@ -642,9 +751,9 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       // compensated float sums in ray order
       float a = 0.f, ca = 0.f, b = 0.f, cb = 0.f;
       float x[FRT_MAX_FREQ], cx[FRT_MAX_FREQ], y[FRT_MAX_FREQ], cy[FRT_MAX_FREQ], z[FRT_MAX_FREQ];
-      for (int nu = 0; nu < F; ++nu) x[nu] = cx[nu] = y[nu] = cy[nu] = z[nu] = 0.f;
+      FRT_FOR_BINS(nu) x[nu] = cx[nu] = y[nu] = cy[nu] = z[nu] = 0.f;
       for (int s = s0; s < s1; ++s) {
-        const int64_t c = cp.perm[s];
+        const int64_t c = s;
         float va, vb;
         if (!LAST) {
           const float2 v = ((const float2*)cp.pair)[c];
@@ -654,14 +763,18 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
           const float* r = (const float*)cp.rec + c * cp.rec_len;
           va = 0.f;
           vb = 0.f;
-          for (int nu = 0; nu < F; ++nu) {
-            va += r[nu];
-            vb += r[F + nu];
-            float yx = r[nu] - cx[nu];
+#pragma unroll
+          for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
+            if (nu >= F) break;
+            const float2 q = *(const float2*)(r + 2 * nu);
+            const float rh = q.x, rhe = q.y;
+            va += rh;
+            vb += rhe;
+            float yx = rh - cx[nu];
             float tx = x[nu] + yx;
             cx[nu] = (tx - x[nu]) - yx;
             x[nu] = tx;
-            float yy = r[F + nu] - cy[nu];
+            float yy = rhe - cy[nu];
             float ty = y[nu] + yy;
             cy[nu] = (ty - y[nu]) - yy;
             y[nu] = ty;
@@ -679,7 +792,7 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       }
       GH = (double)a;
       GHe = (double)b;
-      for (int nu = 0; nu < F; ++nu) {
+      FRT_FOR_BINS(nu) {
         GHnu[nu] = (double)x[nu];
         GHenu[nu] = (double)y[nu];
         NGnu[nu] = (double)z[nu];
@@ -946,7 +1059,7 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, 
     while (e < ncand && key[e] == key[s]) ++e;
     const int b = val[s];
     float NP[FRT_MAX_FREQ];
-    for (int nu = 0; nu < F; ++nu) NP[nu] = rb.nphot[nu * cap + b];
+    FRT_FOR_BINS(nu) NP[nu] = rb.nphot[nu * cap + b];
     const float nb = sum_nphot(rb.nphot, cap, F, b);
     const float rrb = rb.r[b] / g.dx0;
     float R = (float)((double)rrb * (double)rrb * (double)nb);
@@ -958,7 +1071,7 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, 
     for (int64_t j = s + 1; j < e; ++j) {
       const int m = val[j];
       const float nm = sum_nphot(rb.nphot, cap, F, m);
-      for (int nu = 0; nu < F; ++nu) NP[nu] += rb.nphot[nu * cap + m];
+      FRT_FOR_BINS(nu) NP[nu] += rb.nphot[nu * cap + m];
       const float rrm = rb.r[m] / g.dx0;
       R = (float)((double)R + (double)rrm * (double)rrm * (double)nm);
       DR += rb.dr[m] * nm;
@@ -970,7 +1083,7 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, 
       ++merged;
     }
     float ntot = 0.f;
-    for (int nu = 0; nu < F; ++nu) ntot += NP[nu];
+    FRT_FOR_BINS(nu) ntot += NP[nu];
     R = R / ntot;
     R = (float)sqrt((double)R);
     R = R * g.dx0;
@@ -1000,7 +1113,7 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, 
       rb.yhat[b] = oy;
       rb.zhat[b] = oz;
     }
-    for (int nu = 0; nu < F; ++nu) rb.nphot[nu * cap + b] = NP[nu];
+    FRT_FOR_BINS(nu) rb.nphot[nu * cap + b] = NP[nu];
     rb.r[b] = R;
     rb.dr[b] = DR;
     rb.x[b] = X;
@@ -1283,15 +1396,18 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
   if (need <= ctx->seg_cap) return FRT_OK;
   int64_t cap = ctx->seg_cap ? ctx->seg_cap : 1024;
   while (cap < need) cap *= 2;
+  hipStreamSynchronize(ctx->stream);  // the stream may still use the old buffers
   for (int b = 0; b < 2; ++b) {
     hipFree(ctx->seg_key[b]);
     hipFree(ctx->seg_val[b]);
     ctx->seg_key[b] = nullptr;
     ctx->seg_val[b] = nullptr;
   }
-  hipFree(ctx->contrib.pair);
+  hipFree(ctx->contrib.rec);  // base of the record + pair block
+  hipFree(ctx->inv_perm);
   ctx->contrib.pair = nullptr;
   ctx->contrib.rec = nullptr;
+  ctx->inv_perm = nullptr;
   const int F = ctx->g.nfreq;
   const size_t acc = (ctx->cfg.precision == FRT_PREC_MIXED) ? 4 : 8;
   // last-pass record: 3F values rounded up to 16 or 32 (whole cache lines)
@@ -1302,6 +1418,7 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
     e = hipMalloc((void**)&ctx->seg_key[b], (size_t)cap * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&ctx->seg_val[b], (size_t)cap * 4);
   }
+  if (e == hipSuccess) e = hipMalloc((void**)&ctx->inv_perm, (size_t)cap * 4);
   void* blk = nullptr;
   if (e == hipSuccess) e = hipMalloc(&blk, per * (size_t)cap);
   if (e != hipSuccess) {
@@ -1340,6 +1457,7 @@ int launch_segments(frt_ctx* ctx) {
   if (rc) return rc;
   FRT_LAUNCH(ctx, "cell_ranges", grid_for(nseg), k_cell_ranges, ctx->seg_key[1], nseg,
              ctx->cell_start, ctx->cell_end);
+  FRT_LAUNCH(ctx, "inv_perm", sgrid(nseg), k_inv_perm, ctx->seg_val[1], nseg, ctx->inv_perm);
   return FRT_OK;
 }
 
@@ -1360,25 +1478,25 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     if (mixed) {
       if (last)
         FRT_LAUNCH(ctx, "deposit_mixed", sgrid(n), k_deposit_mixed<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
         FRT_LAUNCH(ctx, "deposit_mixed", sgrid(n), k_deposit_mixed<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     } else {
       if (last)
         FRT_LAUNCH(ctx, "deposit_fp64", sgrid(n), k_deposit_fp64<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
         FRT_LAUNCH(ctx, "deposit_fp64", sgrid(n), k_deposit_fp64<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     }
     // algorithmic bytes of this deposit launch (model used for the roofline):
     // per ray: seg_cnt + seg_off (8) + nphot read (4F); per segment: path cell +
-    // dist (8) + avg_nh1/avg_nhe1 (8) + 2 deposits; last pass adds
+    // dist (8) + slot (4) + avg_nh1/avg_nhe1 (8) + 2 deposits; last pass adds
     // 3F per-bin deposits per segment and the ray write-back (4F + 4 + 1).
     {
       const double acc = mixed ? 4. : 8.;

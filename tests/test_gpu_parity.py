@@ -99,3 +99,25 @@ def test_mixed_within_tolerance():
     gg, go = g.download_field("gamma_H1_tot"), o.download_field("gamma_H1_tot")
     scale = float(np.abs(go).max())
     assert float(np.abs(gg - go).max()) <= 1e-3 * scale
+
+
+def test_mixed_repeatable_across_contexts():
+    """Two contexts run the same mixed workload back to back: counters and
+    every field bit-identical (no dependence on timing or on memory left by
+    earlier allocations)."""
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    _print_case("mixed repeat lognormal n=32")
+    wl = inputs.make_workload("lognormal64", n=32, n_src=24, steps=5)
+    cfg = dict(wl["cfg"])
+    cfg["precision"] = 1
+    runs = []
+    for _ in range(2):
+        with FlexRT(cfg) as g:
+            inputs.setup(g, wl)
+            st = [counts(g.step()) for _ in range(wl["steps"])]
+            runs.append((st, fields(g)))
+    (s0, f0), (s1, f1) = runs
+    assert s0 == s1
+    bad = [k for k in STATE_FIELDS if not bits_equal(f0[k], f1[k])]
+    assert not bad, bad
