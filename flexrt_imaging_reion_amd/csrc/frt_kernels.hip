@@ -18,6 +18,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "frt_internal.h"
+#include "frt_fexp.h"
 #include "frt_rng.h"
 
 namespace frt {
@@ -487,20 +488,25 @@ __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, Contri
 #pragma unroll
   for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
     if (nu >= F) break;
-    // absorbed fractions 1 - exp(-dtau) through expm1: in ionized cells dtau is
-    // ~1e-5 and 1 - expf(-dtau) would keep only ~3 significant digits
+    // transmissions and absorbed fractions 1 - exp(-dtau) from one reduction
+    // each (the absorbed fraction through expm1: in ionized cells dtau is
+    // ~1e-5 and 1 - exp(-dtau) would keep only ~3 significant digits)
     const float dth = ah1 * g.sigHf[nu] * dist;
     const float dthe = ahe1 * g.sigHef[nu] * dist;
-    const float omh = -expm1f(-dth);
-    const float omhe = (dthe == 0.f) ? 0.f : -expm1f(-dthe);
-    const float eh = expf(-dth);
-    const float ehe = (dthe == 0.f) ? 1.f : expf(-dthe);
+    float eh, mh, ehe = 1.f, mhe = 0.f;
+    frt_exp_expm1_neg(dth, eh, mh);
+    if (dthe != 0.f) frt_exp_expm1_neg(dthe, ehe, mhe);
+    const float omh = -mh, omhe = -mhe;
     float fh = 0.f, fhe = 0.f;
     if ((dth > 0.f) || (dthe > 0.f)) {
       const float a = omh * ehe;
       const float b = omhe * eh;
       const float den = a + b;
-      if (den > 0.f) {
+      if (den >= 1e-30f) {
+        const float inv = __builtin_amdgcn_rcpf(den);  // 1 ulp
+        fh = a * inv;
+        fhe = b * inv;
+      } else if (den > 0.f) {
         fh = a / den;
         fhe = b / den;
       } else if (g.frac_limit) {
@@ -510,7 +516,8 @@ __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, Contri
         fh = fhe = __int_as_float(0x7fc00000);
       }
     }
-    const float eabs = dump ? nin[nu] : nin[nu] * ((dthe == 0.f) ? omh : -expm1f(-(dth + dthe)));
+    // 1 - exp(-(dth + dthe)) = omh + omhe - omh * omhe (exact identity)
+    const float eabs = dump ? nin[nu] : nin[nu] * fmaf(-omh, omhe, omh + omhe);
     const float gh = fh * eabs * rh1;
     const float ghe = fhe * eabs * rhe1;
     gh_sum += gh;

@@ -51,6 +51,7 @@
 #include <vector>
 
 #include "../flexrt_imaging_reion_amd/csrc/frt_detmath.h"
+#include "../flexrt_imaging_reion_amd/csrc/frt_fexp.h"
 #include "../flexrt_imaging_reion_amd/csrc/frt_healpix.h"
 #include "../flexrt_imaging_reion_amd/csrc/frt_rng.h"
 #include "../include/flexrt_gpu.h"
@@ -1253,6 +1254,11 @@ int orc_download_paths(void* h, int64_t n, int* ray_end, int* cells, float* dist
 
 // ---- unit entry points used by the pin tests ----------------------------------------------------
 double orc_dm_exp(double x) { return dm_exp(x); }
+// The mixed-precision deposit's exp/expm1 helper (frt_fexp.h), compiled for
+// the host so the tests can check it against double-precision exp/expm1.
+void orc_fexp_neg(const float* d, float* e, float* m, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) frt_exp_expm1_neg(d[i], e[i], m[i]);
+}
 double orc_dm_log(double x) { return dm_log(x); }
 double orc_dm_pow(double x, double y) { return dm_pow(x, y); }
 double orc_dm_sin(double x) { return dm_sin(x); }
