@@ -118,6 +118,20 @@ def run_form(wl, precision, steps, warmup, profile):
     return wall, stats, ktimes, state
 
 
+def pmc_traffic(kernel, workload, mixed):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 PMC for this
+    workload and form (profiles/rt_step_<workload>_<form>_traffic.txt: the
+    FETCH_SIZE and WRITE_SIZE counters scaled by a plain-copy calibration)."""
+    path = os.path.join(REPO, "profiles", "rt_step_%s_%s_traffic.txt" % (workload, "mixed" if mixed else "fp64"))
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith(kernel + ":"):
+            kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
+            return float(kv["fetch_bytes"]) + float(kv["write_bytes"]), os.path.relpath(path, REPO)
+    return None, None
+
+
 def roofline(wl, stats, ktimes, mixed):
     """Roofline figure of the kernel with the largest share of GPU time."""
     best = None
@@ -142,10 +156,16 @@ def roofline(wl, stats, ktimes, mixed):
     avg_ms = ms / launches
     avg_bytes = total_bytes / n_launch
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
-            "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": round(avg_bytes),
-            "launches": launches}
+    traffic, source = pmc_traffic(name, wl["name"], mixed)
+    out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 5),
+           "traffic": round(traffic) if traffic is not None else None, "kernel": name,
+           "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": round(avg_bytes),
+           "launches": launches}
+    if source:
+        out["traffic_source"] = source
+        out["traffic_GBps"] = round(traffic / (avg_ms * 1e-3) / 1e9, 2)
+    return out
 
 
 def cpu_baseline(wl, seconds):

@@ -1166,7 +1166,15 @@ void orc_find_avg_unu(void* h) { find_avg_unu(*(State*)h); }
 void orc_merge_rays(void* h) { merge_phase(*(State*)h); }
 void orc_split_rays(void* h) { split_rays(*(State*)h); }
 void orc_update_thermal(void* h) { update_thermal(*(State*)h); }
-void orc_get_stats(void* h, frt_step_stats* st) { *st = ((State*)h)->st; }
+void orc_get_stats(void* h, frt_step_stats* st) {
+  // scalars reflect the state now, also after a single phase call
+  State* s = (State*)h;
+  s->st.nphot_min = s->nphot_min;
+  s->st.nphot_max = s->nphot_max;
+  s->st.nphot_merge = s->nphot_merge;
+  s->st.t = s->t;
+  *st = s->st;
+}
 
 int64_t orc_ray_count(void* h) {
   State* s = (State*)h;
