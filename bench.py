@@ -54,12 +54,13 @@ def kernel_bytes(name, wl, st, mixed):
 
     deposit: per ray 4 (seg count); per moved ray 4 (offset) + 4F (photons in);
       per segment 4 (cell) + 4 (length) + 4 (sorted slot) + 8 (avg_nh1, avg_nhe1)
-      + 2 acc (H and He deposits); the last pass adds 3F acc per segment (per-bin
-      deposits and photons) and 4F + 5 per moved ray (photons out, cell, alive).
+      + 2 acc (H and He deposits) in the first passes, (F+1) acc (photon
+      record) in the last pass, which adds 4F + 5 per moved ray (photons out,
+      cell, alive).
     chem: per cell 8 (run bounds) + 36 (nH, nHe, clump, f_H1, f_He1, recomb x2,
-      avg x2 or their iteration-0 sources) + 8 (avg out); per segment 2 acc; the
-      last pass adds 3F acc per segment, 12F per cell (per-bin fields) and 100
-      per cell (finalize + thermal: ~25 fields).
+      avg x2 or their iteration-0 sources) + 8 (avg out); per segment 2 acc in
+      the first passes, (F+1) acc in the last, which adds 12F per cell (per-bin
+      fields) and 100 per cell (finalize + thermal: ~25 fields).
     march: per ray 56 read (cell, direction, position, time, r, dr) + 5 (flags)
       + 28 written back; per segment 8 (cell, length).
     The per-pass figures are averaged over the itercount passes of the step.
@@ -70,13 +71,13 @@ def kernel_bytes(name, wl, st, mixed):
     n3 = float(wl["cfg"]["n"]) ** 3
     nr, nm, ns = float(st["rays_start"]), float(st["rays_moved"]), float(st["segments"])
     if name.startswith("deposit"):
-        per = nr * 4 + nm * (4 + 4 * F) + ns * (20 + 2 * acc)
-        last = ns * 3 * F * acc + nm * (4 * F + 5)
-        return per + last / I
+        first = nr * 4 + nm * (4 + 4 * F) + ns * (20 + 2 * acc)
+        last = nr * 4 + nm * (4 + 4 * F) + ns * (20 + (F + 1) * acc) + nm * (4 * F + 5)
+        return ((I - 1) * first + last) / I
     if name == "chem":
-        per = n3 * 52 + ns * 2 * acc
-        last = ns * 3 * F * acc + n3 * (12 * F + 100)
-        return per + last / I
+        first = n3 * 52 + ns * 2 * acc
+        last = n3 * 52 + ns * (F + 1) * acc + n3 * (12 * F + 100)
+        return ((I - 1) * first + last) / I
     if name == "march":
         return nr * 89 + ns * 8
     return None

@@ -112,9 +112,9 @@ struct CellPtrs {
 // cell's slots are contiguous and in ray order).
 struct ContribPtrs {
   void* pair;          // [slot] {H, He} deposit summed over bins (2 values)
-  void* rec;           // [slot] last pass: {gh, ghe, photons} per bin, padded to rec_len
+  void* rec;           // [slot] last pass: photons in per bin, then the signed length
   const int32_t* perm; // cell-sorted position -> segment index
-  int rec_len;         // values per record (16 or 32)
+  int rec_len;         // values per record (nfreq + 1 rounded up to even)
 };
 
 struct frt_ctx {

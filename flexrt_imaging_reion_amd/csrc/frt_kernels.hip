@@ -257,23 +257,101 @@ __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict_
 // find_avg_unu ray loop (ray_tracing.cc:701-938), one thread per moved ray.
 //
 // Each segment's deposit goes to its cell-sorted slot (so a cell's slots are
-// contiguous and in ray order): a {H, He} pair of bin sums in the first
-// passes, a record padded to whole cache lines in the last pass: the {gh, ghe}
-// pairs of all bins followed by the photons of all bins, stored per bin as the
-// bins are computed.
+// contiguous and in ray order).  In the first passes the slot holds the
+// {H, He} deposit summed over bins.  In the last pass it holds a photon
+// record: the photons entering the segment in every bin and the segment
+// length, negated on the segment where the ray is dumped (+inf behind a
+// dump).  The cell kernel recomputes the per-bin deposits from the record
+// and its own opacities (the same expressions, so the same bits), which
+// keeps the per-bin Gamma and photon fields without storing 3 values per bin.
 
 // Loop over the frequency bins, unrolled to the compile-time maximum so that
 // per-bin register arrays are indexed statically (no scratch).
 #define FRT_FOR_BINS(nu) \
   _Pragma("unroll") for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) if (nu < F)
 
-#define FRT_MAX_REC 32  // values per record at most (3 * FRT_MAX_FREQ rounded up)
-static_assert(3 * FRT_MAX_FREQ <= FRT_MAX_REC, "record too short");
-
-// Zero record of a segment behind a dump, written with 16-byte stores.
+// Photon record of a segment: nin[0..F-1], then the signed length.
 template <typename T>
-__device__ __forceinline__ void zero_record(T* rec, int rec_len) {
-  for (int k = 0; k < rec_len * (int)sizeof(T); k += 16) *(float4*)((char*)rec + k) = make_float4(0.f, 0.f, 0.f, 0.f);
+__device__ __forceinline__ void store_photon_record(T* rec, const T* nin, int F, float dist, bool dump) {
+  FRT_FOR_BINS(nu) rec[nu] = nin[nu];
+  rec[F] = (T)(dump ? -dist : dist);
+}
+
+template <typename T>
+__device__ __forceinline__ void store_behind_dump(T* rec, int F) {
+  FRT_FOR_BINS(nu) rec[nu] = (T)0;
+  rec[F] = (T)INFINITY;
+}
+
+// Per-bin deposit of one segment, FP64 form (ray_tracing.cc:764-840):
+// HI/HeI split, absorbed photons, Gamma contributions.
+__device__ __forceinline__ void bin_deposit_fp64(const GridConst& g, double dth, double dthe, double nin,
+                                                 bool dump, float ah1, float ahe1, double& gh, double& ghe) {
+  const double dtd = (double)g.dt;
+  double fh = 0., fhe = 0., att = 0.;
+  const bool any = (dth > 0.) || (dthe > 0.);
+  double eh = 1., ehe = 1.;
+  if (any || !dump) {
+    eh = dm_exp(-dth);
+    ehe = (dthe == 0.) ? 1. : dm_exp(-dthe);  // exp(-0) = 1 exactly
+  }
+  if (any) {
+    const double a = (1. - eh) * ehe;
+    const double b = (1. - ehe) * eh;
+    fh = a / (a + b);
+    fhe = b / (a + b);
+    if (g.frac_limit && (a + b) == 0.) {
+      fh = 1. / (1. + dm_exp(dthe - dth));
+      fhe = 1. / (1. + dm_exp(dth - dthe));
+    }
+  }
+  if (dump) {
+    gh = fh * nin / dtd / g.vol / (double)ah1;
+    ghe = fhe * nin / dtd / g.vol / (double)ahe1;
+  } else {
+    // exp(-dth - dthe): with dthe = 0 the argument is exactly -dth
+    att = 1. - ((dthe == 0.) ? eh : dm_exp(-dth - dthe));
+    gh = fh * nin * att / dtd / g.vol / (double)ah1;
+    ghe = fhe * nin * att / dtd / g.vol / (double)ahe1;
+  }
+}
+
+// Per-bin deposit of one segment, mixed form: transmissions eh, ehe and the
+// Gamma contributions gh, ghe (in units of phot_norm photons).
+__device__ __forceinline__ void bin_deposit_mixed(const GridConst& g, float dth, float dthe, float nin, bool dump,
+                                                  float rh1, float rhe1, float& gh, float& ghe, float& eh,
+                                                  float& ehe) {
+  // transmissions and absorbed fractions 1 - exp(-dtau) from one reduction
+  // each (the absorbed fraction through expm1: in ionized cells dtau is
+  // ~1e-5 and 1 - exp(-dtau) would keep only ~3 significant digits)
+  float mh, mhe = 0.f;
+  ehe = 1.f;
+  frt_exp_expm1_neg(dth, eh, mh);
+  if (dthe != 0.f) frt_exp_expm1_neg(dthe, ehe, mhe);
+  const float omh = -mh, omhe = -mhe;
+  float fh = 0.f, fhe = 0.f;
+  if ((dth > 0.f) || (dthe > 0.f)) {
+    const float a = omh * ehe;
+    const float b = omhe * eh;
+    const float den = a + b;
+    if (den >= 1e-30f) {
+      const float inv = __builtin_amdgcn_rcpf(den);  // 1 ulp
+      fh = a * inv;
+      fhe = b * inv;
+    } else if (den > 0.f) {
+      fh = a / den;
+      fhe = b / den;
+    } else if (g.frac_limit) {
+      fh = 1.f / (1.f + expf(dthe - dth));
+      fhe = 1.f / (1.f + expf(dth - dthe));
+    } else {
+      fh = fhe = __int_as_float(0x7fc00000);
+    }
+  }
+  // 1 - exp(-(dth + dthe)) = omh + omhe - omh * omhe (exact identity)
+  const float eabs = dump ? nin : nin * fmaf(-omh, omhe, omh + omhe);
+  gh = fh * eabs * rh1;
+  ghe = fhe * eabs * rhe1;
 }
 
 // Software pipeline over a ray's segments: the path entry (cell, length,
@@ -317,68 +395,36 @@ struct SegPipe {
 };
 
 // One segment of the FP64 deposit; returns true if the ray is dumped here.
+// The last pass stores the segment's photon record instead of its deposit.
 __device__ __forceinline__ bool deposit_segment_fp64(const GridConst& g, ContribPtrs& cp, bool last, int slot,
-                                                     double dist, float ah1, float ahe1, const double* nph,
+                                                     float dist, float ah1, float ahe1, const double* nph,
                                                      double* tauh, double* tauhe) {
   const int F = g.nfreq;
-  const double dtd = (double)g.dt;
+  const double distd = (double)dist;
   double dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ], nin[FRT_MAX_FREQ];
   double tot = 0.;
-#pragma unroll
-  for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
-    if (nu >= F) break;
-    dth[nu] = ((double)ah1 * g.sigH[nu]) * dist;  // avg_nh1*sigma*dist
-    dthe[nu] = ((double)ahe1 * g.sigHe[nu]) * dist;
+  FRT_FOR_BINS(nu) {
+    dth[nu] = ((double)ah1 * g.sigH[nu]) * distd;  // avg_nh1*sigma*dist
+    dthe[nu] = ((double)ahe1 * g.sigHe[nu]) * distd;
     nin[nu] = nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]);
     tot += nin[nu] / FRT_PHOT_NORM;
   }
   const bool dump = tot < g.nphot_min;
-  double gh_sum = 0., ghe_sum = 0.;
-  double* rec = (double*)cp.rec + (int64_t)slot * cp.rec_len;
-#pragma unroll
-  for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
-    if (nu >= F) break;
-    double fh = 0., fhe = 0., att = 0.;
-    const bool any = (dth[nu] > 0.) || (dthe[nu] > 0.);
-    double eh = 1., ehe = 1.;
-    if (any || !dump) {
-      eh = dm_exp(-dth[nu]);
-      ehe = (dthe[nu] == 0.) ? 1. : dm_exp(-dthe[nu]);  // exp(-0) = 1 exactly
+  if (last) {
+    store_photon_record((double*)cp.rec + (int64_t)slot * cp.rec_len, nin, F, dist, dump);
+  } else {
+    double gh_sum = 0., ghe_sum = 0.;
+    FRT_FOR_BINS(nu) {
+      double gh, ghe;
+      bin_deposit_fp64(g, dth[nu], dthe[nu], nin[nu], dump, ah1, ahe1, gh, ghe);
+      gh_sum += gh;
+      ghe_sum += ghe;
     }
-    if (any) {
-      const double a = (1. - eh) * ehe;
-      const double b = (1. - ehe) * eh;
-      fh = a / (a + b);
-      fhe = b / (a + b);
-      if (g.frac_limit && (a + b) == 0.) {
-        fh = 1. / (1. + dm_exp(dthe[nu] - dth[nu]));
-        fhe = 1. / (1. + dm_exp(dth[nu] - dthe[nu]));
-      }
-    }
-    double gh, ghe;
-    if (dump) {
-      gh = fh * nin[nu] / dtd / g.vol / (double)ah1;
-      ghe = fhe * nin[nu] / dtd / g.vol / (double)ahe1;
-    } else {
-      // exp(-dth - dthe): with dthe = 0 the argument is exactly -dth
-      att = 1. - ((dthe[nu] == 0.) ? eh : dm_exp(-dth[nu] - dthe[nu]));
-      gh = fh * nin[nu] * att / dtd / g.vol / (double)ah1;
-      ghe = fhe * nin[nu] * att / dtd / g.vol / (double)ahe1;
-    }
-    gh_sum += gh;
-    ghe_sum += ghe;
-    if (last) {
-      // FP64 record layout: {gh, ghe} per bin, then the photons of every bin
-      *(double2*)(rec + 2 * nu) = make_double2(gh, ghe);
-      rec[2 * F + nu] = nin[nu] / FRT_PHOT_NORM;
-    }
+    ((double2*)cp.pair)[slot] = make_double2(gh_sum, ghe_sum);
+  }
+  FRT_FOR_BINS(nu) {
     tauh[nu] += dth[nu];
     tauhe[nu] += dthe[nu];
-  }
-  if (last) {
-    for (int k = 3 * F; k < cp.rec_len; ++k) rec[k] = 0.;
-  } else {
-    ((double2*)cp.pair)[slot] = make_double2(gh_sum, ghe_sum);
   }
   return dump;
 }
@@ -426,7 +472,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
       for (; p < c; ++p) {
         sp.prefetch(p);
         last_cell = sp.cell0;
-        if (deposit_segment_fp64(g, cp, LAST, sp.slot0, (double)sp.dist0, sp.ah0, sp.ahe0, nph, tauh, tauhe)) {
+        if (deposit_segment_fp64(g, cp, LAST, sp.slot0, sp.dist0, sp.ah0, sp.ahe0, nph, tauh, tauhe)) {
           dumped = true;
           ++p;
           break;
@@ -438,7 +484,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
         const int slot = inv_perm[base + p];
         last_cell = path_cell[(int64_t)p * g.cap + i];
         if (LAST)
-          zero_record((double*)cp.rec + (int64_t)slot * cp.rec_len, cp.rec_len);
+          store_behind_dump((double*)cp.rec + (int64_t)slot * cp.rec_len, F);
         else
           ((double2*)cp.pair)[slot] = make_double2(0., 0.);
       }
@@ -469,68 +515,40 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
 }
 
 // One segment of the mixed deposit; returns true if the ray is dumped here.
+// The last pass stores the segment's photon record instead of its deposit.
 __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, ContribPtrs& cp, bool last, int slot,
                                                       float dist, float ah1, float ahe1, const float* nph,
                                                       float* trans, float inv_dtv, float nmin) {
   const int F = g.nfreq;
-  const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
   float nin[FRT_MAX_FREQ];
   float tot = 0.f;
-#pragma unroll
-  for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
-    if (nu >= F) break;
+  FRT_FOR_BINS(nu) {
     nin[nu] = nph[nu] * trans[nu];
     tot += nin[nu];
   }
   const bool dump = tot < nmin;
-  float gh_sum = 0.f, ghe_sum = 0.f;
-  float* rec = (float*)cp.rec + (int64_t)slot * cp.rec_len;
-#pragma unroll
-  for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
-    if (nu >= F) break;
-    // transmissions and absorbed fractions 1 - exp(-dtau) from one reduction
-    // each (the absorbed fraction through expm1: in ionized cells dtau is
-    // ~1e-5 and 1 - exp(-dtau) would keep only ~3 significant digits)
-    const float dth = ah1 * g.sigHf[nu] * dist;
-    const float dthe = ahe1 * g.sigHef[nu] * dist;
-    float eh, mh, ehe = 1.f, mhe = 0.f;
-    frt_exp_expm1_neg(dth, eh, mh);
-    if (dthe != 0.f) frt_exp_expm1_neg(dthe, ehe, mhe);
-    const float omh = -mh, omhe = -mhe;
-    float fh = 0.f, fhe = 0.f;
-    if ((dth > 0.f) || (dthe > 0.f)) {
-      const float a = omh * ehe;
-      const float b = omhe * eh;
-      const float den = a + b;
-      if (den >= 1e-30f) {
-        const float inv = __builtin_amdgcn_rcpf(den);  // 1 ulp
-        fh = a * inv;
-        fhe = b * inv;
-      } else if (den > 0.f) {
-        fh = a / den;
-        fhe = b / den;
-      } else if (g.frac_limit) {
-        fh = 1.f / (1.f + expf(dthe - dth));
-        fhe = 1.f / (1.f + expf(dth - dthe));
-      } else {
-        fh = fhe = __int_as_float(0x7fc00000);
-      }
-    }
-    // 1 - exp(-(dth + dthe)) = omh + omhe - omh * omhe (exact identity)
-    const float eabs = dump ? nin[nu] : nin[nu] * fmaf(-omh, omhe, omh + omhe);
-    const float gh = fh * eabs * rh1;
-    const float ghe = fhe * eabs * rhe1;
-    gh_sum += gh;
-    ghe_sum += ghe;
-    if (last) {
-      *(float2*)(rec + 2 * nu) = make_float2(gh, ghe);
-      rec[2 * F + nu] = nin[nu];
-    }
-    trans[nu] *= eh * ehe;
-  }
   if (last) {
-    for (int k = 3 * F; k < cp.rec_len; ++k) rec[k] = 0.f;
+    store_photon_record((float*)cp.rec + (int64_t)slot * cp.rec_len, nin, F, dist, dump);
+    FRT_FOR_BINS(nu) {
+      const float dth = ah1 * g.sigHf[nu] * dist;
+      const float dthe = ahe1 * g.sigHef[nu] * dist;
+      float eh, ehe = 1.f, m;
+      frt_exp_expm1_neg(dth, eh, m);
+      if (dthe != 0.f) frt_exp_expm1_neg(dthe, ehe, m);
+      trans[nu] *= eh * ehe;
+    }
   } else {
+    const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
+    float gh_sum = 0.f, ghe_sum = 0.f;
+    FRT_FOR_BINS(nu) {
+      const float dth = ah1 * g.sigHf[nu] * dist;
+      const float dthe = ahe1 * g.sigHef[nu] * dist;
+      float gh, ghe, eh, ehe;
+      bin_deposit_mixed(g, dth, dthe, nin[nu], dump, rh1, rhe1, gh, ghe, eh, ehe);
+      gh_sum += gh;
+      ghe_sum += ghe;
+      trans[nu] *= eh * ehe;
+    }
     ((float2*)cp.pair)[slot] = make_float2(gh_sum, ghe_sum);
   }
   return dump;
@@ -593,7 +611,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
         const int slot = inv_perm[base + p];
         last_cell = path_cell[(int64_t)p * g.cap + i];
         if (LAST)
-          zero_record((float*)cp.rec + (int64_t)slot * cp.rec_len, cp.rec_len);
+          store_behind_dump((float*)cp.rec + (int64_t)slot * cp.rec_len, F);
         else
           ((float2*)cp.pair)[slot] = make_float2(0.f, 0.f);
       }
@@ -716,8 +734,11 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
   if (k < g.n3) {
     const int64_t n3 = g.n3;
     const int s0 = cstart[k], s1 = cend[k];
-    // sums over the cell's segments in cell-sorted (= ray) order
     const int F = g.nfreq;
+    // opacities this pass's deposit used (ray_tracing.cc:656-657,733-743)
+    const float ah1 = FIRST ? cf.f[C_NH1][k] : cf.f[C_AVG_NH1][k];
+    const float ahe1 = FIRST ? cf.f[C_NHE1][k] : cf.f[C_AVG_NHE1][k];
+    // sums over the cell's segments in cell-sorted (= ray) order
     double GH, GHe;
     double GHnu[FRT_MAX_FREQ], GHenu[FRT_MAX_FREQ], NGnu[FRT_MAX_FREQ];
     if (sizeof(ACC) == 8) {
@@ -725,23 +746,29 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       double x[FRT_MAX_FREQ], y[FRT_MAX_FREQ], z[FRT_MAX_FREQ];
       FRT_FOR_BINS(nu) x[nu] = y[nu] = z[nu] = 0.;
       for (int s = s0; s < s1; ++s) {
-        const int64_t c = s;
         if (!LAST) {
-          const double2 v = ((const double2*)cp.pair)[c];
+          const double2 v = ((const double2*)cp.pair)[s];
           a += v.x;
           b += v.y;
         } else {
-          const double* r = (const double*)cp.rec + c * cp.rec_len;
+          // per-bin deposits of the segment from its photon record
+          const double* r = (const double*)cp.rec + (int64_t)s * cp.rec_len;
+          const double d = r[F];
+          if (isinf(d)) continue;  // behind a dump: no deposit
+          const bool dump = d < 0.;
+          const double distd = dump ? -d : d;
           double gs = 0., gse = 0.;
-#pragma unroll
-          for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
-            if (nu >= F) break;
-            const double2 q = *(const double2*)(r + 2 * nu);
-            gs += q.x;
-            gse += q.y;
-            x[nu] += q.x;
-            y[nu] += q.y;
-            z[nu] += r[2 * F + nu];
+          FRT_FOR_BINS(nu) {
+            const double nin = r[nu];
+            const double dth = ((double)ah1 * g.sigH[nu]) * distd;
+            const double dthe = ((double)ahe1 * g.sigHe[nu]) * distd;
+            double gh, ghe;
+            bin_deposit_fp64(g, dth, dthe, nin, dump, ah1, ahe1, gh, ghe);
+            gs += gh;
+            gse += ghe;
+            x[nu] += gh;
+            y[nu] += ghe;
+            z[nu] += nin / FRT_PHOT_NORM;
           }
           a += gs;
           b += gse;
@@ -759,22 +786,28 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       float a = 0.f, ca = 0.f, b = 0.f, cb = 0.f;
       float x[FRT_MAX_FREQ], cx[FRT_MAX_FREQ], y[FRT_MAX_FREQ], cy[FRT_MAX_FREQ], z[FRT_MAX_FREQ];
       FRT_FOR_BINS(nu) x[nu] = cx[nu] = y[nu] = cy[nu] = z[nu] = 0.f;
+      const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
+      const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
       for (int s = s0; s < s1; ++s) {
-        const int64_t c = s;
         float va, vb;
         if (!LAST) {
-          const float2 v = ((const float2*)cp.pair)[c];
+          const float2 v = ((const float2*)cp.pair)[s];
           va = v.x;
           vb = v.y;
         } else {
-          const float* r = (const float*)cp.rec + c * cp.rec_len;
+          const float* r = (const float*)cp.rec + (int64_t)s * cp.rec_len;
+          const float d = r[F];
+          if (isinf(d)) continue;  // behind a dump: no deposit
+          const bool dump = d < 0.f;
+          const float dist = dump ? -d : d;
           va = 0.f;
           vb = 0.f;
-#pragma unroll
-          for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) {
-            if (nu >= F) break;
-            const float2 q = *(const float2*)(r + 2 * nu);
-            const float rh = q.x, rhe = q.y;
+          FRT_FOR_BINS(nu) {
+            const float nin = r[nu];
+            const float dth = ah1 * g.sigHf[nu] * dist;
+            const float dthe = ahe1 * g.sigHef[nu] * dist;
+            float rh, rhe, eh, ehe;
+            bin_deposit_mixed(g, dth, dthe, nin, dump, rh1, rhe1, rh, rhe, eh, ehe);
             va += rh;
             vb += rhe;
             float yx = rh - cx[nu];
@@ -785,7 +818,7 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
             float ty = y[nu] + yy;
             cy[nu] = (ty - y[nu]) - yy;
             y[nu] = ty;
-            z[nu] += r[2 * F + nu];
+            z[nu] += nin;
           }
         }
         float ya = va - ca;
@@ -809,7 +842,7 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
     const float nH = cf.f[C_NH][k];
     const float nHe = cf.f[C_NHE][k];
     const float clump = cf.f[C_CLUMP][k];
-    float rH2, rHe2, ah1, ahe1, dne;
+    float rH2, rHe2, dne;
     if (FIRST) {
       const float T = cf.f[C_TEMP][k];
       const double Tm = (double)frt_maxd(g.temp_min, T);
@@ -825,13 +858,9 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       cf.f[C_RECOMB_HE2][k] = rHe2;
       dne = -cf.f[C_NE][k];
       if (!LAST) cf.f[C_DNE_DT][k] = dne;
-      ah1 = cf.f[C_NH1][k];
-      ahe1 = cf.f[C_NHE1][k];
     } else {
       rH2 = cf.f[C_RECOMB_H2][k];
       rHe2 = cf.f[C_RECOMB_HE2][k];
-      ah1 = cf.f[C_AVG_NH1][k];
-      ahe1 = cf.f[C_AVG_NHE1][k];
       dne = cf.f[C_DNE_DT][k];
     }
     const double dtd = (double)g.dt;
@@ -1417,8 +1446,9 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
   ctx->inv_perm = nullptr;
   const int F = ctx->g.nfreq;
   const size_t acc = (ctx->cfg.precision == FRT_PREC_MIXED) ? 4 : 8;
-  // last-pass record: 3F values rounded up to 16 or 32 (whole cache lines)
-  const int rec_len = (3 * F <= 16) ? 16 : 32;
+  // last-pass photon record: F photon counts and the signed length, rounded
+  // up to an even count of values (8-byte alignment in the FP32 form)
+  const int rec_len = (F + 2) & ~1;
   const size_t per = acc * (size_t)(2 + rec_len);
   hipError_t e = hipSuccess;
   for (int b = 0; b < 2 && e == hipSuccess; ++b) {
@@ -1433,7 +1463,7 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
     return set_error(ctx, FRT_E_NOMEM, "segment buffers: " + std::string(hipGetErrorString(e)));
   }
   char* c = (char*)blk;
-  ctx->contrib.rec = c;                        // cap x rec_len values, 64/128-byte records
+  ctx->contrib.rec = c;                        // cap x rec_len values
   ctx->contrib.pair = c + acc * rec_len * cap;  // cap x 2 values
   ctx->contrib.rec_len = rec_len;
   ctx->contrib.perm = ctx->seg_val[1];
@@ -1503,13 +1533,14 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     }
     // algorithmic bytes of this deposit launch (model used for the roofline):
     // per ray: seg_cnt + seg_off (8) + nphot read (4F); per segment: path cell +
-    // dist (8) + slot (4) + avg_nh1/avg_nhe1 (8) + 2 deposits; last pass adds
-    // 3F per-bin deposits per segment and the ray write-back (4F + 4 + 1).
+    // dist (8) + slot (4) + avg_nh1/avg_nhe1 (8) + the 2-value deposit (first
+    // passes) or the (F+1)-value photon record (last pass); the last pass adds
+    // the ray write-back (4F + 4 + 1).
     {
       const double acc = mixed ? 4. : 8.;
-      double b = (double)n * 8. + (double)ctx->stats.rays_moved * (4. * F) +
-                 (double)ctx->nseg * (20. + 2. * acc);
-      if (last) b += (double)ctx->nseg * 3. * F * acc + (double)ctx->stats.rays_moved * (4. * F + 5.);
+      double b = (double)n * 8. + (double)ctx->stats.rays_moved * (4. * F) + (double)ctx->nseg * 20.;
+      b += (double)ctx->nseg * (last ? (F + 1.) : 2.) * acc;
+      if (last) b += (double)ctx->stats.rays_moved * (4. * F + 5.);
       ctx->deposit_bytes_last += b;
       ctx->deposit_launches_last += 1;
     }
