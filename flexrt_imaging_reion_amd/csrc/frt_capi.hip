@@ -99,29 +99,16 @@ hipError_t dalloc(T** p, size_t count) {
 }
 
 int alloc_rays(frt_ctx* ctx, RaySoA& b, int64_t cap, int nfreq) {
-  hipError_t e = hipSuccess;
-  if (e == hipSuccess) e = dalloc(&b.cell, cap);
-  if (e == hipSuccess) e = dalloc(&b.pixel, cap);
-  if (e == hipSuccess) e = dalloc(&b.order, cap);
-  if (e == hipSuccess) e = dalloc(&b.itheta, cap);
-  if (e == hipSuccess) e = dalloc(&b.xhat, cap);
-  if (e == hipSuccess) e = dalloc(&b.yhat, cap);
-  if (e == hipSuccess) e = dalloc(&b.zhat, cap);
-  if (e == hipSuccess) e = dalloc(&b.nphot, cap * nfreq);
-  if (e == hipSuccess) e = dalloc(&b.r, cap);
-  if (e == hipSuccess) e = dalloc(&b.dr, cap);
-  if (e == hipSuccess) e = dalloc(&b.x, cap);
-  if (e == hipSuccess) e = dalloc(&b.y, cap);
-  if (e == hipSuccess) e = dalloc(&b.z, cap);
-  if (e == hipSuccess) e = dalloc(&b.time, cap);
+  hipError_t e = dalloc(&b.pos, cap);
+  if (e == hipSuccess) e = dalloc(&b.dir, cap);
+  if (e == hipSuccess) e = dalloc(&b.meta, cap);
+  if (e == hipSuccess) e = dalloc(&b.phot, cap * frt_phot_groups(nfreq));
   if (e != hipSuccess) return set_error(ctx, FRT_E_NOMEM, std::string("ray buffer: ") + hipGetErrorString(e));
   return FRT_OK;
 }
 
 void free_rays(RaySoA& b) {
-  hipFree(b.cell); hipFree(b.pixel); hipFree(b.order); hipFree(b.itheta);
-  hipFree(b.xhat); hipFree(b.yhat); hipFree(b.zhat); hipFree(b.nphot);
-  hipFree(b.r); hipFree(b.dr); hipFree(b.x); hipFree(b.y); hipFree(b.z); hipFree(b.time);
+  hipFree(b.pos); hipFree(b.dir); hipFree(b.meta); hipFree(b.phot);
   memset(&b, 0, sizeof(b));
 }
 
@@ -670,20 +657,39 @@ int frt_download_rays(frt_ctx* ctx, frt_ray_view* v) {
   if ((rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync"))) return rc;
   if (n == 0) return FRT_OK;
   RaySoA& b = ctx->buf[ctx->cur];
-  hipError_t e = hipSuccess;
-#define CP(dst, src, sz) if (e == hipSuccess && (dst)) e = stream_copy(ctx, dst, src, (size_t)(sz) * n, hipMemcpyDeviceToHost)
-  CP(v->order, b.order, 2); CP(v->pixel, b.pixel, 4); CP(v->itheta, b.itheta, 2);
-  CP(v->xhat, b.xhat, 4); CP(v->yhat, b.yhat, 4); CP(v->zhat, b.zhat, 4);
-  CP(v->r, b.r, 4); CP(v->dr, b.dr, 4); CP(v->x, b.x, 4); CP(v->y, b.y, 4); CP(v->z, b.z, 4);
-  CP(v->time, b.time, 4); CP(v->cell, b.cell, 4);
-#undef CP
-  if (e == hipSuccess && v->nphot) {
-    const int F = ctx->g.nfreq;
-    std::vector<float> t((size_t)F * n);
-    for (int nu = 0; nu < F && e == hipSuccess; ++nu)
-      e = stream_copy(ctx, t.data() + (size_t)nu * n, b.nphot + (size_t)nu * ctx->g.cap, 4 * (size_t)n, hipMemcpyDeviceToHost);
-    for (int64_t i = 0; i < n; ++i)
-      for (int nu = 0; nu < F; ++nu) v->nphot[i * F + nu] = t[(size_t)nu * n + i];
+  const int F = ctx->g.nfreq;
+  const int G = frt_phot_groups(F);
+  std::vector<float4> pos(n), dir(n), phot((size_t)G * n);
+  std::vector<int4> meta(n);
+  hipError_t e = stream_copy(ctx, pos.data(), b.pos, 16 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = stream_copy(ctx, dir.data(), b.dir, 16 * (size_t)n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = stream_copy(ctx, meta.data(), b.meta, 16 * (size_t)n, hipMemcpyDeviceToHost);
+  for (int v = 0; v < G && e == hipSuccess; ++v)
+    e = stream_copy(ctx, phot.data() + (size_t)v * n, b.phot + (size_t)v * ctx->g.cap, 16 * (size_t)n,
+                    hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_check(ctx, e, "download rays");
+  for (int64_t i = 0; i < n; ++i) {
+    if (v->x) v->x[i] = pos[i].x;
+    if (v->y) v->y[i] = pos[i].y;
+    if (v->z) v->z[i] = pos[i].z;
+    if (v->time) v->time[i] = pos[i].w;
+    if (v->xhat) v->xhat[i] = dir[i].x;
+    if (v->yhat) v->yhat[i] = dir[i].y;
+    if (v->zhat) v->zhat[i] = dir[i].z;
+    if (v->r) v->r[i] = dir[i].w;
+    float dr;
+    memcpy(&dr, &meta[i].x, 4);
+    if (v->dr) v->dr[i] = dr;
+    if (v->cell) v->cell[i] = meta[i].y;
+    if (v->pixel) v->pixel[i] = meta[i].z;
+    if (v->order) v->order[i] = (int16_t)(meta[i].w & 0xffff);
+    if (v->itheta) v->itheta[i] = (uint16_t)((uint32_t)meta[i].w >> 16);
+    if (v->nphot)
+      for (int nu = 0; nu < F; ++nu) {
+        const float4& q = phot[(size_t)(nu >> 2) * n + i];
+        const int k = nu & 3;
+        v->nphot[i * F + nu] = k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w;
+      }
   }
   return hip_check(ctx, e, "download rays");
 }
@@ -704,20 +710,28 @@ int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* v) {
   ctx->cur = 0;
   RaySoA& b = ctx->buf[0];
   hipError_t e = hipSuccess;
-#define CP(dst, src, sz) if (e == hipSuccess) e = stream_copy(ctx, dst, src, (size_t)(sz) * n, hipMemcpyHostToDevice)
   if (n > 0) {
-    CP(b.order, v->order, 2); CP(b.pixel, v->pixel, 4); CP(b.itheta, v->itheta, 2);
-    CP(b.xhat, v->xhat, 4); CP(b.yhat, v->yhat, 4); CP(b.zhat, v->zhat, 4);
-    CP(b.r, v->r, 4); CP(b.dr, v->dr, 4); CP(b.x, v->x, 4); CP(b.y, v->y, 4); CP(b.z, v->z, 4);
-    CP(b.time, v->time, 4); CP(b.cell, v->cell, 4);
     const int F = ctx->g.nfreq;
-    std::vector<float> t((size_t)F * n);
-    for (int64_t i = 0; i < n; ++i)
-      for (int nu = 0; nu < F; ++nu) t[(size_t)nu * n + i] = v->nphot[i * F + nu];
-    for (int nu = 0; nu < F && e == hipSuccess; ++nu)
-      e = stream_copy(ctx, b.nphot + (size_t)nu * ctx->g.cap, t.data() + (size_t)nu * n, 4 * (size_t)n, hipMemcpyHostToDevice);
+    const int G = frt_phot_groups(F);
+    std::vector<float4> pos(n), dir(n), phot((size_t)G * n);
+    std::vector<int4> meta(n);
+    for (int64_t i = 0; i < n; ++i) {
+      pos[i] = make_float4(v->x[i], v->y[i], v->z[i], v->time[i]);
+      dir[i] = make_float4(v->xhat[i], v->yhat[i], v->zhat[i], v->r[i]);
+      int dr;
+      memcpy(&dr, &v->dr[i], 4);
+      meta[i] = make_int4(dr, v->cell[i], v->pixel[i], (v->order[i] & 0xffff) | ((int)v->itheta[i] << 16));
+      float q[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      for (int nu = 0; nu < F; ++nu) q[nu] = v->nphot[i * F + nu];
+      for (int g = 0; g < G; ++g) phot[(size_t)g * n + i] = make_float4(q[4 * g], q[4 * g + 1], q[4 * g + 2], q[4 * g + 3]);
+    }
+    e = stream_copy(ctx, b.pos, pos.data(), 16 * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = stream_copy(ctx, b.dir, dir.data(), 16 * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = stream_copy(ctx, b.meta, meta.data(), 16 * (size_t)n, hipMemcpyHostToDevice);
+    for (int g = 0; g < G && e == hipSuccess; ++g)
+      e = stream_copy(ctx, b.phot + (size_t)g * ctx->g.cap, phot.data() + (size_t)g * n, 16 * (size_t)n,
+                      hipMemcpyHostToDevice);
   }
-#undef CP
   ctx->nrays = n;
   ctx->have_alive = false;
   ctx->have_segments = false;

@@ -17,23 +17,23 @@
 #define FRT_PDF_BINS 10000
 #define FRT_MAX_LEVEL 16
 
-// Ray buffer in structure-of-arrays form; nphot is bin-major [nfreq][cap].
+// Ray buffer: each ray is four 16-byte vectors, each vector in its own array
+// indexed by ray, so a kernel that needs a ray's geometry reads whole 16-byte
+// words and a gather touches one sector per vector instead of one per field.
+//   pos  = {x, y, z, time}
+//   dir  = {xhat, yhat, zhat, r}
+//   meta = {dr (float bits), cell, pixel, order | itheta << 16}
+//   phot = photon counts in groups of four bins: phot[v * cap + i] holds bins
+//          4v..4v+3 of ray i (unused lanes are zero).
 struct RaySoA {
-  int32_t* cell;
-  int32_t* pixel;
-  int16_t* order;
-  uint16_t* itheta;
-  float* xhat;
-  float* yhat;
-  float* zhat;
-  float* nphot;
-  float* r;
-  float* dr;
-  float* x;
-  float* y;
-  float* z;
-  float* time;
+  float4* pos;
+  float4* dir;
+  int4* meta;
+  float4* phot;
 };
+
+// Number of 4-bin photon groups for nfreq bins.
+static inline __host__ __device__ int frt_phot_groups(int nfreq) { return (nfreq + 3) >> 2; }
 
 // Scalars every kernel may need, passed by value.
 struct GridConst {

@@ -64,11 +64,73 @@ __device__ __forceinline__ void rotate3(const float* m, float x, float y, float 
   *oz = m[6] * x + m[7] * y + m[8] * z;
 }
 
+// Loop over the frequency bins, unrolled to the compile-time maximum so that
+// per-bin register arrays are indexed statically (no scratch).
+#define FRT_FOR_BINS(nu) \
+  _Pragma("unroll") for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) if (nu < F)
+
+static_assert(FRT_MAX_FREQ <= 8, "two 4-bin photon groups at most");
+
+// One ray's geometry and identity in registers (pos, dir, meta vectors).
+struct Ray {
+  float x, y, z, time, xhat, yhat, zhat, r, dr;
+  int cell, pixel, order, itheta;
+};
+
+__device__ __forceinline__ void ray_load(const RaySoA& b, int64_t i, Ray& r) {
+  const float4 p = b.pos[i];
+  const float4 d = b.dir[i];
+  const int4 m = b.meta[i];
+  r.x = p.x;
+  r.y = p.y;
+  r.z = p.z;
+  r.time = p.w;
+  r.xhat = d.x;
+  r.yhat = d.y;
+  r.zhat = d.z;
+  r.r = d.w;
+  r.dr = __int_as_float(m.x);
+  r.cell = m.y;
+  r.pixel = m.z;
+  r.order = m.w & 0xffff;
+  r.itheta = (int)((uint32_t)m.w >> 16);
+}
+
+__device__ __forceinline__ void ray_store(const RaySoA& b, int64_t i, const Ray& r) {
+  b.pos[i] = make_float4(r.x, r.y, r.z, r.time);
+  b.dir[i] = make_float4(r.xhat, r.yhat, r.zhat, r.r);
+  b.meta[i] = make_int4(__float_as_int(r.dr), r.cell, r.pixel, (r.order & 0xffff) | (r.itheta << 16));
+}
+
+__device__ __forceinline__ int ray_cell(const RaySoA& b, int64_t i) { return b.meta[i].y; }
+__device__ __forceinline__ void ray_set_cell(const RaySoA& b, int64_t i, int cell) {
+  reinterpret_cast<int*>(b.meta + i)[1] = cell;
+}
+
+// Photon counts of ray i in n[0..7] (bins >= F are zero).
+__device__ __forceinline__ void phot_load(const RaySoA& b, int64_t i, int F, int64_t cap, float* n) {
+  const float* p0 = reinterpret_cast<const float*>(b.phot + i);
+  const float* p1 = reinterpret_cast<const float*>(b.phot + cap + i);
+  FRT_FOR_BINS(nu) n[nu] = (nu < 4) ? p0[nu] : p1[nu - 4];
+  for (int nu = F; nu < 8; ++nu) n[nu] = 0.f;
+}
+
+__device__ __forceinline__ void phot_store(const RaySoA& b, int64_t i, int F, int64_t cap, const float* n) {
+  b.phot[i] = make_float4(n[0], F > 1 ? n[1] : 0.f, F > 2 ? n[2] : 0.f, F > 3 ? n[3] : 0.f);
+  if (F > 4) b.phot[cap + i] = make_float4(n[4], F > 5 ? n[5] : 0.f, F > 6 ? n[6] : 0.f, F > 7 ? n[7] : 0.f);
+}
+
 // sum_nphot (io_funcs.cc:10-16): float sum over bins in bin order.
-__device__ __forceinline__ float sum_nphot(const float* nphot, int64_t cap, int nfreq, int64_t i) {
+__device__ __forceinline__ float phot_sum(const float* n, int F) {
   float s = 0.f;
-  for (int nu = 0; nu < nfreq; ++nu) s += nphot[nu * cap + i];
+  FRT_FOR_BINS(nu) s += n[nu];
   return s;
+}
+
+__device__ __forceinline__ float sum_nphot(const RaySoA& b, int64_t i, int F, int64_t cap) {
+  float n[8];
+  phot_load(b, i, F, cap, n);
+  return phot_sum(n, F);
 }
 
 // Corner correction after a crossing (ray_tracing.cc:1400-1417).
@@ -100,14 +162,16 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
     bool moved = false;
     alive[i] = 1;
     const int n = g.n;
-    const int ind = rb.cell[i];
+    Ray ry;
+    ray_load(rb, i, ry);
+    const int ind = ry.cell;
     const int ci = ind / (n * n);
     const int rem = ind - ci * n * n;
     const int cj = rem / n;
     const int ck = rem - cj * n;
-    const float xhat = rb.xhat[i], yhat = rb.yhat[i], zhat = rb.zhat[i];
-    float x0 = rb.x[i], y0 = rb.y[i], z0 = rb.z[i];
-    const float deltat = rb.time[i] + g.dt;  // ray[x].time += dt
+    const float xhat = ry.xhat, yhat = ry.yhat, zhat = ry.zhat;
+    float x0 = ry.x, y0 = ry.y, z0 = ry.z;
+    const float deltat = ry.time + g.dt;  // ray[x].time += dt
     float dist = g.clight * deltat;
     const float hi = g.nc * g.dx;                             // nc*dx[i] (float)
     const double lo = (1. - (double)g.nc) * (double)g.dx;     // (1. - nc)*dx[i] (double)
@@ -115,11 +179,11 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
     moved = (ax > hi) || (ay > hi) || (az > hi) || ((double)ax < lo) || ((double)ay < lo) ||
             ((double)az < lo);
     if (!moved) {
-      rb.time[i] = deltat;
+      rb.pos[i] = make_float4(ry.x, ry.y, ry.z, deltat);
       seg_cnt[i] = 0;
     } else {
-      float r0 = rb.r[i];
-      float dr = rb.dr[i];
+      float r0 = ry.r;
+      float dr = ry.dr;
       r0 += dist / g.sf;  // comoving radius
       dr -= dist / g.sf;
       int inew = ci, jnew = cj, knew = ck;
@@ -207,12 +271,13 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
       x0 = corner(x0, g.dx, g.min_dist);
       y0 = corner(y0, g.dx, g.min_dist);
       z0 = corner(z0, g.dx, g.min_dist);
-      rb.r[i] = r0;
-      rb.dr[i] = dr;
-      rb.x[i] = x0;
-      rb.y[i] = y0;
-      rb.z[i] = z0;
-      rb.time[i] = 0.f;
+      ry.r = r0;
+      ry.dr = dr;
+      ry.x = x0;
+      ry.y = y0;
+      ry.z = z0;
+      ry.time = 0.f;
+      ray_store(rb, i, ry);
     }
     nmoved += moved ? 1 : 0;
   }
@@ -264,11 +329,6 @@ __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict_
 // dump).  The cell kernel recomputes the per-bin deposits from the record
 // and its own opacities (the same expressions, so the same bits), which
 // keeps the per-bin Gamma and photon fields without storing 3 values per bin.
-
-// Loop over the frequency bins, unrolled to the compile-time maximum so that
-// per-bin register arrays are indexed statically (no scratch).
-#define FRT_FOR_BINS(nu) \
-  _Pragma("unroll") for (int nu = 0; nu < FRT_MAX_FREQ; ++nu) if (nu < F)
 
 // Photon record of a segment: nin[0..F-1], then the signed length.
 template <typename T>
@@ -449,8 +509,10 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
     if (c > 0) {
       const int F = g.nfreq;
       double nph[FRT_MAX_FREQ], tauh[FRT_MAX_FREQ], tauhe[FRT_MAX_FREQ];
+      float n0[8];
+      phot_load(rb, i, F, g.cap, n0);
       FRT_FOR_BINS(nu) {
-        nph[nu] = (double)rb.nphot[nu * g.cap + i] * FRT_PHOT_NORM;
+        nph[nu] = (double)n0[nu] * FRT_PHOT_NORM;
         tauh[nu] = 0.;
         tauhe[nu] = 0.;
       }
@@ -491,14 +553,16 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
       if (LAST) {
         // photon update and removal (ray_tracing.cc:923-935)
         float sum = 0.f;
+        float n1[8];
         FRT_FOR_BINS(nu) {
           const float v = (float)(nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]) / FRT_PHOT_NORM);
-          rb.nphot[nu * g.cap + i] = v;
+          n1[nu] = v;
           sum += v;
         }
+        phot_store(rb, i, F, g.cap, n1);
         dumped_flag = dumped;
         if ((double)sum > g.nphot_min) {
-          rb.cell[i] = last_cell;
+          ray_set_cell(rb, i, last_cell);
         } else {
           alive[i] = 0;
           removed = true;
@@ -576,11 +640,9 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
     const int c = seg_cnt[i];
     if (c > 0) {
       const int F = g.nfreq;
-      float nph[FRT_MAX_FREQ], trans[FRT_MAX_FREQ];
-      FRT_FOR_BINS(nu) {
-        nph[nu] = rb.nphot[nu * g.cap + i];  // in units of phot_norm
-        trans[nu] = 1.f;                     // exp(-tau) carried multiplicatively
-      }
+      float nph[8], trans[FRT_MAX_FREQ];
+      phot_load(rb, i, F, g.cap, nph);  // in units of phot_norm
+      FRT_FOR_BINS(nu) trans[nu] = 1.f;  // exp(-tau) carried multiplicatively
       const int base = seg_off[i];
       SegPipe sp;
       sp.i = i;
@@ -617,14 +679,16 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
       }
       if (LAST) {
         float sum = 0.f;
+        float n1[8];
         FRT_FOR_BINS(nu) {
           const float v = nph[nu] * trans[nu];
-          rb.nphot[nu * g.cap + i] = v;
+          n1[nu] = v;
           sum += v;
         }
+        phot_store(rb, i, F, g.cap, n1);
         dumped_flag = dumped;
         if ((double)sum > g.nphot_min) {
-          rb.cell[i] = last_cell;
+          ray_set_cell(rb, i, last_cell);
         } else {
           alive[i] = 0;
           removed = true;
@@ -1013,7 +1077,7 @@ __global__ void __launch_bounds__(BLK) k_pdf(RaySoA rb, int64_t nrays, GridConst
   __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
     if (!alive[i]) continue;
-    const float s = sum_nphot(rb.nphot, g.cap, g.nfreq, i);
+    const float s = sum_nphot(rb, i, g.nfreq, g.cap);
     // (int) maxd((log10(sum) - lg_min)/del, 0.): maxd narrows to float
     int ind = (int)frt_maxd((float)((dm_log10((double)s) - lg_min) / del), 0.f);
     if (ind > FRT_PDF_BINS - 1) ind = FRT_PDF_BINS - 1;
@@ -1041,14 +1105,15 @@ __global__ void __launch_bounds__(BLK) k_merge_flag(RaySoA rb, int64_t nrays, Gr
   }
   int c = 0;
   if (alive[i]) {
-    const float s = sum_nphot(rb.nphot, g.cap, g.nfreq, i);
+    const float s = sum_nphot(rb, i, g.nfreq, g.cap);
     if (s < g.nphot_merge) {
       c = 1;
+      const float4 d = rb.dir[i];
       float vx, vy, vz;
-      rotate3(inv.m, rb.xhat[i], rb.yhat[i], rb.zhat[i], &vx, &vy, &vz);  // get_unit_vector(.., 6)
+      rotate3(inv.m, d.x, d.y, d.z, &vx, &vy, &vz);  // get_unit_vector(.., 6)
       const int bin = hpx_vec2pix_nest(g.hpx_lvl, (double)vx, (double)vy, (double)vz);
       const int nb = 12 << (2 * g.hpx_lvl);
-      key[i] = (uint32_t)rb.cell[i] * (uint32_t)nb + (uint32_t)bin;
+      key[i] = (uint32_t)ray_cell(rb, i) * (uint32_t)nb + (uint32_t)bin;
     }
   }
   cand[i] = c;
@@ -1094,27 +1159,32 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, 
     int64_t e = s + 1;
     while (e < ncand && key[e] == key[s]) ++e;
     const int b = val[s];
-    float NP[FRT_MAX_FREQ];
-    FRT_FOR_BINS(nu) NP[nu] = rb.nphot[nu * cap + b];
-    const float nb = sum_nphot(rb.nphot, cap, F, b);
-    const float rrb = rb.r[b] / g.dx0;
+    Ray rbase;
+    ray_load(rb, b, rbase);
+    float NP[8];
+    phot_load(rb, b, F, cap, NP);
+    const float nb = phot_sum(NP, F);
+    const float rrb = rbase.r / g.dx0;
     float R = (float)((double)rrb * (double)rrb * (double)nb);
-    float DR = rb.dr[b] * nb;
-    float X = rb.x[b] * nb;
-    float Y = rb.y[b] * nb;
-    float Z = rb.z[b] * nb;
-    float T = rb.time[b] * nb;
+    float DR = rbase.dr * nb;
+    float X = rbase.x * nb;
+    float Y = rbase.y * nb;
+    float Z = rbase.z * nb;
+    float T = rbase.time * nb;
     for (int64_t j = s + 1; j < e; ++j) {
       const int m = val[j];
-      const float nm = sum_nphot(rb.nphot, cap, F, m);
-      FRT_FOR_BINS(nu) NP[nu] += rb.nphot[nu * cap + m];
-      const float rrm = rb.r[m] / g.dx0;
+      float nmv[8];
+      phot_load(rb, m, F, cap, nmv);
+      const float nm = phot_sum(nmv, F);
+      FRT_FOR_BINS(nu) NP[nu] += nmv[nu];
+      const float4 pm = rb.pos[m];
+      const float rrm = rb.dir[m].w / g.dx0;
       R = (float)((double)R + (double)rrm * (double)rrm * (double)nm);
-      DR += rb.dr[m] * nm;
-      X += rb.x[m] * nm;
-      Y += rb.y[m] * nm;
-      Z += rb.z[m] * nm;
-      T += rb.time[m] * nm;
+      DR += __int_as_float(rb.meta[m].x) * nm;
+      X += pm.x * nm;
+      Y += pm.y * nm;
+      Z += pm.z * nm;
+      T += pm.w * nm;
       alive[m] = 0;
       ++merged;
     }
@@ -1142,20 +1212,21 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, 
       hpx_pix2vec_nest(order, pixel, &vx, &vy, &vz);
       float ox, oy, oz;
       rotate3(fwd.m, (float)vx, (float)vy, (float)vz, &ox, &oy, &oz);
-      rb.order[b] = (int16_t)order;
-      rb.pixel[b] = pixel;
-      rb.itheta[b] = (uint16_t)itheta_m;
-      rb.xhat[b] = ox;
-      rb.yhat[b] = oy;
-      rb.zhat[b] = oz;
+      rbase.order = order;
+      rbase.pixel = pixel;
+      rbase.itheta = (int)itheta_m;
+      rbase.xhat = ox;
+      rbase.yhat = oy;
+      rbase.zhat = oz;
     }
-    FRT_FOR_BINS(nu) rb.nphot[nu * cap + b] = NP[nu];
-    rb.r[b] = R;
-    rb.dr[b] = DR;
-    rb.x[b] = X;
-    rb.y[b] = Y;
-    rb.z[b] = Z;
-    rb.time[b] = T;
+    phot_store(rb, b, F, cap, NP);
+    rbase.r = R;
+    rbase.dr = DR;
+    rbase.x = X;
+    rbase.y = Y;
+    rbase.z = Z;
+    rbase.time = T;
+    ray_store(rb, b, rbase);
   }
   block_add(&cnt->groups, head ? 1ull : 0ull);
   block_add(&cnt->merged_away, (unsigned long long)merged);
@@ -1171,10 +1242,10 @@ __global__ void __launch_bounds__(BLK) k_split_flag(RaySoA rb, int64_t nrays, Gr
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i <= nrays; i += (int64_t)gridDim.x * BLK) {
     int out = 0;
     if (i < nrays && alive[i]) {
-      const int l = rb.order[i];
-      const float r = rb.r[i];
+      const int l = rb.meta[i].w & 0xffff;
+      const float r = rb.dir[i].w;
       const bool split = (r > g.rmax_tab[l]) && (l < g.l_max) &&
-                         (sum_nphot(rb.nphot, g.cap, g.nfreq, i) > split_thresh);
+                         (sum_nphot(rb, i, g.nfreq, g.cap) > split_thresh);
       out = split ? 4 : 1;
       nsplit += split ? 1 : 0;
     }
@@ -1193,15 +1264,15 @@ struct ChildGeo {
   float x, y, z;
 };
 
-__device__ __forceinline__ ChildGeo child_geometry(const RaySoA& src, int64_t i, int ch, const GridConst& g,
+__device__ __forceinline__ ChildGeo child_geometry(const Ray& par, int ch, const GridConst& g,
                                                    const float* __restrict__ rot_fwd) {
-  const int l = src.order[i];
-  const int pn = src.pixel[i];
-  const float r0 = src.r[i];
-  const float* m = rot_fwd + 9 * (int64_t)src.itheta[i];
-  const double vox = (double)src.xhat[i], voy = (double)src.yhat[i], voz = (double)src.zhat[i];
+  const int l = par.order;
+  const int pn = par.pixel;
+  const float r0 = par.r;
+  const float* m = rot_fwd + 9 * (int64_t)par.itheta;
+  const double vox = (double)par.xhat, voy = (double)par.yhat, voz = (double)par.zhat;
   const int n = g.n;
-  const int ind = src.cell[i];
+  const int ind = par.cell;
   const int ic = ind / (n * n);
   const int rem = ind - ic * n * n;
   const int jc = rem / n;
@@ -1212,9 +1283,9 @@ __device__ __forceinline__ ChildGeo child_geometry(const RaySoA& src, int64_t i,
   double px, py, pz;
   hpx_pix2vec_nest(l + 1, c.pix, &px, &py, &pz);
   rotate3(m, (float)px, (float)py, (float)pz, &c.vx, &c.vy, &c.vz);
-  float xn = (float)((double)src.x[i] + ((double)c.vx - vox) * (double)rsf);
-  float yn = (float)((double)src.y[i] + ((double)c.vy - voy) * (double)rsf);
-  float zn = (float)((double)src.z[i] + ((double)c.vz - voz) * (double)rsf);
+  float xn = (float)((double)par.x + ((double)c.vx - vox) * (double)rsf);
+  float yn = (float)((double)par.y + ((double)c.vy - voy) * (double)rsf);
+  float zn = (float)((double)par.z + ((double)c.vz - voz) * (double)rsf);
   int icn = ic, jcn = jc, kcn = kc;
   // periodic cell walk (ray_tracing.cc:549-572); the guard only bounds non-finite input
   int w = 0;
@@ -1243,12 +1314,14 @@ __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, int64_t nrays, Gri
     const int no = n_out[i];
     const int o = out_off[i];
     if (no == 1) {
-      key[o] = (uint32_t)src.cell[i];
+      key[o] = (uint32_t)ray_cell(src, i);
       val[o] = o;
       src_of[o] = (uint32_t)i << 2;
     } else if (no == 4) {
+      Ray par;
+      ray_load(src, i, par);
       for (int ch = 0; ch < 4; ++ch) {
-        key[o + ch] = (uint32_t)child_geometry(src, i, ch, g, rot_fwd).cell;
+        key[o + ch] = (uint32_t)child_geometry(par, ch, g, rot_fwd).cell;
         val[o + ch] = o + ch;
         src_of[o + ch] = ((uint32_t)i << 2) | (uint32_t)ch;
       }
@@ -1264,45 +1337,44 @@ __global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, int64_t
                                                  const uint32_t* __restrict__ src_of,
                                                  const float* __restrict__ rot_fwd) {
   const int F = g.nfreq;
+  const int nph = frt_phot_groups(F);
   const int64_t cap = g.cap;
   for (int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x; q < total; q += (int64_t)gridDim.x * BLK) {
     const uint32_t packed = src_of[sorted_pos[q]];
     const int64_t i = (int64_t)(packed >> 2);
     const int ch = (int)(packed & 3u);
     if (n_out[i] == 1) {
-      dst.cell[q] = src.cell[i];
-      dst.pixel[q] = src.pixel[i];
-      dst.order[q] = src.order[i];
-      dst.itheta[q] = src.itheta[i];
-      dst.xhat[q] = src.xhat[i];
-      dst.yhat[q] = src.yhat[i];
-      dst.zhat[q] = src.zhat[i];
-      for (int nu = 0; nu < F; ++nu) dst.nphot[nu * cap + q] = src.nphot[nu * cap + i];
-      dst.r[q] = src.r[i];
-      dst.dr[q] = src.dr[i];
-      dst.x[q] = src.x[i];
-      dst.y[q] = src.y[i];
-      dst.z[q] = src.z[i];
-      dst.time[q] = src.time[i];
+      dst.pos[q] = src.pos[i];
+      dst.dir[q] = src.dir[i];
+      dst.meta[q] = src.meta[i];
+      dst.phot[q] = src.phot[i];
+      if (nph > 1) dst.phot[cap + q] = src.phot[cap + i];
       continue;
     }
-    const int l = src.order[i];
-    const float r0 = src.r[i];
-    const ChildGeo c = child_geometry(src, i, ch, g, rot_fwd);
-    dst.cell[q] = c.cell;
-    dst.pixel[q] = c.pix;
-    dst.order[q] = (int16_t)(l + 1);
-    dst.itheta[q] = src.itheta[i];
-    dst.xhat[q] = c.vx;
-    dst.yhat[q] = c.vy;
-    dst.zhat[q] = c.vz;
-    for (int nu = 0; nu < F; ++nu) dst.nphot[nu * cap + q] = (float)((double)src.nphot[nu * cap + i] / 4.);
-    dst.r[q] = r0;
-    dst.dr[q] = g.rmax_tab[l + 1] - r0;
-    dst.x[q] = c.x;
-    dst.y[q] = c.y;
-    dst.z[q] = c.z;
-    dst.time[q] = src.time[i];
+    Ray par;
+    ray_load(src, i, par);
+    const int l = par.order;
+    const float r0 = par.r;
+    const ChildGeo c = child_geometry(par, ch, g, rot_fwd);
+    Ray kid;
+    kid.cell = c.cell;
+    kid.pixel = c.pix;
+    kid.order = l + 1;
+    kid.itheta = par.itheta;
+    kid.xhat = c.vx;
+    kid.yhat = c.vy;
+    kid.zhat = c.vz;
+    kid.r = r0;
+    kid.dr = g.rmax_tab[l + 1] - r0;
+    kid.x = c.x;
+    kid.y = c.y;
+    kid.z = c.z;
+    kid.time = par.time;
+    ray_store(dst, q, kid);
+    float n[8];
+    phot_load(src, i, F, cap, n);
+    FRT_FOR_BINS(nu) n[nu] = (float)((double)n[nu] / 4.);
+    phot_store(dst, q, F, cap, n);
   }
 }
 
@@ -1327,22 +1399,26 @@ __global__ void __launch_bounds__(BLK) k_cast(RaySoA dst, int64_t base, int nsrc
   const double lum = src_lum[s];
   const double rdt = (double)src_ray_dt[s];
   const int64_t cap = g.cap;
-  for (int nu = 0; nu < g.nfreq; ++nu)
-    dst.nphot[nu * cap + q] = (float)((double)g.spectrum[nu] * lum * rdt / 12. / 1. / FRT_PHOT_NORM);
-  dst.cell[q] = src_cell[s];
-  dst.pixel[q] = pix;
-  dst.order[q] = 0;
-  dst.itheta[q] = (uint16_t)it;
-  dst.xhat[q] = vx;
-  dst.yhat[q] = vy;
-  dst.zhat[q] = vz;
-  dst.r[q] = 0.f;
-  dst.dr[q] = g.rmax_tab[0] - 0.f;
+  const int F = g.nfreq;
+  float n[8];
+  FRT_FOR_BINS(nu) n[nu] = (float)((double)g.spectrum[nu] * lum * rdt / 12. / 1. / FRT_PHOT_NORM);
+  phot_store(dst, q, F, cap, n);
+  Ray ry;
+  ry.cell = src_cell[s];
+  ry.pixel = pix;
+  ry.order = 0;
+  ry.itheta = (int)it;
+  ry.xhat = vx;
+  ry.yhat = vy;
+  ry.zhat = vz;
+  ry.r = 0.f;
+  ry.dr = g.rmax_tab[0] - 0.f;
   const float half = (float)((double)g.dx / 2.);
-  dst.x[q] = half;
-  dst.y[q] = half;
-  dst.z[q] = half;
-  dst.time[q] = 0.f;
+  ry.x = half;
+  ry.y = half;
+  ry.z = half;
+  ry.time = 0.f;
+  ray_store(dst, q, ry);
 }
 
 __global__ void __launch_bounds__(BLK) k_zero_ray_dt(float* __restrict__ ray_dt, int nsrc) {
