@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "frt_internal.h"
+#include "frt_order.h"
 
 namespace frt {
 
@@ -341,6 +342,11 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   }
   ctx->key_bits = 1;
   while ((1ll << ctx->key_bits) < n3) ctx->key_bits++;
+  {
+    const int64_t t = frt_tiles_per_side(c.n);
+    ctx->tile_bits = 1;
+    while ((1ll << ctx->tile_bits) < t * t * t) ctx->tile_bits++;
+  }
   ctx->t = 0.f;
   ctx->nphot_merge = 0.f;
   ctx->nphot_min = 0.;

@@ -150,7 +150,8 @@ struct frt_ctx {
   ContribPtrs contrib;
   int32_t* cell_start;
   int32_t* cell_end;
-  int key_bits;
+  int key_bits;   // bits of a cell index (segment sort)
+  int tile_bits;  // bits of a tile index (ray-list sort)
 
   // merge
   int64_t mcap;

@@ -1,10 +1,11 @@
 // frt_kernels.hip -- gfx950 kernels of the per-timestep RT core.
 //
 // Data layout in HBM
-//   rays    : structure of arrays, one ray per index, list order = the order
-//             in which the reference's serial slot scan would meet them
-//             (survivors keep their order, split children replace their
-//             parent, new casts append in source order);
+//   rays    : four 16-byte vectors per ray, each in its own array (see
+//             RaySoA); after every compaction the list is stably sorted by
+//             the 4x4x4 tile of each ray's cell (frt_order.h), split
+//             children in their parent's place; new casts append in source
+//             order;
 //   paths   : path_cell/path_dist [path_length][capacity], written once per
 //             step by the march and re-read by every C2Ray pass;
 //   segments: stable-sorted by cell once per step; every pass writes each
@@ -19,6 +20,7 @@
 
 #include "frt_internal.h"
 #include "frt_fexp.h"
+#include "frt_order.h"
 #include "frt_rng.h"
 
 namespace frt {
@@ -809,12 +811,29 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       double a = 0., b = 0.;
       double x[FRT_MAX_FREQ], y[FRT_MAX_FREQ], z[FRT_MAX_FREQ];
       FRT_FOR_BINS(nu) x[nu] = y[nu] = z[nu] = 0.;
-      for (int s = s0; s < s1; ++s) {
-        if (!LAST) {
-          const double2 v = ((const double2*)cp.pair)[s];
+      if (!LAST) {
+        // pair loads issued four at a time; the sums keep the slot order
+        const double2* P = (const double2*)cp.pair;
+        int s = s0;
+        for (; s + 4 <= s1; s += 4) {
+          const double2 v0 = P[s], v1 = P[s + 1], v2 = P[s + 2], v3 = P[s + 3];
+          a += v0.x;
+          b += v0.y;
+          a += v1.x;
+          b += v1.y;
+          a += v2.x;
+          b += v2.y;
+          a += v3.x;
+          b += v3.y;
+        }
+        for (; s < s1; ++s) {
+          const double2 v = P[s];
           a += v.x;
           b += v.y;
-        } else {
+        }
+      }
+      for (int s = s0; LAST && s < s1; ++s) {
+        {
           // per-bin deposits of the segment from its photon record
           const double* r = (const double*)cp.rec + (int64_t)s * cp.rec_len;
           const double d = r[F];
@@ -852,13 +871,37 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       FRT_FOR_BINS(nu) x[nu] = cx[nu] = y[nu] = cy[nu] = z[nu] = 0.f;
       const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
       const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
-      for (int s = s0; s < s1; ++s) {
+      // one compensated step of each running sum
+#define FRT_KAHAN2(VA, VB)      \
+  {                             \
+    float ya = (VA) - ca;       \
+    float ta = a + ya;          \
+    ca = (ta - a) - ya;         \
+    a = ta;                     \
+    float yb = (VB) - cb;       \
+    float tb = b + yb;          \
+    cb = (tb - b) - yb;         \
+    b = tb;                     \
+  }
+      if (!LAST) {
+        // pair loads issued four at a time; the sums keep the slot order
+        const float2* P = (const float2*)cp.pair;
+        int s = s0;
+        for (; s + 4 <= s1; s += 4) {
+          const float2 v0 = P[s], v1 = P[s + 1], v2 = P[s + 2], v3 = P[s + 3];
+          FRT_KAHAN2(v0.x, v0.y);
+          FRT_KAHAN2(v1.x, v1.y);
+          FRT_KAHAN2(v2.x, v2.y);
+          FRT_KAHAN2(v3.x, v3.y);
+        }
+        for (; s < s1; ++s) {
+          const float2 v = P[s];
+          FRT_KAHAN2(v.x, v.y);
+        }
+      }
+      for (int s = s0; LAST && s < s1; ++s) {
         float va, vb;
-        if (!LAST) {
-          const float2 v = ((const float2*)cp.pair)[s];
-          va = v.x;
-          vb = v.y;
-        } else {
+        {
           const float* r = (const float*)cp.rec + (int64_t)s * cp.rec_len;
           const float d = r[F];
           if (isinf(d)) continue;  // behind a dump: no deposit
@@ -885,15 +928,9 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
             z[nu] += nin;
           }
         }
-        float ya = va - ca;
-        float ta = a + ya;
-        ca = (ta - a) - ya;
-        a = ta;
-        float yb = vb - cb;
-        float tb = b + yb;
-        cb = (tb - b) - yb;
-        b = tb;
+        FRT_KAHAN2(va, vb);
       }
+#undef FRT_KAHAN2
       GH = (double)a;
       GHe = (double)b;
       FRT_FOR_BINS(nu) {
@@ -1302,8 +1339,9 @@ __device__ __forceinline__ ChildGeo child_geometry(const Ray& par, int ch, const
   return c;
 }
 
-// Cell of every output ray of the compaction (key of the stable cell sort);
-// values are the compacted positions, src records (input ray << 2 | child).
+// Tile of the cell of every output ray of the compaction (key of the stable
+// sort that fixes the list order); values are the compacted positions, src
+// records (input ray << 2 | child).
 __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, int64_t nrays, GridConst g,
                                                   const int32_t* __restrict__ n_out,
                                                   const int32_t* __restrict__ out_off,
@@ -1314,14 +1352,14 @@ __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, int64_t nrays, Gri
     const int no = n_out[i];
     const int o = out_off[i];
     if (no == 1) {
-      key[o] = (uint32_t)ray_cell(src, i);
+      key[o] = frt_tile_key(ray_cell(src, i), g.n);
       val[o] = o;
       src_of[o] = (uint32_t)i << 2;
     } else if (no == 4) {
       Ray par;
       ray_load(src, i, par);
       for (int ch = 0; ch < 4; ++ch) {
-        key[o + ch] = (uint32_t)child_geometry(par, ch, g, rot_fwd).cell;
+        key[o + ch] = frt_tile_key(child_geometry(par, ch, g, rot_fwd).cell, g.n);
         val[o + ch] = o + ch;
         src_of[o + ch] = ((uint32_t)i << 2) | (uint32_t)ch;
       }
@@ -1329,7 +1367,7 @@ __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, int64_t nrays, Gri
   }
 }
 
-// Compaction into the other buffer in stable cell order, as a gather: output q
+// Compaction into the other buffer in stable tile order, as a gather: output q
 // takes the ray (or split child) that sorted to position q.
 __global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, int64_t total, GridConst g,
                                                  const int32_t* __restrict__ n_out,
@@ -1747,10 +1785,10 @@ int launch_split_compact(frt_ctx* ctx, int64_t* n_after) {
                      "ray buffer full: " + std::to_string(total + (int64_t)ctx->nsrc * 12) +
                          " rays needed, capacity " + std::to_string(ctx->g.cap));
   if (total > 0) {
-    // stable sort of the output rays by cell: the list order of every step
+    // stable sort of the output rays by cell tile: the list order of every step
     FRT_LAUNCH(ctx, "out_keys", sgrid(n), k_out_keys, cur_buf(ctx), n, g, ctx->n_out, ctx->out_off,
                ctx->rot_fwd, ctx->m_key[0], ctx->m_val[0], ctx->cand_key_tmp);
-    rc = sort_pairs(ctx, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1], total, ctx->key_bits,
+    rc = sort_pairs(ctx, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1], total, ctx->tile_bits,
                     "sort_rays");
     if (rc) return rc;
     FRT_LAUNCH(ctx, "compact", sgrid(total), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], total, g,

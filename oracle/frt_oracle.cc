@@ -22,11 +22,13 @@
 //   main.cc         step order 117-198
 //
 // Deviations (each applied identically on the GPU, none changes the physics):
-//  * Ray buffer: an ordered list instead of 30 N^3 slots.  The order is the
-//    serial scan order: survivors keep their order, split children take
-//    their parent's place, merged rays stay at the first member's place,
-//    new casts append in source order.  The reference's order depends on
-//    its OpenMP thread schedule.
+//  * Ray buffer: an ordered list instead of 30 N^3 slots.  At every
+//    compaction (split_rays, and removal of dead rays) the list is stably
+//    sorted by the 4x4x4 tile of each ray's cell (frt_order.h): survivors,
+//    split children (in their parent's place) and merged rays (at the first
+//    member's place) keep their relative order within a tile; new casts
+//    append in source order.  The reference's order depends on its OpenMP
+//    thread schedule.
 //  * rand() % Ntheta is replaced by frt_itheta_hash (seed, pass, source).
 //  * Transcendentals go through frt_detmath.h (tested against glibc), so the
 //    GPU can reproduce this file bit for bit.
@@ -52,6 +54,7 @@
 
 #include "../flexrt_imaging_reion_amd/csrc/frt_detmath.h"
 #include "../flexrt_imaging_reion_amd/csrc/frt_fexp.h"
+#include "../flexrt_imaging_reion_amd/csrc/frt_order.h"
 #include "../flexrt_imaging_reion_amd/csrc/frt_healpix.h"
 #include "../flexrt_imaging_reion_amd/csrc/frt_rng.h"
 #include "../include/flexrt_gpu.h"
@@ -846,8 +849,11 @@ static void split_rays(State& s) {
     out.push_back(ray);
   }
   for (auto& ray : out) ray.active = true;  // set_to_true
-  // list order of the next step: stable by current cell
-  std::stable_sort(out.begin(), out.end(), [](const Ray& a, const Ray& b) { return a.path_ind[0] < b.path_ind[0]; });
+  // list order of the next step: stable by the tile of the current cell
+  const int N = s.N;
+  std::stable_sort(out.begin(), out.end(), [N](const Ray& a, const Ray& b) {
+    return frt_tile_key(a.path_ind[0], N) < frt_tile_key(b.path_ind[0], N);
+  });
   s.ray.swap(out);
   s.st.rays_split = splits;
   s.dirty = false;
@@ -980,14 +986,17 @@ static void new_rays(State& s) {
   s.st.rays_new = (int64_t)s.ray.size() - before;
 }
 
-// Compaction outside split_rays: drop removed rays and restore the cell order.
+// Compaction outside split_rays: drop removed rays and restore the tile order.
 static void drop_dead(State& s) {
   if (!s.dirty) return;
   std::vector<Ray> out;
   out.reserve(s.ray.size());
   for (auto& r : s.ray)
     if (r.tag) out.push_back(r);
-  std::stable_sort(out.begin(), out.end(), [](const Ray& a, const Ray& b) { return a.path_ind[0] < b.path_ind[0]; });
+  const int N = s.N;
+  std::stable_sort(out.begin(), out.end(), [N](const Ray& a, const Ray& b) {
+    return frt_tile_key(a.path_ind[0], N) < frt_tile_key(b.path_ind[0], N);
+  });
   s.ray.swap(out);
   s.dirty = false;
 }
