@@ -78,6 +78,30 @@ def test_fp64_steps_bit_exact(name, n, n_src, steps):
     assert not bad, {k: max_rel(fg[k], fo[k]) for k in bad}
 
 
+@pytest.mark.parametrize("rpc,steps", [(12, 6), (13, 14)])
+def test_fp64_merging_bit_exact(rpc, steps):
+    """Merge path (calc_nphot_merge, merge) exercised on a small grid by a low
+    rays-per-cell target: num_thresh = n^3 (rpc - 12), so rpc = 12 merges
+    from the first step and rpc = 13 from step 10 on."""
+    _print_case("fp64 merging lognormal n=16 src=64 rpc=%d steps=%d" % (rpc, steps))
+    wl = inputs.make_workload("lognormal64", n=16, n_src=64, steps=steps)
+    wl["cfg"]["ray_per_cell_max"] = rpc
+    g, o = make_pair(wl, precision=0)
+    merged = 0
+    for s in range(wl["steps"]):
+        sg = g.step()
+        so = o.step()
+        assert counts(sg) == counts(so), "step %d" % s
+        assert sg["nphot_merge"] == so["nphot_merge"]
+        merged += so["rays_merged_away"]
+    assert merged > 0, "the workload must exercise merging"
+    err = compare_rays(g.rays(), o.rays())
+    assert err is None, err
+    fg, fo = fields(g), fields(o)
+    bad = [k for k in STATE_FIELDS if not bits_equal(fg[k], fo[k])]
+    assert not bad, {k: max_rel(fg[k], fo[k]) for k in bad}
+
+
 def test_mixed_within_tolerance():
     """Mixed form against the FP64 checker.
 
