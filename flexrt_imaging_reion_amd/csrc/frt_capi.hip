@@ -298,6 +298,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   GridConst& g = ctx->g;
   memset(&g, 0, sizeof(g));
   g.n = c.n;
+  g.lay = frt_cell_layout(c.n);
   g.n3 = (int)n3;
   g.nfreq = c.nfreq;
   g.P = c.path_length;
@@ -474,10 +475,19 @@ int frt_set_sources(frt_ctx* ctx, int32_t nsrc, const int32_t* cell, const doubl
   hipError_t e = dalloc(&ctx->src_cell, nsrc);
   if (e == hipSuccess) e = dalloc(&ctx->src_lum, nsrc);
   if (e == hipSuccess) e = dalloc(&ctx->src_ray_dt, nsrc);
-  if (e == hipSuccess && nsrc > 0) e = stream_copy(ctx, ctx->src_cell, cell, 4 * (size_t)nsrc, hipMemcpyHostToDevice);
+  std::vector<int32_t> icell(cell, cell + nsrc);
+  for (auto& c : icell) c = frt_cell_from_rowmajor(ctx->g.lay, c);
+  if (e == hipSuccess && nsrc > 0) e = stream_copy(ctx, ctx->src_cell, icell.data(), 4 * (size_t)nsrc, hipMemcpyHostToDevice);
   if (e == hipSuccess && nsrc > 0) e = stream_copy(ctx, ctx->src_lum, lum, 8 * (size_t)nsrc, hipMemcpyHostToDevice);
   if (e == hipSuccess && nsrc > 0) e = stream_zero(ctx, ctx->src_ray_dt, 4 * (size_t)nsrc);
   return hip_check(ctx, e, "sources");
+}
+
+// Internal cell index of every row-major cell (the C ABI's numbering).
+static void cell_permutation(const frt_ctx* ctx, std::vector<int32_t>& perm) {
+  const int64_t n3 = ctx->g.n3;
+  perm.resize((size_t)n3);
+  for (int64_t r = 0; r < n3; ++r) perm[(size_t)r] = frt_cell_from_rowmajor(ctx->g.lay, (int)r);
 }
 
 int frt_upload_field(frt_ctx* ctx, int32_t field, const float* host) {
@@ -485,16 +495,14 @@ int frt_upload_field(frt_ctx* ctx, int32_t field, const float* host) {
   float* d = field_ptr(ctx, field);
   if (!d) return set_error(ctx, FRT_E_ARG, "unknown field");
   const size_t n3 = (size_t)ctx->g.n3;
-  hipError_t e;
-  if (field < C_NSCALAR) {
-    e = stream_copy(ctx, d, host, 4 * n3, hipMemcpyHostToDevice);
-  } else {
-    const int F = ctx->g.nfreq;
-    std::vector<float> t((size_t)F * n3);
-    for (size_t k = 0; k < n3; ++k)
-      for (int nu = 0; nu < F; ++nu) t[(size_t)nu * n3 + k] = host[k * F + nu];
-    e = stream_copy(ctx, d, t.data(), 4 * t.size(), hipMemcpyHostToDevice);
-  }
+  std::vector<int32_t> perm;
+  cell_permutation(ctx, perm);
+  const int F = field < C_NSCALAR ? 1 : ctx->g.nfreq;
+  // per-bin fields: [cell][nfreq] row-major in, [nfreq][cell] internal on the device
+  std::vector<float> t((size_t)F * n3);
+  for (size_t r = 0; r < n3; ++r)
+    for (int nu = 0; nu < F; ++nu) t[(size_t)nu * n3 + (size_t)perm[r]] = host[r * F + nu];
+  const hipError_t e = stream_copy(ctx, d, t.data(), 4 * t.size(), hipMemcpyHostToDevice);
   return hip_check(ctx, e, "upload field");
 }
 
@@ -505,16 +513,14 @@ int frt_download_field(frt_ctx* ctx, int32_t field, float* host) {
   const size_t n3 = (size_t)ctx->g.n3;
   int rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
   if (rc) return rc;
-  hipError_t e;
-  if (field < C_NSCALAR) {
-    e = stream_copy(ctx, host, d, 4 * n3, hipMemcpyDeviceToHost);
-  } else {
-    const int F = ctx->g.nfreq;
-    std::vector<float> t((size_t)F * n3);
-    e = stream_copy(ctx, t.data(), d, 4 * t.size(), hipMemcpyDeviceToHost);
-    for (size_t k = 0; k < n3; ++k)
-      for (int nu = 0; nu < F; ++nu) host[k * F + nu] = t[(size_t)nu * n3 + k];
-  }
+  std::vector<int32_t> perm;
+  cell_permutation(ctx, perm);
+  const int F = field < C_NSCALAR ? 1 : ctx->g.nfreq;
+  std::vector<float> t((size_t)F * n3);
+  const hipError_t e = stream_copy(ctx, t.data(), d, 4 * t.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    for (size_t r = 0; r < n3; ++r)
+      for (int nu = 0; nu < F; ++nu) host[r * F + nu] = t[(size_t)nu * n3 + (size_t)perm[r]];
   return hip_check(ctx, e, "download field");
 }
 
@@ -686,7 +692,7 @@ int frt_download_rays(frt_ctx* ctx, frt_ray_view* v) {
     float dr;
     memcpy(&dr, &meta[i].x, 4);
     if (v->dr) v->dr[i] = dr;
-    if (v->cell) v->cell[i] = meta[i].y;
+    if (v->cell) v->cell[i] = frt_cell_to_rowmajor(ctx->g.lay, meta[i].y);
     if (v->pixel) v->pixel[i] = meta[i].z;
     if (v->order) v->order[i] = (int16_t)(meta[i].w & 0xffff);
     if (v->itheta) v->itheta[i] = (uint16_t)((uint32_t)meta[i].w >> 16);
@@ -726,7 +732,8 @@ int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* v) {
       dir[i] = make_float4(v->xhat[i], v->yhat[i], v->zhat[i], v->r[i]);
       int dr;
       memcpy(&dr, &v->dr[i], 4);
-      meta[i] = make_int4(dr, v->cell[i], v->pixel[i], (v->order[i] & 0xffff) | ((int)v->itheta[i] << 16));
+      meta[i] = make_int4(dr, frt_cell_from_rowmajor(ctx->g.lay, v->cell[i]), v->pixel[i],
+                          (v->order[i] & 0xffff) | ((int)v->itheta[i] << 16));
       float q[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       for (int nu = 0; nu < F; ++nu) q[nu] = v->nphot[i * F + nu];
       for (int g = 0; g < G; ++g) phot[(size_t)g * n + i] = make_float4(q[4 * g], q[4 * g + 1], q[4 * g + 2], q[4 * g + 3]);
@@ -763,7 +770,7 @@ int frt_download_paths(frt_ctx* ctx, int64_t n, int32_t* ray_end, int32_t* cells
   for (int64_t i = 0; i < n; ++i)
     for (int p = 0; p < P; ++p) {
       const bool used = p < ray_end[i];
-      cells[i * P + p] = used ? c[(size_t)p * n + i] : -1;
+      cells[i * P + p] = used ? frt_cell_to_rowmajor(ctx->g.lay, c[(size_t)p * n + i]) : -1;
       dists[i * P + p] = used ? d[(size_t)p * n + i] : 0.f;
     }
   return FRT_OK;

@@ -11,6 +11,7 @@
 #include "frt_detmath.h"
 #include "frt_healpix.h"
 #include "frt_rates.h"
+#include "frt_order.h"
 
 #define FRT_NALPHA 35
 #define FRT_PHOT_NORM 1e60
@@ -38,6 +39,7 @@ static inline __host__ __device__ int frt_phot_groups(int nfreq) { return (nfreq
 // Scalars every kernel may need, passed by value.
 struct GridConst {
   int n;            // cells per side
+  FrtCellLayout lay;  // internal cell numbering (frt_order.h)
   int n3;           // n^3
   int nfreq;
   int P;            // path_length

@@ -13,7 +13,8 @@
 //             cell sums its run in ray order (no float atomics, so the sums
 //             do not depend on scheduling);
 //   cells   : one float array per reference field, [nfreq][n^3] for per-bin
-//             fields.
+//             fields, cells numbered tile by tile (frt_order.h) so that the
+//             cells a ray list tile touches are contiguous.
 // Every floating-point expression keeps the reference's float/double
 // promotions (noted per line); the library is built with -ffp-contract=off.
 #include <hipcub/hipcub.hpp>
@@ -166,11 +167,8 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
     const int n = g.n;
     Ray ry;
     ray_load(rb, i, ry);
-    const int ind = ry.cell;
-    const int ci = ind / (n * n);
-    const int rem = ind - ci * n * n;
-    const int cj = rem / n;
-    const int ck = rem - cj * n;
+    int ci, cj, ck;
+    frt_cell_ijk(g.lay, ry.cell, ci, cj, ck);
     const float xhat = ry.xhat, yhat = ry.yhat, zhat = ry.zhat;
     float x0 = ry.x, y0 = ry.y, z0 = ry.z;
     const float deltat = ry.time + g.dt;  // ray[x].time += dt
@@ -211,7 +209,7 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
         if ((tlast + tmin) < deltat) {
           dist = tmin * clgt;
           if (p < g.P) {
-            path_cell[(int64_t)p * g.cap + i] = inew * n * n + jnew * n + knew;
+            path_cell[(int64_t)p * g.cap + i] = frt_cell_index(g.lay, inew, jnew, knew);
             path_dist[(int64_t)p * g.cap + i] = dist;
           } else {
             overflow = true;
@@ -261,7 +259,7 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
       dist = (deltat - tlast + tmin) * g.clight;
       dist = frt_maxd(dist, g.min_dist * g.dx);
       if (p < g.P) {
-        path_cell[(int64_t)p * g.cap + i] = inew * n * n + jnew * n + knew;
+        path_cell[(int64_t)p * g.cap + i] = frt_cell_index(g.lay, inew, jnew, knew);
         path_dist[(int64_t)p * g.cap + i] = dist;
       } else {
         overflow = true;
@@ -1309,11 +1307,8 @@ __device__ __forceinline__ ChildGeo child_geometry(const Ray& par, int ch, const
   const float* m = rot_fwd + 9 * (int64_t)par.itheta;
   const double vox = (double)par.xhat, voy = (double)par.yhat, voz = (double)par.zhat;
   const int n = g.n;
-  const int ind = par.cell;
-  const int ic = ind / (n * n);
-  const int rem = ind - ic * n * n;
-  const int jc = rem / n;
-  const int kc = rem - jc * n;
+  int ic, jc, kc;
+  frt_cell_ijk(g.lay, par.cell, ic, jc, kc);
   const float rsf = r0 * g.sf;
   ChildGeo c;
   c.pix = 4 * pn + ch;
@@ -1332,7 +1327,7 @@ __device__ __forceinline__ ChildGeo child_geometry(const Ray& par, int ch, const
   while ((double)yn < 0. && w < 4096) { yn += g.dx; jcn = wrap_mod(jcn - 1, n); ++w; }
   while (zn > g.dx && w < 4096) { zn -= g.dx; kcn = wrap_mod(kcn + 1, n); ++w; }
   while ((double)zn < 0. && w < 4096) { zn += g.dx; kcn = wrap_mod(kcn - 1, n); ++w; }
-  c.cell = icn * n * n + jcn * n + kcn;
+  c.cell = frt_cell_index(g.lay, icn, jcn, kcn);
   c.x = xn;
   c.y = yn;
   c.z = zn;
@@ -1352,14 +1347,14 @@ __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, int64_t nrays, Gri
     const int no = n_out[i];
     const int o = out_off[i];
     if (no == 1) {
-      key[o] = frt_tile_key(ray_cell(src, i), g.n);
+      key[o] = frt_tile_key_internal(g.lay, ray_cell(src, i));
       val[o] = o;
       src_of[o] = (uint32_t)i << 2;
     } else if (no == 4) {
       Ray par;
       ray_load(src, i, par);
       for (int ch = 0; ch < 4; ++ch) {
-        key[o + ch] = frt_tile_key(child_geometry(par, ch, g, rot_fwd).cell, g.n);
+        key[o + ch] = frt_tile_key_internal(g.lay, child_geometry(par, ch, g, rot_fwd).cell);
         val[o + ch] = o + ch;
         src_of[o + ch] = ((uint32_t)i << 2) | (uint32_t)ch;
       }

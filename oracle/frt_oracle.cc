@@ -1273,6 +1273,10 @@ int orc_download_paths(void* h, int64_t n, int* ray_end, int* cells, float* dist
 double orc_dm_exp(double x) { return dm_exp(x); }
 // The mixed-precision deposit's exp/expm1 helper (frt_fexp.h), compiled for
 // the host so the tests can check it against double-precision exp/expm1.
+// The GPU library's internal cell numbering (frt_order.h), for the tests.
+int orc_cell_index(int n, int i, int j, int k) { return frt_cell_index(frt_cell_layout(n), i, j, k); }
+void orc_cell_ijk(int n, int c, int* ijk) { frt_cell_ijk(frt_cell_layout(n), c, ijk[0], ijk[1], ijk[2]); }
+unsigned orc_tile_key(int n, int rowmajor_cell) { return frt_tile_key(rowmajor_cell, n); }
 void orc_fexp_neg(const float* d, float* e, float* m, int64_t n) {
   for (int64_t i = 0; i < n; ++i) frt_exp_expm1_neg(d[i], e[i], m[i]);
 }

@@ -43,6 +43,9 @@ _PROTOS = {
     "orc_download_paths": (C.c_int, [_VP, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float)]),
     "orc_dm_exp": (C.c_double, [C.c_double]),
+    "orc_cell_index": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_cell_ijk": (None, [C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "orc_tile_key": (C.c_uint, [C.c_int, C.c_int]),
     "orc_fexp_neg": (None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64]),
     "orc_dm_log": (C.c_double, [C.c_double]),
     "orc_dm_pow": (C.c_double, [C.c_double, C.c_double]),
