@@ -307,14 +307,19 @@ __global__ void __launch_bounds__(BLK) k_fill_i32(int32_t* a, int64_t n, int32_t
   if (i < n) a[i] = v;
 }
 
-__global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict__ key, int64_t nseg,
+// Per-cell slot ranges of the sorted segments, and the slot of every segment
+// (the inverse of the sort permutation).
+__global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict__ key,
+                                                     const int32_t* __restrict__ val, int64_t nseg,
                                                      int32_t* __restrict__ start,
-                                                     int32_t* __restrict__ end) {
+                                                     int32_t* __restrict__ end,
+                                                     int32_t* __restrict__ inv) {
   const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (s >= nseg) return;
   const uint32_t k = key[s];
   if (s == 0 || key[s - 1] != k) start[k] = (int32_t)s;
   if (s == nseg - 1 || key[s + 1] != k) end[k] = (int32_t)(s + 1);
+  inv[val[s]] = (int32_t)s;
 }
 
 
@@ -704,12 +709,6 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
   }
 }
 
-// Inverse of the cell sort of segments: slot of each segment.
-__global__ void __launch_bounds__(BLK) k_inv_perm(const int32_t* __restrict__ val, int64_t nseg,
-                                                  int32_t* __restrict__ inv) {
-  for (int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * BLK)
-    inv[val[s]] = (int32_t)s;
-}
 
 // ---------------------------------------------------------------------------
 // update_thermal for one cell (gas_funcs.cc:362-419 with set_Treion_default
@@ -830,11 +829,26 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
           b += v.y;
         }
       }
+      // photon records: the next slot's record is requested before the
+      // current one is processed
+      const double* R = (const double*)cp.rec;
+      double qn[FRT_MAX_FREQ], dn = 0.;
+      if (LAST && s0 < s1) {
+        const double* r = R + (int64_t)s0 * cp.rec_len;
+        FRT_FOR_BINS(nu) qn[nu] = r[nu];
+        dn = r[F];
+      }
       for (int s = s0; LAST && s < s1; ++s) {
         {
           // per-bin deposits of the segment from its photon record
-          const double* r = (const double*)cp.rec + (int64_t)s * cp.rec_len;
-          const double d = r[F];
+          double r[FRT_MAX_FREQ];
+          FRT_FOR_BINS(nu) r[nu] = qn[nu];
+          const double d = dn;
+          if (s + 1 < s1) {
+            const double* rn = R + (int64_t)(s + 1) * cp.rec_len;
+            FRT_FOR_BINS(nu) qn[nu] = rn[nu];
+            dn = rn[F];
+          }
           if (isinf(d)) continue;  // behind a dump: no deposit
           const bool dump = d < 0.;
           const double distd = dump ? -d : d;
@@ -897,18 +911,33 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
           FRT_KAHAN2(v.x, v.y);
         }
       }
+      // photon records: the next slot's record is requested before the
+      // current one is processed
+      const float* R = (const float*)cp.rec;
+      float qn[FRT_MAX_FREQ], dn = 0.f;
+      if (LAST && s0 < s1) {
+        const float* r = R + (int64_t)s0 * cp.rec_len;
+        FRT_FOR_BINS(nu) qn[nu] = r[nu];
+        dn = r[F];
+      }
       for (int s = s0; LAST && s < s1; ++s) {
         float va, vb;
         {
-          const float* r = (const float*)cp.rec + (int64_t)s * cp.rec_len;
-          const float d = r[F];
+          float q[FRT_MAX_FREQ];
+          FRT_FOR_BINS(nu) q[nu] = qn[nu];
+          const float d = dn;
+          if (s + 1 < s1) {
+            const float* r = R + (int64_t)(s + 1) * cp.rec_len;
+            FRT_FOR_BINS(nu) qn[nu] = r[nu];
+            dn = r[F];
+          }
           if (isinf(d)) continue;  // behind a dump: no deposit
           const bool dump = d < 0.f;
           const float dist = dump ? -d : d;
           va = 0.f;
           vb = 0.f;
           FRT_FOR_BINS(nu) {
-            const float nin = r[nu];
+            const float nin = q[nu];
             const float dth = ah1 * g.sigHf[nu] * dist;
             const float dthe = ahe1 * g.sigHef[nu] * dist;
             float rh, rhe, eh, ehe;
@@ -1102,7 +1131,7 @@ __global__ void __launch_bounds__(BLK) k_count_alive(const uint8_t* __restrict__
 // Photon-count histogram of calc_nphot_merge: privatized per block in LDS,
 // non-empty bins flushed with one global atomic each (integer sums: the
 // result does not depend on order).
-static constexpr unsigned PDF_GRID = 256;
+static constexpr unsigned PDF_GRID = 1024;
 
 __global__ void __launch_bounds__(BLK) k_pdf(RaySoA rb, int64_t nrays, GridConst g,
                                              const uint8_t* __restrict__ alive, double lg_min,
@@ -1601,9 +1630,8 @@ int launch_segments(frt_ctx* ctx) {
   rc = sort_pairs(ctx, ctx->seg_key[0], ctx->seg_key[1], ctx->seg_val[0], ctx->seg_val[1], nseg,
                   ctx->key_bits, "sort_segments");
   if (rc) return rc;
-  FRT_LAUNCH(ctx, "cell_ranges", grid_for(nseg), k_cell_ranges, ctx->seg_key[1], nseg,
-             ctx->cell_start, ctx->cell_end);
-  FRT_LAUNCH(ctx, "inv_perm", sgrid(nseg), k_inv_perm, ctx->seg_val[1], nseg, ctx->inv_perm);
+  FRT_LAUNCH(ctx, "cell_ranges", grid_for(nseg), k_cell_ranges, ctx->seg_key[1], ctx->seg_val[1], nseg,
+             ctx->cell_start, ctx->cell_end, ctx->inv_perm);
   return FRT_OK;
 }
 
