@@ -116,7 +116,7 @@ struct ContribPtrs {
   void* pair;          // [slot] {H, He} deposit summed over bins (2 values)
   void* rec;           // [slot] last pass: photons in per bin, then the signed length
   const int32_t* perm; // cell-sorted position -> segment index
-  int rec_len;         // values per record (nfreq + 1 rounded up to even)
+  int rec_len;         // values per record (nfreq + 1 rounded up to a multiple of 4)
 };
 
 struct frt_ctx {

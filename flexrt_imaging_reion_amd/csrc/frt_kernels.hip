@@ -335,17 +335,45 @@ __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict_
 // and its own opacities (the same expressions, so the same bits), which
 // keeps the per-bin Gamma and photon fields without storing 3 values per bin.
 
-// Photon record of a segment: nin[0..F-1], then the signed length.
+// Photon record of a segment: nin[0..F-1], the signed length at F, zeros up
+// to the record length (a multiple of 4 values), written as 16-byte stores.
+#define FRT_MAX_REC 12  // record values at most: FRT_MAX_FREQ + 1 rounded up to 4
+static_assert(FRT_MAX_FREQ + 1 <= FRT_MAX_REC, "record too short");
+
 template <typename T>
-__device__ __forceinline__ void store_photon_record(T* rec, const T* nin, int F, float dist, bool dump) {
-  FRT_FOR_BINS(nu) rec[nu] = nin[nu];
-  rec[F] = (T)(dump ? -dist : dist);
+__device__ __forceinline__ T rec_value(const T* nin, int F, T last, int k) {
+  if (k < FRT_MAX_FREQ && k < F) return nin[k < FRT_MAX_FREQ ? k : 0];
+  return (k == F) ? last : (T)0;
+}
+
+__device__ __forceinline__ void store_record_values(float* rec, const float* nin, int F, float last, int rec_len) {
+#pragma unroll
+  for (int k = 0; k < FRT_MAX_REC; k += 4) {
+    if (k >= rec_len) break;
+    *(float4*)(rec + k) = make_float4(rec_value(nin, F, last, k), rec_value(nin, F, last, k + 1),
+                                      rec_value(nin, F, last, k + 2), rec_value(nin, F, last, k + 3));
+  }
+}
+
+__device__ __forceinline__ void store_record_values(double* rec, const double* nin, int F, double last,
+                                                    int rec_len) {
+#pragma unroll
+  for (int k = 0; k < FRT_MAX_REC; k += 2) {
+    if (k >= rec_len) break;
+    *(double2*)(rec + k) = make_double2(rec_value(nin, F, last, k), rec_value(nin, F, last, k + 1));
+  }
 }
 
 template <typename T>
-__device__ __forceinline__ void store_behind_dump(T* rec, int F) {
-  FRT_FOR_BINS(nu) rec[nu] = (T)0;
-  rec[F] = (T)INFINITY;
+__device__ __forceinline__ void store_photon_record(T* rec, const T* nin, int F, float dist, bool dump,
+                                                    int rec_len) {
+  store_record_values(rec, nin, F, (T)(dump ? -dist : dist), rec_len);
+}
+
+template <typename T>
+__device__ __forceinline__ void store_behind_dump(T* rec, int F, int rec_len) {
+  const T zero[FRT_MAX_FREQ] = {};
+  store_record_values(rec, zero, F, (T)INFINITY, rec_len);
 }
 
 // Per-bin deposit of one segment, FP64 form (ray_tracing.cc:764-840):
@@ -476,7 +504,7 @@ __device__ __forceinline__ bool deposit_segment_fp64(const GridConst& g, Contrib
   }
   const bool dump = tot < g.nphot_min;
   if (last) {
-    store_photon_record((double*)cp.rec + (int64_t)slot * cp.rec_len, nin, F, dist, dump);
+    store_photon_record((double*)cp.rec + (int64_t)slot * cp.rec_len, nin, F, dist, dump, cp.rec_len);
   } else {
     double gh_sum = 0., ghe_sum = 0.;
     FRT_FOR_BINS(nu) {
@@ -551,7 +579,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
         const int slot = inv_perm[base + p];
         last_cell = path_cell[(int64_t)p * g.cap + i];
         if (LAST)
-          store_behind_dump((double*)cp.rec + (int64_t)slot * cp.rec_len, F);
+          store_behind_dump((double*)cp.rec + (int64_t)slot * cp.rec_len, F, cp.rec_len);
         else
           ((double2*)cp.pair)[slot] = make_double2(0., 0.);
       }
@@ -597,7 +625,7 @@ __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, Contri
   }
   const bool dump = tot < nmin;
   if (last) {
-    store_photon_record((float*)cp.rec + (int64_t)slot * cp.rec_len, nin, F, dist, dump);
+    store_photon_record((float*)cp.rec + (int64_t)slot * cp.rec_len, nin, F, dist, dump, cp.rec_len);
     FRT_FOR_BINS(nu) {
       const float dth = ah1 * g.sigHf[nu] * dist;
       const float dthe = ahe1 * g.sigHef[nu] * dist;
@@ -678,7 +706,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
         const int slot = inv_perm[base + p];
         last_cell = path_cell[(int64_t)p * g.cap + i];
         if (LAST)
-          store_behind_dump((float*)cp.rec + (int64_t)slot * cp.rec_len, F);
+          store_behind_dump((float*)cp.rec + (int64_t)slot * cp.rec_len, F, cp.rec_len);
         else
           ((float2*)cp.pair)[slot] = make_float2(0.f, 0.f);
       }
@@ -1585,8 +1613,8 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
   const int F = ctx->g.nfreq;
   const size_t acc = (ctx->cfg.precision == FRT_PREC_MIXED) ? 4 : 8;
   // last-pass photon record: F photon counts and the signed length, rounded
-  // up to an even count of values (8-byte alignment in the FP32 form)
-  const int rec_len = (F + 2) & ~1;
+  // up to a multiple of 4 values (whole 16-byte stores)
+  const int rec_len = (F + 4) & ~3;
   const size_t per = acc * (size_t)(2 + rec_len);
   hipError_t e = hipSuccess;
   for (int b = 0; b < 2 && e == hipSuccess; ++b) {
