@@ -412,6 +412,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = set_error(ctx, FRT_E_HIP, std::string("init: ") + hipGetErrorString(e));
   }
+  if (rc == FRT_OK) rc = reserve_workspace(ctx);
   if (rc != FRT_OK) {
     std::string msg = ctx->err;
     frt_destroy(ctx);

@@ -31,9 +31,21 @@ FRT_FEXP_HD inline float frt_pow2f(int k) {
 //   q = expm1(r) by its Taylor series through r^7 (truncation below 2e-8
 //   relative for |r| <= 0.35);
 //   exp(-d) = 2^k (1 + q),  expm1(-d) = 2^k q + (2^k - 1)  (one fma).
+// Below d = 1/32 (most segments of an ionized region) the reduction is
+// skipped: 1 - exp(-d) = d (1 - d/2 + d^2/6 - d^3/24 + d^4/120) with the
+// next term below 1.3e-9 relative, and exp(-d) = 1 - that.
 // d = 0 gives e = 1, m = 0 exactly.  Above d = 87 exp(-d) leaves the normal
 // range: e = 0 and m = -1 are returned.  NaN propagates.
 FRT_FEXP_HD inline void frt_exp_expm1_neg(float d, float& e, float& m) {
+  if (d < 0.03125f) {
+    float q = fmaf(d, 1.f / 120.f, -1.f / 24.f);
+    q = fmaf(q, d, 1.f / 6.f);
+    q = fmaf(q, d, -0.5f);
+    q = fmaf(q, d, 1.f);
+    m = fmaf(-d, q, 0.f);  // +0 at d = 0
+    e = 1.f + m;
+    return;
+  }
   if (d > 87.f) {
     e = 0.f;
     m = -1.f;

@@ -211,6 +211,7 @@ uint32_t itheta_hash(uint64_t seed, uint64_t stream, uint64_t counter);
 
 int launch_march(frt_ctx* ctx);
 int launch_segments(frt_ctx* ctx);
+int reserve_workspace(frt_ctx* ctx);
 int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal);
 int launch_thermal(frt_ctx* ctx);
 int launch_merge(frt_ctx* ctx);

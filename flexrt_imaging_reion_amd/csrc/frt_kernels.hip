@@ -59,6 +59,83 @@ __device__ __forceinline__ void block_add(unsigned long long* ctr, unsigned long
   __syncthreads();
 }
 
+// Ray order of the deposit kernels.  A wavefront runs as long as its longest
+// path.  The FP64 deposit is bound by FP64 issue, so each block takes CHUNK
+// consecutive rays at a time and reorders them by decreasing segment count
+// (LDS counting sort, one bucket per count): the lanes of a wavefront then
+// walk paths of (nearly) the same length, and rays without segments are
+// skipped.  The mixed deposit keeps the list order (for_rays), where
+// neighbouring lanes read neighbouring path entries; spreading a wavefront's
+// rays over a chunk costs it more in memory requests than it saves in idle
+// lanes.  The order changes no result: each ray writes only its own slots,
+// and the counters are sums.
+static constexpr int DEP_CHUNK_FP64 = 4096;
+static constexpr int DEP_BUCKETS = 32;  // counts >= 31 share the first bucket
+static inline unsigned chunk_grid(int64_t n, int chunk) {
+  const int64_t g = (n + chunk - 1) / chunk;
+  return (unsigned)(g < (int64_t)MAX_GRID ? g : (int64_t)MAX_GRID);
+}
+
+// Calls body(i, seg_cnt[i]) for every ray i < nrays with seg_cnt[i] > 0, in
+// list order (grid-stride).
+template <typename Body>
+__device__ __forceinline__ void for_rays(const int32_t* __restrict__ seg_cnt, int64_t nrays, Body&& body) {
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+    const int c = seg_cnt[i];
+    if (c > 0) body(i, c);
+  }
+}
+
+// Calls body(i, seg_cnt[i]) for every ray i < nrays with seg_cnt[i] > 0, in
+// decreasing segment count within each chunk.  Every thread of the block
+// must reach the call.
+template <int CHUNK, typename Body>
+__device__ __forceinline__ void for_rays_by_count(const int32_t* __restrict__ seg_cnt, int64_t nrays,
+                                                  Body&& body) {
+  static_assert(CHUNK % BLK == 0 && CHUNK <= 32768, "chunk of whole blocks, int16 local indices");
+  constexpr int PER = CHUNK / BLK;
+  __shared__ int s_cnt[DEP_BUCKETS];
+  __shared__ int s_off[DEP_BUCKETS];
+  __shared__ int16_t s_idx[CHUNK];
+  for (int64_t base = (int64_t)blockIdx.x * CHUNK; base < nrays; base += (int64_t)gridDim.x * CHUNK) {
+    if (threadIdx.x < DEP_BUCKETS) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int bkt[PER], rank[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int64_t i = base + k * BLK + threadIdx.x;
+      const int c = i < nrays ? seg_cnt[i] : 0;
+      bkt[k] = c <= 0 ? DEP_BUCKETS - 1 : (c >= DEP_BUCKETS - 1 ? 0 : DEP_BUCKETS - 1 - c);
+      rank[k] = atomicAdd(&s_cnt[bkt[k]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < DEP_BUCKETS) {  // exclusive scan of the bucket sizes
+      const int own = s_cnt[threadIdx.x];
+      int v = own;
+#pragma unroll
+      for (int off = 1; off < DEP_BUCKETS; off <<= 1) {
+        const int t = __shfl_up(v, off, DEP_BUCKETS);
+        if ((int)threadIdx.x >= off) v += t;
+      }
+      s_off[threadIdx.x] = v - own;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) s_idx[s_off[bkt[k]] + rank[k]] = (int16_t)(k * BLK + threadIdx.x);
+    __syncthreads();
+    const int nact = s_off[DEP_BUCKETS - 1];  // rays with segments come first
+#pragma unroll 1
+    for (int k = 0; k < PER; ++k) {
+      const int pos = k * BLK + threadIdx.x;
+      if (pos < nact) {
+        const int64_t i = base + s_idx[pos];
+        body(i, seg_cnt[i]);
+      }
+    }
+    __syncthreads();  // the LDS arrays are reused by the next chunk
+  }
+}
+
 // v' = M v in float with the reference's left-to-right sums (healpix_funcs.cc:112-133).
 __device__ __forceinline__ void rotate3(const float* m, float x, float y, float z, float* ox,
                                         float* oy, float* oz) {
@@ -536,10 +613,9 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
                                                       DevCounters* cnt) {
   unsigned long long ndumped = 0, nremoved = 0;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+  for_rays_by_count<DEP_CHUNK_FP64>(seg_cnt, nrays, [&](int64_t i, int c) {
     bool dumped_flag = false, removed = false;
-    const int c = seg_cnt[i];
-    if (c > 0) {
+    {
       const int F = g.nfreq;
       double nph[FRT_MAX_FREQ], tauh[FRT_MAX_FREQ], tauhe[FRT_MAX_FREQ];
       float n0[8];
@@ -604,7 +680,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
     }
     ndumped += dumped_flag ? 1 : 0;
     nremoved += removed ? 1 : 0;
-  }
+  });
   if (LAST) {
     block_add(&cnt->dumped, ndumped);
     block_add(&cnt->terminated, nremoved);
@@ -635,7 +711,8 @@ __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, Contri
       trans[nu] *= eh * ehe;
     }
   } else {
-    const float rh1 = inv_dtv / ah1, rhe1 = inv_dtv / ahe1;
+    // 1/(dt vol n) with the hardware reciprocal (1 ulp)
+    const float rh1 = inv_dtv * __builtin_amdgcn_rcpf(ah1), rhe1 = inv_dtv * __builtin_amdgcn_rcpf(ahe1);
     float gh_sum = 0.f, ghe_sum = 0.f;
     FRT_FOR_BINS(nu) {
       const float dth = ah1 * g.sigHf[nu] * dist;
@@ -668,10 +745,9 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
   // 1/(dt vol) in units of phot_norm photons
   const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
   const float nmin = (float)g.nphot_min;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+  for_rays(seg_cnt, nrays, [&](int64_t i, int c) {
     bool dumped_flag = false, removed = false;
-    const int c = seg_cnt[i];
-    if (c > 0) {
+    {
       const int F = g.nfreq;
       float nph[8], trans[FRT_MAX_FREQ];
       phot_load(rb, i, F, g.cap, nph);  // in units of phot_norm
@@ -730,7 +806,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
     }
     ndumped += dumped_flag ? 1 : 0;
     nremoved += removed ? 1 : 0;
-  }
+  });
   if (LAST) {
     block_add(&cnt->dumped, ndumped);
     block_add(&cnt->terminated, nremoved);
@@ -1637,6 +1713,32 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
   return FRT_OK;
 }
 
+// Up-front sizing of the segment buffers and of the scan/sort workspace, so
+// that the step loop does not reallocate (each reallocation waits for the
+// stream and costs milliseconds) while the ray and segment counts grow:
+// two segments per ray at full ray capacity, capped at 2^27 segments, and
+// the workspace of the largest scan and sorts.  Counts beyond that still
+// grow the buffers on demand.
+int reserve_workspace(frt_ctx* ctx) {
+  const int64_t cap = ctx->g.cap;
+  const int64_t segs = std::min<int64_t>(cap * std::min(ctx->cfg.path_length, 2), (int64_t)1 << 27);
+  int rc = grow_segments(ctx, segs);
+  if (rc) return rc;
+  size_t need = 0, b = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, ctx->seg_key[0], ctx->seg_key[1], ctx->seg_val[0],
+                                                    ctx->seg_val[1], (int)segs, 0, 32, ctx->stream);
+  if (e != hipSuccess) return hip_check(ctx, e, "workspace size");
+  need = std::max(need, b);
+  e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1],
+                                         (int)cap, 0, 32, ctx->stream);
+  if (e != hipSuccess) return hip_check(ctx, e, "workspace size");
+  need = std::max(need, b);
+  e = hipcub::DeviceScan::ExclusiveSum(nullptr, b, ctx->seg_cnt, ctx->seg_off, (int)(cap + 1), ctx->stream);
+  if (e != hipSuccess) return hip_check(ctx, e, "workspace size");
+  need = std::max(need, b);
+  return ensure_cub_tmp(ctx, need);
+}
+
 // Segment offsets, cell sort and cell ranges of the current paths.
 int launch_segments(frt_ctx* ctx) {
   const int64_t n = ctx->nrays;
@@ -1688,11 +1790,11 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
                    ctx->alive, ctx->dcnt);
     } else {
       if (last)
-        FRT_LAUNCH(ctx, "deposit_fp64", sgrid(n), k_deposit_fp64<true>, rb, n, g, ctx->path_cell,
+        FRT_LAUNCH(ctx, "deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64<true>, rb, n, g, ctx->path_cell,
                    ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
-        FRT_LAUNCH(ctx, "deposit_fp64", sgrid(n), k_deposit_fp64<false>, rb, n, g, ctx->path_cell,
+        FRT_LAUNCH(ctx, "deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64<false>, rb, n, g, ctx->path_cell,
                    ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     }

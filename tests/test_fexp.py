@@ -31,7 +31,7 @@ def samples():
     d = np.concatenate([
         10.0 ** rng.uniform(-12, 1.9, 200000),  # optical depths 1e-12 .. ~80
         rng.uniform(0, 1, 50000),
-        np.array([0.0, 1e-30, 1e-8, 0.34657, 0.34658, 0.5, 0.6931472, 1.0, 10.0, 50.0, 87.0]),
+        np.array([0.0, 1e-30, 1e-8, 0.0312499, 0.03125, 0.0312501, 0.34657, 0.34658, 0.5, 0.6931472, 1.0, 10.0, 50.0, 87.0]),
     ]).astype(np.float32)
     return d
 
