@@ -114,13 +114,12 @@ void free_rays(RaySoA& b) {
 }
 
 int read_counters(frt_ctx* ctx) {
-  int rc = hip_check(ctx, hipMemcpyAsync(ctx->hcnt, ctx->dcnt, sizeof(DevCounters), hipMemcpyDeviceToHost, ctx->stream), "counters");
-  if (rc) return rc;
-  return hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  return read_device_counters(ctx);
 }
 
 int reset_counters(frt_ctx* ctx) {
-  return hip_check(ctx, hipMemsetAsync(ctx->dcnt, 0, sizeof(DevCounters), ctx->stream), "reset counters");
+  return hip_check(ctx, hipMemsetAsync(ctx->dcnt, 0, FRT_CTR_COPIES * sizeof(DevCounters), ctx->stream),
+                   "reset counters");
 }
 
 // Rays held after removals: drop dead rays (no splits) so the buffer holds live rays only.
@@ -388,8 +387,8 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e == hipSuccess) e = dalloc(&ctx->cell_block, nf);
     if (e == hipSuccess) e = dalloc(&ctx->rot_fwd, 9 * FRT_NTHETA);
     if (e == hipSuccess) e = dalloc(&ctx->rot_inv, 9 * FRT_NTHETA);
-    if (e == hipSuccess) e = dalloc(&ctx->dcnt, 1);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hcnt, sizeof(DevCounters), hipHostMallocDefault);
+    if (e == hipSuccess) e = dalloc(&ctx->dcnt, FRT_CTR_COPIES);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hcnt, FRT_CTR_COPIES * sizeof(DevCounters), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hscalar, 64, hipHostMallocDefault);
     if (e != hipSuccess) rc = set_error(ctx, FRT_E_NOMEM, std::string("allocation: ") + hipGetErrorString(e));
   }
@@ -408,7 +407,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     build_rotation_tables(fwd.data(), inv.data());
     if (e == hipSuccess) e = stream_copy(ctx, ctx->rot_fwd, fwd.data(), 4 * fwd.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = stream_copy(ctx, ctx->rot_inv, inv.data(), 4 * inv.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemsetAsync(ctx->dcnt, 0, sizeof(DevCounters), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->dcnt, 0, FRT_CTR_COPIES * sizeof(DevCounters), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = set_error(ctx, FRT_E_HIP, std::string("init: ") + hipGetErrorString(e));
   }

@@ -89,6 +89,10 @@ struct DevCounters {
   unsigned int path_overflow;
   unsigned int pad;
 };
+// The device holds FRT_CTR_COPIES copies of the counters; a block adds into
+// copy blockIdx % FRT_CTR_COPIES, so a large grid does not serialize on one
+// atomic address.  The host sums the copies (read_device_counters).
+#define FRT_CTR_COPIES 16
 
 struct ProfRec {
   const char* name;
@@ -186,8 +190,8 @@ struct frt_ctx {
   int spectrum_set;
   frt_step_stats stats;
 
-  DevCounters* dcnt;       // device
-  DevCounters* hcnt;       // pinned host mirror
+  DevCounters* dcnt;       // device, FRT_CTR_COPIES copies
+  DevCounters* hcnt;       // pinned host: FRT_CTR_COPIES raw copies, their sum in hcnt[0]
   int64_t* hscalar;        // pinned host scratch
   void* cub_tmp;
   size_t cub_tmp_bytes;
@@ -212,6 +216,7 @@ uint32_t itheta_hash(uint64_t seed, uint64_t stream, uint64_t counter);
 int launch_march(frt_ctx* ctx);
 int launch_segments(frt_ctx* ctx);
 int reserve_workspace(frt_ctx* ctx);
+int read_device_counters(frt_ctx* ctx);
 int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal);
 int launch_thermal(frt_ctx* ctx);
 int launch_merge(frt_ctx* ctx);

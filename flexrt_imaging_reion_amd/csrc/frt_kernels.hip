@@ -35,16 +35,20 @@ __device__ __forceinline__ int wrap_mod(int a, int b) {
   return a % b;
 }
 
-// Grid-stride launches are capped at this many blocks, so a per-block counter
-// flush costs at most MAX_GRID atomics per kernel on one address.
+// Grid-stride launches of the light per-ray kernels are capped at this many
+// blocks.  The march, deposit and compaction kernels (long, latency-bound
+// threads) launch one thread per ray instead: the dispatcher then refills
+// the CUs with fresh wavefronts to the end, where a capped grid leaves a
+// partial last round of long-running wavefronts.
 static constexpr unsigned MAX_GRID = 4096;
 static inline unsigned sgrid(int64_t n) {
   const int64_t g = (n + BLK - 1) / BLK;
   return (unsigned)(g < (int64_t)MAX_GRID ? g : (int64_t)MAX_GRID);
 }
 
-// Add v summed over the block to *ctr with one atomic.  Every thread of the
-// block must reach the call.
+// Add v summed over the block to counter *ctr (a field of DevCounters copy 0)
+// with one atomic on the block's copy.  Every thread of the block must reach
+// the call.
 __device__ __forceinline__ void block_add(unsigned long long* ctr, unsigned long long v) {
   __shared__ unsigned long long part[BLK / 64];
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -54,7 +58,9 @@ __device__ __forceinline__ void block_add(unsigned long long* ctr, unsigned long
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
     for (int k = 0; k < BLK / 64; ++k) t += part[k];
-    if (t) atomicAdd(ctr, t);
+    constexpr unsigned stride = sizeof(DevCounters) / sizeof(unsigned long long);
+    static_assert(sizeof(DevCounters) % sizeof(unsigned long long) == 0, "whole-word counter copies");
+    if (t) atomicAdd(ctr + (blockIdx.x % FRT_CTR_COPIES) * stride, t);
   }
   __syncthreads();
 }
@@ -1627,7 +1633,7 @@ int launch_add_ray_dt(frt_ctx* ctx) {
 }
 
 int launch_march(frt_ctx* ctx) {
-  FRT_LAUNCH(ctx, "march", sgrid(ctx->nrays), k_march, cur_buf(ctx), ctx->nrays, ctx->g,
+  FRT_LAUNCH(ctx, "march", grid_for(ctx->nrays), k_march, cur_buf(ctx), ctx->nrays, ctx->g,
              ctx->path_cell, ctx->path_dist, ctx->seg_cnt, ctx->alive, ctx->dcnt);
   return FRT_OK;
 }
@@ -1713,6 +1719,28 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
   return FRT_OK;
 }
 
+// Copy the counter copies to the host and sum them into hcnt[0].
+int read_device_counters(frt_ctx* ctx) {
+  int rc = hip_check(ctx, hipMemcpyAsync(ctx->hcnt, ctx->dcnt, FRT_CTR_COPIES * sizeof(DevCounters),
+                                         hipMemcpyDeviceToHost, ctx->stream), "counters");
+  if (rc) return rc;
+  rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  if (rc) return rc;
+  DevCounters* h = ctx->hcnt;
+  for (int r = 1; r < FRT_CTR_COPIES; ++r) {
+    h[0].moved += h[r].moved;
+    h[0].dumped += h[r].dumped;
+    h[0].terminated += h[r].terminated;
+    h[0].alive += h[r].alive;
+    h[0].merged_away += h[r].merged_away;
+    h[0].groups += h[r].groups;
+    h[0].splits += h[r].splits;
+    h[0].substeps += h[r].substeps;
+    h[0].path_overflow |= h[r].path_overflow;
+  }
+  return FRT_OK;
+}
+
 // Up-front sizing of the segment buffers and of the scan/sort workspace, so
 // that the step loop does not reallocate (each reallocation waits for the
 // stream and costs milliseconds) while the ray and segment counts grow:
@@ -1781,11 +1809,11 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     const float* aHe = first ? ctx->c.f[C_NHE1] : ctx->c.f[C_AVG_NHE1];
     if (mixed) {
       if (last)
-        FRT_LAUNCH(ctx, "deposit_mixed", sgrid(n), k_deposit_mixed<true>, rb, n, g, ctx->path_cell,
+        FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<true>, rb, n, g, ctx->path_cell,
                    ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
-        FRT_LAUNCH(ctx, "deposit_mixed", sgrid(n), k_deposit_mixed<false>, rb, n, g, ctx->path_cell,
+        FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<false>, rb, n, g, ctx->path_cell,
                    ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     } else {
@@ -1853,9 +1881,7 @@ int launch_merge(frt_ctx* ctx) {
   int rc;
   // count_rays
   FRT_LAUNCH(ctx, "count_alive", sgrid(n), k_count_alive, ctx->alive, n, ctx->dcnt);
-  rc = hip_check(ctx, hipMemcpyAsync(ctx->hcnt, ctx->dcnt, sizeof(DevCounters), hipMemcpyDeviceToHost, ctx->stream), "counters");
-  if (rc) return rc;
-  rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+  rc = read_device_counters(ctx);
   if (rc) return rc;
   const int64_t count = (int64_t)ctx->hcnt->alive;
   const int64_t nb = 12ll << (2 * ctx->cfg.hpx_lvl);
@@ -1944,7 +1970,7 @@ int launch_split_compact(frt_ctx* ctx, int64_t* n_after) {
     rc = sort_pairs(ctx, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1], total, ctx->tile_bits,
                     "sort_rays");
     if (rc) return rc;
-    FRT_LAUNCH(ctx, "compact", sgrid(total), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], total, g,
+    FRT_LAUNCH(ctx, "compact", grid_for(total), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], total, g,
                ctx->n_out, ctx->m_val[1], ctx->cand_key_tmp, ctx->rot_fwd);
   }
   ctx->cur = 1 - ctx->cur;
