@@ -53,7 +53,7 @@ def parse():
 def kernel_bytes(name, wl, st, mixed):
     """Algorithmic HBM bytes of one launch of `name` in a step with stats `st`.
 
-    deposit: per ray 4 (seg count); per moved ray 4 (offset) + 4F (photons in);
+    deposit: per ray 4 (seg count); per moved ray 4F (photons in);
       per segment 4 (cell) + 4 (length) + 4 (sorted slot) + 8 (avg_nh1, avg_nhe1)
       + 2 acc (H and He deposits) in the first passes, (F+1) acc (photon
       record) in the last pass, which adds 4F + 5 per moved ray (photons out,
@@ -72,8 +72,8 @@ def kernel_bytes(name, wl, st, mixed):
     n3 = float(wl["cfg"]["n"]) ** 3
     nr, nm, ns = float(st["rays_start"]), float(st["rays_moved"]), float(st["segments"])
     if name.startswith("deposit"):
-        first = nr * 4 + nm * (4 + 4 * F) + ns * (20 + 2 * acc)
-        last = nr * 4 + nm * (4 + 4 * F) + ns * (20 + (F + 1) * acc) + nm * (4 * F + 5)
+        first = nr * 4 + nm * 4 * F + ns * (20 + 2 * acc)
+        last = nr * 4 + nm * 4 * F + ns * (20 + (F + 1) * acc) + nm * (4 * F + 5)
         return ((I - 1) * first + last) / I
     if name == "chem":
         first = n3 * 52 + ns * 2 * acc

@@ -368,6 +368,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   if (rc == FRT_OK) {
     if (e == hipSuccess) e = dalloc(&ctx->path_cell, (size_t)cap * P);
     if (e == hipSuccess) e = dalloc(&ctx->path_dist, (size_t)cap * P);
+    if (e == hipSuccess) e = dalloc(&ctx->path_slot, (size_t)cap * P);
     if (e == hipSuccess) e = dalloc(&ctx->seg_cnt, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->seg_off, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->alive, cap + 1);
@@ -432,7 +433,7 @@ int frt_destroy(frt_ctx* ctx) {
   for (int b = 0; b < 2; ++b) {
     hipFree(ctx->seg_key[b]); hipFree(ctx->seg_val[b]); hipFree(ctx->m_key[b]); hipFree(ctx->m_val[b]);
   }
-  hipFree(ctx->contrib.rec); hipFree(ctx->inv_perm); hipFree(ctx->pdf);  // rec is the block base
+  hipFree(ctx->contrib.rec); hipFree(ctx->path_slot); hipFree(ctx->pdf);  // rec is the block base
   hipFree(ctx->cell_start); hipFree(ctx->cell_end); hipFree(ctx->cell_block);
   hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
   hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt);

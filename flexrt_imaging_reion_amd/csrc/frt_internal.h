@@ -119,7 +119,6 @@ struct CellPtrs {
 struct ContribPtrs {
   void* pair;          // [slot] {H, He} deposit summed over bins (2 values)
   void* rec;           // [slot] last pass: photons in per bin, then the signed length
-  const int32_t* perm; // cell-sorted position -> segment index
   int rec_len;         // values per record (nfreq + 1 rounded up to a multiple of 4)
 };
 
@@ -152,7 +151,7 @@ struct frt_ctx {
   int64_t seg_cap;
   uint32_t* seg_key[2];
   int32_t* seg_val[2];
-  int32_t* inv_perm;   // segment index -> cell-sorted slot
+  int32_t* path_slot;  // [path_length][capacity]: cell-sorted slot of each path entry
   ContribPtrs contrib;
   int32_t* cell_start;
   int32_t* cell_end;

@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(BLK) k_seg_keys(int64_t nrays, int64_t cap,
   const int base = seg_off[i];
   for (int p = 0; p < c; ++p) {
     key[base + p] = (uint32_t)path_cell[(int64_t)p * cap + i];
-    val[base + p] = base + p;
+    val[base + p] = (int32_t)((int64_t)p * cap + i);  // path-array index (< cap * path_length < 2^31)
   }
 }
 
@@ -390,8 +390,8 @@ __global__ void __launch_bounds__(BLK) k_fill_i32(int32_t* a, int64_t n, int32_t
   if (i < n) a[i] = v;
 }
 
-// Per-cell slot ranges of the sorted segments, and the slot of every segment
-// (the inverse of the sort permutation).
+// Per-cell slot ranges of the sorted segments, and the slot of every path
+// entry (the inverse of the sort permutation, laid out like path_cell).
 __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict__ key,
                                                      const int32_t* __restrict__ val, int64_t nseg,
                                                      int32_t* __restrict__ start,
@@ -535,10 +535,10 @@ __device__ __forceinline__ void bin_deposit_mixed(const GridConst& g, float dth,
 // before segment p is computed, so the dependent loads overlap the math.
 struct SegPipe {
   int64_t i, stride;
-  int c, base;
+  int c;
   const int32_t* path_cell;
   const float* path_dist;
-  const int32_t* inv_perm;
+  const int32_t* path_slot;
   const float* avgH;
   const float* avgHe;
   // stage registers: segment p (ready), p+1 (path + opacity), p+2 (path)
@@ -548,7 +548,7 @@ struct SegPipe {
   __device__ __forceinline__ void load_path(int p, int& cell, float& dist, int& slot) const {
     cell = path_cell[(int64_t)p * stride + i];
     dist = path_dist[(int64_t)p * stride + i];
-    slot = inv_perm[base + p];
+    slot = path_slot[(int64_t)p * stride + i];
   }
   __device__ __forceinline__ void start() {
     load_path(0, cell0, dist0, slot0);
@@ -612,8 +612,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
                                                       const int32_t* __restrict__ path_cell,
                                                       const float* __restrict__ path_dist,
                                                       const int32_t* __restrict__ seg_cnt,
-                                                      const int32_t* __restrict__ seg_off,
-                                                      const int32_t* __restrict__ inv_perm,
+                                                      const int32_t* __restrict__ path_slot,
                                                       const float* __restrict__ avgH,
                                                       const float* __restrict__ avgHe,
                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
@@ -631,15 +630,13 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
         tauh[nu] = 0.;
         tauhe[nu] = 0.;
       }
-      const int base = seg_off[i];
       SegPipe sp;
       sp.i = i;
       sp.stride = g.cap;
       sp.c = c;
-      sp.base = base;
       sp.path_cell = path_cell;
       sp.path_dist = path_dist;
-      sp.inv_perm = inv_perm;
+      sp.path_slot = path_slot;
       sp.avgH = avgH;
       sp.avgHe = avgHe;
       sp.start();
@@ -658,7 +655,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
       }
       // segments behind a dump deposit nothing this pass
       for (; p < c; ++p) {
-        const int slot = inv_perm[base + p];
+        const int slot = path_slot[(int64_t)p * g.cap + i];
         last_cell = path_cell[(int64_t)p * g.cap + i];
         if (LAST)
           store_behind_dump((double*)cp.rec + (int64_t)slot * cp.rec_len, F, cp.rec_len);
@@ -741,8 +738,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
                                                        const int32_t* __restrict__ path_cell,
                                                        const float* __restrict__ path_dist,
                                                        const int32_t* __restrict__ seg_cnt,
-                                                       const int32_t* __restrict__ seg_off,
-                                                       const int32_t* __restrict__ inv_perm,
+                                                       const int32_t* __restrict__ path_slot,
                                                        const float* __restrict__ avgH,
                                                        const float* __restrict__ avgHe,
                                                        ContribPtrs cp, uint8_t* __restrict__ alive,
@@ -758,15 +754,13 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
       float nph[8], trans[FRT_MAX_FREQ];
       phot_load(rb, i, F, g.cap, nph);  // in units of phot_norm
       FRT_FOR_BINS(nu) trans[nu] = 1.f;  // exp(-tau) carried multiplicatively
-      const int base = seg_off[i];
       SegPipe sp;
       sp.i = i;
       sp.stride = g.cap;
       sp.c = c;
-      sp.base = base;
       sp.path_cell = path_cell;
       sp.path_dist = path_dist;
-      sp.inv_perm = inv_perm;
+      sp.path_slot = path_slot;
       sp.avgH = avgH;
       sp.avgHe = avgHe;
       sp.start();
@@ -785,7 +779,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
       }
       // segments behind a dump deposit nothing this pass
       for (; p < c; ++p) {
-        const int slot = inv_perm[base + p];
+        const int slot = path_slot[(int64_t)p * g.cap + i];
         last_cell = path_cell[(int64_t)p * g.cap + i];
         if (LAST)
           store_behind_dump((float*)cp.rec + (int64_t)slot * cp.rec_len, F, cp.rec_len);
@@ -1688,10 +1682,8 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
     ctx->seg_val[b] = nullptr;
   }
   hipFree(ctx->contrib.rec);  // base of the record + pair block
-  hipFree(ctx->inv_perm);
   ctx->contrib.pair = nullptr;
   ctx->contrib.rec = nullptr;
-  ctx->inv_perm = nullptr;
   const int F = ctx->g.nfreq;
   const size_t acc = (ctx->cfg.precision == FRT_PREC_MIXED) ? 4 : 8;
   // last-pass photon record: F photon counts and the signed length, rounded
@@ -1703,7 +1695,6 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
     e = hipMalloc((void**)&ctx->seg_key[b], (size_t)cap * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&ctx->seg_val[b], (size_t)cap * 4);
   }
-  if (e == hipSuccess) e = hipMalloc((void**)&ctx->inv_perm, (size_t)cap * 4);
   void* blk = nullptr;
   if (e == hipSuccess) e = hipMalloc(&blk, per * (size_t)cap);
   if (e != hipSuccess) {
@@ -1714,7 +1705,6 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
   ctx->contrib.rec = c;                        // cap x rec_len values
   ctx->contrib.pair = c + acc * rec_len * cap;  // cap x 2 values
   ctx->contrib.rec_len = rec_len;
-  ctx->contrib.perm = ctx->seg_val[1];
   ctx->seg_cap = cap;
   return FRT_OK;
 }
@@ -1789,7 +1779,7 @@ int launch_segments(frt_ctx* ctx) {
                   ctx->key_bits, "sort_segments");
   if (rc) return rc;
   FRT_LAUNCH(ctx, "cell_ranges", grid_for(nseg), k_cell_ranges, ctx->seg_key[1], ctx->seg_val[1], nseg,
-             ctx->cell_start, ctx->cell_end, ctx->inv_perm);
+             ctx->cell_start, ctx->cell_end, ctx->path_slot);
   return FRT_OK;
 }
 
@@ -1810,30 +1800,30 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     if (mixed) {
       if (last)
         FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
         FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     } else {
       if (last)
         FRT_LAUNCH(ctx, "deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
         FRT_LAUNCH(ctx, "deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->seg_off, ctx->inv_perm, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, aH, aHe, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     }
     // algorithmic bytes of this deposit launch (model used for the roofline):
-    // per ray: seg_cnt + seg_off (8) + nphot read (4F); per segment: path cell +
-    // dist (8) + slot (4) + avg_nh1/avg_nhe1 (8) + the 2-value deposit (first
+    // per ray: seg_cnt (4) + nphot read (4F); per segment: path cell + dist (8)
+    // + slot (4) + avg_nh1/avg_nhe1 (8) + the 2-value deposit (first
     // passes) or the (F+1)-value photon record (last pass); the last pass adds
     // the ray write-back (4F + 4 + 1).
     {
       const double acc = mixed ? 4. : 8.;
-      double b = (double)n * 8. + (double)ctx->stats.rays_moved * (4. * F) + (double)ctx->nseg * 20.;
+      double b = (double)n * 4. + (double)ctx->stats.rays_moved * (4. * F) + (double)ctx->nseg * 20.;
       b += (double)ctx->nseg * (last ? (F + 1.) : 2.) * acc;
       if (last) b += (double)ctx->stats.rays_moved * (4. * F + 5.);
       ctx->deposit_bytes_last += b;
