@@ -46,6 +46,19 @@ static inline unsigned sgrid(int64_t n) {
   return (unsigned)(g < (int64_t)MAX_GRID ? g : (int64_t)MAX_GRID);
 }
 
+// Block index remapped so that each XCD runs a contiguous eighth of the
+// grid.  Workgroups are dealt to the 8 XCDs round-robin in block order, so
+// without the remap neighbouring blocks, which handle neighbouring rays and
+// write neighbouring slots, sit in different L2s, and each L2 writes back
+// partial lines of the same region.  A permutation of [0, gridDim.x), so a
+// grid-stride loop over it still visits every index once.
+__device__ __forceinline__ int64_t xcd_block() {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned G8 = G & ~7u;
+  if (b >= G8) return b;
+  return (int64_t)(b & 7u) * (G8 >> 3) + (b >> 3);
+}
+
 // Add v summed over the block to counter *ctr (a field of DevCounters copy 0)
 // with one atomic on the block's copy.  Every thread of the block must reach
 // the call.
@@ -86,7 +99,7 @@ static inline unsigned chunk_grid(int64_t n, int chunk) {
 // list order (grid-stride).
 template <typename Body>
 __device__ __forceinline__ void for_rays(const int32_t* __restrict__ seg_cnt, int64_t nrays, Body&& body) {
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+  for (int64_t i = xcd_block() * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
     const int c = seg_cnt[i];
     if (c > 0) body(i, c);
   }
@@ -244,7 +257,7 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
                                                DevCounters* cnt) {
   unsigned long long nmoved = 0;
   bool overflow = false;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+  for (int64_t i = xcd_block() * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
     bool moved = false;
     alive[i] = 1;
     const int n = g.n;
@@ -397,7 +410,7 @@ __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict_
                                                      int32_t* __restrict__ start,
                                                      int32_t* __restrict__ end,
                                                      int32_t* __restrict__ inv) {
-  const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const int64_t s = xcd_block() * BLK + threadIdx.x;
   if (s >= nseg) return;
   const uint32_t k = key[s];
   if (s == 0 || key[s - 1] != k) start[k] = (int32_t)s;
