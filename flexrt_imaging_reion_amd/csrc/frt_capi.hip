@@ -386,6 +386,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e == hipSuccess) e = dalloc(&ctx->cell_end, n3);
     const size_t nf = (size_t)(C_NSCALAR + 3 * c.nfreq) * (size_t)n3;
     if (e == hipSuccess) e = dalloc(&ctx->cell_block, nf);
+    if (e == hipSuccess) e = dalloc(&ctx->c.avg2, (size_t)n3);
     if (e == hipSuccess) e = dalloc(&ctx->rot_fwd, 9 * FRT_NTHETA);
     if (e == hipSuccess) e = dalloc(&ctx->rot_inv, 9 * FRT_NTHETA);
     if (e == hipSuccess) e = dalloc(&ctx->dcnt, FRT_CTR_COPIES);
@@ -434,7 +435,7 @@ int frt_destroy(frt_ctx* ctx) {
     hipFree(ctx->seg_key[b]); hipFree(ctx->seg_val[b]); hipFree(ctx->m_key[b]); hipFree(ctx->m_val[b]);
   }
   hipFree(ctx->contrib.rec); hipFree(ctx->path_slot); hipFree(ctx->pdf);  // rec is the block base
-  hipFree(ctx->cell_start); hipFree(ctx->cell_end); hipFree(ctx->cell_block);
+  hipFree(ctx->cell_start); hipFree(ctx->cell_end); hipFree(ctx->cell_block); hipFree(ctx->c.avg2);
   hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
   hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt);
   if (ctx->hcnt) hipHostFree(ctx->hcnt);

@@ -112,6 +112,7 @@ struct CellPtrs {
   float* gamma_h1_nu;   // [nfreq][n3]
   float* gamma_he1_nu;  // [nfreq][n3]
   float* ngam_nu;       // [nfreq][n3]
+  float2* avg2;         // [n3] {avg_nh1, avg_nhe1} of the next deposit pass, one 8-byte gather
 };
 
 // Deposits of path segments, one slot per segment in cell-sorted order (a

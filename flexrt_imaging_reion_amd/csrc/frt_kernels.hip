@@ -398,6 +398,13 @@ __global__ void __launch_bounds__(BLK) k_seg_keys(int64_t nrays, int64_t cap,
   }
 }
 
+// {n_HI, n_HeI} packed for the first deposit pass of a step.
+__global__ void __launch_bounds__(BLK) k_pack_avg(const float* __restrict__ a, const float* __restrict__ b,
+                                                  float2* __restrict__ out, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (k < n) out[k] = make_float2(a[k], b[k]);
+}
+
 __global__ void __launch_bounds__(BLK) k_fill_i32(int32_t* a, int64_t n, int32_t v) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i < n) a[i] = v;
@@ -552,8 +559,7 @@ struct SegPipe {
   const int32_t* path_cell;
   const float* path_dist;
   const int32_t* path_slot;
-  const float* avgH;
-  const float* avgHe;
+  const float2* avg2;
   // stage registers: segment p (ready), p+1 (path + opacity), p+2 (path)
   int cell0, cell1, cell2, slot0, slot1, slot2;
   float dist0, dist1, dist2, ah0, ahe0, ah1, ahe1;
@@ -566,15 +572,17 @@ struct SegPipe {
   __device__ __forceinline__ void start() {
     load_path(0, cell0, dist0, slot0);
     if (c > 1) load_path(1, cell1, dist1, slot1);
-    ah0 = avgH[cell0];
-    ahe0 = avgHe[cell0];
+    const float2 a = avg2[cell0];
+    ah0 = a.x;
+    ahe0 = a.y;
   }
   // issue the loads of the next stages before segment p is computed
   __device__ __forceinline__ void prefetch(int p) {
     if (p + 2 < c) load_path(p + 2, cell2, dist2, slot2);
     if (p + 1 < c) {
-      ah1 = avgH[cell1];
-      ahe1 = avgHe[cell1];
+      const float2 a = avg2[cell1];
+      ah1 = a.x;
+      ahe1 = a.y;
     }
   }
   __device__ __forceinline__ void advance() {
@@ -626,8 +634,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
                                                       const float* __restrict__ path_dist,
                                                       const int32_t* __restrict__ seg_cnt,
                                                       const int32_t* __restrict__ path_slot,
-                                                      const float* __restrict__ avgH,
-                                                      const float* __restrict__ avgHe,
+                                                      const float2* __restrict__ avg2,
                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
                                                       DevCounters* cnt) {
   unsigned long long ndumped = 0, nremoved = 0;
@@ -650,8 +657,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
       sp.path_cell = path_cell;
       sp.path_dist = path_dist;
       sp.path_slot = path_slot;
-      sp.avgH = avgH;
-      sp.avgHe = avgHe;
+      sp.avg2 = avg2;
       sp.start();
       bool dumped = false;
       int p = 0;
@@ -752,8 +758,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
                                                        const float* __restrict__ path_dist,
                                                        const int32_t* __restrict__ seg_cnt,
                                                        const int32_t* __restrict__ path_slot,
-                                                       const float* __restrict__ avgH,
-                                                       const float* __restrict__ avgHe,
+                                                       const float2* __restrict__ avg2,
                                                        ContribPtrs cp, uint8_t* __restrict__ alive,
                                                        DevCounters* cnt) {
   unsigned long long ndumped = 0, nremoved = 0;
@@ -774,8 +779,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
       sp.path_cell = path_cell;
       sp.path_dist = path_dist;
       sp.path_slot = path_slot;
-      sp.avgH = avgH;
-      sp.avgHe = avgHe;
+      sp.avg2 = avg2;
       sp.start();
       bool dumped = false;
       int p = 0;
@@ -1124,6 +1128,7 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
     const float avg_he1 = (float)((double)nHe * (1. - (xeqhe + tihe / dtd * (x0he - xeqhe) * (1. - e_he))));
     cf.f[C_AVG_NH1][k] = avg_h1;
     cf.f[C_AVG_NHE1][k] = avg_he1;
+    cf.avg2[k] = make_float2(avg_h1, avg_he1);
 
     if (LAST) {
       const float nH1n = (float)((double)nH * (1. - (xeq + (x0 - xeq) * e_h)));
@@ -1808,25 +1813,25 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   ctx->deposit_launches_last = 0;
   for (int it = 0; it < I; ++it) {
     const bool first = (it == 0), last = (it == I - 1);
-    const float* aH = first ? ctx->c.f[C_NH1] : ctx->c.f[C_AVG_NH1];
-    const float* aHe = first ? ctx->c.f[C_NHE1] : ctx->c.f[C_AVG_NHE1];
+    if (first)  // the first pass attenuates with n_HI, n_HeI themselves
+      FRT_LAUNCH(ctx, "pack_avg", gc, k_pack_avg, ctx->c.f[C_NH1], ctx->c.f[C_NHE1], ctx->c.avg2, g.n3);
     if (mixed) {
       if (last)
         FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
         FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     } else {
       if (last)
         FRT_LAUNCH(ctx, "deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib,
                    ctx->alive, ctx->dcnt);
       else
         FRT_LAUNCH(ctx, "deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, aH, aHe, ctx->contrib,
+                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib,
                    ctx->alive, ctx->dcnt);
     }
     // algorithmic bytes of this deposit launch (model used for the roofline):
