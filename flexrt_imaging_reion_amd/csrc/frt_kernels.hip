@@ -479,6 +479,26 @@ __device__ __forceinline__ void store_behind_dump(T* rec, int F, int rec_len) {
   store_record_values(rec, zero, F, (T)INFINITY, rec_len);
 }
 
+// Photon record as registers: q[0..F-1] the photons in, d the signed length
+// (16-byte loads; F + 1 <= rec_len <= FRT_MAX_REC).
+__device__ __forceinline__ void load_photon_record(const float* rec, int F, int rec_len, float* q, float& d) {
+  float v[FRT_MAX_REC];
+#pragma unroll
+  for (int k = 0; k < FRT_MAX_REC; k += 4) {
+    float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (k < rec_len) t = *(const float4*)(rec + k);
+    v[k] = t.x;
+    v[k + 1] = t.y;
+    v[k + 2] = t.z;
+    v[k + 3] = t.w;
+  }
+  FRT_FOR_BINS(nu) q[nu] = v[nu];
+  d = 0.f;
+#pragma unroll
+  for (int k = 1; k < FRT_MAX_REC; ++k)
+    if (k == F) d = v[k];
+}
+
 // Per-bin deposit of one segment, FP64 form (ray_tracing.cc:764-840):
 // HI/HeI split, absorbed photons, Gamma contributions.
 __device__ __forceinline__ void bin_deposit_fp64(const GridConst& g, double dth, double dthe, double nin,
@@ -1036,22 +1056,14 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       // current one is processed
       const float* R = (const float*)cp.rec;
       float qn[FRT_MAX_FREQ], dn = 0.f;
-      if (LAST && s0 < s1) {
-        const float* r = R + (int64_t)s0 * cp.rec_len;
-        FRT_FOR_BINS(nu) qn[nu] = r[nu];
-        dn = r[F];
-      }
+      if (LAST && s0 < s1) load_photon_record(R + (int64_t)s0 * cp.rec_len, F, cp.rec_len, qn, dn);
       for (int s = s0; LAST && s < s1; ++s) {
         float va, vb;
         {
           float q[FRT_MAX_FREQ];
           FRT_FOR_BINS(nu) q[nu] = qn[nu];
           const float d = dn;
-          if (s + 1 < s1) {
-            const float* r = R + (int64_t)(s + 1) * cp.rec_len;
-            FRT_FOR_BINS(nu) qn[nu] = r[nu];
-            dn = r[F];
-          }
+          if (s + 1 < s1) load_photon_record(R + (int64_t)(s + 1) * cp.rec_len, F, cp.rec_len, qn, dn);
           if (isinf(d)) continue;  // behind a dump: no deposit
           const bool dump = d < 0.f;
           const float dist = dump ? -d : d;
