@@ -1499,29 +1499,41 @@ __device__ __forceinline__ ChildGeo child_geometry(const Ray& par, int ch, const
 
 // Tile of the cell of every output ray of the compaction (key of the stable
 // sort that fixes the list order); values are the compacted positions, src
-// records (input ray << 2 | child).
+// records (input ray << 2 | child).  One thread per input ray.  The four
+// children of the split rays of a wavefront are spread over its lanes (a
+// split is rare, and its child geometry costs hundreds of instructions), so
+// a wavefront with k splits computes 4k children in ceil(4k / 64) rounds
+// instead of four rounds per split lane.
 __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, int64_t nrays, GridConst g,
                                                   const int32_t* __restrict__ n_out,
                                                   const int32_t* __restrict__ out_off,
                                                   const float* __restrict__ rot_fwd,
                                                   uint32_t* __restrict__ key, int32_t* __restrict__ val,
                                                   uint32_t* __restrict__ src_of) {
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
-    const int no = n_out[i];
+  __shared__ int s_par[BLK];  // split rays of each wavefront, in lane order
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wbase = threadIdx.x & ~63;
+  const int no = i < nrays ? n_out[i] : 0;
+  if (no == 1) {
     const int o = out_off[i];
-    if (no == 1) {
-      key[o] = frt_tile_key_internal(g.lay, ray_cell(src, i));
-      val[o] = o;
-      src_of[o] = (uint32_t)i << 2;
-    } else if (no == 4) {
-      Ray par;
-      ray_load(src, i, par);
-      for (int ch = 0; ch < 4; ++ch) {
-        key[o + ch] = frt_tile_key_internal(g.lay, child_geometry(par, ch, g, rot_fwd).cell);
-        val[o + ch] = o + ch;
-        src_of[o + ch] = ((uint32_t)i << 2) | (uint32_t)ch;
-      }
-    }
+    key[o] = frt_tile_key_internal(g.lay, ray_cell(src, i));
+    val[o] = o;
+    src_of[o] = (uint32_t)i << 2;
+  }
+  const unsigned long long mask = __ballot(no == 4);
+  if (no == 4) s_par[wbase + __popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
+  __syncthreads();
+  const int nsplit = __popcll(mask);
+  for (int t = lane; t < 4 * nsplit; t += 64) {
+    const int ip = s_par[wbase + (t >> 2)];
+    const int ch = t & 3;
+    const int o = out_off[ip];
+    Ray par;
+    ray_load(src, ip, par);
+    key[o + ch] = frt_tile_key_internal(g.lay, child_geometry(par, ch, g, rot_fwd).cell);
+    val[o + ch] = o + ch;
+    src_of[o + ch] = ((uint32_t)ip << 2) | (uint32_t)ch;
   }
 }
 
@@ -1985,7 +1997,7 @@ int launch_split_compact(frt_ctx* ctx, int64_t* n_after) {
                          " rays needed, capacity " + std::to_string(ctx->g.cap));
   if (total > 0) {
     // stable sort of the output rays by cell tile: the list order of every step
-    FRT_LAUNCH(ctx, "out_keys", sgrid(n), k_out_keys, cur_buf(ctx), n, g, ctx->n_out, ctx->out_off,
+    FRT_LAUNCH(ctx, "out_keys", grid_for(n), k_out_keys, cur_buf(ctx), n, g, ctx->n_out, ctx->out_off,
                ctx->rot_fwd, ctx->m_key[0], ctx->m_val[0], ctx->cand_key_tmp);
     rc = sort_pairs(ctx, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1], total, ctx->tile_bits,
                     "sort_rays");
