@@ -7,7 +7,11 @@ bins, clumping C=3, 20 steps of 0.5 Myr.  One bench "step" is one RT timestep
 (march, 5 C2Ray passes, merge, split, thermal update, casting).
 
 The chosen (mixed) form is timed over --steps steps from the initial state
-after --warmup untimed steps; the FP64 form is timed the same way beside it
+after --warmup untimed steps, without instrumentation (`value`); the same
+steps are then run again from the initial state with HIP events around every
+launch, which give the per-kernel times of `roofline` (that pass's wall time
+is reported as ms_per_step_with_launch_events).  The FP64 form is timed the
+same way beside it
 and its final state is the reference for the mixed-form errors (the FP64 GPU
 path is bit-identical to the CPU checker, see tests/test_gpu_parity.py).
 The CPU checker runs a bounded sample of the same workload on one host core.
@@ -89,6 +93,10 @@ def log(msg):
 
 
 def run_form(wl, precision, steps, warmup, profile):
+    """Warm up, then time `steps` steps of the workload from its initial state
+    without instrumentation (the throughput), then, if `profile`, run the same
+    steps again from the initial state with every launch bracketed by HIP
+    events on the library's stream (per-kernel times for the roofline)."""
     import torch
 
     from flexrt_imaging_reion_amd import inputs
@@ -101,23 +109,32 @@ def run_form(wl, precision, steps, warmup, profile):
     log("precision %d: context ready, %d warmup steps" % (precision, warmup))
     for _ in range(warmup):
         g.step()
-    inputs.setup(g, wl)  # back to the initial state: time steps 1..K of the workload
-    g.synchronize()
-    g.set_profiling(profile)
-    g.reset_kernel_times()
-    stats = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        stats.append(g.step())
-    g.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+
+    def timed(prof):
+        inputs.setup(g, wl)  # back to the initial state: time steps 1..K of the workload
+        g.synchronize()
+        g.set_profiling(prof)
+        g.reset_kernel_times()
+        st = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st.append(g.step())
+        g.synchronize()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, st
+
+    wall, stats = timed(False)
     log("precision %d: %d steps in %.3f s, %d rays at the end" % (precision, steps, wall, stats[-1]["rays_end"]))
-    ktimes = {k: g.kernel_time(k) for k in KERNELS}
     state = {f: g.download_field(f) for f in ("nH1", "nH", "temp", "gamma_H1_tot")}
+    ktimes, wall_prof = {}, None
+    if profile:
+        wall_prof, _ = timed(True)
+        log("precision %d: profiled pass %.3f s" % (precision, wall_prof))
+        ktimes = {k: g.kernel_time(k) for k in KERNELS}
+        g.set_profiling(False)
     g.close()
-    return wall, stats, ktimes, state
+    return wall, stats, ktimes, state, wall_prof
 
 
 def pmc_traffic(kernel, workload, mixed):
@@ -204,7 +221,7 @@ def main():
     log("workload %s built: %d source cells" % (a.workload, len(wl["src_cells"])))
     steps = a.steps
     n3 = float(wl["cfg"]["n"]) ** 3
-    wall_m, st_m, kt_m, state_m = run_form(wl, 1, steps, a.warmup, True)
+    wall_m, st_m, kt_m, state_m, wp_m = run_form(wl, 1, steps, a.warmup, True)
     out = {
         "metric": METRIC,
         "value": round(n3 * steps / wall_m, 1),
@@ -224,11 +241,12 @@ def main():
                                   len(wl["src_cells"]), steps, wl["params"]["dt_myr"], wl["params"]["clump"]),
                    "parallelism": "single GPU"},
         "roofline": roofline(wl, st_m, kt_m, True),
+        "ms_per_step_with_launch_events": round(1e3 * wp_m / steps, 3),
         "rays_end": int(st_m[-1]["rays_end"]),
         "segments_per_step": int(np.mean([s["segments"] for s in st_m])),
     }
     if not a.no_fp64:
-        wall_f, st_f, kt_f, state_f = run_form(wl, 0, steps, a.warmup, True)
+        wall_f, st_f, kt_f, state_f, wp_f = run_form(wl, 0, steps, a.warmup, True)
         xm = state_m["nH1"] / state_m["nH"]
         xf = state_f["nH1"] / state_f["nH"]
         rel = lambda u, v: float(np.max(np.abs(u.astype(np.float64) - v) / np.maximum(np.abs(v), 1e-300)))
