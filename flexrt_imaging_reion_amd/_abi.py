@@ -17,6 +17,7 @@ FRT_E_CAPACITY = 3
 FRT_E_PATH_OVERFLOW = 4
 FRT_E_STATE = 5
 FRT_E_NOMEM = 6
+FRT_E_IO = 7
 
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -71,6 +72,40 @@ class FrtRayView(C.Structure):
         ("x", C.POINTER(C.c_float)), ("y", C.POINTER(C.c_float)), ("z", C.POINTER(C.c_float)),
         ("time", C.POINTER(C.c_float)), ("cell", C.POINTER(C.c_int32)),
     ]
+
+
+# frt_file_extras: per-cell fields of the restart files the library does not
+# evolve, in struct order.
+EXTRA_FLOAT_FIELDS = ["rho", "f_He3", "mfp", "mfp_912", "fion", "tcross", "treion_out"]
+EXTRA_FLAG_FIELDS = ["tcross_flag", "neutral_flag"]
+
+
+class FrtFileExtras(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_float)) for n in EXTRA_FLOAT_FIELDS] + \
+               [(n, C.POINTER(C.c_uint8)) for n in EXTRA_FLAG_FIELDS]
+
+
+def make_extras(n3, arrays=None, alloc=False):
+    """An frt_file_extras over numpy arrays: the given ones (write), or fresh
+    zero arrays for every field (alloc=True, read).  Returns (struct, arrays)."""
+    arrays = dict(arrays or {})
+    if alloc:
+        for n in EXTRA_FLOAT_FIELDS:
+            arrays.setdefault(n, np.zeros(n3, dtype=np.float32))
+        for n in EXTRA_FLAG_FIELDS:
+            arrays.setdefault(n, np.zeros(n3, dtype=np.uint8))
+    ex = FrtFileExtras()
+    for n in EXTRA_FLOAT_FIELDS:
+        if n in arrays:
+            arrays[n] = np.ascontiguousarray(arrays[n], dtype=np.float32).reshape(-1)
+            assert arrays[n].size == n3, n
+            setattr(ex, n, arrays[n].ctypes.data_as(C.POINTER(C.c_float)))
+    for n in EXTRA_FLAG_FIELDS:
+        if n in arrays:
+            arrays[n] = np.ascontiguousarray(arrays[n], dtype=np.uint8).reshape(-1)
+            assert arrays[n].size == n3, n
+            setattr(ex, n, arrays[n].ctypes.data_as(C.POINTER(C.c_uint8)))
+    return ex, arrays
 
 
 RAY_FIELDS = [

@@ -6,7 +6,7 @@ import of any GPU entry point raises: there is no CPU fallback.
 import ctypes as C
 import os
 
-from ._abi import FrtConfig, FrtRayView, FrtStepStats
+from ._abi import FrtConfig, FrtFileExtras, FrtRayView, FrtStepStats
 from ._build import LIB_PATH
 
 _lib = None
@@ -24,6 +24,7 @@ _PROTOS = {
     "frt_download_field": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_float)]),
     "frt_set_time": (C.c_int, [_CTX, C.c_float]),
     "frt_get_time": (C.c_int, [_CTX, C.POINTER(C.c_float)]),
+    "frt_set_step": (C.c_int, [_CTX, C.c_int64]),
     "frt_init_rt": (C.c_int, [_CTX]),
     "frt_get_nphot_min_max": (C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "frt_new_rays": (C.c_int, [_CTX]),
@@ -39,6 +40,12 @@ _PROTOS = {
     "frt_upload_rays": (C.c_int, [_CTX, C.POINTER(FrtRayView)]),
     "frt_download_paths": (C.c_int, [_CTX, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float)]),
+    "frt_write_ray_file": (C.c_int, [_CTX, C.c_char_p]),
+    "frt_read_ray_file": (C.c_int, [_CTX, C.c_char_p]),
+    "frt_write_gas_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),
+    "frt_read_gas_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),
+    "frt_write_treion_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),
+    "frt_read_treion_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),
     "frt_set_profiling": (C.c_int, [_CTX, C.c_int32]),
     "frt_kernel_time": (C.c_int, [_CTX, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "frt_reset_kernel_times": (C.c_int, [_CTX]),

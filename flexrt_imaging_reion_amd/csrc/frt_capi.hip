@@ -230,6 +230,40 @@ float* field_ptr(frt_ctx* ctx, int field) {
   return nullptr;
 }
 
+// Whole-file reads and writes of the restart files.
+int write_file(frt_ctx* ctx, const char* path, const void* data, size_t bytes) {
+  FILE* f = fopen(path, "wb");
+  if (!f) return set_error(ctx, FRT_E_IO, std::string("cannot open ") + path + " for writing");
+  const size_t w = bytes ? fwrite(data, 1, bytes, f) : 0;
+  const int closed = fclose(f);
+  if (w != bytes || closed != 0) return set_error(ctx, FRT_E_IO, std::string("short write to ") + path);
+  return FRT_OK;
+}
+
+int read_file(frt_ctx* ctx, const char* path, std::vector<uint8_t>& data) {
+  FILE* f = fopen(path, "rb");
+  if (!f) return set_error(ctx, FRT_E_IO, std::string("cannot open ") + path);
+  fseek(f, 0, SEEK_END);
+  const long size = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  if (size < 0) {
+    fclose(f);
+    return set_error(ctx, FRT_E_IO, std::string("cannot size ") + path);
+  }
+  data.resize((size_t)size);
+  const size_t r = size ? fread(data.data(), 1, (size_t)size, f) : 0;
+  fclose(f);
+  if (r != (size_t)size) return set_error(ctx, FRT_E_IO, std::string("short read of ") + path);
+  return FRT_OK;
+}
+
+template <typename T>
+T get_le(const uint8_t* p) {
+  T v;
+  memcpy(&v, p, sizeof(T));
+  return v;
+}
+
 }  // namespace
 
 extern "C" {
@@ -532,6 +566,13 @@ int frt_set_time(frt_ctx* ctx, float t) {
   return FRT_OK;
 }
 
+int frt_set_step(frt_ctx* ctx, int64_t step) {
+  if (!ctx || step < 0) return FRT_E_ARG;
+  ctx->step = step;
+  ctx->cast_calls = (uint64_t)step + 1;
+  return FRT_OK;
+}
+
 int frt_get_time(const frt_ctx* ctx, float* t) {
   if (!ctx || !t) return FRT_E_ARG;
   *t = ctx->t;
@@ -830,6 +871,202 @@ int frt_copy_kernel(frt_ctx* ctx, int64_t nbytes, int32_t repeats) {
 int frt_synchronize(frt_ctx* ctx) {
   if (!ctx) return FRT_E_ARG;
   return hip_check(ctx, hipStreamSynchronize(ctx->stream), "synchronize");
+}
+
+// ---- restart files ----------------------------------------------------------
+
+int frt_write_ray_file(frt_ctx* ctx, const char* path) {
+  if (!ctx || !path) return FRT_E_ARG;
+  int rc = ensure_compact(ctx);
+  if (rc) return rc;
+  const int64_t n = ctx->nrays;
+  const size_t rec = FRT_RAY_FILE_FIXED + 4 * (size_t)ctx->g.nfreq;
+  std::vector<uint8_t> h((size_t)n * rec + 8);
+  if (n > 0) {
+    uint8_t* d = nullptr;
+    if (hipMalloc((void**)&d, (size_t)n * rec) != hipSuccess)
+      return set_error(ctx, FRT_E_NOMEM, "ray file staging buffer");
+    rc = launch_pack_ray_file(ctx, d);
+    hipError_t e = rc ? hipSuccess : stream_copy(ctx, h.data(), d, (size_t)n * rec, hipMemcpyDeviceToHost);
+    hipStreamSynchronize(ctx->stream);
+    hipFree(d);
+    if (rc) return rc;
+    rc = hip_check(ctx, e, "ray file download");
+    if (rc) return rc;
+  }
+  const int64_t end = -1;  // end marker of read_rays_binary
+  memcpy(h.data() + (size_t)n * rec, &end, 8);
+  return write_file(ctx, path, h.data(), h.size());
+}
+
+int frt_read_ray_file(frt_ctx* ctx, const char* path) {
+  if (!ctx || !path) return FRT_E_ARG;
+  std::vector<uint8_t> h;
+  int rc = read_file(ctx, path, h);
+  if (rc) return rc;
+  const int F = ctx->g.nfreq;
+  const size_t rec = FRT_RAY_FILE_FIXED + 4 * (size_t)F;
+  if (h.size() < 8 || (h.size() - 8) % rec != 0 || get_le<int64_t>(h.data() + h.size() - 8) != -1)
+    return set_error(ctx, FRT_E_IO, std::string(path) + ": not a ray file of " + std::to_string(F) + " bins");
+  const int64_t n = (int64_t)((h.size() - 8) / rec);
+  std::vector<int16_t> order(n);
+  std::vector<int32_t> pixel(n), cell(n);
+  std::vector<uint16_t> itheta(n);
+  std::vector<float> xh(n), yh(n), zh(n), nph((size_t)n * F), r(n), dr(n), x(n), y(n), z(n), tm(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const uint8_t* o = h.data() + (size_t)i * rec;
+    if (get_le<int64_t>(o) < 0)
+      return set_error(ctx, FRT_E_IO, std::string(path) + ": end marker before the end of the file");
+    order[i] = get_le<int16_t>(o + 9);
+    pixel[i] = get_le<int32_t>(o + 11);
+    itheta[i] = get_le<uint16_t>(o + 15);
+    xh[i] = get_le<float>(o + 17);
+    yh[i] = get_le<float>(o + 21);
+    zh[i] = get_le<float>(o + 25);
+    for (int nu = 0; nu < F; ++nu) nph[(size_t)i * F + nu] = get_le<float>(o + 29 + 4 * nu);
+    const uint8_t* q = o + 29 + 4 * F;
+    r[i] = get_le<float>(q);
+    dr[i] = get_le<float>(q + 4);
+    x[i] = get_le<float>(q + 8);
+    y[i] = get_le<float>(q + 12);
+    z[i] = get_le<float>(q + 16);
+    tm[i] = get_le<float>(q + 20);
+    cell[i] = get_le<int32_t>(q + 24);
+  }
+  frt_ray_view v;
+  memset(&v, 0, sizeof(v));
+  v.count = n;
+  v.order = order.data();
+  v.pixel = pixel.data();
+  v.itheta = itheta.data();
+  v.xhat = xh.data();
+  v.yhat = yh.data();
+  v.zhat = zh.data();
+  v.nphot = nph.data();
+  v.r = r.data();
+  v.dr = dr.data();
+  v.x = x.data();
+  v.y = y.data();
+  v.z = z.data();
+  v.time = tm.data();
+  v.cell = cell.data();
+  return frt_upload_rays(ctx, &v);
+}
+
+int frt_write_gas_file(frt_ctx* ctx, const char* path, const frt_file_extras* extras) {
+  if (!ctx || !path) return FRT_E_ARG;
+  const int64_t n3 = ctx->g.n3;
+  const int F = ctx->g.nfreq;
+  const size_t nval = 1 + (size_t)n3 * (FRT_GAS_FILE_FIXED + F);
+  const float* src[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (extras) {
+    src[0] = extras->rho;
+    src[1] = extras->f_He3;
+    src[2] = extras->mfp;
+    src[3] = extras->mfp_912;
+    src[4] = extras->fion;
+    src[5] = extras->tcross;
+    src[6] = extras->treion_out;
+  }
+  unsigned have = 0;
+  for (int j = 0; j < 7; ++j)
+    if (src[j]) have |= 1u << j;
+  float *d = nullptr, *dext = nullptr;
+  if (hipMalloc((void**)&d, nval * 4) != hipSuccess) return set_error(ctx, FRT_E_NOMEM, "gas file staging buffer");
+  hipError_t e = hipSuccess;
+  if (have && hipMalloc((void**)&dext, (size_t)7 * n3 * 4) != hipSuccess) e = hipErrorOutOfMemory;
+  for (int j = 0; j < 7 && e == hipSuccess; ++j)
+    if (src[j]) e = stream_copy(ctx, dext + (size_t)j * n3, src[j], (size_t)n3 * 4, hipMemcpyHostToDevice);
+  std::vector<float> h(nval);
+  int rc = (e == hipSuccess) ? launch_pack_gas_file(ctx, dext, have, d) : FRT_OK;
+  if (e == hipSuccess && rc == FRT_OK) e = stream_copy(ctx, h.data(), d, nval * 4, hipMemcpyDeviceToHost);
+  hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  hipFree(dext);
+  if (rc) return rc;
+  rc = hip_check(ctx, e, "gas file");
+  if (rc) return rc;
+  h[0] = ctx->t;
+  return write_file(ctx, path, h.data(), nval * 4);
+}
+
+int frt_read_gas_file(frt_ctx* ctx, const char* path, frt_file_extras* extras) {
+  if (!ctx || !path) return FRT_E_ARG;
+  std::vector<uint8_t> raw;
+  int rc = read_file(ctx, path, raw);
+  if (rc) return rc;
+  const int64_t n3 = ctx->g.n3;
+  const int F = ctx->g.nfreq;
+  const int W = FRT_GAS_FILE_FIXED + F;
+  if (raw.size() != 4 * (1 + (size_t)n3 * W))
+    return set_error(ctx, FRT_E_IO, std::string(path) + ": not a gas file of this grid and bin count");
+  const float* v = reinterpret_cast<const float*>(raw.data());
+  std::vector<float> col((size_t)n3);
+  auto column = [&](int j) {
+    for (int64_t c = 0; c < n3; ++c) col[(size_t)c] = v[1 + (size_t)c * W + j];
+  };
+  // (file column, field) of the fields this library holds
+  static const int held[][2] = {{1, FRT_F_TEMP},          {2, FRT_F_NH1},           {3, FRT_F_FH1},
+                                {4, FRT_F_FHE1},          {5, FRT_F_FH2},           {6, FRT_F_FHE2},
+                                {8, FRT_F_GAMMA_H1_TOT},  {9, FRT_F_GAMMA_HE1_TOT}, {12, FRT_F_HEAT_RATE},
+                                {13, FRT_F_COOL_RATE},    {14, FRT_F_AVG_NH1}};
+  for (const auto& hf : held) {
+    column(hf[0]);
+    rc = frt_upload_field(ctx, hf[1], col.data());
+    if (rc) return rc;
+  }
+  std::vector<float> ng((size_t)n3 * F);
+  for (int64_t c = 0; c < n3; ++c)
+    for (int nu = 0; nu < F; ++nu) ng[(size_t)c * F + nu] = v[1 + (size_t)c * W + FRT_GAS_FILE_FIXED + nu];
+  rc = frt_upload_field(ctx, FRT_F_NGAM_NU, ng.data());
+  if (rc) return rc;
+  if (extras) {
+    float* dst[7] = {extras->rho, extras->f_He3, extras->mfp, extras->mfp_912, extras->fion, extras->tcross,
+                     extras->treion_out};
+    static const int cols[7] = {0, 7, 10, 11, 15, 16, 17};
+    for (int j = 0; j < 7; ++j)
+      if (dst[j])
+        for (int64_t c = 0; c < n3; ++c) dst[j][c] = v[1 + (size_t)c * W + cols[j]];
+  }
+  return frt_set_time(ctx, v[0]);
+}
+
+int frt_write_treion_file(frt_ctx* ctx, const char* path, const frt_file_extras* extras) {
+  if (!ctx || !path) return FRT_E_ARG;
+  const int64_t n3 = ctx->g.n3;
+  std::vector<float> tre((size_t)n3), flag((size_t)n3);
+  int rc = frt_download_field(ctx, FRT_F_TREION, tre.data());
+  if (!rc) rc = frt_download_field(ctx, FRT_F_TREION_FLAG, flag.data());
+  if (rc) return rc;
+  std::vector<uint8_t> h((size_t)n3 * 7);
+  for (int64_t c = 0; c < n3; ++c) {
+    uint8_t* o = h.data() + (size_t)c * 7;
+    memcpy(o, &tre[(size_t)c], 4);
+    o[4] = flag[(size_t)c] != 0.f ? 1 : 0;
+    o[5] = (extras && extras->tcross_flag) ? (extras->tcross_flag[c] ? 1 : 0) : 0;
+    o[6] = (extras && extras->neutral_flag) ? (extras->neutral_flag[c] ? 1 : 0) : 0;
+  }
+  return write_file(ctx, path, h.data(), h.size());
+}
+
+int frt_read_treion_file(frt_ctx* ctx, const char* path, frt_file_extras* extras) {
+  if (!ctx || !path) return FRT_E_ARG;
+  std::vector<uint8_t> h;
+  int rc = read_file(ctx, path, h);
+  if (rc) return rc;
+  const int64_t n3 = ctx->g.n3;
+  if (h.size() != (size_t)n3 * 7) return set_error(ctx, FRT_E_IO, std::string(path) + ": not a treion file of this grid");
+  std::vector<float> tre((size_t)n3), flag((size_t)n3);
+  for (int64_t c = 0; c < n3; ++c) {
+    const uint8_t* o = h.data() + (size_t)c * 7;
+    memcpy(&tre[(size_t)c], o, 4);
+    flag[(size_t)c] = o[4] ? 1.f : 0.f;
+    if (extras && extras->tcross_flag) extras->tcross_flag[c] = o[5];
+    if (extras && extras->neutral_flag) extras->neutral_flag[c] = o[6];
+  }
+  rc = frt_upload_field(ctx, FRT_F_TREION, tre.data());
+  if (!rc) rc = frt_upload_field(ctx, FRT_F_TREION_FLAG, flag.data());
+  return rc;
 }
 
 }  // extern "C"

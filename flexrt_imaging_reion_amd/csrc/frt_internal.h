@@ -224,6 +224,14 @@ int launch_split_compact(frt_ctx* ctx, int64_t* n_after);
 int launch_cast(frt_ctx* ctx);
 int launch_add_ray_dt(frt_ctx* ctx);
 int launch_copy(frt_ctx* ctx, int64_t nbytes, int repeats);
+// Restart files (io_funcs.cc): fixed bytes of a ray record before the
+// nfreq photon counts are added, and fixed floats of a gas-file cell record.
+#define FRT_RAY_FILE_FIXED 57
+#define FRT_GAS_FILE_FIXED 18
+#define FRT_M_H 1.672622e-24  // global_constants.h:19
+#define FRT_Y_HE 0.246        // global_constants.h:28
+int launch_pack_ray_file(frt_ctx* ctx, uint8_t* out);
+int launch_pack_gas_file(frt_ctx* ctx, const float* ext, unsigned have, float* out);
 
 // helpers shared with the C-ABI file
 int ensure_cub_tmp(frt_ctx* ctx, size_t bytes);

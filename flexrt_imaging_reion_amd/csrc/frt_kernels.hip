@@ -2025,6 +2025,92 @@ int launch_cast(frt_ctx* ctx) {
   return FRT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Restart files (io_funcs.cc), packed on the GPU.
+
+template <typename T>
+__device__ __forceinline__ void put_le(uint8_t* dst, T v) {
+  __builtin_memcpy(dst, &v, sizeof(T));  // byte stores: records are not aligned
+}
+
+// write_rays_binary (io_funcs.cc:274-303): one record of
+// FRT_RAY_FILE_FIXED + 4 nfreq bytes per ray, fields in the order and widths
+// of the reference's ray_type; the slot index is the list position and
+// active is TRUE (every ray between steps is ready to march).
+__global__ void __launch_bounds__(BLK) k_pack_ray_file(RaySoA rb, int64_t n, GridConst g, uint8_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const int F = g.nfreq;
+  const int64_t rec = FRT_RAY_FILE_FIXED + 4 * F;
+  Ray r;
+  ray_load(rb, i, r);
+  float nph[8];
+  phot_load(rb, i, F, g.cap, nph);
+  uint8_t* o = out + i * rec;
+  put_le<int64_t>(o, i);
+  put_le<uint8_t>(o + 8, (uint8_t)1);
+  put_le<int16_t>(o + 9, (int16_t)r.order);
+  put_le<int32_t>(o + 11, (int32_t)r.pixel);
+  put_le<uint16_t>(o + 15, (uint16_t)r.itheta);
+  put_le<float>(o + 17, r.xhat);
+  put_le<float>(o + 21, r.yhat);
+  put_le<float>(o + 25, r.zhat);
+  FRT_FOR_BINS(nu) put_le<float>(o + 29 + 4 * nu, nph[nu]);
+  uint8_t* q = o + 29 + 4 * F;
+  put_le<float>(q, r.r);
+  put_le<float>(q + 4, r.dr);
+  put_le<float>(q + 8, r.x);
+  put_le<float>(q + 12, r.y);
+  put_le<float>(q + 16, r.z);
+  put_le<float>(q + 20, r.time);
+  put_le<int32_t>(q + 24, (int32_t)frt_cell_to_rowmajor(g.lay, r.cell));
+}
+
+// write_gas_binary (io_funcs.cc:350-392): per row-major cell FRT_GAS_FILE_FIXED
+// floats and ngam_nu[nfreq]; out[0] is the time, written by the host.  The
+// fields this library does not evolve come from ext ([7][n3], row-major) when
+// bit j of have is set, else from their C2Ray-mode values.
+__global__ void __launch_bounds__(BLK) k_pack_gas_file(CellPtrs cf, GridConst g, const float* __restrict__ ext,
+                                                       unsigned have, float* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (r >= g.n3) return;
+  const int F = g.nfreq;
+  const int64_t n3 = g.n3;
+  const int64_t k = frt_cell_from_rowmajor(g.lay, (int)r);
+  float* o = out + 1 + r * (FRT_GAS_FILE_FIXED + F);
+  auto extra = [&](int j, float dflt) { return (have >> j & 1u) ? ext[(int64_t)j * n3 + r] : dflt; };
+  // rho defaults to nH m_H / (1 - Y), the inverse of init (init_funcs.cc)
+  o[0] = extra(0, (float)((double)cf.f[C_NH][k] * FRT_M_H / (1. - FRT_Y_HE)));
+  o[1] = cf.f[C_TEMP][k];
+  o[2] = cf.f[C_NH1][k];
+  o[3] = cf.f[C_FH1][k];
+  o[4] = cf.f[C_FHE1][k];
+  o[5] = cf.f[C_FH2][k];
+  o[6] = cf.f[C_FHE2][k];
+  o[7] = extra(1, 0.f);  // f_He3
+  o[8] = cf.f[C_GAMMA_H1_TOT][k];
+  o[9] = cf.f[C_GAMMA_HE1_TOT][k];
+  o[10] = extra(2, 0.f);  // mfp
+  o[11] = extra(3, 0.f);  // mfp_912
+  o[12] = cf.f[C_HEAT][k];
+  o[13] = cf.f[C_COOL][k];
+  o[14] = cf.f[C_AVG_NH1][k];
+  o[15] = extra(4, 1.f);  // fion = 1 with subgrid off (rt_funcs.cc:50-57)
+  o[16] = extra(5, 0.f);  // tcross
+  o[17] = extra(6, 0.f);  // treion_out
+  FRT_FOR_BINS(nu) o[FRT_GAS_FILE_FIXED + nu] = cf.ngam_nu[(int64_t)nu * n3 + k];
+}
+
+int launch_pack_ray_file(frt_ctx* ctx, uint8_t* out) {
+  FRT_LAUNCH(ctx, "pack_ray_file", grid_for(ctx->nrays), k_pack_ray_file, cur_buf(ctx), ctx->nrays, ctx->g, out);
+  return FRT_OK;
+}
+
+int launch_pack_gas_file(frt_ctx* ctx, const float* ext, unsigned have, float* out) {
+  FRT_LAUNCH(ctx, "pack_gas_file", grid_for(ctx->g.n3), k_pack_gas_file, ctx->c, ctx->g, ext, have, out);
+  return FRT_OK;
+}
+
 int launch_copy(frt_ctx* ctx, int64_t nbytes, int repeats) {
   const int64_t n16 = nbytes / 16;
   unsigned grid = (unsigned)std::min<int64_t>((n16 + BLK - 1) / BLK, 256 * 64);

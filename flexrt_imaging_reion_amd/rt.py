@@ -6,12 +6,13 @@ gas_funcs.cc) and take and return numpy arrays in the reference's layouts.
 Every call goes through the C ABI; a missing library raises at construction.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
 from . import _lib
-from ._abi import (FIELD_ID, PER_BIN, PREC_FP64, PREC_MIXED, FrtConfig, FrtStepStats,
-                   config_struct, make_ray_view)
+from ._abi import (EXTRA_FLAG_FIELDS, EXTRA_FLOAT_FIELDS, FIELD_ID, PER_BIN, PREC_FP64, PREC_MIXED,
+                   FrtConfig, FrtStepStats, config_struct, make_extras, make_ray_view)
 
 
 class FlexRTError(RuntimeError):
@@ -105,6 +106,10 @@ class FlexRT:
     def set_time(self, t):
         self._check(self._lib.frt_set_time(self._h, float(t)), "set_time")
 
+    def set_step(self, step):
+        """Restart: completed steps (they key the casting and merging angle streams)."""
+        self._check(self._lib.frt_set_step(self._h, int(step)), "set_step")
+
     def get_time(self):
         t = C.c_float()
         self._check(self._lib.frt_get_time(self._h, C.byref(t)), "get_time")
@@ -176,6 +181,40 @@ class FlexRT:
             cells.ctypes.data_as(C.POINTER(C.c_int32)),
             dists.ctypes.data_as(C.POINTER(C.c_float))), "download_paths")
         return end, cells, dists
+
+    # -- restart files (io_funcs.cc formats) --------------------------------------------
+    def write_ray_file(self, path):
+        """write_rays_binary (io_funcs.cc:274-303)."""
+        self._check(self._lib.frt_write_ray_file(self._h, os.fsencode(path)), "write_ray_file")
+
+    def read_ray_file(self, path):
+        """read_rays_binary (io_funcs.cc:112-139): replaces the ray buffer."""
+        self._check(self._lib.frt_read_ray_file(self._h, os.fsencode(path)), "read_ray_file")
+
+    def write_gas_file(self, path, extras=None):
+        """write_gas_binary (io_funcs.cc:350-392); extras: the fields the
+        library does not evolve (see frt_file_extras), row-major."""
+        ex, keep = make_extras(self.n ** 3, extras)
+        self._check(self._lib.frt_write_gas_file(self._h, os.fsencode(path), C.byref(ex)), "write_gas_file")
+
+    def read_gas_file(self, path):
+        """read_grid_binary (io_funcs.cc:79-111); returns the extras of the file."""
+        ex, arrays = make_extras(self.n ** 3, alloc=True)
+        self._check(self._lib.frt_read_gas_file(self._h, os.fsencode(path), C.byref(ex)), "read_gas_file")
+        return {k: arrays[k] for k in EXTRA_FLOAT_FIELDS}
+
+    def write_treion_file(self, path, extras=None):
+        """write_treion (io_funcs.cc:330-347)."""
+        ex, keep = make_extras(self.n ** 3, extras)
+        self._check(self._lib.frt_write_treion_file(self._h, os.fsencode(path), C.byref(ex)),
+                    "write_treion_file")
+
+    def read_treion_file(self, path):
+        """read_treion (io_funcs.cc:64-77); returns tcross_flag, neutral_flag."""
+        ex, arrays = make_extras(self.n ** 3, alloc=True)
+        self._check(self._lib.frt_read_treion_file(self._h, os.fsencode(path), C.byref(ex)),
+                    "read_treion_file")
+        return {k: arrays[k] for k in EXTRA_FLAG_FIELDS}
 
     # -- measurement -------------------------------------------------------------------
     def set_profiling(self, on):

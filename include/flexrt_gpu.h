@@ -40,7 +40,8 @@ typedef enum {
   FRT_E_CAPACITY = 3,      /* ray buffer full ("processor full", ray_tracing.cc:275-287) */
   FRT_E_PATH_OVERFLOW = 4, /* a ray crossed more than path_length cells in a step (ray_tracing.cc:1323) */
   FRT_E_STATE = 5,         /* call out of order (e.g. find_avg_unu before update_rays) */
-  FRT_E_NOMEM = 6          /* device allocation failed */
+  FRT_E_NOMEM = 6,         /* device allocation failed */
+  FRT_E_IO = 7             /* a restart file could not be written, read or parsed */
 } frt_status;
 
 typedef enum { FRT_PREC_FP64 = 0, FRT_PREC_MIXED = 1 } frt_precision;
@@ -183,6 +184,11 @@ int frt_upload_field(frt_ctx* ctx, int32_t field, const float* host);
 int frt_download_field(frt_ctx* ctx, int32_t field, float* host);
 int frt_set_time(frt_ctx* ctx, float t);
 int frt_get_time(const frt_ctx* ctx, float* t);
+/* Restart: the number of completed steps, which keys the deterministic angle
+ * streams that replace rand() in casting and merging.  Sets the state of a
+ * context that ran `step` calls of frt_step after one frt_new_rays; the
+ * step is reported in frt_step_stats.step. */
+int frt_set_step(frt_ctx* ctx, int64_t step);
 
 /* ---- reference routines on the per-step path ----------------------------- */
 /* init_rt (rt_funcs.cc:36-126), fresh start: empty ray buffer, ray_dt = dt
@@ -225,6 +231,43 @@ int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* view);
  * cells/dists laid out [ray][path_length] (ray[x].path, ray_end[x];
  * ray_tracing.cc:1323-1327,1439-1442,1471).  n must equal the ray count. */
 int frt_download_paths(frt_ctx* ctx, int64_t n, int32_t* ray_end, int32_t* cells, float* dists);
+
+/* ---- restart files (the reference's formats, io_funcs.cc) ----------------- */
+/* Per-cell fields of the gas and treion files that this library does not
+ * evolve (C2Ray mode, subgrid off), row-major [n^3]; any may be NULL.  A write
+ * then uses the value noted; a read fills the non-NULL ones. */
+typedef struct {
+  float* rho;            /* g/cm^3; NULL: nH m_H / (1 - Y) */
+  float* f_He3;          /* NULL: 0 */
+  float* mfp;            /* NULL: 0 */
+  float* mfp_912;        /* NULL: 0 */
+  float* fion;           /* NULL: 1 (fion = 1 with subgrid off) */
+  float* tcross;         /* NULL: 0 */
+  float* treion_out;     /* NULL: 0 */
+  uint8_t* tcross_flag;  /* treion file; NULL: 0 */
+  uint8_t* neutral_flag; /* treion file; NULL: 0 */
+} frt_file_extras;
+/* write_rays_binary (io_funcs.cc:274-303): per ray its index (long), active
+ * (bool, TRUE), order (short), pixel (int), itheta (unsigned short), xhat,
+ * yhat, zhat, nphot[nfreq], r, dr, x, y, z, time (float) and its cell
+ * path[0].ind (int, row-major), packed without padding; then the long -1.
+ * The records are packed on the GPU and written with one host write. */
+int frt_write_ray_file(frt_ctx* ctx, const char* path);
+/* read_rays_binary (io_funcs.cc:112-139): the rays of a ray file, in file
+ * order, replace the ray buffer (the indices are not used). */
+int frt_read_ray_file(frt_ctx* ctx, const char* path);
+/* write_gas_binary (io_funcs.cc:350-392): the time t (float), then per cell
+ * in row-major order rho, temp, nH1, f_H1, f_He1, f_H2, f_He2, f_He3,
+ * gamma_H1_tot, gamma_He1_tot, mfp, mfp_912, heat_rate, cool_rate, avg_nh1,
+ * fion, tcross, treion_out and ngam_nu[nfreq] (float).  Packed on the GPU. */
+int frt_write_gas_file(frt_ctx* ctx, const char* path, const frt_file_extras* extras);
+/* read_grid_binary (io_funcs.cc:79-111): sets t and the fields of the file
+ * that this library holds; the others go to extras (may be NULL). */
+int frt_read_gas_file(frt_ctx* ctx, const char* path, frt_file_extras* extras);
+/* write_treion / read_treion (io_funcs.cc:330-347,64-77): per row-major cell
+ * treion (float), treion_flag, tcross_flag, neutral_flag (bool). */
+int frt_write_treion_file(frt_ctx* ctx, const char* path, const frt_file_extras* extras);
+int frt_read_treion_file(frt_ctx* ctx, const char* path, frt_file_extras* extras);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* When enabled, every kernel launch is bracketed by HIP events on the

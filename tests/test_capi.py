@@ -7,7 +7,7 @@ import re
 import pytest
 
 from flexrt_imaging_reion_amd import _lib, inputs
-from flexrt_imaging_reion_amd._abi import FIELDS, FrtConfig, FrtRayView, FrtStepStats
+from flexrt_imaging_reion_amd._abi import FIELDS, FrtConfig, FrtFileExtras, FrtRayView, FrtStepStats
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(REPO, "include", "flexrt_gpu.h")
@@ -42,6 +42,7 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(FrtConfig) == 16 * 4 + 8 + 8 + 12 * 8
     assert C.sizeof(FrtStepStats) == 15 * 8 + 2 * 8 + 2 * 4
     assert C.sizeof(FrtRayView) == 8 + 14 * 8
+    assert C.sizeof(FrtFileExtras) == 9 * 8
 
 
 def test_config_default_without_gpu():

@@ -1,0 +1,152 @@
+"""Restart files in the reference's formats (io_funcs.cc), through the C ABI.
+
+The layouts are checked against the order and widths in which the reference's
+readers consume the files (read_rays_binary io_funcs.cc:112-139,
+read_grid_binary :79-111, read_treion :64-77), by parsing what the library
+writes with numpy structured types built from those readers; then each file
+is read back into a fresh context, and a run restarted from a ray file
+continues bit for bit like the run that wrote it.
+"""
+import numpy as np
+import pytest
+
+from flexrt_imaging_reion_amd import inputs
+from flexrt_imaging_reion_amd._abi import FIELDS
+from parity_util import STATE_FIELDS, bits_equal, compare_rays, counts
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(*a):
+    print("CASE", *a, flush=True)
+
+
+def ray_record(nfreq):
+    """read_rays_binary's sequence (io_funcs.cc:118-135), packed."""
+    return np.dtype([("x", "<i8"), ("active", "u1"), ("order", "<i2"), ("pixel", "<i4"), ("itheta", "<u2"),
+                     ("xhat", "<f4"), ("yhat", "<f4"), ("zhat", "<f4"), ("nphot", "<f4", (nfreq,)),
+                     ("r", "<f4"), ("dr", "<f4"), ("px", "<f4"), ("py", "<f4"), ("pz", "<f4"),
+                     ("time", "<f4"), ("cell", "<i4")])
+
+
+GAS_COLUMNS = ["rho", "temp", "nH1", "f_H1", "f_He1", "f_H2", "f_He2", "f_He3", "gamma_H1_tot",
+               "gamma_He1_tot", "mfp", "mfp_912", "heat_rate", "cool_rate", "avg_nh1", "fion", "tcross",
+               "treion_out"]  # read_grid_binary, io_funcs.cc:87-104, then ngam_nu[Nfreq]
+
+
+def run(precision, steps, n=16, n_src=16):
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    wl = inputs.make_workload("lognormal64", n=n, n_src=n_src, steps=steps)
+    g = FlexRT(dict(wl["cfg"], precision=precision))
+    inputs.setup(g, wl)
+    for _ in range(steps):
+        g.step()
+    return wl, g
+
+
+def test_ray_file_layout_and_round_trip(tmp_path):
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    _print_case("ray file lognormal n=16 src=16 steps=4")
+    wl, g = run(0, 4)
+    path = tmp_path / "ray_z=07.000"
+    try:
+        g.write_ray_file(path)
+        rays = g.rays()
+        F = int(wl["cfg"]["nfreq"])
+        raw = path.read_bytes()
+        dt = ray_record(F)
+        n = len(rays["cell"])
+        assert len(raw) == n * dt.itemsize + 8 and dt.itemsize == 57 + 4 * F
+        assert np.frombuffer(raw[-8:], "<i8")[0] == -1
+        rec = np.frombuffer(raw[:-8], dt)
+        assert np.array_equal(rec["x"], np.arange(n)) and np.all(rec["active"] == 1)
+        for f, k in [("order", "order"), ("pixel", "pixel"), ("itheta", "itheta"), ("xhat", "xhat"),
+                     ("yhat", "yhat"), ("zhat", "zhat"), ("r", "r"), ("dr", "dr"), ("px", "x"), ("py", "y"),
+                     ("pz", "z"), ("time", "time"), ("cell", "cell")]:
+            assert bits_equal(rec[f], np.asarray(rays[k]).astype(rec[f].dtype)), f
+        assert bits_equal(rec["nphot"].reshape(n, F), np.asarray(rays["nphot"]).reshape(n, F))
+        with FlexRT(dict(wl["cfg"], precision=0)) as h:
+            inputs.setup(h, wl)
+            h.read_ray_file(path)
+            assert compare_rays(h.rays(), rays) is None
+    finally:
+        g.close()
+
+
+def test_gas_and_treion_files(tmp_path):
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    _print_case("gas/treion files lognormal n=16 src=16 steps=4")
+    wl, g = run(0, 4)
+    n3 = int(wl["cfg"]["n"]) ** 3
+    F = int(wl["cfg"]["nfreq"])
+    rng = np.random.default_rng(4)
+    extras = {"rho": rng.uniform(1e-31, 1e-29, n3).astype(np.float32),
+              "mfp": rng.uniform(0, 1e24, n3).astype(np.float32)}
+    flags = {"tcross_flag": (rng.uniform(size=n3) < 0.3).astype(np.uint8),
+             "neutral_flag": (rng.uniform(size=n3) < 0.6).astype(np.uint8)}
+    gas, tre = tmp_path / "gas_z=07.000", tmp_path / "tre_z=07.000"
+    try:
+        g.write_gas_file(gas, extras)
+        g.write_treion_file(tre, flags)
+        raw = np.frombuffer(gas.read_bytes(), "<f4")
+        assert raw.size == 1 + n3 * (18 + F)
+        assert raw[0] == np.float32(g.get_time())
+        cols = raw[1:].reshape(n3, 18 + F)
+        want = {k: g.download_field(k) for k in GAS_COLUMNS if k in FIELDS}
+        for j, name in enumerate(GAS_COLUMNS):
+            if name in want:
+                assert bits_equal(cols[:, j], want[name]), name
+        assert bits_equal(cols[:, 0], extras["rho"]) and bits_equal(cols[:, 10], extras["mfp"])
+        assert np.all(cols[:, 7] == 0) and np.all(cols[:, 11] == 0) and np.all(cols[:, 15] == 1)
+        assert bits_equal(cols[:, 18:], g.download_field("ngam_nu").reshape(n3, F))
+        t = np.frombuffer(tre.read_bytes(), np.dtype([("treion", "<f4"), ("treion_flag", "u1"),
+                                                      ("tcross_flag", "u1"), ("neutral_flag", "u1")]))
+        assert t.size == n3
+        assert bits_equal(t["treion"], g.download_field("treion"))
+        assert np.array_equal(t["treion_flag"], (g.download_field("treion_flag") != 0).astype(np.uint8))
+        assert np.array_equal(t["tcross_flag"], flags["tcross_flag"])
+        assert np.array_equal(t["neutral_flag"], flags["neutral_flag"])
+        with FlexRT(dict(wl["cfg"], precision=0)) as h:
+            inputs.setup(h, wl)
+            got = h.read_gas_file(gas)
+            fl = h.read_treion_file(tre)
+            assert np.float32(h.get_time()) == np.float32(g.get_time())
+            for name in list(want) + ["ngam_nu", "treion", "treion_flag"]:
+                assert bits_equal(h.download_field(name), g.download_field(name)), name
+            assert bits_equal(got["rho"], extras["rho"]) and np.all(got["fion"] == 1)
+            assert np.array_equal(fl["neutral_flag"], flags["neutral_flag"])
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_restart_from_ray_file_continues_bit_exact(tmp_path, precision):
+    """A context given the writer's cell state and its ray file continues the
+    run exactly as the writer does (the ray file holds the whole ray state)."""
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    _print_case("restart precision=%d lognormal n=16 src=16 3+3 steps" % precision)
+    wl, a = run(precision, 3)
+    path = tmp_path / "ray_restart"
+    try:
+        a.write_ray_file(path)
+        done = a.stats()["step"]
+        with FlexRT(dict(wl["cfg"], precision=precision)) as b:
+            inputs.setup(b, wl)
+            for name in FIELDS:
+                b.upload_field(name, a.download_field(name))
+            b.read_ray_file(path)
+            b.set_time(a.get_time())
+            b.set_step(done)
+            b.get_nphot_min_max()
+            for s in range(3):
+                sa, sb = a.step(), b.step()
+                assert counts(sa) == counts(sb), "step %d" % s
+            assert compare_rays(a.rays(), b.rays()) is None
+            bad = [k for k in STATE_FIELDS if not bits_equal(a.download_field(k), b.download_field(k))]
+            assert not bad, bad
+    finally:
+        a.close()
