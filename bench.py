@@ -15,6 +15,9 @@ same way beside it
 and its final state is the reference for the mixed-form errors (the FP64 GPU
 path is bit-identical to the CPU checker, see tests/test_gpu_parity.py).
 The CPU checker runs a bounded sample of the same workload on one host core.
+With --restart-files DIR, the mixed run's restart files (ray, gas, treion in
+the reference's binary formats) are also written and read back, and the CPU
+checker's ray-file writer is timed on a sample of the same rays.
 
 Prints one JSON line.
 """
@@ -48,6 +51,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--restart-files", default=None, metavar="DIR",
+                    help="after the mixed run, time writing and reading its restart files "
+                         "(ray, gas, treion) in DIR (removed afterwards), and the CPU checker's "
+                         "ray-file writer on a sample of the same rays")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the runs, copy MIB MiB through the library's plain copy kernel "
                          "(reference dispatch for HBM counter calibration)")
@@ -92,7 +99,7 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-def run_form(wl, precision, steps, warmup, profile):
+def run_form(wl, precision, steps, warmup, profile, files_dir=None):
     """Warm up, then time `steps` steps of the workload from its initial state
     without instrumentation (the throughput), then, if `profile`, run the same
     steps again from the initial state with every launch bracketed by HIP
@@ -133,8 +140,71 @@ def run_form(wl, precision, steps, warmup, profile):
         log("precision %d: profiled pass %.3f s" % (precision, wall_prof))
         ktimes = {k: g.kernel_time(k) for k in KERNELS}
         g.set_profiling(False)
+    files = restart_files(g, wl, files_dir) if files_dir else None
     g.close()
-    return wall, stats, ktimes, state, wall_prof
+    return wall, stats, ktimes, state, wall_prof, files
+
+
+def restart_files(g, wl, d, sample=2_000_000):
+    """Write and read back the context's restart files in the reference's
+    formats (C ABI frt_write_ray_file & co.: pack on the GPU, one copy to the
+    host, one file write), best of 3, then the CPU checker's restatement of
+    write_rays_binary (one stream write per field) on the first `sample` rays
+    of the same list, whose bytes must equal the library file's prefix."""
+    from oracle.pyoracle import Oracle
+
+    os.makedirs(d, exist_ok=True)
+    paths = {k: os.path.join(d, k) for k in ("ray", "gas", "treion", "ray_cpu")}
+    out = {}
+
+    def best(fn):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    try:
+        for name, write, read in [("ray", g.write_ray_file, g.read_ray_file),
+                                  ("gas", g.write_gas_file, g.read_gas_file),
+                                  ("treion", g.write_treion_file, g.read_treion_file)]:
+            tw = best(lambda: write(paths[name]))
+            size = os.path.getsize(paths[name])
+            tr = best(lambda: read(paths[name]))
+            out[name] = {"bytes": size, "write_s": round(tw, 4), "read_s": round(tr, 4),
+                         "write_GBps": round(size / tw / 1e9, 3), "read_GBps": round(size / tr / 1e9, 3)}
+            log("restart %s file: %d bytes, write %.3f s, read %.3f s" % (name, size, tw, tr))
+        rays = g.rays()
+        n = len(rays["cell"])
+        m = min(n, sample)
+        part = {k: np.ascontiguousarray(v[:m] if k != "nphot" else v[: m * int(wl["cfg"]["nfreq"])])
+                for k, v in rays.items()}
+        o = Oracle(wl["cfg"])
+        try:
+            o.upload_rays(part)
+            t0 = time.perf_counter()
+            o.write_ray_file(paths["ray_cpu"])
+            tc = time.perf_counter() - t0
+        finally:
+            o.close()
+        rec = 57 + 4 * int(wl["cfg"]["nfreq"])
+        with open(paths["ray"], "rb") as f:
+            head = f.read(m * rec)
+        with open(paths["ray_cpu"], "rb") as f:
+            cpu = f.read()
+        out["ray"]["rays"] = n
+        out["cpu_baseline"] = {"kind": "port", "rays": m, "write_s": round(tc, 4),
+                                  "rays_per_s": round(m / tc, 1), "GBps": round(len(cpu) / tc / 1e9, 3),
+                                  "same_bytes_as_library": cpu[:-8] == head and cpu[-8:] == (-1).to_bytes(
+                                      8, "little", signed=True),
+                                  "cores": 1}
+        log("cpu checker ray file: %d rays in %.3f s" % (m, tc))
+    finally:
+        for p in paths.values():
+            if os.path.exists(p):
+                os.remove(p)
+    return out
 
 
 def pmc_traffic(kernel, workload, mixed):
@@ -221,7 +291,7 @@ def main():
     log("workload %s built: %d source cells" % (a.workload, len(wl["src_cells"])))
     steps = a.steps
     n3 = float(wl["cfg"]["n"]) ** 3
-    wall_m, st_m, kt_m, state_m, wp_m = run_form(wl, 1, steps, a.warmup, True)
+    wall_m, st_m, kt_m, state_m, wp_m, files = run_form(wl, 1, steps, a.warmup, True, a.restart_files)
     out = {
         "metric": METRIC,
         "value": round(n3 * steps / wall_m, 1),
@@ -246,7 +316,7 @@ def main():
         "segments_per_step": int(np.mean([s["segments"] for s in st_m])),
     }
     if not a.no_fp64:
-        wall_f, st_f, kt_f, state_f, wp_f = run_form(wl, 0, steps, a.warmup, True)
+        wall_f, st_f, kt_f, state_f, wp_f, _ = run_form(wl, 0, steps, a.warmup, True)
         xm = state_m["nH1"] / state_m["nH"]
         xf = state_f["nH1"] / state_f["nH"]
         rel = lambda u, v: float(np.max(np.abs(u.astype(np.float64) - v) / np.maximum(np.abs(v), 1e-300)))
@@ -271,6 +341,8 @@ def main():
             "T_max_rel": rel(state_m["temp"], state_f["temp"]),
             "ray_counts_exact": all(all(sm[k] == sf[k] for k in keys) for sm, sf in zip(st_m, st_f)),
         }
+    if files:
+        out["restart_files"] = files
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
     if a.calibrate_copy:

@@ -48,6 +48,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <algorithm>
 #include <map>
 #include <vector>
@@ -1249,6 +1250,45 @@ int orc_upload_rays(void* h, const frt_ray_view* v) {
     s->ray.push_back(r);
   }
   return 0;
+}
+
+// write_rays_binary (io_funcs.cc:274-303): every tagged ray, in buffer
+// order, as its index (long), then each field with its own stream write in the
+// reference's order and widths, then the long -1.  The checker's buffer has no
+// free slots, so the index is the list position.  Returns 0, or 1 if the file
+// cannot be written.
+int orc_write_ray_file(void* h, const char* path) {
+  State* s = (State*)h;
+  drop_dead(*s);
+  std::ofstream file(path, std::ios::out | std::ios::binary);
+  if (!file) return 1;
+  const int F = s->Nfreq;
+  long int x = 0;
+  for (auto& r : s->ray) {
+    if (!r.tag) continue;
+    const bool active = true;  // set_to_true at the end of a step
+    file.write((const char*)&x, sizeof(long int));
+    file.write((const char*)&active, sizeof(bool));
+    file.write((const char*)&r.order, sizeof(short int));
+    file.write((const char*)&r.pixel, sizeof(int));
+    file.write((const char*)&r.itheta, sizeof(unsigned short int));
+    file.write((const char*)&r.xhat, sizeof(float));
+    file.write((const char*)&r.yhat, sizeof(float));
+    file.write((const char*)&r.zhat, sizeof(float));
+    for (int nu = 0; nu < F; nu++) file.write((const char*)&r.nphot[nu], sizeof(float));
+    file.write((const char*)&r.r, sizeof(float));
+    file.write((const char*)&r.dr, sizeof(float));
+    file.write((const char*)&r.x, sizeof(float));
+    file.write((const char*)&r.y, sizeof(float));
+    file.write((const char*)&r.z, sizeof(float));
+    file.write((const char*)&r.time, sizeof(float));
+    file.write((const char*)&r.path_ind[0], sizeof(int));
+    x++;
+  }
+  const long int end = -1;
+  file.write((const char*)&end, sizeof(long int));
+  file.close();
+  return file ? 0 : 1;
 }
 
 // Paths of the last update_rays: ray_end[i] and [ray][P] cells / lengths.

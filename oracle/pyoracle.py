@@ -58,6 +58,7 @@ _PROTOS = {
     "orc_hubble": (C.c_float, [C.c_float, C.c_double, C.c_double, C.c_double]),
     "orc_rotate": (None, [C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "orc_c2ray_cell": (None, [C.c_float] * 9 + [C.c_double, C.c_double, C.c_float, C.POINTER(C.c_float)]),
+    "orc_write_ray_file": (C.c_int, [C.c_void_p, C.c_char_p]),
     "orc_itheta_hash": (C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint64]),
     "orc_derived": (None, [_VP, C.POINTER(C.c_double)]),
 }
@@ -193,6 +194,11 @@ class Oracle:
         v, arrays = make_ray_view(n, self.nfreq, rays)
         self._keep = arrays
         assert self._lib.orc_upload_rays(self._h, C.byref(v)) == 0
+
+    def write_ray_file(self, path):
+        rc = self._lib.orc_write_ray_file(self._h, str(path).encode())
+        if rc:
+            raise RuntimeError("oracle: cannot write %s" % path)
 
     def paths(self):
         n = self.ray_count()

@@ -12,21 +12,13 @@ import pytest
 
 from flexrt_imaging_reion_amd import inputs
 from flexrt_imaging_reion_amd._abi import FIELDS
-from parity_util import STATE_FIELDS, bits_equal, compare_rays, counts
+from parity_util import STATE_FIELDS, bits_equal, check_ray_file, compare_rays, counts, make_pair
 
 pytestmark = pytest.mark.gpu
 
 
 def _print_case(*a):
     print("CASE", *a, flush=True)
-
-
-def ray_record(nfreq):
-    """read_rays_binary's sequence (io_funcs.cc:118-135), packed."""
-    return np.dtype([("x", "<i8"), ("active", "u1"), ("order", "<i2"), ("pixel", "<i4"), ("itheta", "<u2"),
-                     ("xhat", "<f4"), ("yhat", "<f4"), ("zhat", "<f4"), ("nphot", "<f4", (nfreq,)),
-                     ("r", "<f4"), ("dr", "<f4"), ("px", "<f4"), ("py", "<f4"), ("pz", "<f4"),
-                     ("time", "<f4"), ("cell", "<i4")])
 
 
 GAS_COLUMNS = ["rho", "temp", "nH1", "f_H1", "f_He1", "f_H2", "f_He2", "f_He3", "gamma_H1_tot",
@@ -54,25 +46,33 @@ def test_ray_file_layout_and_round_trip(tmp_path):
     try:
         g.write_ray_file(path)
         rays = g.rays()
-        F = int(wl["cfg"]["nfreq"])
-        raw = path.read_bytes()
-        dt = ray_record(F)
-        n = len(rays["cell"])
-        assert len(raw) == n * dt.itemsize + 8 and dt.itemsize == 57 + 4 * F
-        assert np.frombuffer(raw[-8:], "<i8")[0] == -1
-        rec = np.frombuffer(raw[:-8], dt)
-        assert np.array_equal(rec["x"], np.arange(n)) and np.all(rec["active"] == 1)
-        for f, k in [("order", "order"), ("pixel", "pixel"), ("itheta", "itheta"), ("xhat", "xhat"),
-                     ("yhat", "yhat"), ("zhat", "zhat"), ("r", "r"), ("dr", "dr"), ("px", "x"), ("py", "y"),
-                     ("pz", "z"), ("time", "time"), ("cell", "cell")]:
-            assert bits_equal(rec[f], np.asarray(rays[k]).astype(rec[f].dtype)), f
-        assert bits_equal(rec["nphot"].reshape(n, F), np.asarray(rays["nphot"]).reshape(n, F))
+        assert check_ray_file(path.read_bytes(), rays, int(wl["cfg"]["nfreq"])) is None
         with FlexRT(dict(wl["cfg"], precision=0)) as h:
             inputs.setup(h, wl)
             h.read_ray_file(path)
             assert compare_rays(h.rays(), rays) is None
     finally:
         g.close()
+
+
+def test_ray_file_same_bytes_as_checker(tmp_path):
+    """After the same FP64 steps, the library's ray file and the checker's
+    restatement of write_rays_binary (one stream write per field) are the
+    same bytes."""
+    _print_case("ray file bytes vs checker lognormal n=16 src=16 steps=5")
+    wl = inputs.make_workload("lognormal64", n=16, n_src=16, steps=5)
+    g, o = make_pair(wl, precision=0)
+    try:
+        for _ in range(5):
+            g.step()
+            o.step()
+        g.write_ray_file(tmp_path / "gpu")
+        o.write_ray_file(tmp_path / "cpu")
+        a, b = (tmp_path / "gpu").read_bytes(), (tmp_path / "cpu").read_bytes()
+        assert len(a) == len(b) and a == b
+    finally:
+        g.close()
+        o.close()
 
 
 def test_gas_and_treion_files(tmp_path):
