@@ -2066,7 +2066,7 @@ __global__ void __launch_bounds__(BLK) k_pack_ray_file(RaySoA rb, int64_t n, Gri
   put_le<int32_t>(q + 24, (int32_t)frt_cell_to_rowmajor(g.lay, r.cell));
 }
 
-// write_gas_binary (io_funcs.cc:350-392): per row-major cell FRT_GAS_FILE_FIXED
+// write_gas_binary (io_funcs.cc:358-393): per row-major cell FRT_GAS_FILE_FIXED
 // floats and ngam_nu[nfreq]; out[0] is the time, written by the host.  The
 // fields this library does not evolve come from ext ([7][n3], row-major) when
 // bit j of have is set, else from their C2Ray-mode values.

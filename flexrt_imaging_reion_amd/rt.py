@@ -31,6 +31,16 @@ def default_config(n):
     return {name: getattr(s, name) for name, _ in FrtConfig._fields_}
 
 
+def checkpoint_paths(output_dir, zz):
+    """File names of the driver's checkpoint at redshift zz
+    (gas_funcs.cc:617-645): output_dir (concatenated as given, so it carries
+    its own separator) + "gas_z=" / "ray_z=" / "tre_z=" + zz with six decimals
+    less the last two, zero-padded to two integer digits below z = 10."""
+    zs = "%f" % float(zz)
+    pad = "0" if float(zz) < 10.0 else ""
+    return {k: str(output_dir) + k + "_z=" + pad + zs[:-2] for k in ("gas", "ray", "tre")}
+
+
 class FlexRT:
     """One GPU RT context (device state resident in HBM)."""
 
@@ -188,33 +198,45 @@ class FlexRT:
         self._check(self._lib.frt_write_ray_file(self._h, os.fsencode(path)), "write_ray_file")
 
     def read_ray_file(self, path):
-        """read_rays_binary (io_funcs.cc:112-139): replaces the ray buffer."""
+        """read_rays_binary (io_funcs.cc:119-145): replaces the ray buffer."""
         self._check(self._lib.frt_read_ray_file(self._h, os.fsencode(path)), "read_ray_file")
 
     def write_gas_file(self, path, extras=None):
-        """write_gas_binary (io_funcs.cc:350-392); extras: the fields the
+        """write_gas_binary (io_funcs.cc:358-393); extras: the fields the
         library does not evolve (see frt_file_extras), row-major."""
         ex, keep = make_extras(self.n ** 3, extras)
         self._check(self._lib.frt_write_gas_file(self._h, os.fsencode(path), C.byref(ex)), "write_gas_file")
 
     def read_gas_file(self, path):
-        """read_grid_binary (io_funcs.cc:79-111); returns the extras of the file."""
+        """read_grid_binary (io_funcs.cc:81-116); returns the extras of the file."""
         ex, arrays = make_extras(self.n ** 3, alloc=True)
         self._check(self._lib.frt_read_gas_file(self._h, os.fsencode(path), C.byref(ex)), "read_gas_file")
         return {k: arrays[k] for k in EXTRA_FLOAT_FIELDS}
 
     def write_treion_file(self, path, extras=None):
-        """write_treion (io_funcs.cc:330-347)."""
+        """write_treion (io_funcs.cc:338-355)."""
         ex, keep = make_extras(self.n ** 3, extras)
         self._check(self._lib.frt_write_treion_file(self._h, os.fsencode(path), C.byref(ex)),
                     "write_treion_file")
 
     def read_treion_file(self, path):
-        """read_treion (io_funcs.cc:64-77); returns tcross_flag, neutral_flag."""
+        """read_treion (io_funcs.cc:65-77); returns tcross_flag, neutral_flag."""
         ex, arrays = make_extras(self.n ** 3, alloc=True)
         self._check(self._lib.frt_read_treion_file(self._h, os.fsencode(path), C.byref(ex)),
                     "read_treion_file")
         return {k: arrays[k] for k in EXTRA_FLAG_FIELDS}
+
+    def write_checkpoint(self, output_dir, zz, extras=None, rays=True):
+        """The end-of-hydro-step output of the driver (gas_funcs.cc:617-656):
+        gas file, ray file (the driver writes it only for some snapshots, hence
+        `rays`), treion file, named as checkpoint_paths names them.  Returns
+        the paths written."""
+        paths = checkpoint_paths(output_dir, zz)
+        self.write_gas_file(paths["gas"], extras)
+        if rays:
+            self.write_ray_file(paths["ray"])
+        self.write_treion_file(paths["tre"], extras)
+        return paths
 
     # -- measurement -------------------------------------------------------------------
     def set_profiling(self, on):

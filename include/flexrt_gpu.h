@@ -253,18 +253,18 @@ typedef struct {
  * path[0].ind (int, row-major), packed without padding; then the long -1.
  * The records are packed on the GPU and written with one host write. */
 int frt_write_ray_file(frt_ctx* ctx, const char* path);
-/* read_rays_binary (io_funcs.cc:112-139): the rays of a ray file, in file
+/* read_rays_binary (io_funcs.cc:119-145): the rays of a ray file, in file
  * order, replace the ray buffer (the indices are not used). */
 int frt_read_ray_file(frt_ctx* ctx, const char* path);
-/* write_gas_binary (io_funcs.cc:350-392): the time t (float), then per cell
+/* write_gas_binary (io_funcs.cc:358-393): the time t (float), then per cell
  * in row-major order rho, temp, nH1, f_H1, f_He1, f_H2, f_He2, f_He3,
  * gamma_H1_tot, gamma_He1_tot, mfp, mfp_912, heat_rate, cool_rate, avg_nh1,
  * fion, tcross, treion_out and ngam_nu[nfreq] (float).  Packed on the GPU. */
 int frt_write_gas_file(frt_ctx* ctx, const char* path, const frt_file_extras* extras);
-/* read_grid_binary (io_funcs.cc:79-111): sets t and the fields of the file
+/* read_grid_binary (io_funcs.cc:81-116): sets t and the fields of the file
  * that this library holds; the others go to extras (may be NULL). */
 int frt_read_gas_file(frt_ctx* ctx, const char* path, frt_file_extras* extras);
-/* write_treion / read_treion (io_funcs.cc:330-347,64-77): per row-major cell
+/* write_treion / read_treion (io_funcs.cc:338-355,65-77): per row-major cell
  * treion (float), treion_flag, tcross_flag, neutral_flag (bool). */
 int frt_write_treion_file(frt_ctx* ctx, const char* path, const frt_file_extras* extras);
 int frt_read_treion_file(frt_ctx* ctx, const char* path, frt_file_extras* extras);

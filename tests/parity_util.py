@@ -22,7 +22,7 @@ RAY_FILE_FIELDS = [("order", "order"), ("pixel", "pixel"), ("itheta", "itheta"),
 
 def ray_record(nfreq):
     """One ray of a ray file, in the order and widths read_rays_binary
-    consumes them (io_funcs.cc:118-135), packed."""
+    consumes them (io_funcs.cc:122-142), packed."""
     return np.dtype([("x", "<i8"), ("active", "u1"), ("order", "<i2"), ("pixel", "<i4"), ("itheta", "<u2"),
                      ("xhat", "<f4"), ("yhat", "<f4"), ("zhat", "<f4"), ("nphot", "<f4", (nfreq,)),
                      ("r", "<f4"), ("dr", "<f4"), ("px", "<f4"), ("py", "<f4"), ("pz", "<f4"),

@@ -1,8 +1,8 @@
 """Restart files in the reference's formats (io_funcs.cc), through the C ABI.
 
 The layouts are checked against the order and widths in which the reference's
-readers consume the files (read_rays_binary io_funcs.cc:112-139,
-read_grid_binary :79-111, read_treion :64-77), by parsing what the library
+readers consume the files (read_rays_binary io_funcs.cc:119-145,
+read_grid_binary :81-116, read_treion :65-77), by parsing what the library
 writes with numpy structured types built from those readers; then each file
 is read back into a fresh context, and a run restarted from a ray file
 continues bit for bit like the run that wrote it.
@@ -23,7 +23,7 @@ def _print_case(*a):
 
 GAS_COLUMNS = ["rho", "temp", "nH1", "f_H1", "f_He1", "f_H2", "f_He2", "f_He3", "gamma_H1_tot",
                "gamma_He1_tot", "mfp", "mfp_912", "heat_rate", "cool_rate", "avg_nh1", "fion", "tcross",
-               "treion_out"]  # read_grid_binary, io_funcs.cc:87-104, then ngam_nu[Nfreq]
+               "treion_out"]  # read_grid_binary, io_funcs.cc:89-109, then ngam_nu[Nfreq]
 
 
 def run(precision, steps, n=16, n_src=16):
