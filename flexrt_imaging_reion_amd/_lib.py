@@ -25,6 +25,9 @@ _PROTOS = {
     "frt_set_time": (C.c_int, [_CTX, C.c_float]),
     "frt_get_time": (C.c_int, [_CTX, C.POINTER(C.c_float)]),
     "frt_set_step": (C.c_int, [_CTX, C.c_int64]),
+    "frt_set_dt": (C.c_int, [_CTX, C.c_float]),
+    "frt_get_cell_size": (C.c_int, [_CTX, C.POINTER(C.c_float)]),
+    "frt_update_cosmo_expansion": (C.c_int, [_CTX, C.c_float, C.c_float]),
     "frt_init_rt": (C.c_int, [_CTX]),
     "frt_get_nphot_min_max": (C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "frt_new_rays": (C.c_int, [_CTX]),

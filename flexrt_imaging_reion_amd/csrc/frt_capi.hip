@@ -222,12 +222,24 @@ int finish_stats(frt_ctx* ctx) {
   return FRT_OK;
 }
 
+// The C ABI's ids 0..FRT_F_INC_FLUX are the internal scalar slots of the
+// same number; the per-bin fields follow, then the later scalar fields.
 float* field_ptr(frt_ctx* ctx, int field) {
-  if (field >= 0 && field < C_NSCALAR) return ctx->c.f[field];
-  if (field == FRT_F_GAMMA_H1_NU) return ctx->c.gamma_h1_nu;
-  if (field == FRT_F_GAMMA_HE1_NU) return ctx->c.gamma_he1_nu;
-  if (field == FRT_F_NGAM_NU) return ctx->c.ngam_nu;
-  return nullptr;
+  if (field >= 0 && field <= FRT_F_INC_FLUX) return ctx->c.f[field];
+  switch (field) {
+    case FRT_F_GAMMA_H1_NU: return ctx->c.gamma_h1_nu;
+    case FRT_F_GAMMA_HE1_NU: return ctx->c.gamma_he1_nu;
+    case FRT_F_NGAM_NU: return ctx->c.ngam_nu;
+    case FRT_F_RHO: return ctx->c.f[C_RHO];
+    case FRT_F_FHE3: return ctx->c.f[C_FHE3];
+    case FRT_F_MFP: return ctx->c.f[C_MFP];
+    case FRT_F_MFP_912: return ctx->c.f[C_MFP912];
+    default: return nullptr;
+  }
+}
+
+static bool per_bin(int field) {
+  return field == FRT_F_GAMMA_H1_NU || field == FRT_F_GAMMA_HE1_NU || field == FRT_F_NGAM_NU;
 }
 
 // Whole-file reads and writes of the restart files.
@@ -368,6 +380,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   g.compton_coef = frt_compton_rate((double)zz);
   g.t_cmb = 2.726 * (double)(1.f + zz);
   g.nhe_to_nh = (float)((FRT_Y_HE / 4.) / (1. - FRT_Y_HE));
+  g.sig912 = frt_sigmapi_H1(13.6 / FRT_H_EV);  // mfp_912 (ray_tracing.cc:1080)
   const float safety = (float)c.safety_factor;
   for (int l = 0; l < FRT_MAX_LEVEL; ++l) {
     // rmax_from_l / rmin_from_l (ray_tracing.cc:66-79)
@@ -394,6 +407,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->mcap = cap;
   ctx->have_segments = false;
   ctx->have_alive = false;
+  ctx->have_rho = false;
 
   int rc = FRT_OK;
   for (int b = 0; b < 2 && rc == FRT_OK; ++b) rc = alloc_rays(ctx, ctx->buf[b], cap, c.nfreq);
@@ -533,12 +547,13 @@ int frt_upload_field(frt_ctx* ctx, int32_t field, const float* host) {
   const size_t n3 = (size_t)ctx->g.n3;
   std::vector<int32_t> perm;
   cell_permutation(ctx, perm);
-  const int F = field < C_NSCALAR ? 1 : ctx->g.nfreq;
+  const int F = per_bin(field) ? ctx->g.nfreq : 1;
   // per-bin fields: [cell][nfreq] row-major in, [nfreq][cell] internal on the device
   std::vector<float> t((size_t)F * n3);
   for (size_t r = 0; r < n3; ++r)
     for (int nu = 0; nu < F; ++nu) t[(size_t)nu * n3 + (size_t)perm[r]] = host[r * F + nu];
   const hipError_t e = stream_copy(ctx, d, t.data(), 4 * t.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && field == FRT_F_RHO) ctx->have_rho = true;
   return hip_check(ctx, e, "upload field");
 }
 
@@ -551,7 +566,7 @@ int frt_download_field(frt_ctx* ctx, int32_t field, float* host) {
   if (rc) return rc;
   std::vector<int32_t> perm;
   cell_permutation(ctx, perm);
-  const int F = field < C_NSCALAR ? 1 : ctx->g.nfreq;
+  const int F = per_bin(field) ? ctx->g.nfreq : 1;
   std::vector<float> t((size_t)F * n3);
   const hipError_t e = stream_copy(ctx, t.data(), d, 4 * t.size(), hipMemcpyDeviceToHost);
   if (e == hipSuccess)
@@ -571,6 +586,35 @@ int frt_set_step(frt_ctx* ctx, int64_t step) {
   ctx->step = step;
   ctx->cast_calls = (uint64_t)step + 1;
   return FRT_OK;
+}
+
+int frt_set_dt(frt_ctx* ctx, float dt) {
+  if (!ctx || !(dt > 0.f)) return FRT_E_ARG;
+  ctx->g.dt = dt;
+  return FRT_OK;
+}
+
+int frt_get_cell_size(const frt_ctx* ctx, float* dx) {
+  if (!ctx || !dx) return FRT_E_ARG;
+  *dx = ctx->g.dx;
+  return FRT_OK;
+}
+
+int frt_update_cosmo_expansion(frt_ctx* ctx, float a_current, float a_prev) {
+  if (!ctx) return FRT_E_ARG;
+  if (!(a_current > 0.f) || !(a_prev > 0.f)) return set_error(ctx, FRT_E_ARG, "scale factors must be positive");
+  if (!ctx->have_rho) return set_error(ctx, FRT_E_ARG, "update_cosmo_expansion needs rho: upload FRT_F_RHO first");
+  // dx[i] *= a_current/a_prev (float); rho *= pow(a_prev/a_current, 3.): the
+  // square of a float is exact in double, so q*q*q is the correctly rounded cube
+  const float ratio = a_current / a_prev;
+  const double q = (double)(a_prev / a_current);
+  const double rho_scale = q * q * q;
+  ctx->g.dx = ctx->g.dx * ratio;
+  ctx->g.sf = ctx->g.dx / ctx->g.dx0;
+  ctx->g.vol = (double)ctx->g.dx * (double)ctx->g.dx * (double)ctx->g.dx;
+  int rc = launch_cosmo_expansion(ctx, ratio, rho_scale);
+  if (rc) return rc;
+  return hip_check(ctx, hipGetLastError(), "update_cosmo_expansion");
 }
 
 int frt_get_time(const frt_ctx* ctx, float* t) {
@@ -978,7 +1022,7 @@ int frt_write_gas_file(frt_ctx* ctx, const char* path, const frt_file_extras* ex
   for (int j = 0; j < 7 && e == hipSuccess; ++j)
     if (src[j]) e = stream_copy(ctx, dext + (size_t)j * n3, src[j], (size_t)n3 * 4, hipMemcpyHostToDevice);
   std::vector<float> h(nval);
-  int rc = (e == hipSuccess) ? launch_pack_gas_file(ctx, dext, have, d) : FRT_OK;
+  int rc = (e == hipSuccess) ? launch_pack_gas_file(ctx, dext, have, ctx->have_rho ? 1 : 0, d) : FRT_OK;
   if (e == hipSuccess && rc == FRT_OK) e = stream_copy(ctx, h.data(), d, nval * 4, hipMemcpyDeviceToHost);
   hipStreamSynchronize(ctx->stream);
   hipFree(d);
@@ -1006,10 +1050,11 @@ int frt_read_gas_file(frt_ctx* ctx, const char* path, frt_file_extras* extras) {
     for (int64_t c = 0; c < n3; ++c) col[(size_t)c] = v[1 + (size_t)c * W + j];
   };
   // (file column, field) of the fields this library holds
-  static const int held[][2] = {{1, FRT_F_TEMP},          {2, FRT_F_NH1},           {3, FRT_F_FH1},
-                                {4, FRT_F_FHE1},          {5, FRT_F_FH2},           {6, FRT_F_FHE2},
-                                {8, FRT_F_GAMMA_H1_TOT},  {9, FRT_F_GAMMA_HE1_TOT}, {12, FRT_F_HEAT_RATE},
-                                {13, FRT_F_COOL_RATE},    {14, FRT_F_AVG_NH1}};
+  static const int held[][2] = {{0, FRT_F_RHO},           {1, FRT_F_TEMP},          {2, FRT_F_NH1},
+                                {3, FRT_F_FH1},           {4, FRT_F_FHE1},          {5, FRT_F_FH2},
+                                {6, FRT_F_FHE2},          {7, FRT_F_FHE3},          {8, FRT_F_GAMMA_H1_TOT},
+                                {9, FRT_F_GAMMA_HE1_TOT}, {10, FRT_F_MFP},          {11, FRT_F_MFP_912},
+                                {12, FRT_F_HEAT_RATE},    {13, FRT_F_COOL_RATE},    {14, FRT_F_AVG_NH1}};
   for (const auto& hf : held) {
     column(hf[0]);
     rc = frt_upload_field(ctx, hf[1], col.data());
@@ -1044,7 +1089,8 @@ int frt_write_treion_file(frt_ctx* ctx, const char* path, const frt_file_extras*
     memcpy(o, &tre[(size_t)c], 4);
     o[4] = flag[(size_t)c] != 0.f ? 1 : 0;
     o[5] = (extras && extras->tcross_flag) ? (extras->tcross_flag[c] ? 1 : 0) : 0;
-    o[6] = (extras && extras->neutral_flag) ? (extras->neutral_flag[c] ? 1 : 0) : 0;
+    // neutral_flag starts TRUE (rt_funcs.cc:69) and only the I-front mode clears it
+    o[6] = (extras && extras->neutral_flag) ? (extras->neutral_flag[c] ? 1 : 0) : 1;
   }
   return write_file(ctx, path, h.data(), h.size());
 }

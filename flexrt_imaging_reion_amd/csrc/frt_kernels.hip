@@ -1176,6 +1176,10 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       cf.f[C_FHE2][k] = nHe2n / nHe;
       cf.f[C_FH1][k] = nH1n / nH;
       cf.f[C_FHE1][k] = nHe1n / nHe;
+      cf.f[C_FHE3][k] = 0.f / nHe;  // nHe3 = 0 (ray_tracing.cc:1128,1194)
+      // mean free paths from the opacity this pass used (ray_tracing.cc:1078-1081)
+      cf.f[C_MFP][k] = (float)(1. / g.sigH[0] / (double)ah1);
+      cf.f[C_MFP912][k] = (float)(1. / g.sig912 / (double)ah1);
       if (THERMAL) {
         if (g.temp_ev) {
           ThermalIn in;
@@ -2071,7 +2075,7 @@ __global__ void __launch_bounds__(BLK) k_pack_ray_file(RaySoA rb, int64_t n, Gri
 // fields this library does not evolve come from ext ([7][n3], row-major) when
 // bit j of have is set, else from their C2Ray-mode values.
 __global__ void __launch_bounds__(BLK) k_pack_gas_file(CellPtrs cf, GridConst g, const float* __restrict__ ext,
-                                                       unsigned have, float* __restrict__ out) {
+                                                       unsigned have, int held_rho, float* __restrict__ out) {
   const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (r >= g.n3) return;
   const int F = g.nfreq;
@@ -2079,26 +2083,85 @@ __global__ void __launch_bounds__(BLK) k_pack_gas_file(CellPtrs cf, GridConst g,
   const int64_t k = frt_cell_from_rowmajor(g.lay, (int)r);
   float* o = out + 1 + r * (FRT_GAS_FILE_FIXED + F);
   auto extra = [&](int j, float dflt) { return (have >> j & 1u) ? ext[(int64_t)j * n3 + r] : dflt; };
-  // rho defaults to nH m_H / (1 - Y), the inverse of init (init_funcs.cc)
-  o[0] = extra(0, (float)((double)cf.f[C_NH][k] * FRT_M_H / (1. - FRT_Y_HE)));
+  // rho: given, held, or nH m_H / (1 - Y), the inverse of init (init_funcs.cc:64)
+  o[0] = extra(0, held_rho ? cf.f[C_RHO][k] : (float)((double)cf.f[C_NH][k] * FRT_M_H / (1. - FRT_Y_HE)));
   o[1] = cf.f[C_TEMP][k];
   o[2] = cf.f[C_NH1][k];
   o[3] = cf.f[C_FH1][k];
   o[4] = cf.f[C_FHE1][k];
   o[5] = cf.f[C_FH2][k];
   o[6] = cf.f[C_FHE2][k];
-  o[7] = extra(1, 0.f);  // f_He3
+  o[7] = extra(1, cf.f[C_FHE3][k]);
   o[8] = cf.f[C_GAMMA_H1_TOT][k];
   o[9] = cf.f[C_GAMMA_HE1_TOT][k];
-  o[10] = extra(2, 0.f);  // mfp
-  o[11] = extra(3, 0.f);  // mfp_912
+  o[10] = extra(2, cf.f[C_MFP][k]);
+  o[11] = extra(3, cf.f[C_MFP912][k]);
   o[12] = cf.f[C_HEAT][k];
   o[13] = cf.f[C_COOL][k];
   o[14] = cf.f[C_AVG_NH1][k];
   o[15] = extra(4, 1.f);  // fion = 1 with subgrid off (rt_funcs.cc:50-57)
   o[16] = extra(5, 0.f);  // tcross
-  o[17] = extra(6, 0.f);  // treion_out
+  o[17] = extra(6, cf.f[C_TREION][k]);  // treion_out is set with treion (ray_tracing.cc:1129-1132)
   FRT_FOR_BINS(nu) o[FRT_GAS_FILE_FIXED + nu] = cf.ngam_nu[(int64_t)nu * n3 + k];
+}
+
+// update_cosmo_expansion's cell loop (gas_funcs.cc:483-512) with fion = 1
+// (rt_funcs.cc:55): rho scaled by the cube of a_prev/a_current, the number
+// densities recomputed from it and the ionized fractions, in the reference's
+// float / double promotions.  The cell order does not matter here.
+__global__ void __launch_bounds__(BLK) k_cosmo_cells(CellPtrs cf, int64_t n3, double rho_scale) {
+  const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (k >= n3) return;
+  const float rho = (float)((double)cf.f[C_RHO][k] * rho_scale);
+  const float nH = (float)((1. - FRT_Y_HE) * (double)rho / FRT_M_H);
+  const float nHe = (float)(FRT_Y_HE * (double)rho / (4. * FRT_M_H));
+  const float nH2 = cf.f[C_FH2][k] * nH;  // / fion, fion = 1
+  const float nHe2 = cf.f[C_FHE2][k] * nHe;
+  const float nHe3 = cf.f[C_FHE3][k] * nHe;
+  const float nH1 = nH - nH2;
+  const float nHe1 = nHe - nHe2 - nHe3;
+  const float ne = (float)((double)(nH2 + nHe2) + 2. * (double)nHe3);
+  cf.f[C_RHO][k] = rho;
+  cf.f[C_NH][k] = nH;
+  cf.f[C_NHE][k] = nHe;
+  cf.f[C_NH2][k] = nH2;
+  cf.f[C_NHE2][k] = nHe2;
+  cf.f[C_NH1][k] = nH1;
+  cf.f[C_AVG_NH1][k] = nH1;
+  cf.f[C_NHE1][k] = nHe1;
+  cf.f[C_AVG_NHE1][k] = nHe1;
+  cf.f[C_NE][k] = ne;
+  cf.f[C_NTOT][k] = nH + nHe + ne;
+  cf.f[C_NH1_PREV][k] = nH1;
+  cf.avg2[k] = make_float2(nH1, nHe1);
+}
+
+// Ray loop of update_cosmo_expansion (gas_funcs.cc:519-550): positions within
+// the cell scaled by a_current/a_prev and moved off the faces, with g.dx
+// already the new cell size.
+__device__ __forceinline__ float cosmo_clamp(float x, float dx, float min_dist) {
+  if (x <= 0.f) x = min_dist * dx;
+  if ((double)x >= (double)dx * (1. - (double)min_dist / 2.)) x = (float)((double)dx * (1. - (double)min_dist));
+  return x;
+}
+
+__global__ void __launch_bounds__(BLK) k_cosmo_rays(float4* __restrict__ pos, int64_t n, float ratio, float dx,
+                                                    float min_dist) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  float4 p = pos[i];
+  p.x = cosmo_clamp(p.x * ratio, dx, min_dist);
+  p.y = cosmo_clamp(p.y * ratio, dx, min_dist);
+  p.z = cosmo_clamp(p.z * ratio, dx, min_dist);
+  pos[i] = p;
+}
+
+int launch_cosmo_expansion(frt_ctx* ctx, float ratio, double rho_scale) {
+  FRT_LAUNCH(ctx, "cosmo_cells", grid_for(ctx->g.n3), k_cosmo_cells, ctx->c, (int64_t)ctx->g.n3, rho_scale);
+  if (ctx->nrays > 0)
+    FRT_LAUNCH(ctx, "cosmo_rays", grid_for(ctx->nrays), k_cosmo_rays, cur_buf(ctx).pos, ctx->nrays, ratio,
+               ctx->g.dx, ctx->g.min_dist);
+  return FRT_OK;
 }
 
 int launch_pack_ray_file(frt_ctx* ctx, uint8_t* out) {
@@ -2106,8 +2169,8 @@ int launch_pack_ray_file(frt_ctx* ctx, uint8_t* out) {
   return FRT_OK;
 }
 
-int launch_pack_gas_file(frt_ctx* ctx, const float* ext, unsigned have, float* out) {
-  FRT_LAUNCH(ctx, "pack_gas_file", grid_for(ctx->g.n3), k_pack_gas_file, ctx->c, ctx->g, ext, have, out);
+int launch_pack_gas_file(frt_ctx* ctx, const float* ext, unsigned have, int held_rho, float* out) {
+  FRT_LAUNCH(ctx, "pack_gas_file", grid_for(ctx->g.n3), k_pack_gas_file, ctx->c, ctx->g, ext, have, held_rho, out);
   return FRT_OK;
 }
 

@@ -173,7 +173,9 @@ def grid_sources(cells, lum, n3):
 # ---- gas initialization (init_funcs.cc:10-143, fresh start) -----------------------------------
 def init_gas(rho, temp0, clump):
     """Fields of init() for a fresh start.  fion is still 0 when init() runs
-    (init_rt sets it later, main.cc:54,72), so the fion == 0 branch applies."""
+    (init_rt sets it later, main.cc:54,72), so the fion == 0 branch applies.
+    f_He3 = 1 - f_He1 - f_He2 (init_funcs.cc:41).  mfp and mfp_912 are left 0
+    here: they are outputs (gas file) that the first C2Ray step sets."""
     rho = np.asarray(rho, dtype=np.float32).reshape(-1)
     n3 = rho.size
     f64 = np.float64
@@ -191,9 +193,10 @@ def init_gas(rho, temp0, clump):
     f_He2 = np.full(n3, FHE2_0, dtype=np.float32)
     temp = np.broadcast_to(np.asarray(temp0, dtype=np.float32), (n3,)).astype(np.float32).copy()
     clump = np.broadcast_to(np.asarray(clump, dtype=np.float32), (n3,)).astype(np.float32).copy()
+    f_He3 = np.full(n3, np.float32(1.0 - f64(FHE1_0) - f64(FHE2_0)), dtype=np.float32)
     return dict(nH=nH, nHe=nHe, nH1=nH1, nHe1=nHe1, nH2=nH2, nHe2=nHe2, ne=ne, n_tot=n_tot,
-                f_H1=f_H1, f_H2=f_H2, f_He1=f_He1, f_He2=f_He2, temp=temp, clump=clump,
-                nH1_prev=nH1.copy(), avg_nh1=nH1.copy(), avg_nhe1=nHe1.copy())
+                f_H1=f_H1, f_H2=f_H2, f_He1=f_He1, f_He2=f_He2, f_He3=f_He3, temp=temp, clump=clump,
+                nH1_prev=nH1.copy(), avg_nh1=nH1.copy(), avg_nhe1=nHe1.copy(), rho=rho.copy())
 
 
 # ---- workloads ----------------------------------------------------------------------------------

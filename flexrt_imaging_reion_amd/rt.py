@@ -120,6 +120,22 @@ class FlexRT:
         """Restart: completed steps (they key the casting and merging angle streams)."""
         self._check(self._lib.frt_set_step(self._h, int(step)), "set_step")
 
+    def set_dt(self, dt):
+        """update_dt (rt_funcs.cc:9-12): the driver's new time step."""
+        self._check(self._lib.frt_set_dt(self._h, float(dt)), "set_dt")
+
+    def cell_size(self):
+        """dx[0], cm (grows with update_cosmo_expansion)."""
+        v = C.c_float()
+        self._check(self._lib.frt_get_cell_size(self._h, C.byref(v)), "get_cell_size")
+        return v.value
+
+    def update_cosmo_expansion(self, a_current, a_prev):
+        """update_cosmo_expansion (gas_funcs.cc:460-552) for the driver's scale
+        factors now and one dt earlier; needs rho uploaded."""
+        self._check(self._lib.frt_update_cosmo_expansion(self._h, float(a_current), float(a_prev)),
+                    "update_cosmo_expansion")
+
     def get_time(self):
         t = C.c_float()
         self._check(self._lib.frt_get_time(self._h, C.byref(t)), "get_time")

@@ -105,7 +105,10 @@ typedef struct frt_step_stats {
 } frt_step_stats;
 
 /* Cell fields (float32, n^3 values each, or n^3*nfreq for the _NU fields).
- * Names follow global_variables.cc. */
+ * Names follow global_variables.cc.  RHO and F_HE3 are inputs of the cosmic
+ * expansion (gas_funcs.cc:460-552) and of the gas file; MFP and MFP_912 are
+ * written by the last C2Ray pass (ray_tracing.cc:1078-1081) for the gas
+ * file. */
 typedef enum {
   FRT_F_NH = 0,        /* nH */
   FRT_F_NHE,           /* nHe */
@@ -140,6 +143,10 @@ typedef enum {
   FRT_F_GAMMA_H1_NU,   /* gamma_H1_nu[cell][nfreq] */
   FRT_F_GAMMA_HE1_NU,  /* gamma_He1_nu[cell][nfreq] */
   FRT_F_NGAM_NU,       /* ngam_nu[cell][nfreq] */
+  FRT_F_RHO,           /* rho, g/cm^3 */
+  FRT_F_FHE3,          /* f_He3 (0 after a C2Ray step, ray_tracing.cc:1128,1194) */
+  FRT_F_MFP,           /* mfp = 1/sigma_HI(nu_phot[0])/avg_nh1 */
+  FRT_F_MFP_912,       /* mfp_912 = 1/sigma_HI(13.6 eV)/avg_nh1 */
   FRT_F_COUNT
 } frt_field;
 
@@ -189,6 +196,19 @@ int frt_get_time(const frt_ctx* ctx, float* t);
  * context that ran `step` calls of frt_step after one frt_new_rays; the
  * step is reported in frt_step_stats.step. */
 int frt_set_step(frt_ctx* ctx, int64_t step);
+/* update_dt (rt_funcs.cc:9-12) computes dt on the host from dx; the driver
+ * hands the new value over (dt > 0). */
+int frt_set_dt(frt_ctx* ctx, float dt);
+int frt_get_cell_size(const frt_ctx* ctx, float* dx);
+/* update_cosmo_expansion (gas_funcs.cc:460-552) for the scale factors the
+ * driver computed (scale_factor(t + cosmic_time(1/(1+zinit))) and the same
+ * one dt earlier, cosmo_funcs.cc:42-78): the cell size grows by
+ * a_current/a_prev; rho falls by (a_prev/a_current)^3 and nH, nHe, the ion
+ * densities (from f_H2, f_He2, f_He3 with fion = 1), avg_nh1, avg_nhe1, ne,
+ * n_tot and nH1_prev are recomputed from it; ray positions within their
+ * cells are scaled and kept off the cell faces.  Requires RHO to have been
+ * uploaded (or read from a gas file). */
+int frt_update_cosmo_expansion(frt_ctx* ctx, float a_current, float a_prev);
 
 /* ---- reference routines on the per-step path ----------------------------- */
 /* init_rt (rt_funcs.cc:36-126), fresh start: empty ray buffer, ray_dt = dt

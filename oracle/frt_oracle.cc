@@ -69,6 +69,7 @@ const double h_eV = 4.135667e-15;
 const double k_B = 1.38065e-16;
 const double cl = 2.998e10;
 const double Y = 0.246;
+const double m_H = 1.672622e-24;  // global_constants.h:19
 const double mpc_to_km = 3.086e19;
 const double kpc_to_cm = 3.086e21;
 const double ev_to_erg = 1.6022e-12;
@@ -164,7 +165,7 @@ struct State {
   std::vector<float> nH, nHe, nH1, nHe1, nH2, nHe2, ne, n_tot, f_H1, f_He1, f_H2, f_He2, temp, dne_dt,
       clump, nH1_prev, avg_nh1, avg_nhe1, recomb_H2, recomb_He2, gamma_H1_tot, gamma_He1_tot,
       recrate_H2, recrate_He2, treion, treion_flag, heat_rate, cool_rate, IF_speed, Inc_Flux,
-      gamma_H1_nu, gamma_He1_nu, ngam_nu;
+      gamma_H1_nu, gamma_He1_nu, ngam_nu, rho, f_He3, mfp, mfp_912;
   std::vector<double> avg_gamma, avg_gamma_he1, avg_gamma_nu, avg_gamma_nu_he1, ngam_acc;
   std::vector<Ray> ray;
   std::vector<int> src_cell;
@@ -216,6 +217,10 @@ static std::vector<float>* field_vec(State& s, int f) {
     case FRT_F_GAMMA_H1_NU: return &s.gamma_H1_nu;
     case FRT_F_GAMMA_HE1_NU: return &s.gamma_He1_nu;
     case FRT_F_NGAM_NU: return &s.ngam_nu;
+    case FRT_F_RHO: return &s.rho;
+    case FRT_F_FHE3: return &s.f_He3;
+    case FRT_F_MFP: return &s.mfp;
+    case FRT_F_MFP_912: return &s.mfp_912;
     default: return nullptr;
   }
 }
@@ -602,7 +607,11 @@ static void find_avg_unu(State& s) {
       }
     }
     // cell update (:941-1214), C2Ray branch
+    const double sig0 = sigmapi_H1(s.nu_phot[0]), sig912 = sigmapi_H1(13.6 / h_eV);
     for (int c = 0; c < s.N3; c++) {
+      // :1078-1081, from the opacity of this pass
+      s.mfp[c] = (float)(1. / sig0 / s.avg_nh1[c]);
+      s.mfp_912[c] = (float)(1. / sig912 / s.avg_nh1[c]);
       CellOut o = c2ray_cell(s.nH[c], s.nHe[c], s.avg_nh1[c], s.avg_nhe1[c], s.f_H1[c], s.f_He1[c], s.clump[c],
                              s.recomb_H2[c], s.recomb_He2[c], s.avg_gamma[c], s.avg_gamma_he1[c], s.dt);
       s.avg_nh1[c] = o.avg_nh1;
@@ -633,6 +642,7 @@ static void find_avg_unu(State& s) {
         s.f_He2[c] = s.nHe2[c] / s.nHe[c];
         s.f_H1[c] = s.nH1[c] / s.nH[c];
         s.f_He1[c] = s.nHe1[c] / s.nHe[c];
+        s.f_He3[c] = 0.f / s.nHe[c];  // nHe3 = 0 (:1128), f_He3 = nHe3/nHe (:1194)
       }
     }
   }
@@ -1084,7 +1094,8 @@ void* orc_create(const frt_config* cfg) {
   const size_t n3 = (size_t)s->N3, nf = n3 * s->Nfreq;
   for (int f = 0; f < FRT_F_COUNT; f++) {
     std::vector<float>* v = field_vec(*s, f);
-    v->assign((f >= FRT_F_GAMMA_H1_NU) ? nf : n3, 0.f);
+    const bool per_bin = f == FRT_F_GAMMA_H1_NU || f == FRT_F_GAMMA_HE1_NU || f == FRT_F_NGAM_NU;
+    v->assign(per_bin ? nf : n3, 0.f);
   }
   s->clump.assign(n3, 1.0f);
   s->avg_gamma.assign(n3, 0.);
@@ -1290,6 +1301,45 @@ int orc_write_ray_file(void* h, const char* path) {
   file.close();
   return file ? 0 : 1;
 }
+
+// update_cosmo_expansion (gas_funcs.cc:460-552) for given a_current, a_prev,
+// with fion = 1 (rt_funcs.cc:55): dx grows, rho falls, the number densities
+// follow from rho and the ionized fractions, ray positions scale within their
+// cells and are moved off the faces.  pow(q, 3.) of a float q is written
+// q*q*q in double, which is exact up to its one final rounding.
+void orc_update_cosmo_expansion(void* h, float a_current, float a_prev) {
+  State& s = *(State*)h;
+  s.dx *= a_current / a_prev;
+  const double q = (double)(a_prev / a_current);
+  for (int c = 0; c < s.N3; c++) {
+    s.rho[c] *= q * q * q;
+    s.nH[c] = (1. - Y) * s.rho[c] / m_H;
+    s.nHe[c] = Y * s.rho[c] / (4. * m_H);
+    const float fion = 1.f;
+    s.nH2[c] = s.f_H2[c] * s.nH[c] / fion;
+    s.nHe2[c] = s.f_He2[c] * s.nHe[c] / fion;
+    const float nHe3 = s.f_He3[c] * s.nHe[c] / fion;
+    s.nH1[c] = s.nH[c] - s.nH2[c];
+    s.avg_nh1[c] = s.nH1[c];
+    s.nHe1[c] = s.nHe[c] - s.nHe2[c] - nHe3;
+    s.avg_nhe1[c] = s.nHe1[c];
+    s.ne[c] = s.nH2[c] + s.nHe2[c] + 2. * nHe3;
+    s.n_tot[c] = s.nH[c] + s.nHe[c] + s.ne[c];
+    s.nH1_prev[c] = s.nH1[c];
+  }
+  for (auto& r : s.ray) {
+    if (!r.tag) continue;
+    float* x[3] = {&r.x, &r.y, &r.z};
+    for (float* v : x) {
+      *v *= a_current / a_prev;
+      if (*v <= 0.) *v = s.min_dist * s.dx;
+      if (*v >= s.dx * (1. - s.min_dist / 2.)) *v = s.dx * (1. - s.min_dist);
+    }
+  }
+}
+
+void orc_set_dt(void* h, float dt) { ((State*)h)->dt = dt; }
+float orc_get_cell_size(void* h) { return ((State*)h)->dx; }
 
 // Paths of the last update_rays: ray_end[i] and [ray][P] cells / lengths.
 int orc_download_paths(void* h, int64_t n, int* ray_end, int* cells, float* dists) {

@@ -59,6 +59,9 @@ _PROTOS = {
     "orc_rotate": (None, [C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "orc_c2ray_cell": (None, [C.c_float] * 9 + [C.c_double, C.c_double, C.c_float, C.POINTER(C.c_float)]),
     "orc_write_ray_file": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "orc_update_cosmo_expansion": (None, [C.c_void_p, C.c_float, C.c_float]),
+    "orc_set_dt": (None, [C.c_void_p, C.c_float]),
+    "orc_get_cell_size": (C.c_float, [C.c_void_p]),
     "orc_itheta_hash": (C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint64]),
     "orc_derived": (None, [_VP, C.POINTER(C.c_double)]),
 }
@@ -194,6 +197,15 @@ class Oracle:
         v, arrays = make_ray_view(n, self.nfreq, rays)
         self._keep = arrays
         assert self._lib.orc_upload_rays(self._h, C.byref(v)) == 0
+
+    def update_cosmo_expansion(self, a_current, a_prev):
+        self._lib.orc_update_cosmo_expansion(self._h, float(a_current), float(a_prev))
+
+    def set_dt(self, dt):
+        self._lib.orc_set_dt(self._h, float(dt))
+
+    def cell_size(self):
+        return float(self._lib.orc_get_cell_size(self._h))
 
     def write_ray_file(self, path):
         rc = self._lib.orc_write_ray_file(self._h, str(path).encode())

@@ -12,7 +12,7 @@ STATE_FIELDS = ["nH1", "nHe1", "nH2", "nHe2", "ne", "n_tot", "f_H1", "f_He1", "t
                 "avg_nh1", "avg_nhe1", "gamma_H1_tot", "gamma_He1_tot", "gamma_H1_nu",
                 "gamma_He1_nu", "recrate_H2", "recrate_He2", "treion", "treion_flag",
                 "heat_rate", "cool_rate", "IF_speed", "Inc_Flux", "nH1_prev", "recomb_H2",
-                "recomb_He2", "ngam_nu"]
+                "recomb_He2", "ngam_nu", "rho", "f_He3", "mfp", "mfp_912"]
 
 
 RAY_FILE_FIELDS = [("order", "order"), ("pixel", "pixel"), ("itheta", "itheta"), ("xhat", "xhat"),

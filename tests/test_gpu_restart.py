@@ -97,10 +97,12 @@ def test_gas_and_treion_files(tmp_path):
         cols = raw[1:].reshape(n3, 18 + F)
         want = {k: g.download_field(k) for k in GAS_COLUMNS if k in FIELDS}
         for j, name in enumerate(GAS_COLUMNS):
-            if name in want:
+            if name in want and name not in extras:
                 assert bits_equal(cols[:, j], want[name]), name
         assert bits_equal(cols[:, 0], extras["rho"]) and bits_equal(cols[:, 10], extras["mfp"])
-        assert np.all(cols[:, 7] == 0) and np.all(cols[:, 11] == 0) and np.all(cols[:, 15] == 1)
+        assert np.all(cols[:, 7] == 0) and np.all(cols[:, 15] == 1) and np.all(cols[:, 16] == 0)
+        assert bits_equal(cols[:, 11], g.download_field("mfp_912"))
+        assert bits_equal(cols[:, 17], g.download_field("treion"))  # treion_out is set with treion
         assert bits_equal(cols[:, 18:], g.download_field("ngam_nu").reshape(n3, F))
         t = np.frombuffer(tre.read_bytes(), np.dtype([("treion", "<f4"), ("treion_flag", "u1"),
                                                       ("tcross_flag", "u1"), ("neutral_flag", "u1")]))
@@ -109,13 +111,23 @@ def test_gas_and_treion_files(tmp_path):
         assert np.array_equal(t["treion_flag"], (g.download_field("treion_flag") != 0).astype(np.uint8))
         assert np.array_equal(t["tcross_flag"], flags["tcross_flag"])
         assert np.array_equal(t["neutral_flag"], flags["neutral_flag"])
+        # without extras: the held rho and mfp, neutral_flag TRUE (rt_funcs.cc:69)
+        g.write_gas_file(gas)
+        g.write_treion_file(tre)
+        cols = np.frombuffer(gas.read_bytes(), "<f4")[1:].reshape(n3, 18 + F)
+        assert bits_equal(cols[:, 0], g.download_field("rho"))
+        assert bits_equal(cols[:, 10], g.download_field("mfp"))
+        assert np.all(np.frombuffer(tre.read_bytes(), "u1").reshape(n3, 7)[:, 6] == 1)
+        g.write_gas_file(gas, extras)
+        g.write_treion_file(tre, flags)
         with FlexRT(dict(wl["cfg"], precision=0)) as h:
             inputs.setup(h, wl)
             got = h.read_gas_file(gas)
             fl = h.read_treion_file(tre)
             assert np.float32(h.get_time()) == np.float32(g.get_time())
             for name in list(want) + ["ngam_nu", "treion", "treion_flag"]:
-                assert bits_equal(h.download_field(name), g.download_field(name)), name
+                ref = extras[name] if name in extras else g.download_field(name)
+                assert bits_equal(h.download_field(name), ref), name
             assert bits_equal(got["rho"], extras["rho"]) and np.all(got["fion"] == 1)
             assert np.array_equal(fl["neutral_flag"], flags["neutral_flag"])
     finally:
