@@ -17,7 +17,9 @@ path is bit-identical to the CPU checker, see tests/test_gpu_parity.py).
 The CPU checker runs a bounded sample of the same workload on one host core.
 With --restart-files DIR, the mixed run's restart files (ray, gas, treion in
 the reference's binary formats) are also written and read back, and the CPU
-checker's ray-file writer is timed on a sample of the same rays.
+checker's ray-file writer is timed on a sample of the same rays.  With
+--driver-ops, the driver's per-step device ops (cosmic expansion, hydro-step
+reload) are timed on the final state.
 
 Prints one JSON line.
 """
@@ -55,6 +57,10 @@ def parse():
                     help="after the mixed run, time writing and reading its restart files "
                          "(ray, gas, treion) in DIR (removed afterwards), and the CPU checker's "
                          "ray-file writer on a sample of the same rays")
+    ap.add_argument("--driver-ops", action="store_true",
+                    help="after the mixed run, time the driver's per-step device ops "
+                         "(update_cosmo_expansion, update_hydro_step) on its state, and the CPU "
+                         "checker's on the same grid")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the runs, copy MIB MiB through the library's plain copy kernel "
                          "(reference dispatch for HBM counter calibration)")
@@ -99,7 +105,7 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-def run_form(wl, precision, steps, warmup, profile, files_dir=None):
+def run_form(wl, precision, steps, warmup, profile, files_dir=None, driver_ops=False):
     """Warm up, then time `steps` steps of the workload from its initial state
     without instrumentation (the throughput), then, if `profile`, run the same
     steps again from the initial state with every launch bracketed by HIP
@@ -141,8 +147,63 @@ def run_form(wl, precision, steps, warmup, profile, files_dir=None):
         ktimes = {k: g.kernel_time(k) for k in KERNELS}
         g.set_profiling(False)
     files = restart_files(g, wl, files_dir) if files_dir else None
+    if driver_ops:
+        files = dict(files or {}, driver_ops=time_driver_ops(g, wl))
     g.close()
     return wall, stats, ktimes, state, wall_prof, files
+
+
+def time_driver_ops(g, wl, reps=5, sample_rays=2_000_000):
+    """update_cosmo_expansion and update_hydro_step on the context's state
+    (HIP-event kernel times, mean of `reps` calls; algorithmic bytes: per cell
+    4 fields read and 12 fields + the packed opacity pair written = 72 B, per
+    ray one 16-byte position read and written = 32 B), and the CPU checker's
+    update_cosmo_expansion on the same cells and a sample of the same rays."""
+    from oracle.pyoracle import Oracle
+
+    n3 = int(wl["cfg"]["n"]) ** 3
+    nrays = g.ray_count()
+    a0 = float(wl["cfg"]["a_rt"])
+    g.synchronize()
+    g.set_profiling(True)
+    g.reset_kernel_times()
+    for r in range(reps):
+        g.update_cosmo_expansion(np.float32(a0 * 1.001 ** (r + 1)), np.float32(a0 * 1.001 ** r))
+    g.synchronize()
+    kc, nc = g.kernel_time("cosmo_cells")
+    kr, nr = g.kernel_time("cosmo_rays")
+    g.reset_kernel_times()
+    g.update_hydro_step(np.float32(wl["cfg"]["zz"]), np.float32(a0))
+    g.synchronize()
+    kh, _ = g.kernel_time("cosmo_cells")
+    g.set_profiling(False)
+    cell_ms, ray_ms = kc / max(nc, 1), kr / max(nr, 1)
+    out = {"cells": n3, "rays": nrays,
+           "cosmo_cells_ms": round(cell_ms, 4), "cosmo_cells_GBps": round(72 * n3 / (cell_ms * 1e-3) / 1e9, 1),
+           "cosmo_rays_ms": round(ray_ms, 4), "cosmo_rays_GBps": round(32 * nrays / (ray_ms * 1e-3) / 1e9, 1),
+           "hydro_step_cells_ms": round(kh, 4)}
+    rays = g.rays()
+    m = min(len(rays["cell"]), sample_rays)
+    nf = int(wl["cfg"]["nfreq"])
+    part = {k: np.ascontiguousarray(v[:m] if k != "nphot" else v[: m * nf]) for k, v in rays.items()}
+    o = Oracle(wl["cfg"])
+    try:
+        for k in ("rho", "f_H2", "f_He2", "f_He3"):
+            o.upload_field(k, g.download_field(k))
+        t0 = time.perf_counter()
+        o.update_cosmo_expansion(np.float32(a0 * 1.001), np.float32(a0))
+        tc = time.perf_counter() - t0
+        o.upload_rays(part)
+        t0 = time.perf_counter()
+        o.update_cosmo_expansion(np.float32(a0 * 1.001), np.float32(a0))
+        tr = time.perf_counter() - t0 - tc
+    finally:
+        o.close()
+    out["cpu_baseline"] = {"kind": "port", "cores": 1, "cells_ms": round(1e3 * tc, 2),
+                           "rays_sample": m, "rays_ms_per_million": round(1e3 * tr / m * 1e6, 2)}
+    log("driver ops: cells %.3f ms, rays %.3f ms (GPU); cells %.1f ms (CPU)" % (cell_ms, ray_ms, 1e3 * tc))
+    return out
+
 
 
 def restart_files(g, wl, d, sample=2_000_000):
@@ -291,7 +352,8 @@ def main():
     log("workload %s built: %d source cells" % (a.workload, len(wl["src_cells"])))
     steps = a.steps
     n3 = float(wl["cfg"]["n"]) ** 3
-    wall_m, st_m, kt_m, state_m, wp_m, files = run_form(wl, 1, steps, a.warmup, True, a.restart_files)
+    wall_m, st_m, kt_m, state_m, wp_m, files = run_form(wl, 1, steps, a.warmup, True, a.restart_files,
+                                                        a.driver_ops)
     out = {
         "metric": METRIC,
         "value": round(n3 * steps / wall_m, 1),
@@ -341,6 +403,8 @@ def main():
             "T_max_rel": rel(state_m["temp"], state_f["temp"]),
             "ray_counts_exact": all(all(sm[k] == sf[k] for k in keys) for sm, sf in zip(st_m, st_f)),
         }
+    if files and "driver_ops" in files:
+        out["driver_ops"] = files.pop("driver_ops")
     if files:
         out["restart_files"] = files
     if not a.no_cpu:

@@ -28,7 +28,7 @@ FIELDS = [
     "temp", "dne_dt", "clump", "nH1_prev", "avg_nh1", "avg_nhe1", "recomb_H2", "recomb_He2",
     "gamma_H1_tot", "gamma_He1_tot", "recrate_H2", "recrate_He2", "treion", "treion_flag",
     "heat_rate", "cool_rate", "IF_speed", "Inc_Flux", "gamma_H1_nu", "gamma_He1_nu", "ngam_nu",
-    "rho", "f_He3", "mfp", "mfp_912",
+    "rho", "f_He3", "mfp", "mfp_912", "avg_gamma_prev",
 ]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 PER_BIN = {"gamma_H1_nu", "gamma_He1_nu", "ngam_nu"}
@@ -79,6 +79,27 @@ class FrtRayView(C.Structure):
 # evolve, in struct order.
 EXTRA_FLOAT_FIELDS = ["rho", "f_He3", "mfp", "mfp_912", "fion", "tcross", "treion_out"]
 EXTRA_FLAG_FIELDS = ["tcross_flag", "neutral_flag"]
+
+
+# write_otf's averages (frt_otf): (name, values); float fields except npht, nsrc.
+OTF_FIELDS = [("fH1", 2), ("fH2", 2), ("fHe1", 2), ("fHe2", 2), ("fHe3", 2), ("nH1", 1), ("nHe1", 1),
+              ("nH", 3), ("nHe", 3), ("ne", 1), ("GH1", 7), ("RH2", 5), ("T", 5), ("MFP", 6),
+              ("MFP_912", 4), ("Heat", 5), ("Cool", 5)]
+
+
+class FrtOtf(C.Structure):
+    _fields_ = [(n, C.c_float * k if k > 1 else C.c_float) for n, k in OTF_FIELDS] + \
+               [("npht", C.c_double), ("nsrc", C.c_int32), ("fion", C.c_float * 2)]
+
+    def as_dict(self):
+        d = {}
+        for n, k in OTF_FIELDS:
+            v = getattr(self, n)
+            d[n] = list(v) if k > 1 else v
+        d["npht"] = self.npht
+        d["nsrc"] = self.nsrc
+        d["fion"] = list(self.fion)
+        return d
 
 
 class FrtFileExtras(C.Structure):

@@ -6,7 +6,7 @@ import of any GPU entry point raises: there is no CPU fallback.
 import ctypes as C
 import os
 
-from ._abi import FrtConfig, FrtFileExtras, FrtRayView, FrtStepStats
+from ._abi import FrtConfig, FrtFileExtras, FrtOtf, FrtRayView, FrtStepStats
 from ._build import LIB_PATH
 
 _lib = None
@@ -28,6 +28,9 @@ _PROTOS = {
     "frt_set_dt": (C.c_int, [_CTX, C.c_float]),
     "frt_get_cell_size": (C.c_int, [_CTX, C.POINTER(C.c_float)]),
     "frt_update_cosmo_expansion": (C.c_int, [_CTX, C.c_float, C.c_float]),
+    "frt_update_hydro_step": (C.c_int, [_CTX, C.c_float, C.c_float]),
+    "frt_otf_averages": (C.c_int, [_CTX, C.c_float, C.c_float, C.POINTER(FrtOtf)]),
+    "frt_write_otf": (C.c_int, [_CTX, C.c_char_p, C.c_float, C.c_float]),
     "frt_init_rt": (C.c_int, [_CTX]),
     "frt_get_nphot_min_max": (C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "frt_new_rays": (C.c_int, [_CTX]),
@@ -43,6 +46,7 @@ _PROTOS = {
     "frt_upload_rays": (C.c_int, [_CTX, C.POINTER(FrtRayView)]),
     "frt_download_paths": (C.c_int, [_CTX, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float)]),
+    "frt_write_vif_file": (C.c_int, [_CTX, C.c_char_p]),
     "frt_write_ray_file": (C.c_int, [_CTX, C.c_char_p]),
     "frt_read_ray_file": (C.c_int, [_CTX, C.c_char_p]),
     "frt_write_gas_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),

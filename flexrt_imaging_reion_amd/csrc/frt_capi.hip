@@ -1,5 +1,6 @@
 // frt_capi.hip -- context management and the extern "C" entry points.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 
@@ -234,6 +235,7 @@ float* field_ptr(frt_ctx* ctx, int field) {
     case FRT_F_FHE3: return ctx->c.f[C_FHE3];
     case FRT_F_MFP: return ctx->c.f[C_MFP];
     case FRT_F_MFP_912: return ctx->c.f[C_MFP912];
+    case FRT_F_GAMMA_PREV: return ctx->c.f[C_GAMMA_PREV];
     default: return nullptr;
   }
 }
@@ -316,6 +318,23 @@ int frt_config_default(frt_config* cfg, int32_t n) {
   return FRT_OK;
 }
 
+// dx = (Lx0*a)/Nx*kpc_to_cm with float Lx0, a (gas_funcs.cc:592,710,
+// init_funcs.cc:22-24), and what follows from it.
+static void set_cell_size(GridConst& g, const frt_config& c, float a_rt) {
+  const float Lx = (float)c.box_ckpc * a_rt;
+  g.dx = (float)((double)(Lx / (float)c.n) * FRT_KPC_TO_CM);
+  g.sf = g.dx / g.dx0;
+  g.vol = (double)g.dx * (double)g.dx * (double)g.dx;
+}
+
+// Redshift-dependent constants of update_thermal (gas_funcs.cc:362-458).
+static void set_redshift(GridConst& g, const frt_config& c, float zz) {
+  g.zz = zz;
+  g.hubble = frt_hubble((float)(1. / (1. + (double)zz)), c.hh, c.omega_m, c.omega_l);
+  g.compton_coef = frt_compton_rate((double)zz);
+  g.t_cmb = 2.726 * (double)(1.f + zz);
+}
+
 int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   if (!cfg_in || !out) return FRT_E_ARG;
   *out = nullptr;
@@ -353,9 +372,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   const float Lx0 = (float)c.box_ckpc;
   const float a_rt = (float)c.a_rt;
   g.dx0 = (float)((double)(Lx0 / (float)c.n) * FRT_KPC_TO_CM);
-  const float Lx = Lx0 * a_rt;
-  g.dx = (float)((double)(Lx / (float)c.n) * FRT_KPC_TO_CM);
-  g.sf = g.dx / g.dx0;
+  set_cell_size(g, c, a_rt);
   const float clf = (float)c.cl_factor;
   g.clight = (float)((double)clf * FRT_CL);
   g.nc = (float)c.nl_const;
@@ -365,7 +382,6 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     g.dt = (float)((double)(1.0f * g.dx / clf) / FRT_CL);
   g.min_dist = (float)c.min_dist;
   g.temp_min = (float)c.temp_min;
-  g.vol = (double)g.dx * (double)g.dx * (double)g.dx;
   g.l_max = c.l_max;
   g.hpx_lvl = c.hpx_lvl;
   g.frac_limit = c.frac_limit;
@@ -374,11 +390,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   g.recomb_cool = c.recomb_cool;
   g.free_free = c.free_free;
   g.compton = c.compton;
-  const float zz = (float)c.zz;
-  g.zz = zz;
-  g.hubble = frt_hubble((float)(1. / (1. + (double)zz)), c.hh, c.omega_m, c.omega_l);
-  g.compton_coef = frt_compton_rate((double)zz);
-  g.t_cmb = 2.726 * (double)(1.f + zz);
+  set_redshift(g, c, (float)c.zz);
   g.nhe_to_nh = (float)((FRT_Y_HE / 4.) / (1. - FRT_Y_HE));
   g.sig912 = frt_sigmapi_H1(13.6 / FRT_H_EV);  // mfp_912 (ray_tracing.cc:1080)
   const float safety = (float)c.safety_factor;
@@ -408,6 +420,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->have_segments = false;
   ctx->have_alive = false;
   ctx->have_rho = false;
+  ctx->otf_buf = nullptr;
 
   int rc = FRT_OK;
   for (int b = 0; b < 2 && rc == FRT_OK; ++b) rc = alloc_rays(ctx, ctx->buf[b], cap, c.nfreq);
@@ -484,6 +497,7 @@ int frt_destroy(frt_ctx* ctx) {
   }
   hipFree(ctx->contrib.rec); hipFree(ctx->path_slot); hipFree(ctx->pdf);  // rec is the block base
   hipFree(ctx->cell_start); hipFree(ctx->cell_end); hipFree(ctx->cell_block); hipFree(ctx->c.avg2);
+  hipFree(ctx->otf_buf);
   hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
   hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt);
   if (ctx->hcnt) hipHostFree(ctx->hcnt);
@@ -604,17 +618,124 @@ int frt_update_cosmo_expansion(frt_ctx* ctx, float a_current, float a_prev) {
   if (!ctx) return FRT_E_ARG;
   if (!(a_current > 0.f) || !(a_prev > 0.f)) return set_error(ctx, FRT_E_ARG, "scale factors must be positive");
   if (!ctx->have_rho) return set_error(ctx, FRT_E_ARG, "update_cosmo_expansion needs rho: upload FRT_F_RHO first");
-  // dx[i] *= a_current/a_prev (float); rho *= pow(a_prev/a_current, 3.): the
-  // square of a float is exact in double, so q*q*q is the correctly rounded cube
+  // dx[i] *= a_current/a_prev (float); rho *= pow(a_prev/a_current, 3.), the
+  // scalar computed on the host with the C library's pow, as the reference does
   const float ratio = a_current / a_prev;
-  const double q = (double)(a_prev / a_current);
-  const double rho_scale = q * q * q;
+  const double rho_scale = std::pow((double)(a_prev / a_current), 3.);
   ctx->g.dx = ctx->g.dx * ratio;
   ctx->g.sf = ctx->g.dx / ctx->g.dx0;
   ctx->g.vol = (double)ctx->g.dx * (double)ctx->g.dx * (double)ctx->g.dx;
-  int rc = launch_cosmo_expansion(ctx, ratio, rho_scale);
+  int rc = launch_cosmo_expansion(ctx, ratio, rho_scale, true);
   if (rc) return rc;
   return hip_check(ctx, hipGetLastError(), "update_cosmo_expansion");
+}
+
+int frt_update_hydro_step(frt_ctx* ctx, float zz, float a_rt) {
+  if (!ctx) return FRT_E_ARG;
+  if (!(a_rt > 0.f) || !(zz > -1.f)) return set_error(ctx, FRT_E_ARG, "need a_rt > 0 and zz > -1");
+  if (!ctx->have_rho) return set_error(ctx, FRT_E_ARG, "update_hydro_step needs rho: upload FRT_F_RHO first");
+  ctx->cfg.zz = zz;
+  ctx->cfg.a_rt = a_rt;
+  set_redshift(ctx->g, ctx->cfg, zz);
+  set_cell_size(ctx->g, ctx->cfg, a_rt);  // Lx = Lx0*a_rt, dx = Lx/Nx*kpc_to_cm
+  int rc = launch_cosmo_expansion(ctx, 1.f, std::pow((double)a_rt, -3.), false);
+  if (rc) return rc;
+  return hip_check(ctx, hipGetLastError(), "update_hydro_step");
+}
+
+int frt_otf_averages(frt_ctx* ctx, float avg_rho, float temp_0, frt_otf* out) {
+  if (!ctx || !out) return FRT_E_ARG;
+  int rc = ensure_compact(ctx);  // count_photons_io counts the rays in the buffer
+  if (rc) return rc;
+  const size_t nd = (size_t)FRT_OTF_BLOCKS * (FRT_OTF_NV + 1) + 64;
+  if (!ctx->otf_buf && hipMalloc((void**)&ctx->otf_buf, nd * sizeof(double)) != hipSuccess) {
+    ctx->otf_buf = nullptr;
+    return set_error(ctx, FRT_E_NOMEM, "otf buffer");
+  }
+  double* dout = ctx->otf_buf + (size_t)FRT_OTF_BLOCKS * (FRT_OTF_NV + 1);
+  rc = launch_otf(ctx, avg_rho, temp_0, ctx->otf_buf, dout);
+  if (rc) return rc;
+  double S[FRT_OTF_NV + 1];
+  rc = hip_check(ctx, stream_copy(ctx, S, dout, sizeof(S), hipMemcpyDeviceToHost), "otf");
+  if (rc) return rc;
+  // calc_vol_avg: (float)tot / pow(N, 3.); calc_mass_avg: (float)tot / m_tot
+  const double N3 = (double)ctx->g.n3;
+  auto vol = [&](int i) { return (float)((double)(float)S[i] / N3); };
+  auto mass = [&](int i, int m) { return (float)((double)(float)S[i] / S[m]); };
+  frt_otf& o = *out;
+  memset(&o, 0, sizeof(o));
+  o.fH1[0] = vol(0); o.fH1[1] = mass(17, 7);
+  o.fH2[0] = vol(1); o.fH2[1] = mass(18, 7);
+  o.fHe1[0] = vol(2); o.fHe1[1] = mass(19, 8);
+  o.fHe2[0] = vol(3); o.fHe2[1] = mass(20, 8);
+  o.fHe3[0] = vol(4); o.fHe3[1] = mass(21, 8);
+  o.nH1 = vol(5);
+  o.nHe1 = vol(6);
+  // fion = 1: weighting by fion is the volume average (m_tot = N^3 exactly)
+  o.nH[0] = o.nH[1] = vol(7); o.nH[2] = mass(23, 22);
+  o.nHe[0] = o.nHe[1] = vol(8); o.nHe[2] = mass(24, 22);
+  o.ne = vol(9);
+  const float GH1[7] = {vol(10), vol(10), mass(25, 22), mass(32, 7), mass(34, 33), (float)S[46], (float)S[45]};
+  const float RH2[5] = {vol(11), vol(11), mass(26, 22), mass(35, 7), mass(37, 36)};
+  const float T[5] = {vol(12), vol(12), mass(27, 22), mass(38, 7), mass(40, 39)};
+  const float MFP[6] = {vol(13), vol(13), mass(28, 22), mass(41, 7), (float)S[48], (float)S[47]};
+  const float M912[4] = {vol(14), vol(14), mass(29, 22), mass(42, 7)};
+  const float Heat[5] = {vol(15), vol(15), mass(30, 22), mass(43, 7), mass(43, 7)};
+  const float Cool[5] = {vol(16), vol(16), mass(31, 22), mass(44, 7), mass(44, 7)};
+  memcpy(o.GH1, GH1, sizeof(GH1));
+  memcpy(o.RH2, RH2, sizeof(RH2));
+  memcpy(o.T, T, sizeof(T));
+  memcpy(o.MFP, MFP, sizeof(MFP));
+  memcpy(o.MFP_912, M912, sizeof(M912));
+  memcpy(o.Heat, Heat, sizeof(Heat));
+  memcpy(o.Cool, Cool, sizeof(Cool));
+  o.npht = S[FRT_OTF_NV] * FRT_PHOT_NORM;
+  o.nsrc = ctx->nsrc;
+  o.fion[0] = (float)((double)(float)N3 / N3);
+  o.fion[1] = (float)((double)(float)S[7] / S[7]);
+  return FRT_OK;
+}
+
+int frt_write_otf(frt_ctx* ctx, const char* path, float avg_rho, float temp_0) {
+  if (!ctx || !path) return FRT_E_ARG;
+  frt_otf o;
+  int rc = frt_otf_averages(ctx, avg_rho, temp_0, &o);
+  if (rc) return rc;
+  FILE* f = fopen(path, "a");
+  if (!f) return set_error(ctx, FRT_E_IO, std::string("cannot open ") + path + " for appending");
+  const double yr_to_s = 3.154e7;  // global_constants.h:43
+  // the reference's lines (io_funcs.cc:557-562, 633-654)
+  fprintf(f, "Step Number    : %d\n", (int)ctx->step);
+  fprintf(f, "Redshift       : %le\n", (double)ctx->g.zz);
+  fprintf(f, "Time [Myr]     : %le\n", ctx->t / yr_to_s / 1e6);
+  fprintf(f, "Time Step [Myr]: %le\n", ctx->g.dt / yr_to_s / 1e6);
+  fprintf(f, "\n");
+  fprintf(f, "Global Averages\n");
+  fprintf(f, "fH1     : %le    %le\n", o.fH1[0], o.fH1[1]);
+  fprintf(f, "fH2     : %le    %le\n", o.fH2[0], o.fH2[1]);
+  fprintf(f, "fHe1    : %le    %le\n", o.fHe1[0], o.fHe1[1]);
+  fprintf(f, "fHe2    : %le    %le\n", o.fHe2[0], o.fHe2[1]);
+  fprintf(f, "fHe3    : %le    %le\n", o.fHe3[0], o.fHe3[1]);
+  fprintf(f, "nH1     : %le       \n", o.nH1);
+  fprintf(f, "nHe1    : %le       \n", o.nHe1);
+  fprintf(f, "nH      : %le    %le    %le\n", o.nH[0], o.nH[1], o.nH[2]);
+  fprintf(f, "nHe     : %le    %le    %le\n", o.nHe[0], o.nHe[1], o.nHe[2]);
+  fprintf(f, "ne      : %le       \n", o.ne);
+  fprintf(f, "GH1     : %le    %le    %le    %le    %le    %le    %le\n", o.GH1[0], o.GH1[1], o.GH1[2], o.GH1[3],
+          o.GH1[4], o.GH1[5], o.GH1[6]);
+  fprintf(f, "RH2     : %le    %le    %le    %le    %le\n", o.RH2[0], o.RH2[1], o.RH2[2], o.RH2[3], o.RH2[4]);
+  fprintf(f, "T       : %le    %le    %le    %le    %le\n", o.T[0], o.T[1], o.T[2], o.T[3], o.T[4]);
+  fprintf(f, "MFP     : %le    %le    %le    %le    %le    %le\n", o.MFP[0], o.MFP[1], o.MFP[2], o.MFP[3], o.MFP[4],
+          o.MFP[5]);
+  fprintf(f, "MFP_912 : %le    %le    %le    %le\n", o.MFP_912[0], o.MFP_912[1], o.MFP_912[2], o.MFP_912[3]);
+  fprintf(f, "Heat    : %le    %le    %le     %le    %le\n", o.Heat[0], o.Heat[1], o.Heat[2], o.Heat[3], o.Heat[4]);
+  fprintf(f, "Cool    : %le    %le    %le     %le    %le\n", o.Cool[0], o.Cool[1], o.Cool[2], o.Cool[3], o.Cool[4]);
+  fprintf(f, "npht    : %le       \n", o.npht);
+  fprintf(f, "nsrc    : %d        \n", o.nsrc);
+  fprintf(f, "fion    : %le    %le\n", o.fion[0], o.fion[1]);
+  fprintf(f, "\n");
+  if (fclose(f) != 0) return set_error(ctx, FRT_E_IO, std::string("write to ") + path + " failed");
+  return FRT_OK;
 }
 
 int frt_get_time(const frt_ctx* ctx, float* t) {
@@ -914,7 +1035,9 @@ int frt_copy_kernel(frt_ctx* ctx, int64_t nbytes, int32_t repeats) {
 
 int frt_synchronize(frt_ctx* ctx) {
   if (!ctx) return FRT_E_ARG;
-  return hip_check(ctx, hipStreamSynchronize(ctx->stream), "synchronize");
+  const int rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "synchronize");
+  if (rc == FRT_OK && ctx->prof) prof_resolve(ctx);  // kernel times of calls outside frt_step
+  return rc;
 }
 
 // ---- restart files ----------------------------------------------------------
@@ -1060,6 +1183,10 @@ int frt_read_gas_file(frt_ctx* ctx, const char* path, frt_file_extras* extras) {
     rc = frt_upload_field(ctx, hf[1], col.data());
     if (rc) return rc;
   }
+  // restart: avg_gamma = gamma_H1_tot, avg_gamma_prev = avg_gamma (init_funcs.cc:115-118)
+  column(8);
+  rc = frt_upload_field(ctx, FRT_F_GAMMA_PREV, col.data());
+  if (rc) return rc;
   std::vector<float> ng((size_t)n3 * F);
   for (int64_t c = 0; c < n3; ++c)
     for (int nu = 0; nu < F; ++nu) ng[(size_t)c * F + nu] = v[1 + (size_t)c * W + FRT_GAS_FILE_FIXED + nu];
@@ -1093,6 +1220,22 @@ int frt_write_treion_file(frt_ctx* ctx, const char* path, const frt_file_extras*
     o[6] = (extras && extras->neutral_flag) ? (extras->neutral_flag[c] ? 1 : 0) : 1;
   }
   return write_file(ctx, path, h.data(), h.size());
+}
+
+int frt_write_vif_file(frt_ctx* ctx, const char* path) {
+  if (!ctx || !path) return FRT_E_ARG;
+  const size_t n3 = (size_t)ctx->g.n3;
+  float4* d = nullptr;
+  if (hipMalloc((void**)&d, n3 * sizeof(float4)) != hipSuccess) return set_error(ctx, FRT_E_NOMEM, "vIF staging buffer");
+  std::vector<float> h(4 * n3);
+  int rc = launch_pack_vif_file(ctx, d);
+  hipError_t e = (rc == FRT_OK) ? stream_copy(ctx, h.data(), d, n3 * sizeof(float4), hipMemcpyDeviceToHost) : hipSuccess;
+  hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (rc) return rc;
+  rc = hip_check(ctx, e, "vIF file");
+  if (rc) return rc;
+  return write_file(ctx, path, h.data(), h.size() * 4);
 }
 
 int frt_read_treion_file(frt_ctx* ctx, const char* path, frt_file_extras* extras) {

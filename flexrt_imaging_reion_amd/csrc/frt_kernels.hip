@@ -2105,10 +2105,156 @@ __global__ void __launch_bounds__(BLK) k_pack_gas_file(CellPtrs cf, GridConst g,
   FRT_FOR_BINS(nu) o[FRT_GAS_FILE_FIXED + nu] = cf.ngam_nu[(int64_t)nu * n3 + k];
 }
 
-// update_cosmo_expansion's cell loop (gas_funcs.cc:483-512) with fion = 1
-// (rt_funcs.cc:55): rho scaled by the cube of a_prev/a_current, the number
-// densities recomputed from it and the ionized fractions, in the reference's
-// float / double promotions.  The cell order does not matter here.
+// write_vIF_binary (io_funcs.cc:396-430): per row-major cell IF_speed,
+// Inc_Flux, fion and Delta_fion; without I-front tracking fion = 1
+// (rt_funcs.cc:55) and Delta_fion = 0 (ray_tracing.cc:645).
+__global__ void __launch_bounds__(BLK) k_pack_vif_file(CellPtrs cf, GridConst g, float4* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (r >= g.n3) return;
+  const int64_t k = frt_cell_from_rowmajor(g.lay, (int)r);
+  out[r] = make_float4(cf.f[C_IF_SPEED][k], cf.f[C_INC_FLUX][k], 1.f, 0.f);
+}
+
+int launch_pack_vif_file(frt_ctx* ctx, float4* out) {
+  FRT_LAUNCH(ctx, "pack_vif_file", grid_for(ctx->g.n3), k_pack_vif_file, ctx->c, ctx->g, out);
+  return FRT_OK;
+}
+
+// ---- write_otf reductions (io_funcs.cc:500-657) ------------------------------
+// Each thread accumulates FRT_OTF_NV values over a grid-stride range of
+// cells: double sums (calc_vol_avg / calc_mass_avg, data_funcs.cc:75-112)
+// and float extremes (maxarr from 0, minarr from 1e32, data_funcs.cc:22-51);
+// the block reduces them with a fixed butterfly and writes one partial per
+// value; k_otf_final adds the partials in block order.  The result does not
+// depend on the schedule; it differs from a serial double sum only by the
+// order of the additions.
+__device__ __forceinline__ bool otf_is_max(int i) { return i == FRT_OTF_NSUM || i == FRT_OTF_NSUM + 2; }
+__device__ __forceinline__ bool otf_is_min(int i) { return i == FRT_OTF_NSUM + 1 || i == FRT_OTF_NSUM + 3; }
+
+__device__ __forceinline__ void otf_block_store(double (&v)[FRT_OTF_NV], double* __restrict__ part) {
+  __shared__ double red[BLK / 64][FRT_OTF_NV];
+#pragma unroll
+  for (int i = 0; i < FRT_OTF_NV; ++i) {
+    double x = v[i];
+    for (int off = 32; off > 0; off >>= 1) {
+      const double y = __shfl_xor(x, off, 64);
+      x = otf_is_max(i) ? fmax(x, y) : otf_is_min(i) ? fmin(x, y) : x + y;
+    }
+    v[i] = x;
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int i = 0; i < FRT_OTF_NV; ++i) red[w][i] = v[i];
+  }
+  __syncthreads();
+  const int i = threadIdx.x;
+  if (i < FRT_OTF_NV) {
+    double x = red[0][i];
+    for (int q = 1; q < BLK / 64; ++q)
+      x = otf_is_max(i) ? fmax(x, red[q][i]) : otf_is_min(i) ? fmin(x, red[q][i]) : x + red[q][i];
+    part[(int64_t)blockIdx.x * FRT_OTF_NV + i] = x;
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_otf_cells(CellPtrs cf, int64_t n3, float avg_rho, float temp_0,
+                                                   double* __restrict__ part) {
+  double v[FRT_OTF_NV];
+#pragma unroll
+  for (int i = 0; i < FRT_OTF_NV; ++i) v[i] = 0.;
+  v[FRT_OTF_NSUM + 1] = v[FRT_OTF_NSUM + 3] = (double)1e32f;
+  for (int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x; k < n3; k += (int64_t)gridDim.x * BLK) {
+    const double fH1 = cf.f[C_FH1][k], fH2 = cf.f[C_FH2][k], fHe1 = cf.f[C_FHE1][k];
+    const double fHe2 = cf.f[C_FHE2][k], fHe3 = cf.f[C_FHE3][k];
+    const double nH = cf.f[C_NH][k], nHe = cf.f[C_NHE][k], nH1 = cf.f[C_NH1][k], nH2 = cf.f[C_NH2][k];
+    const float Gf = cf.f[C_GAMMA_H1_TOT][k], Gp = cf.f[C_GAMMA_PREV][k];
+    const float Tf = cf.f[C_TEMP][k], mfpf = cf.f[C_MFP][k];
+    const double G = Gf, rec = cf.f[C_RECRATE_H2][k], T = Tf, mfp = mfpf, mfp912 = cf.f[C_MFP912][k];
+    const double heat = cf.f[C_HEAT][k], cool = cf.f[C_COOL][k];
+    // binary (io_funcs.cc:528-533) with fion = 1
+    const bool bin = (Gf >= 3e-14f) && (Gp >= 3e-14f) && (fabs(dm_log10(Gf) - dm_log10(Gp)) < 1.);
+    const double b = bin ? 1. : 0.;
+    // mean-density cells and their temperature (:535-549, subgrid off)
+    const float rr = cf.f[C_RHO][k] / avg_rho;
+    const bool rm = ((double)rr > 0.9) && ((double)rr < 1.1);
+    const float fH2f = cf.f[C_FH2][k];
+    const double temp0 = rm ? (double)(float)((double)(Tf * fH2f) + (double)temp_0 * (1. - (double)fH2f))
+                            : (double)temp_0;
+    const double r = rm ? 1. : 0.;
+    v[0] += fH1; v[1] += fH2; v[2] += fHe1; v[3] += fHe2; v[4] += fHe3;
+    v[5] += (double)cf.f[C_AVG_NH1][k]; v[6] += (double)cf.f[C_AVG_NHE1][k];
+    v[7] += nH; v[8] += nHe; v[9] += (double)cf.f[C_NE][k];
+    v[10] += G; v[11] += rec; v[12] += T; v[13] += mfp; v[14] += mfp912; v[15] += heat; v[16] += cool;
+    v[17] += fH1 * nH; v[18] += fH2 * nH; v[19] += fHe1 * nHe; v[20] += fHe2 * nHe; v[21] += fHe3 * nHe;
+    v[22] += b; v[23] += nH * b; v[24] += nHe * b; v[25] += G * b; v[26] += rec * b; v[27] += T * b;
+    v[28] += mfp * b; v[29] += mfp912 * b; v[30] += heat * b; v[31] += cool * b;
+    v[32] += G * nH; v[33] += nH1; v[34] += G * nH1; v[35] += rec * nH; v[36] += nH2; v[37] += rec * nH2;
+    v[38] += T * nH; v[39] += r; v[40] += temp0 * r; v[41] += mfp * nH; v[42] += mfp912 * nH;
+    v[43] += heat * nH; v[44] += cool * nH;
+    if (Gf > (float)v[45]) v[45] = Gf;
+    if (Gf < (float)v[46]) v[46] = Gf;
+    if (mfpf > (float)v[47]) v[47] = mfpf;
+    if (mfpf < (float)v[48]) v[48] = mfpf;
+  }
+  otf_block_store(v, part);
+}
+
+// count_photons_io (io_funcs.cc:18-31): per ray the float sum over bins
+// (sum_nphot, :10-16), added in double.
+__global__ void __launch_bounds__(BLK) k_otf_rays(RaySoA rb, int64_t n, int F, int64_t cap,
+                                                  double* __restrict__ part) {
+  __shared__ double red[BLK / 64];
+  double acc = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    float nph[8];
+    phot_load(rb, i, F, cap, nph);
+    float s = 0.f;
+    FRT_FOR_BINS(nu) s += nph[nu];
+    acc += (double)s;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = red[0];
+    for (int q = 1; q < BLK / 64; ++q) x += red[q];
+    part[blockIdx.x] = x;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_otf_final(const double* __restrict__ part, int nb, const double* __restrict__ rpart,
+                                                  int nbr, double* __restrict__ out) {
+  const int i = threadIdx.x;
+  if (i < FRT_OTF_NV) {
+    double x = part[i];
+    for (int b = 1; b < nb; ++b) {
+      const double y = part[(int64_t)b * FRT_OTF_NV + i];
+      x = otf_is_max(i) ? fmax(x, y) : otf_is_min(i) ? fmin(x, y) : x + y;
+    }
+    out[i] = x;
+  } else if (i == FRT_OTF_NV) {
+    double x = 0.;
+    for (int b = 0; b < nbr; ++b) x += rpart[b];
+    out[i] = x;
+  }
+}
+
+int launch_otf(frt_ctx* ctx, float avg_rho, float temp_0, double* part, double* out) {
+  const unsigned nb = (unsigned)std::min<int64_t>((ctx->g.n3 + BLK - 1) / BLK, FRT_OTF_BLOCKS);
+  const unsigned nbr = (unsigned)std::max<int64_t>(1, std::min<int64_t>((ctx->nrays + BLK - 1) / BLK, FRT_OTF_BLOCKS));
+  double* rpart = part + (size_t)FRT_OTF_BLOCKS * FRT_OTF_NV;
+  FRT_LAUNCH(ctx, "otf_cells", nb, k_otf_cells, ctx->c, (int64_t)ctx->g.n3, avg_rho, temp_0, part);
+  FRT_LAUNCH(ctx, "otf_rays", nbr, k_otf_rays, cur_buf(ctx), ctx->nrays, ctx->g.nfreq, ctx->g.cap, rpart);
+  hipLaunchKernelGGL(k_otf_final, dim3(1), dim3(64), 0, ctx->stream, part, (int)nb, rpart, (int)nbr, out);
+  return FRT_OK;
+}
+
+// update_cosmo_expansion's cell loop (gas_funcs.cc:483-512), and
+// update_hydro_step's (:688-744) for a new density field, with fion = 1
+// (rt_funcs.cc:55): rho scaled (by (a_prev/a_current)^3, or a_rt^-3 for the
+// comoving density just read), the number densities recomputed from it and the
+// ionized fractions, in the reference's float / double promotions.  The cell
+// order does not matter here.
 __global__ void __launch_bounds__(BLK) k_cosmo_cells(CellPtrs cf, int64_t n3, double rho_scale) {
   const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (k >= n3) return;
@@ -2156,9 +2302,9 @@ __global__ void __launch_bounds__(BLK) k_cosmo_rays(float4* __restrict__ pos, in
   pos[i] = p;
 }
 
-int launch_cosmo_expansion(frt_ctx* ctx, float ratio, double rho_scale) {
+int launch_cosmo_expansion(frt_ctx* ctx, float ratio, double rho_scale, bool rays) {
   FRT_LAUNCH(ctx, "cosmo_cells", grid_for(ctx->g.n3), k_cosmo_cells, ctx->c, (int64_t)ctx->g.n3, rho_scale);
-  if (ctx->nrays > 0)
+  if (rays && ctx->nrays > 0)
     FRT_LAUNCH(ctx, "cosmo_rays", grid_for(ctx->nrays), k_cosmo_rays, cur_buf(ctx).pos, ctx->nrays, ratio,
                ctx->g.dx, ctx->g.min_dist);
   return FRT_OK;

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from . import _lib
-from ._abi import (EXTRA_FLAG_FIELDS, EXTRA_FLOAT_FIELDS, FIELD_ID, PER_BIN, PREC_FP64, PREC_MIXED,
+from ._abi import (EXTRA_FLAG_FIELDS, EXTRA_FLOAT_FIELDS, FIELD_ID, PER_BIN, PREC_FP64, PREC_MIXED, FrtOtf,
                    FrtConfig, FrtStepStats, config_struct, make_extras, make_ray_view)
 
 
@@ -39,6 +39,13 @@ def checkpoint_paths(output_dir, zz):
     zs = "%f" % float(zz)
     pad = "0" if float(zz) < 10.0 else ""
     return {k: str(output_dir) + k + "_z=" + pad + zs[:-2] for k in ("gas", "ray", "tre")}
+
+
+def vif_path(output_dir, zz, z_rt):
+    """write_vIF_binary's file name (io_funcs.cc:399-414): the digits are those
+    of z_rt = 1/a_rt - 1, the zero padding follows the snapshot redshift zz."""
+    pad = "0" if float(zz) < 10.0 else ""
+    return str(output_dir) + "vIF_z=" + pad + ("%f" % float(z_rt))[:-2]
 
 
 class FlexRT:
@@ -135,6 +142,22 @@ class FlexRT:
         factors now and one dt earlier; needs rho uploaded."""
         self._check(self._lib.frt_update_cosmo_expansion(self._h, float(a_current), float(a_prev)),
                     "update_cosmo_expansion")
+
+    def update_hydro_step(self, zz, a_rt):
+        """Device part of update_hydro_step (gas_funcs.cc:554-752): call after
+        uploading the next snapshot's comoving density as "rho" (and its
+        sources with set_sources)."""
+        self._check(self._lib.frt_update_hydro_step(self._h, float(zz), float(a_rt)), "update_hydro_step")
+
+    def otf_averages(self, avg_rho, temp_0=100.0):
+        """write_otf's global averages (io_funcs.cc:500-632), reduced on the GPU."""
+        o = FrtOtf()
+        self._check(self._lib.frt_otf_averages(self._h, float(avg_rho), float(temp_0), C.byref(o)), "otf_averages")
+        return o.as_dict()
+
+    def write_otf(self, path, avg_rho, temp_0=100.0):
+        """write_otf: appends this step's block to the text file `path`."""
+        self._check(self._lib.frt_write_otf(self._h, os.fsencode(path), float(avg_rho), float(temp_0)), "write_otf")
 
     def get_time(self):
         t = C.c_float()
@@ -241,6 +264,10 @@ class FlexRT:
         self._check(self._lib.frt_read_treion_file(self._h, os.fsencode(path), C.byref(ex)),
                     "read_treion_file")
         return {k: arrays[k] for k in EXTRA_FLAG_FIELDS}
+
+    def write_vif_file(self, path):
+        """write_vIF_binary (io_funcs.cc:396-430)."""
+        self._check(self._lib.frt_write_vif_file(self._h, os.fsencode(path)), "write_vif_file")
 
     def write_checkpoint(self, output_dir, zz, extras=None, rays=True):
         """The end-of-hydro-step output of the driver (gas_funcs.cc:617-656):

@@ -147,6 +147,7 @@ typedef enum {
   FRT_F_FHE3,          /* f_He3 (0 after a C2Ray step, ray_tracing.cc:1128,1194) */
   FRT_F_MFP,           /* mfp = 1/sigma_HI(nu_phot[0])/avg_nh1 */
   FRT_F_MFP_912,       /* mfp_912 = 1/sigma_HI(13.6 eV)/avg_nh1 */
+  FRT_F_GAMMA_PREV,    /* avg_gamma_prev (0 on a fresh start, Gamma_HI of the gas file on a restart) */
   FRT_F_COUNT
 } frt_field;
 
@@ -209,6 +210,46 @@ int frt_get_cell_size(const frt_ctx* ctx, float* dx);
  * cells are scaled and kept off the cell faces.  Requires RHO to have been
  * uploaded (or read from a gas file). */
 int frt_update_cosmo_expansion(frt_ctx* ctx, float a_current, float a_prev);
+/* The device part of update_hydro_step (gas_funcs.cc:554-752) once the driver
+ * has read the next snapshot: upload its comoving density into FRT_F_RHO
+ * (get_ugas io_funcs.cc:33-62) and its sources (frt_set_sources), then call
+ * this with the snapshot's redshift and a_rt = scale_factor(t + tinit): rho
+ * becomes proper (times a_rt^-3), the cell size Lx0 a_rt / Nx, nH, nHe, the
+ * ion densities, avg_nh1, avg_nhe1, ne, n_tot and nH1_prev are recomputed,
+ * and the redshift-dependent rates of update_thermal follow zz. */
+int frt_update_hydro_step(frt_ctx* ctx, float zz, float a_rt);
+
+/* Global averages of write_otf (io_funcs.cc:500-657), reduced on the GPU,
+ * each in the reference's form: volume average (float)(sum) / N^3 and mass
+ * average (float)(sum f m) / sum m (calc_vol_avg / calc_mass_avg,
+ * data_funcs.cc:75-112, sums in double), min / max as minarr / maxarr
+ * (data_funcs.cc:22-51).  "fion"-weighted averages use fion = 1, "binary"
+ * marks cells whose Gamma_HI and avg_gamma_prev (FRT_F_GAMMA_PREV) are both
+ * >= 3e-14 /s and within a decade (io_funcs.cc:524-532); "mean_den" is the
+ * temperature of cells within 10% of avg_rho with temp_0 for the neutral
+ * share (:535-549). */
+typedef struct frt_otf {
+  float fH1[2], fH2[2], fHe1[2], fHe2[2], fHe3[2]; /* volume, mass */
+  float nH1, nHe1;                                 /* volume averages of avg_nh1, avg_nhe1 */
+  float nH[3], nHe[3];                             /* volume, fion, binary */
+  float ne;
+  float GH1[7];     /* volume, fion, binary, mass(nH), mass(nH1), min, max */
+  float RH2[5];     /* recrate_H2: volume, fion, binary, mass(nH), mass(nH2) */
+  float T[5];       /* volume, fion, binary, mass(nH), mean_den */
+  float MFP[6];     /* volume, fion, binary, mass(nH), min, max */
+  float MFP_912[4]; /* volume, fion, binary, mass(nH) */
+  float Heat[5];    /* volume, fion, binary, mass(nH), mass(nH) */
+  float Cool[5];
+  double npht;      /* count_photons_io: photons in all rays */
+  int32_t nsrc;
+  float fion[2];
+} frt_otf;
+/* avg_rho: the driver's volume-average density (io_funcs.cc:61, scaled with
+ * the expansion); temp_0: the initial gas temperature (user_inputs.h:96). */
+int frt_otf_averages(frt_ctx* ctx, float avg_rho, float temp_0, frt_otf* out);
+/* write_otf: appends the step's block of text to path, in the reference's
+ * format (step, redshift, time and time step in Myr, then the averages). */
+int frt_write_otf(frt_ctx* ctx, const char* path, float avg_rho, float temp_0);
 
 /* ---- reference routines on the per-step path ----------------------------- */
 /* init_rt (rt_funcs.cc:36-126), fresh start: empty ray buffer, ray_dt = dt
@@ -288,6 +329,9 @@ int frt_read_gas_file(frt_ctx* ctx, const char* path, frt_file_extras* extras);
  * treion (float), treion_flag, tcross_flag, neutral_flag (bool). */
 int frt_write_treion_file(frt_ctx* ctx, const char* path, const frt_file_extras* extras);
 int frt_read_treion_file(frt_ctx* ctx, const char* path, frt_file_extras* extras);
+/* write_vIF_binary (io_funcs.cc:396-430): per row-major cell IF_speed,
+ * Inc_Flux, fion (1 without I-front tracking) and Delta_fion (0), float. */
+int frt_write_vif_file(frt_ctx* ctx, const char* path);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* When enabled, every kernel launch is bracketed by HIP events on the
@@ -302,7 +346,8 @@ int frt_deposit_bytes(const frt_ctx* ctx, double* bytes_per_launch, int64_t* lau
 /* Device-to-device copy of nbytes through a plain copy kernel, used to
  * calibrate HBM counters. */
 int frt_copy_kernel(frt_ctx* ctx, int64_t nbytes, int32_t repeats);
-/* Block until all work queued on the context stream has finished. */
+/* Block until all work queued on the context stream has finished; with
+ * profiling on, the kernel times of the finished launches are then counted. */
 int frt_synchronize(frt_ctx* ctx);
 
 #ifdef __cplusplus

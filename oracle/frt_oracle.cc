@@ -165,7 +165,7 @@ struct State {
   std::vector<float> nH, nHe, nH1, nHe1, nH2, nHe2, ne, n_tot, f_H1, f_He1, f_H2, f_He2, temp, dne_dt,
       clump, nH1_prev, avg_nh1, avg_nhe1, recomb_H2, recomb_He2, gamma_H1_tot, gamma_He1_tot,
       recrate_H2, recrate_He2, treion, treion_flag, heat_rate, cool_rate, IF_speed, Inc_Flux,
-      gamma_H1_nu, gamma_He1_nu, ngam_nu, rho, f_He3, mfp, mfp_912;
+      gamma_H1_nu, gamma_He1_nu, ngam_nu, rho, f_He3, mfp, mfp_912, gamma_prev;
   std::vector<double> avg_gamma, avg_gamma_he1, avg_gamma_nu, avg_gamma_nu_he1, ngam_acc;
   std::vector<Ray> ray;
   std::vector<int> src_cell;
@@ -221,6 +221,7 @@ static std::vector<float>* field_vec(State& s, int f) {
     case FRT_F_FHE3: return &s.f_He3;
     case FRT_F_MFP: return &s.mfp;
     case FRT_F_MFP_912: return &s.mfp_912;
+    case FRT_F_GAMMA_PREV: return &s.gamma_prev;
     default: return nullptr;
   }
 }
@@ -1302,17 +1303,13 @@ int orc_write_ray_file(void* h, const char* path) {
   return file ? 0 : 1;
 }
 
-// update_cosmo_expansion (gas_funcs.cc:460-552) for given a_current, a_prev,
-// with fion = 1 (rt_funcs.cc:55): dx grows, rho falls, the number densities
-// follow from rho and the ionized fractions, ray positions scale within their
-// cells and are moved off the faces.  pow(q, 3.) of a float q is written
-// q*q*q in double, which is exact up to its one final rounding.
-void orc_update_cosmo_expansion(void* h, float a_current, float a_prev) {
-  State& s = *(State*)h;
-  s.dx *= a_current / a_prev;
-  const double q = (double)(a_prev / a_current);
+// Number densities from rho (scaled by rho_scale first) and the ionized
+// fractions with fion = 1 (rt_funcs.cc:55): the cell loops of
+// update_cosmo_expansion (gas_funcs.cc:483-512) and update_hydro_step
+// (:688-744, without the subgrid-only ddelta_dt).
+static void densities_from_rho(State& s, double rho_scale) {
   for (int c = 0; c < s.N3; c++) {
-    s.rho[c] *= q * q * q;
+    s.rho[c] *= rho_scale;
     s.nH[c] = (1. - Y) * s.rho[c] / m_H;
     s.nHe[c] = Y * s.rho[c] / (4. * m_H);
     const float fion = 1.f;
@@ -1327,6 +1324,15 @@ void orc_update_cosmo_expansion(void* h, float a_current, float a_prev) {
     s.n_tot[c] = s.nH[c] + s.nHe[c] + s.ne[c];
     s.nH1_prev[c] = s.nH1[c];
   }
+}
+
+// update_cosmo_expansion (gas_funcs.cc:460-552) for given a_current, a_prev:
+// dx grows, rho falls by pow(a_prev/a_current, 3.), the densities follow,
+// ray positions scale within their cells and are moved off the faces.
+void orc_update_cosmo_expansion(void* h, float a_current, float a_prev) {
+  State& s = *(State*)h;
+  s.dx *= a_current / a_prev;
+  densities_from_rho(s, std::pow(a_prev / a_current, 3.));
   for (auto& r : s.ray) {
     if (!r.tag) continue;
     float* x[3] = {&r.x, &r.y, &r.z};
@@ -1336,6 +1342,110 @@ void orc_update_cosmo_expansion(void* h, float a_current, float a_prev) {
       if (*v >= s.dx * (1. - s.min_dist / 2.)) *v = s.dx * (1. - s.min_dist);
     }
   }
+}
+
+// update_hydro_step (gas_funcs.cc:554-752) after a new comoving density was
+// uploaded into rho: redshift zz, Lx = Lx0*a_rt, rho *= pow(a_rt, -3.), the
+// densities recomputed.
+void orc_update_hydro_step(void* h, float zz, float a_rt) {
+  State& s = *(State*)h;
+  s.zz = zz;
+  const float Lx = (float)s.cfg.box_ckpc * a_rt;
+  s.dx = (float)(Lx / s.N * kpc_to_cm);
+  densities_from_rho(s, std::pow(a_rt, -3.));
+}
+
+// write_otf's averages (io_funcs.cc:500-632) with fion = 1, serial over the
+// row-major cells: calc_vol_avg / calc_mass_avg (data_funcs.cc:75-112),
+// minarr / maxarr (:22-51), count_photons_io (io_funcs.cc:18-31).
+static float vol_avg(const State& s, const std::vector<float>& f) {
+  double tot = 0.;
+  for (int c = 0; c < s.N3; c++) tot += (double)f[c];
+  return (float)tot / std::pow((double)s.N, 3.);
+}
+static float mass_avg(const State& s, const std::vector<float>& f, const std::vector<float>& den) {
+  double tot = 0, m_tot = 0;
+  for (int c = 0; c < s.N3; c++) {
+    const double m = (double)den[c];
+    m_tot += m;
+    tot += (double)f[c] * m;
+  }
+  return (float)tot / m_tot;
+}
+static float maxarr(const State& s, const std::vector<float>& a) {
+  float out = 0.;
+  for (int c = 0; c < s.N3; c++)
+    if (a[c] > out) out = a[c];
+  return out;
+}
+static float minarr(const State& s, const std::vector<float>& a) {
+  float out = 1e32;
+  for (int c = 0; c < s.N3; c++)
+    if (a[c] < out) out = a[c];
+  return out;
+}
+
+void orc_otf_averages(void* h, float avg_rho, float temp_0, frt_otf* o) {
+  State& s = *(State*)h;
+  drop_dead(s);
+  const float gamma_min_intg = 3e-14;
+  const size_t n3 = (size_t)s.N3;
+  std::vector<float> fion(n3, 1.f), inH1(n3), inH(n3), binary(n3), rho_mean(n3), temp0(n3);
+  for (size_t c = 0; c < n3; c++) {
+    inH1[c] = s.nH1[c];  // fion == 1
+    inH[c] = s.nH[c];
+    const float g = s.gamma_H1_tot[c], gp = s.gamma_prev[c];
+    binary[c] = ((g >= gamma_min_intg) && (gp >= gamma_min_intg) && (std::fabs(dm_log10(g) - dm_log10(gp)) < 1.))
+                    ? 1.f : 0.f;
+    if ((s.rho[c] / avg_rho > 0.9) && (s.rho[c] / avg_rho < 1.1)) {
+      rho_mean[c] = 1.;
+      temp0[c] = s.temp[c] * s.f_H2[c] + temp_0 * (1. - s.f_H2[c]);
+    } else {
+      rho_mean[c] = 0.;
+      temp0[c] = temp_0;
+    }
+  }
+  std::memset(o, 0, sizeof(*o));
+  o->fH1[0] = vol_avg(s, s.f_H1); o->fH1[1] = mass_avg(s, s.f_H1, s.nH);
+  o->fH2[0] = vol_avg(s, s.f_H2); o->fH2[1] = mass_avg(s, s.f_H2, s.nH);
+  o->fHe1[0] = vol_avg(s, s.f_He1); o->fHe1[1] = mass_avg(s, s.f_He1, s.nHe);
+  o->fHe2[0] = vol_avg(s, s.f_He2); o->fHe2[1] = mass_avg(s, s.f_He2, s.nHe);
+  o->fHe3[0] = vol_avg(s, s.f_He3); o->fHe3[1] = mass_avg(s, s.f_He3, s.nHe);
+  o->nH1 = vol_avg(s, s.avg_nh1);
+  o->nHe1 = vol_avg(s, s.avg_nhe1);
+  o->nH[0] = vol_avg(s, s.nH); o->nH[1] = mass_avg(s, s.nH, fion); o->nH[2] = mass_avg(s, s.nH, binary);
+  o->nHe[0] = vol_avg(s, s.nHe); o->nHe[1] = mass_avg(s, s.nHe, fion); o->nHe[2] = mass_avg(s, s.nHe, binary);
+  o->ne = vol_avg(s, s.ne);
+  const std::vector<float>& G = s.gamma_H1_tot;
+  const float GH1[7] = {vol_avg(s, G), mass_avg(s, G, fion), mass_avg(s, G, binary), mass_avg(s, G, inH),
+                        mass_avg(s, G, s.nH1), minarr(s, G), maxarr(s, G)};
+  const std::vector<float>& R = s.recrate_H2;
+  const float RH2[5] = {vol_avg(s, R), mass_avg(s, R, fion), mass_avg(s, R, binary), mass_avg(s, R, inH),
+                        mass_avg(s, R, s.nH2)};
+  const float T[5] = {vol_avg(s, s.temp), mass_avg(s, s.temp, fion), mass_avg(s, s.temp, binary),
+                      mass_avg(s, s.temp, s.nH), mass_avg(s, temp0, rho_mean)};
+  const float MFP[6] = {vol_avg(s, s.mfp), mass_avg(s, s.mfp, fion), mass_avg(s, s.mfp, binary),
+                        mass_avg(s, s.mfp, inH), minarr(s, s.mfp), maxarr(s, s.mfp)};
+  const float M912[4] = {vol_avg(s, s.mfp_912), mass_avg(s, s.mfp_912, fion), mass_avg(s, s.mfp_912, binary),
+                         mass_avg(s, s.mfp_912, inH)};
+  const float Heat[5] = {vol_avg(s, s.heat_rate), mass_avg(s, s.heat_rate, fion), mass_avg(s, s.heat_rate, binary),
+                         mass_avg(s, s.heat_rate, s.nH), mass_avg(s, s.heat_rate, inH)};
+  const float Cool[5] = {vol_avg(s, s.cool_rate), mass_avg(s, s.cool_rate, fion), mass_avg(s, s.cool_rate, binary),
+                         mass_avg(s, s.cool_rate, s.nH), mass_avg(s, s.cool_rate, inH)};
+  std::memcpy(o->GH1, GH1, sizeof(GH1));
+  std::memcpy(o->RH2, RH2, sizeof(RH2));
+  std::memcpy(o->T, T, sizeof(T));
+  std::memcpy(o->MFP, MFP, sizeof(MFP));
+  std::memcpy(o->MFP_912, M912, sizeof(M912));
+  std::memcpy(o->Heat, Heat, sizeof(Heat));
+  std::memcpy(o->Cool, Cool, sizeof(Cool));
+  double photon_counter = 0.;
+  for (auto& r : s.ray)
+    if (r.tag) photon_counter += (double)sum_nphot(s, r);
+  o->npht = photon_counter * phot_norm;
+  o->nsrc = (int32_t)s.src_cell.size();
+  o->fion[0] = vol_avg(s, fion);
+  o->fion[1] = mass_avg(s, fion, s.nH);
 }
 
 void orc_set_dt(void* h, float dt) { ((State*)h)->dt = dt; }

@@ -7,7 +7,7 @@ import re
 import pytest
 
 from flexrt_imaging_reion_amd import _lib, inputs
-from flexrt_imaging_reion_amd._abi import FIELDS, FrtConfig, FrtFileExtras, FrtRayView, FrtStepStats
+from flexrt_imaging_reion_amd._abi import FIELDS, FrtConfig, FrtFileExtras, FrtOtf, FrtRayView, FrtStepStats
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(REPO, "include", "flexrt_gpu.h")
@@ -43,6 +43,7 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(FrtStepStats) == 15 * 8 + 2 * 8 + 2 * 4
     assert C.sizeof(FrtRayView) == 8 + 14 * 8
     assert C.sizeof(FrtFileExtras) == 9 * 8
+    assert C.sizeof(FrtOtf) == 56 * 4 + 8 + 4 + 2 * 4 + 4  # 56 floats, npht, nsrc, fion[2], padding
 
 
 def test_config_default_without_gpu():

@@ -106,3 +106,33 @@ def test_mean_free_paths_of_single_pass():
         assert np.all(o.download_field("f_He3") == 0)
     finally:
         o.close()
+
+
+def test_hydro_step_reload():
+    """update_hydro_step after a new comoving density: the cell is Lx0 a_rt / N,
+    rho is the proper density rho_c a_rt^-3, and the densities follow it with
+    the ionized fractions kept."""
+    wl, o = stepped()
+    try:
+        n3 = 16 ** 3
+        rng = np.random.default_rng(11)
+        rho_c = (o.download_field("rho").astype(np.float64) * (1 / 8.0) ** 3
+                 * rng.uniform(0.8, 1.25, n3)).astype(F32)  # a comoving snapshot
+        o.upload_field("rho", rho_c)
+        fr = {k: o.download_field(k).astype(np.float64) for k in ("f_H2", "f_He2")}
+        zz, a = F32(6.8), F32(1 / 7.8)
+        o.update_hydro_step(zz, a)
+        kpc = 3.086e21
+        dx_want = float(F32(wl["cfg"]["box_ckpc"]) * a) / 16 * kpc
+        assert abs(o.cell_size() / dx_want - 1) < 2e-7
+        rho = o.download_field("rho").astype(np.float64)
+        assert np.max(np.abs(rho / (rho_c.astype(np.float64) * float(a) ** -3) - 1)) < 2e-7
+        nH = o.download_field("nH").astype(np.float64)
+        assert np.max(np.abs(nH * M_H / rho / (1 - Y) - 1)) < 2e-7
+        assert np.max(np.abs(o.download_field("nH2") / nH / fr["f_H2"] - 1)) < 2e-7
+        nHe = o.download_field("nHe").astype(np.float64)
+        assert np.max(np.abs(o.download_field("nHe2") / nHe / fr["f_He2"] - 1)) < 2e-7
+        o.step()  # the new state and redshift step on
+        assert np.all(np.isfinite(o.download_field("temp")))
+    finally:
+        o.close()

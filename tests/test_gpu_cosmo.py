@@ -84,3 +84,80 @@ def test_expansion_needs_rho():
         with pytest.raises(FlexRTError) as ei:
             g.update_cosmo_expansion(0.13, 0.125)
         assert ei.value.code == FRT_E_ARG
+
+
+def test_fp64_hydro_step_bit_exact():
+    """Three steps, a new comoving density snapshot and redshift
+    (update_hydro_step), three more steps: bit for bit like the checker."""
+    _print_case("fp64 hydro step lognormal n=16 src=16 3+3 steps")
+    wl = inputs.make_workload("lognormal64", n=16, n_src=16, steps=6)
+    g, o = make_pair(wl, precision=0)
+    a_new = F32(1 / 7.6)
+    rng = np.random.default_rng(5)
+    try:
+        for s in range(3):
+            sg, so = g.step(), o.step()
+            assert counts(sg) == counts(so), "step %d" % s
+        rho_c = (g.download_field("rho").astype(np.float64) * float(wl["cfg"]["a_rt"]) ** 3
+                 * rng.uniform(0.9, 1.1, 16 ** 3)).astype(F32)
+        for rt in (g, o):
+            rt.upload_field("rho", rho_c)
+            rt.update_hydro_step(F32(6.6), a_new)
+            rt.get_nphot_min_max()
+        assert F32(g.cell_size()) == F32(o.cell_size())
+        for s in range(3, 6):
+            sg, so = g.step(), o.step()
+            assert counts(sg) == counts(so), "step %d" % s
+        err = compare_rays(g.rays(), o.rays())
+        assert err is None, err
+        fg, fo = fields(g), fields(o)
+        bad = [k for k in STATE_FIELDS if not bits_equal(fg[k], fo[k])]
+        assert not bad, {k: max_rel(fg[k], fo[k]) for k in bad}
+    finally:
+        g.close()
+        o.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_otf_averages_match_checker(tmp_path, precision):
+    """write_otf's averages reduced on the GPU against the checker's serial
+    sums over the same state (for the mixed form the GPU state is copied into
+    the checker first), so only the order of the double additions differs;
+    and the text block written in the reference's format."""
+    import math
+
+    from flexrt_imaging_reion_amd._abi import FIELDS
+
+    _print_case("otf averages precision=%d lognormal n=16 src=16 steps=4" % precision)
+    wl = inputs.make_workload("lognormal64", n=16, n_src=16, steps=4)
+    g, o = make_pair(wl, precision=precision)
+    try:
+        for _ in range(4):
+            g.step()
+            o.step()
+        if precision:
+            for k in FIELDS:
+                o.upload_field(k, g.download_field(k))
+            o.upload_rays(g.rays())
+        for rt in (g, o):
+            rt.upload_field("avg_gamma_prev", (rt.download_field("gamma_H1_tot") * F32(1.5)).astype(F32))
+        avg_rho = F32(g.download_field("rho").astype(np.float64).mean())
+        rg, ro = g.otf_averages(avg_rho), o.otf_averages(avg_rho)
+        tol = 1e-6
+        for k in rg:
+            a = np.atleast_1d(np.asarray(rg[k], dtype=np.float64))
+            b = np.atleast_1d(np.asarray(ro[k], dtype=np.float64))
+            for x, y in zip(a, b):
+                assert (math.isnan(x) and math.isnan(y)) or abs(x - y) <= tol * abs(y), (k, x, y)
+        path = tmp_path / "otf.txt"
+        g.write_otf(path, avg_rho)
+        g.write_otf(path, avg_rho)
+        text = path.read_text().split("\n")
+        assert text.count("Global Averages") == 2
+        assert text[0] == "Step Number    : 4"
+        line = next(t for t in text if t.startswith("GH1"))
+        got = [float(x) for x in line.split(":")[1].split()]
+        assert got == [float("%e" % v) for v in rg["GH1"]]
+    finally:
+        g.close()
+        o.close()

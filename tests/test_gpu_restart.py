@@ -162,3 +162,21 @@ def test_restart_from_ray_file_continues_bit_exact(tmp_path, precision):
             assert not bad, bad
     finally:
         a.close()
+
+
+def test_vif_file(tmp_path):
+    """write_vIF_binary's cube (io_funcs.cc:420-427): per row-major cell
+    IF_speed, Inc_Flux, fion = 1, Delta_fion = 0."""
+    _print_case("vIF file lognormal n=16 src=16 steps=4")
+    wl, g = run(0, 4)
+    try:
+        path = tmp_path / "vIF_z=07.0000"
+        g.write_vif_file(path)
+        n3 = int(wl["cfg"]["n"]) ** 3
+        v = np.frombuffer(path.read_bytes(), "<f4").reshape(n3, 4)
+        assert bits_equal(v[:, 0], g.download_field("IF_speed"))
+        assert bits_equal(v[:, 1], g.download_field("Inc_Flux"))
+        assert np.all(v[:, 2] == 1) and np.all(v[:, 3] == 0)
+        assert np.any(v[:, 0] > 0), "the workload must move ionization fronts"
+    finally:
+        g.close()

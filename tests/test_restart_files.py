@@ -42,3 +42,11 @@ def test_checkpoint_names():
     assert p == {"gas": "out/gas_z=07.0000", "ray": "out/ray_z=07.0000", "tre": "out/tre_z=07.0000"}
     assert checkpoint_paths("/d/", 12.345678)["gas"] == "/d/gas_z=12.3456"
     assert checkpoint_paths("", 9.99999)["tre"] == "tre_z=09.9999"
+
+
+def test_vif_name():
+    """io_funcs.cc:399-414: digits of z_rt, zero padding from zz."""
+    from flexrt_imaging_reion_amd.rt import vif_path
+
+    assert vif_path("o/", 7.0, 6.987654) == "o/vIF_z=06.9876"
+    assert vif_path("o/", 10.5, 10.48) == "o/vIF_z=10.4800"
