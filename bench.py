@@ -176,20 +176,41 @@ def time_driver_ops(g, wl, reps=5, sample_rays=2_000_000):
     g.update_hydro_step(np.float32(wl["cfg"]["zz"]), np.float32(a0))
     g.synchronize()
     kh, _ = g.kernel_time("cosmo_cells")
+    g.reset_kernel_times()
+    avg_rho = np.float32(g.download_field("rho").astype(np.float64).mean())
+    for _ in range(reps):
+        g.otf_averages(avg_rho)
+    g.synchronize()
+    ko, no = g.kernel_time("otf_cells")
+    kor, _ = g.kernel_time("otf_rays")
     g.set_profiling(False)
+    t0 = time.perf_counter()
+    otf_gpu = g.otf_averages(avg_rho)
+    otf_wall = time.perf_counter() - t0
     cell_ms, ray_ms = kc / max(nc, 1), kr / max(nr, 1)
     out = {"cells": n3, "rays": nrays,
            "cosmo_cells_ms": round(cell_ms, 4), "cosmo_cells_GBps": round(72 * n3 / (cell_ms * 1e-3) / 1e9, 1),
            "cosmo_rays_ms": round(ray_ms, 4), "cosmo_rays_GBps": round(32 * nrays / (ray_ms * 1e-3) / 1e9, 1),
-           "hydro_step_cells_ms": round(kh, 4)}
+           "hydro_step_cells_ms": round(kh, 4),
+           # write_otf: 21 cell fields of 4 B read; per ray its photon vector(s)
+           "otf_cells_ms": round(ko / max(no, 1), 4),
+           "otf_cells_GBps": round(84 * n3 / (ko / max(no, 1) * 1e-3) / 1e9, 1),
+           "otf_rays_ms": round(kor / max(no, 1), 4),
+           "otf_call_wall_ms": round(1e3 * otf_wall, 3)}
     rays = g.rays()
     m = min(len(rays["cell"]), sample_rays)
     nf = int(wl["cfg"]["nfreq"])
     part = {k: np.ascontiguousarray(v[:m] if k != "nphot" else v[: m * nf]) for k, v in rays.items()}
     o = Oracle(wl["cfg"])
     try:
-        for k in ("rho", "f_H2", "f_He2", "f_He3"):
+        from flexrt_imaging_reion_amd._abi import FIELDS
+
+        for k in FIELDS:
             o.upload_field(k, g.download_field(k))
+        o.set_sources(wl["src_cells"], wl["src_lum"])
+        t0 = time.perf_counter()
+        otf_cpu = o.otf_averages(avg_rho)
+        to = time.perf_counter() - t0
         t0 = time.perf_counter()
         o.update_cosmo_expansion(np.float32(a0 * 1.001), np.float32(a0))
         tc = time.perf_counter() - t0
@@ -200,7 +221,11 @@ def time_driver_ops(g, wl, reps=5, sample_rays=2_000_000):
     finally:
         o.close()
     out["cpu_baseline"] = {"kind": "port", "cores": 1, "cells_ms": round(1e3 * tc, 2),
-                           "rays_sample": m, "rays_ms_per_million": round(1e3 * tr / m * 1e6, 2)}
+                           "rays_sample": m, "rays_ms_per_million": round(1e3 * tr / m * 1e6, 2),
+                           "otf_cells_ms": round(1e3 * to, 2)}
+    out["otf_max_rel_vs_checker"] = max(
+        abs(float(x) - float(y)) / abs(float(y)) for k in ("fH1", "fH2", "nH", "GH1", "T", "MFP", "Heat", "Cool")
+        for x, y in zip(np.atleast_1d(otf_gpu[k])[:2], np.atleast_1d(otf_cpu[k])[:2]) if y)
     log("driver ops: cells %.3f ms, rays %.3f ms (GPU); cells %.1f ms (CPU)" % (cell_ms, ray_ms, 1e3 * tc))
     return out
 
