@@ -47,6 +47,10 @@ _PROTOS = {
     "frt_download_paths": (C.c_int, [_CTX, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float)]),
     "frt_write_vif_file": (C.c_int, [_CTX, C.c_char_p]),
+    "frt_update_gadget_var": (C.c_int, [_CTX]),
+    "frt_write_gadget_file": (C.c_int, [_CTX, C.c_char_p]),
+    "frt_reset_gadget_var": (C.c_int, [_CTX]),
+    "frt_get_gadget_step_time": (C.c_int, [_CTX, C.POINTER(C.c_float)]),
     "frt_write_ray_file": (C.c_int, [_CTX, C.c_char_p]),
     "frt_read_ray_file": (C.c_int, [_CTX, C.c_char_p]),
     "frt_write_gas_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),

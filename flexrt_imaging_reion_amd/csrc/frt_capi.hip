@@ -236,6 +236,8 @@ float* field_ptr(frt_ctx* ctx, int field) {
     case FRT_F_MFP: return ctx->c.f[C_MFP];
     case FRT_F_MFP_912: return ctx->c.f[C_MFP912];
     case FRT_F_GAMMA_PREV: return ctx->c.f[C_GAMMA_PREV];
+    case FRT_F_GADGET_GAMMA: return ctx->c.f[C_GADGET_GAMMA];
+    case FRT_F_GADGET_HEAT: return ctx->c.f[C_GADGET_HEAT];
     default: return nullptr;
   }
 }
@@ -421,6 +423,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->have_alive = false;
   ctx->have_rho = false;
   ctx->otf_buf = nullptr;
+  ctx->gadget_step_time = 0.f;
 
   int rc = FRT_OK;
   for (int b = 0; b < 2 && rc == FRT_OK; ++b) rc = alloc_rays(ctx, ctx->buf[b], cap, c.nfreq);
@@ -1236,6 +1239,43 @@ int frt_write_vif_file(frt_ctx* ctx, const char* path) {
   rc = hip_check(ctx, e, "vIF file");
   if (rc) return rc;
   return write_file(ctx, path, h.data(), h.size() * 4);
+}
+
+int frt_update_gadget_var(frt_ctx* ctx) {
+  if (!ctx) return FRT_E_ARG;
+  if (!ctx->have_rho) return set_error(ctx, FRT_E_ARG, "update_gadget_var needs rho: upload FRT_F_RHO first");
+  ctx->gadget_step_time += ctx->g.dt;  // io_funcs.cc:436
+  return launch_gadget(ctx, 0, nullptr);
+}
+
+int frt_write_gadget_file(frt_ctx* ctx, const char* path) {
+  if (!ctx || !path) return FRT_E_ARG;
+  const size_t n3 = (size_t)ctx->g.n3;
+  float* d = nullptr;
+  if (hipMalloc((void**)&d, 2 * n3 * sizeof(float)) != hipSuccess)
+    return set_error(ctx, FRT_E_NOMEM, "gadget file staging buffer");
+  std::vector<float> h(2 * n3);
+  int rc = launch_gadget(ctx, 1, d);
+  hipError_t e = (rc == FRT_OK) ? stream_copy(ctx, h.data(), d, 2 * n3 * sizeof(float), hipMemcpyDeviceToHost)
+                                : hipSuccess;
+  hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (rc) return rc;
+  rc = hip_check(ctx, e, "gadget file");
+  if (rc) return rc;
+  return write_file(ctx, path, h.data(), h.size() * 4);
+}
+
+int frt_reset_gadget_var(frt_ctx* ctx) {
+  if (!ctx) return FRT_E_ARG;
+  ctx->gadget_step_time = 0.f;
+  return launch_gadget(ctx, 2, nullptr);
+}
+
+int frt_get_gadget_step_time(const frt_ctx* ctx, float* step_time) {
+  if (!ctx || !step_time) return FRT_E_ARG;
+  *step_time = ctx->gadget_step_time;
+  return FRT_OK;
 }
 
 int frt_read_treion_file(frt_ctx* ctx, const char* path, frt_file_extras* extras) {

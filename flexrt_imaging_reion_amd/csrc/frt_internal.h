@@ -105,7 +105,8 @@ enum CellField {
   C_NH = 0, C_NHE, C_NH1, C_NHE1, C_NH2, C_NHE2, C_NE, C_NTOT, C_FH1, C_FHE1, C_FH2, C_FHE2,
   C_TEMP, C_DNE_DT, C_CLUMP, C_NH1_PREV, C_AVG_NH1, C_AVG_NHE1, C_RECOMB_H2, C_RECOMB_HE2,
   C_GAMMA_H1_TOT, C_GAMMA_HE1_TOT, C_RECRATE_H2, C_RECRATE_HE2, C_TREION, C_TREION_FLAG,
-  C_HEAT, C_COOL, C_IF_SPEED, C_INC_FLUX, C_RHO, C_FHE3, C_MFP, C_MFP912, C_GAMMA_PREV, C_NSCALAR
+  C_HEAT, C_COOL, C_IF_SPEED, C_INC_FLUX, C_RHO, C_FHE3, C_MFP, C_MFP912, C_GAMMA_PREV, C_GADGET_GAMMA,
+  C_GADGET_HEAT, C_NSCALAR
 };
 
 struct CellPtrs {
@@ -147,6 +148,7 @@ struct frt_ctx {
   uint32_t* cand_key_tmp;  // [cap]
   bool have_rho;       // rho was uploaded or read from a gas file
   double* otf_buf;     // write_otf partial sums (allocated at first use)
+  float gadget_step_time;  // MP-Gadget coupling: dt summed since the last reset
   bool have_segments;  // update_rays ran and find_avg_unu has not yet consumed the paths
   bool have_alive;     // find_avg_unu ran in this step (alive flags valid)
   int64_t nseg;
@@ -236,6 +238,7 @@ int launch_copy(frt_ctx* ctx, int64_t nbytes, int repeats);
 int launch_pack_ray_file(frt_ctx* ctx, uint8_t* out);
 int launch_pack_gas_file(frt_ctx* ctx, const float* ext, unsigned have, int held_rho, float* out);
 int launch_pack_vif_file(frt_ctx* ctx, float4* out);
+int launch_gadget(frt_ctx* ctx, int mode, float* out);  // 0 accumulate, 1 average + pack, 2 clear
 // Sums and extremes of write_otf: FRT_OTF_NSUM double sums, then max/min of
 // gamma_H1_tot and of mfp; FRT_OTF_NV values in all, then the photon count.
 #define FRT_OTF_NSUM 45

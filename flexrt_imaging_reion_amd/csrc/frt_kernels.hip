@@ -2249,6 +2249,37 @@ int launch_otf(frt_ctx* ctx, float avg_rho, float temp_0, double* part, double* 
   return FRT_OK;
 }
 
+// MP-Gadget coupling (io_funcs.cc:434-496).  mode 0: update_gadget_var's
+// sums (float, gadget_ini_heat = 0); mode 1: write_gadget_binary's division
+// by the step time, in place, and the two row-major cubes into out; mode 2:
+// reset_gadget_var.
+__global__ void __launch_bounds__(BLK) k_gadget(CellPtrs cf, GridConst g, int mode, float step_time,
+                                                float* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (k >= g.n3) return;
+  float gg = cf.f[C_GADGET_GAMMA][k], gh = cf.f[C_GADGET_HEAT][k];
+  if (mode == 0) {
+    gg += cf.f[C_GAMMA_H1_TOT][k] * g.dt;
+    gh += 0.f + cf.f[C_HEAT][k] * g.dt / cf.f[C_RHO][k];
+  } else if (mode == 1) {
+    gg /= step_time;
+    gh /= step_time;
+    const int64_t r = frt_cell_to_rowmajor(g.lay, (int)k);
+    out[r] = gg;
+    out[g.n3 + r] = gh;
+  } else {
+    gg = 0.f;
+    gh = 0.f;
+  }
+  cf.f[C_GADGET_GAMMA][k] = gg;
+  cf.f[C_GADGET_HEAT][k] = gh;
+}
+
+int launch_gadget(frt_ctx* ctx, int mode, float* out) {
+  FRT_LAUNCH(ctx, "gadget", grid_for(ctx->g.n3), k_gadget, ctx->c, ctx->g, mode, ctx->gadget_step_time, out);
+  return FRT_OK;
+}
+
 // update_cosmo_expansion's cell loop (gas_funcs.cc:483-512), and
 // update_hydro_step's (:688-744) for a new density field, with fion = 1
 // (rt_funcs.cc:55): rho scaled (by (a_prev/a_current)^3, or a_rt^-3 for the

@@ -6,6 +6,7 @@ gas_funcs.cc) and take and return numpy arrays in the reference's layouts.
 Every call goes through the C ABI; a missing library raises at construction.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -39,6 +40,16 @@ def checkpoint_paths(output_dir, zz):
     zs = "%f" % float(zz)
     pad = "0" if float(zz) < 10.0 else ""
     return {k: str(output_dir) + k + "_z=" + pad + zs[:-2] for k in ("gas", "ray", "tre")}
+
+
+def gadget_path(output_dir, current_time, step_time, hh, omega_m):
+    """write_gadget_binary's file name (io_funcs.cc:465-467): the redshift of
+    current_time - step_time/2 by toRedshift (cosmo_funcs.cc:108-113, an
+    Einstein-de Sitter + Lambda closed form), six decimals."""
+    H0 = 3.24086e-18
+    t = float(current_time) - float(step_time) / 2.0
+    atime = (math.sqrt(omega_m / (1.0 - omega_m)) * math.sinh(1.5 * H0 * hh * math.sqrt(1.0 - omega_m) * t)) ** (2.0 / 3.0)
+    return str(output_dir) + "small_gas_rt_z=" + "%f" % (1.0 / atime - 1.0)
 
 
 def vif_path(output_dir, zz, z_rt):
@@ -268,6 +279,24 @@ class FlexRT:
     def write_vif_file(self, path):
         """write_vIF_binary (io_funcs.cc:396-430)."""
         self._check(self._lib.frt_write_vif_file(self._h, os.fsencode(path)), "write_vif_file")
+
+    # -- MP-Gadget coupling (io_funcs.cc:434-496) ----------------------------------------
+    def update_gadget_var(self):
+        """Adds this step's Gamma_HI dt and heat_rate dt / rho (needs rho)."""
+        self._check(self._lib.frt_update_gadget_var(self._h), "update_gadget_var")
+
+    def write_gadget_file(self, path):
+        """write_gadget_binary: averages over the step time, in place, then the
+        two row-major cubes (gadget_gamma, gadget_heat)."""
+        self._check(self._lib.frt_write_gadget_file(self._h, os.fsencode(path)), "write_gadget_file")
+
+    def reset_gadget_var(self):
+        self._check(self._lib.frt_reset_gadget_var(self._h), "reset_gadget_var")
+
+    def gadget_step_time(self):
+        v = C.c_float()
+        self._check(self._lib.frt_get_gadget_step_time(self._h, C.byref(v)), "get_gadget_step_time")
+        return v.value
 
     def write_checkpoint(self, output_dir, zz, extras=None, rays=True):
         """The end-of-hydro-step output of the driver (gas_funcs.cc:617-656):

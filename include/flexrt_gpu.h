@@ -148,6 +148,8 @@ typedef enum {
   FRT_F_MFP,           /* mfp = 1/sigma_HI(nu_phot[0])/avg_nh1 */
   FRT_F_MFP_912,       /* mfp_912 = 1/sigma_HI(13.6 eV)/avg_nh1 */
   FRT_F_GAMMA_PREV,    /* avg_gamma_prev (0 on a fresh start, Gamma_HI of the gas file on a restart) */
+  FRT_F_GADGET_GAMMA,  /* gadget_gamma: sum of Gamma_HI dt since the last reset (then the average) */
+  FRT_F_GADGET_HEAT,   /* gadget_heat: sum of heat_rate dt / rho since the last reset (then the average) */
   FRT_F_COUNT
 } frt_field;
 
@@ -332,6 +334,19 @@ int frt_read_treion_file(frt_ctx* ctx, const char* path, frt_file_extras* extras
 /* write_vIF_binary (io_funcs.cc:396-430): per row-major cell IF_speed,
  * Inc_Flux, fion (1 without I-front tracking) and Delta_fion (0), float. */
 int frt_write_vif_file(frt_ctx* ctx, const char* path);
+
+/* MP-Gadget coupling (io_funcs.cc:434-496), main.cc:229-240 with
+ * MP_GADGET_HEAT: after each step update_gadget_var adds Gamma_HI dt and
+ * heat_rate dt / rho (the I-front heat injection gadget_ini_heat is 0 without
+ * the subgrid model, subgrid.cc:674-686) and the step's dt to the step time;
+ * write_gadget_binary divides both by the step time, in place, and writes the
+ * two row-major float cubes; reset_gadget_var clears them.  rho must have
+ * been uploaded. */
+int frt_update_gadget_var(frt_ctx* ctx);
+int frt_write_gadget_file(frt_ctx* ctx, const char* path);
+int frt_reset_gadget_var(frt_ctx* ctx);
+/* gadget_step_time: the summed time steps since the last reset, s. */
+int frt_get_gadget_step_time(const frt_ctx* ctx, float* step_time);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* When enabled, every kernel launch is bracketed by HIP events on the

@@ -165,7 +165,8 @@ struct State {
   std::vector<float> nH, nHe, nH1, nHe1, nH2, nHe2, ne, n_tot, f_H1, f_He1, f_H2, f_He2, temp, dne_dt,
       clump, nH1_prev, avg_nh1, avg_nhe1, recomb_H2, recomb_He2, gamma_H1_tot, gamma_He1_tot,
       recrate_H2, recrate_He2, treion, treion_flag, heat_rate, cool_rate, IF_speed, Inc_Flux,
-      gamma_H1_nu, gamma_He1_nu, ngam_nu, rho, f_He3, mfp, mfp_912, gamma_prev;
+      gamma_H1_nu, gamma_He1_nu, ngam_nu, rho, f_He3, mfp, mfp_912, gamma_prev, gadget_gamma, gadget_heat;
+  float gadget_step_time = 0.f;
   std::vector<double> avg_gamma, avg_gamma_he1, avg_gamma_nu, avg_gamma_nu_he1, ngam_acc;
   std::vector<Ray> ray;
   std::vector<int> src_cell;
@@ -222,6 +223,8 @@ static std::vector<float>* field_vec(State& s, int f) {
     case FRT_F_MFP: return &s.mfp;
     case FRT_F_MFP_912: return &s.mfp_912;
     case FRT_F_GAMMA_PREV: return &s.gamma_prev;
+    case FRT_F_GADGET_GAMMA: return &s.gadget_gamma;
+    case FRT_F_GADGET_HEAT: return &s.gadget_heat;
     default: return nullptr;
   }
 }
@@ -1447,6 +1450,32 @@ void orc_otf_averages(void* h, float avg_rho, float temp_0, frt_otf* o) {
   o->fion[0] = vol_avg(s, fion);
   o->fion[1] = mass_avg(s, fion, s.nH);
 }
+
+// MP-Gadget coupling (io_funcs.cc:434-496) with gadget_ini_heat = 0 (only
+// the subgrid model sets it, subgrid.cc:674-686).
+void orc_update_gadget_var(void* h) {
+  State& s = *(State*)h;
+  s.gadget_step_time += s.dt;
+  const float gadget_ini_heat = 0.f;
+  for (int c = 0; c < s.N3; c++) {
+    s.gadget_gamma[c] += (s.gamma_H1_tot[c] * s.dt);
+    s.gadget_heat[c] += (gadget_ini_heat + s.heat_rate[c] * s.dt / s.rho[c]);
+  }
+}
+// write_gadget_binary's averaging, in place (the file holds the two cubes).
+void orc_gadget_average(void* h) {
+  State& s = *(State*)h;
+  for (int c = 0; c < s.N3; c++) {
+    s.gadget_gamma[c] /= s.gadget_step_time;
+    s.gadget_heat[c] /= s.gadget_step_time;
+  }
+}
+void orc_reset_gadget_var(void* h) {
+  State& s = *(State*)h;
+  s.gadget_step_time = 0.0;
+  for (int c = 0; c < s.N3; c++) s.gadget_gamma[c] = s.gadget_heat[c] = 0.0;
+}
+float orc_get_gadget_step_time(void* h) { return ((State*)h)->gadget_step_time; }
 
 void orc_set_dt(void* h, float dt) { ((State*)h)->dt = dt; }
 float orc_get_cell_size(void* h) { return ((State*)h)->dx; }

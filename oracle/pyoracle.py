@@ -61,6 +61,10 @@ _PROTOS = {
     "orc_write_ray_file": (C.c_int, [C.c_void_p, C.c_char_p]),
     "orc_update_cosmo_expansion": (None, [C.c_void_p, C.c_float, C.c_float]),
     "orc_set_dt": (None, [C.c_void_p, C.c_float]),
+    "orc_update_gadget_var": (None, [C.c_void_p]),
+    "orc_gadget_average": (None, [C.c_void_p]),
+    "orc_reset_gadget_var": (None, [C.c_void_p]),
+    "orc_get_gadget_step_time": (C.c_float, [C.c_void_p]),
     "orc_otf_averages": (None, [C.c_void_p, C.c_float, C.c_float, C.POINTER(FrtOtf)]),
     "orc_update_hydro_step": (None, [C.c_void_p, C.c_float, C.c_float]),
     "orc_get_cell_size": (C.c_float, [C.c_void_p]),
@@ -210,6 +214,18 @@ class Oracle:
         o = FrtOtf()
         self._lib.orc_otf_averages(self._h, float(avg_rho), float(temp_0), C.byref(o))
         return o.as_dict()
+
+    def update_gadget_var(self):
+        self._lib.orc_update_gadget_var(self._h)
+
+    def gadget_average(self):
+        self._lib.orc_gadget_average(self._h)
+
+    def reset_gadget_var(self):
+        self._lib.orc_reset_gadget_var(self._h)
+
+    def gadget_step_time(self):
+        return float(self._lib.orc_get_gadget_step_time(self._h))
 
     def set_dt(self, dt):
         self._lib.orc_set_dt(self._h, float(dt))

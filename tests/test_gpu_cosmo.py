@@ -161,3 +161,31 @@ def test_otf_averages_match_checker(tmp_path, precision):
     finally:
         g.close()
         o.close()
+
+
+def test_gadget_coupling_bit_exact(tmp_path):
+    """update_gadget_var after each of 3 FP64 steps, write_gadget_binary's
+    averaged cubes and reset_gadget_var: bit for bit like the checker."""
+    _print_case("fp64 gadget coupling lognormal n=16 src=16 steps=3")
+    wl = inputs.make_workload("lognormal64", n=16, n_src=16, steps=3)
+    g, o = make_pair(wl, precision=0)
+    try:
+        for _ in range(3):
+            g.step()
+            o.step()
+            g.update_gadget_var()
+            o.update_gadget_var()
+        assert g.gadget_step_time() == o.gadget_step_time()
+        path = tmp_path / "small_gas_rt_z=7.000000"
+        g.write_gadget_file(path)
+        o.gadget_average()
+        n3 = 16 ** 3
+        cubes = np.frombuffer(path.read_bytes(), "<f4").reshape(2, n3)
+        assert bits_equal(cubes[0], o.download_field("gadget_gamma"))
+        assert bits_equal(cubes[1], o.download_field("gadget_heat"))
+        assert bits_equal(g.download_field("gadget_gamma"), cubes[0])  # averaged in place
+        g.reset_gadget_var()
+        assert g.gadget_step_time() == 0.0 and not np.any(g.download_field("gadget_heat"))
+    finally:
+        g.close()
+        o.close()
