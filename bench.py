@@ -183,6 +183,7 @@ def time_driver_ops(g, wl, reps=5, sample_rays=2_000_000):
     g.synchronize()
     ko, no = g.kernel_time("otf_cells")
     kor, _ = g.kernel_time("otf_rays")
+    kof, _ = g.kernel_time("otf_final")
     g.set_profiling(False)
     t0 = time.perf_counter()
     otf_gpu = g.otf_averages(avg_rho)
@@ -196,6 +197,7 @@ def time_driver_ops(g, wl, reps=5, sample_rays=2_000_000):
            "otf_cells_ms": round(ko / max(no, 1), 4),
            "otf_cells_GBps": round(84 * n3 / (ko / max(no, 1) * 1e-3) / 1e9, 1),
            "otf_rays_ms": round(kor / max(no, 1), 4),
+           "otf_final_ms": round(kof / max(no, 1), 4),
            "otf_call_wall_ms": round(1e3 * otf_wall, 3)}
     rays = g.rays()
     m = min(len(rays["cell"]), sample_rays)

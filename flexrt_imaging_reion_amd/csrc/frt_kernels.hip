@@ -2125,8 +2125,8 @@ int launch_pack_vif_file(frt_ctx* ctx, float4* out) {
 // cells: double sums (calc_vol_avg / calc_mass_avg, data_funcs.cc:75-112)
 // and float extremes (maxarr from 0, minarr from 1e32, data_funcs.cc:22-51);
 // the block reduces them with a fixed butterfly and writes one partial per
-// value; k_otf_final adds the partials in block order.  The result does not
-// depend on the schedule; it differs from a serial double sum only by the
+// value; k_otf_final combines the partials in a fixed order.  The result does
+// not depend on the schedule; it differs from a serial double sum only by the
 // order of the additions.
 __device__ __forceinline__ bool otf_is_max(int i) { return i == FRT_OTF_NSUM || i == FRT_OTF_NSUM + 2; }
 __device__ __forceinline__ bool otf_is_min(int i) { return i == FRT_OTF_NSUM + 1 || i == FRT_OTF_NSUM + 3; }
@@ -2222,20 +2222,31 @@ __global__ void __launch_bounds__(BLK) k_otf_rays(RaySoA rb, int64_t n, int F, i
   }
 }
 
-__global__ void __launch_bounds__(64) k_otf_final(const double* __restrict__ part, int nb, const double* __restrict__ rpart,
-                                                  int nbr, double* __restrict__ out) {
-  const int i = threadIdx.x;
-  if (i < FRT_OTF_NV) {
-    double x = part[i];
-    for (int b = 1; b < nb; ++b) {
-      const double y = part[(int64_t)b * FRT_OTF_NV + i];
-      x = otf_is_max(i) ? fmax(x, y) : otf_is_min(i) ? fmin(x, y) : x + y;
-    }
-    out[i] = x;
-  } else if (i == FRT_OTF_NV) {
-    double x = 0.;
-    for (int b = 0; b < nbr; ++b) x += rpart[b];
-    out[i] = x;
+// One block per value: each thread combines a fixed strided subset of the
+// block partials in order, then a fixed butterfly and the four wave results
+// in order (the value FRT_OTF_NV is the photon count of k_otf_rays).
+__global__ void __launch_bounds__(BLK) k_otf_final(const double* __restrict__ part, int nb, const double* __restrict__ rpart,
+                                                   int nbr, double* __restrict__ out) {
+  __shared__ double red[BLK / 64];
+  const int i = blockIdx.x;
+  const bool rays = i == FRT_OTF_NV;
+  const bool mx = otf_is_max(i), mn = otf_is_min(i);
+  const int n = rays ? nbr : nb;
+  double x = mn ? (double)1e32f : 0.;
+  for (int b = threadIdx.x; b < n; b += BLK) {
+    const double y = rays ? rpart[b] : part[(int64_t)b * FRT_OTF_NV + i];
+    x = mx ? fmax(x, y) : mn ? fmin(x, y) : x + y;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double y = __shfl_xor(x, off, 64);
+    x = mx ? fmax(x, y) : mn ? fmin(x, y) : x + y;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+    for (int q = 1; q < BLK / 64; ++q) t = mx ? fmax(t, red[q]) : mn ? fmin(t, red[q]) : t + red[q];
+    out[i] = t;
   }
 }
 
@@ -2245,7 +2256,7 @@ int launch_otf(frt_ctx* ctx, float avg_rho, float temp_0, double* part, double* 
   double* rpart = part + (size_t)FRT_OTF_BLOCKS * FRT_OTF_NV;
   FRT_LAUNCH(ctx, "otf_cells", nb, k_otf_cells, ctx->c, (int64_t)ctx->g.n3, avg_rho, temp_0, part);
   FRT_LAUNCH(ctx, "otf_rays", nbr, k_otf_rays, cur_buf(ctx), ctx->nrays, ctx->g.nfreq, ctx->g.cap, rpart);
-  hipLaunchKernelGGL(k_otf_final, dim3(1), dim3(64), 0, ctx->stream, part, (int)nb, rpart, (int)nbr, out);
+  FRT_LAUNCH(ctx, "otf_final", FRT_OTF_NV + 1, k_otf_final, part, (int)nb, rpart, (int)nbr, out);
   return FRT_OK;
 }
 
