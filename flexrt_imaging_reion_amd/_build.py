@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libflexrt_gpu.so")
 ORACLE_SRC = os.path.join(REPO_DIR, "oracle", "frt_oracle.cc")
 ORACLE_LIB = os.path.join(REPO_DIR, "oracle", "liboracle.so")
 
-HIP_SOURCES = ["frt_kernels.hip", "frt_capi.hip"]
+HIP_SOURCES = ["frt_kernels.hip", "frt_capi.hip", "frt_adjoint.hip"]
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
     # no implicit a*b+c fusion: the path must reproduce the CPU arithmetic

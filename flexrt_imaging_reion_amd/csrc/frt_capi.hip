@@ -424,6 +424,11 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->have_rho = false;
   ctx->otf_buf = nullptr;
   ctx->gadget_step_time = 0.f;
+  ctx->adj_on = ctx->adj_valid = false;
+  ctx->adj_iters = 0;
+  ctx->adj_nrays = 0;
+  ctx->adj_tape = nullptr;
+  ctx->adj_nph = nullptr;
 
   int rc = FRT_OK;
   for (int b = 0; b < 2 && rc == FRT_OK; ++b) rc = alloc_rays(ctx, ctx->buf[b], cap, c.nfreq);
@@ -501,6 +506,8 @@ int frt_destroy(frt_ctx* ctx) {
   hipFree(ctx->contrib.rec); hipFree(ctx->path_slot); hipFree(ctx->pdf);  // rec is the block base
   hipFree(ctx->cell_start); hipFree(ctx->cell_end); hipFree(ctx->cell_block); hipFree(ctx->c.avg2);
   hipFree(ctx->otf_buf);
+  hipFree(ctx->adj_tape);
+  hipFree(ctx->adj_nph);
   hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
   hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt);
   if (ctx->hcnt) hipHostFree(ctx->hcnt);

@@ -149,6 +149,13 @@ struct frt_ctx {
   bool have_rho;       // rho was uploaded or read from a gas file
   double* otf_buf;     // write_otf partial sums (allocated at first use)
   float gadget_step_time;  // MP-Gadget coupling: dt summed since the last reset
+  // adjoint of find_avg_unu (FP64 form): tape of the last recorded call
+  bool adj_on;         // record a tape in find_avg_unu
+  bool adj_valid;      // the tape and the paths belong to the last find_avg_unu
+  int adj_iters;       // C2Ray passes on the tape
+  int64_t adj_nrays;   // rays on the tape
+  double* adj_tape;    // [itercount][4][n3] + [2][n3]
+  float* adj_nph;      // [capacity][nfreq] photons at the start of the step
   bool have_segments;  // update_rays ran and find_avg_unu has not yet consumed the paths
   bool have_alive;     // find_avg_unu ran in this step (alive flags valid)
   int64_t nseg;

@@ -405,6 +405,17 @@ __global__ void __launch_bounds__(BLK) k_pack_avg(const float* __restrict__ a, c
   if (k < n) out[k] = make_float2(a[k], b[k]);
 }
 
+// Adjoint tape: a pass's opacities (or the step's starting fractions) as
+// doubles, and the rays' photons at the start of the step.
+__global__ void __launch_bounds__(BLK) k_tape_pair(const float* __restrict__ a, const float* __restrict__ b,
+                                                   double* __restrict__ out, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (k < n) {
+    out[k] = (double)a[k];
+    out[n + k] = (double)b[k];
+  }
+}
+
 __global__ void __launch_bounds__(BLK) k_fill_i32(int32_t* a, int64_t n, int32_t v) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i < n) a[i] = v;
@@ -932,7 +943,7 @@ template <typename ACC, bool FIRST, bool LAST, bool THERMAL>
 __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribPtrs cp,
                                               const int32_t* __restrict__ cstart,
                                               const int32_t* __restrict__ cend,
-                                              DevCounters* cnt) {
+                                              DevCounters* cnt, double* __restrict__ tape_g) {
   const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
   int nsub = 0;
   if (k < g.n3) {
@@ -1100,6 +1111,10 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       }
     }
 
+    if (tape_g) {  // adjoint tape: this pass's Gamma sums
+      tape_g[k] = GH;
+      tape_g[n3 + k] = GHe;
+    }
     const float nH = cf.f[C_NH][k];
     const float nHe = cf.f[C_NHE][k];
     const float clump = cf.f[C_CLUMP][k];
@@ -1673,6 +1688,7 @@ int launch_add_ray_dt(frt_ctx* ctx) {
 }
 
 int launch_march(frt_ctx* ctx) {
+  ctx->adj_valid = false;  // new paths: a recorded adjoint tape no longer matches
   FRT_LAUNCH(ctx, "march", grid_for(ctx->nrays), k_march, cur_buf(ctx), ctx->nrays, ctx->g,
              ctx->path_cell, ctx->path_dist, ctx->seg_cnt, ctx->alive, ctx->dcnt);
   return FRT_OK;
@@ -1829,6 +1845,14 @@ int launch_segments(frt_ctx* ctx) {
   return FRT_OK;
 }
 
+__global__ void __launch_bounds__(BLK) k_tape_nph(RaySoA rb, int64_t n, int F, int64_t cap, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  float v[8];
+  phot_load(rb, i, F, cap, v);
+  FRT_FOR_BINS(nu) out[i * F + nu] = v[nu];
+}
+
 int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   const int I = ctx->cfg.itercount;
   const int64_t n = ctx->nrays;
@@ -1839,8 +1863,20 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   const int F = g.nfreq;
   ctx->deposit_bytes_last = 0.;
   ctx->deposit_launches_last = 0;
+  // adjoint tape (FP64 form): per pass [opacity HI, HeI, Gamma_HI, Gamma_HeI][n3],
+  // then the starting f_H1, f_He1, and the rays' starting photons
+  double* tape = (ctx->adj_on && !mixed) ? ctx->adj_tape : nullptr;
+  if (tape) {
+    FRT_LAUNCH(ctx, "adj_tape", gc, k_tape_pair, ctx->c.f[C_FH1], ctx->c.f[C_FHE1], tape + (size_t)4 * I * g.n3,
+               (int64_t)g.n3);
+    FRT_LAUNCH(ctx, "adj_tape", grid_for(n), k_tape_nph, rb, n, F, g.cap, ctx->adj_nph);
+  }
   for (int it = 0; it < I; ++it) {
     const bool first = (it == 0), last = (it == I - 1);
+    double* tk = tape ? tape + (size_t)4 * it * g.n3 : nullptr;
+    if (tk)
+      FRT_LAUNCH(ctx, "adj_tape", gc, k_tape_pair, ctx->c.f[first ? C_NH1 : C_AVG_NH1],
+                 ctx->c.f[first ? C_NHE1 : C_AVG_NHE1], tk, (int64_t)g.n3);
     if (first)  // the first pass attenuates with n_HI, n_HeI themselves
       FRT_LAUNCH(ctx, "pack_avg", gc, k_pack_avg, ctx->c.f[C_NH1], ctx->c.f[C_NHE1], ctx->c.avg2, g.n3);
     if (mixed) {
@@ -1877,7 +1913,7 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     }
 #define FRT_CHEM(ACC, F1, L1, TH)                                                           \
   FRT_LAUNCH(ctx, "chem", gc, (k_chem<ACC, F1, L1, TH>), ctx->c, g, ctx->contrib, ctx->cell_start, \
-             ctx->cell_end, ctx->dcnt)
+             ctx->cell_end, ctx->dcnt, tk ? tk + 2 * g.n3 : nullptr)
     if (mixed) {
       if (first && last) {
         if (fuse_thermal) FRT_CHEM(float, true, true, true); else FRT_CHEM(float, true, true, false);
@@ -1900,6 +1936,11 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
       }
     }
 #undef FRT_CHEM
+  }
+  if (tape) {
+    ctx->adj_valid = true;
+    ctx->adj_iters = I;
+    ctx->adj_nrays = n;
   }
   return FRT_OK;
 }

@@ -280,6 +280,29 @@ class FlexRT:
         """write_vIF_binary (io_funcs.cc:396-430)."""
         self._check(self._lib.frt_write_vif_file(self._h, os.fsencode(path)), "write_vif_file")
 
+    # -- adjoint of find_avg_unu (FP64 form) ------------------------------------------------
+    def adj_enable(self, on=True):
+        """Record an adjoint tape in every following find_avg_unu."""
+        self._check(self._lib.frt_adj_enable(self._h, 1 if on else 0), "adj_enable")
+
+    def adj_find_avg_unu(self, a_nh1):
+        """Gradient of <a_nh1, n_HI after the last find_avg_unu> with respect to
+        the rays' starting photons ([ray][nfreq], stored units), clump, nH, nHe
+        and the starting nH1, nHe1 (row-major cells); paths, dumps and
+        recombination coefficients held fixed."""
+        n3 = self.n ** 3
+        a = np.ascontiguousarray(np.asarray(a_nh1, dtype=np.float64).reshape(n3))
+        out = {k: np.zeros(n3) for k in ("clump", "nH", "nHe", "nH1", "nHe1")}
+        nr = C.c_int64()
+        self._check(self._lib.frt_adj_ray_count(self._h, C.byref(nr)), "adj_ray_count")
+        nph = np.zeros(nr.value * self.nfreq + 1)
+        P = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        self._check(self._lib.frt_adj_find_avg_unu(self._h, P(a), P(nph), P(out["clump"]), P(out["nH"]),
+                                                   P(out["nHe"]), P(out["nH1"]), P(out["nHe1"])),
+                    "adj_find_avg_unu")
+        out["nphot"] = nph[: nr.value * self.nfreq].reshape(nr.value, self.nfreq)
+        return out
+
     # -- MP-Gadget coupling (io_funcs.cc:434-496) ----------------------------------------
     def update_gadget_var(self):
         """Adds this step's Gamma_HI dt and heat_rate dt / rho (needs rho)."""

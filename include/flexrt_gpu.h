@@ -348,6 +348,24 @@ int frt_reset_gadget_var(frt_ctx* ctx);
 /* gadget_step_time: the summed time steps since the last reset, s. */
 int frt_get_gadget_step_time(const frt_ctx* ctx, float* step_time);
 
+/* ---- adjoint of find_avg_unu (FP64 form) --------------------------------------
+ * No counterpart in the reference: the gradient of one find_avg_unu
+ * (ray_tracing.cc:623-1216, C2Ray branch) with the ray paths, the dump
+ * decisions and the recombination coefficients held at their forward values.
+ * frt_adj_enable(ctx, 1) makes every following find_avg_unu (also inside
+ * frt_step) record a tape; frt_adj_find_avg_unu then takes the adjoint of the
+ * n_HI field that call produced (a_nh1, row-major) and returns, for any
+ * non-NULL output, the gradient with respect to the rays' starting photons
+ * (g_nphot, [ray][nfreq] in the list order of that call, per unit of the
+ * stored photons), the clumping factor (g_clump) and the inputs nH, nHe and
+ * the starting n_HI, n_HeI (row-major cells).  Valid until the next
+ * update_rays. */
+int frt_adj_enable(frt_ctx* ctx, int32_t on);
+/* Rays on the tape (the length of g_nphot in rays); 0 without a tape. */
+int frt_adj_ray_count(const frt_ctx* ctx, int64_t* n);
+int frt_adj_find_avg_unu(frt_ctx* ctx, const double* a_nh1, double* g_nphot, double* g_clump, double* g_nH,
+                         double* g_nHe, double* g_nH1, double* g_nHe1);
+
 /* ---- measurement ----------------------------------------------------------- */
 /* When enabled, every kernel launch is bracketed by HIP events on the
  * context stream and its duration accumulated per kernel name. */

@@ -1,0 +1,98 @@
+"""Adjoint of find_avg_unu (FP64 form) against central finite differences of
+the forward itself, through the torch.autograd.Function wrapper: the
+directional derivative of a weighted sum of x_HI along random directions in
+the ray photons, the clumping factors and the per-cell log-density, with the
+ray paths, dumps and recombination coefficients frozen (the perturbations are
+small enough not to move a dump).
+
+The forward stores n_HI in float, so a finite difference of a sum over many
+cells carries ~1e-7 relative rounding per cell; the weights sit on the
+cells the radiation has reached (ionized fraction above 1 %), where x_HI
+moves most, which keeps that noise well below the 1 % tolerance."""
+import numpy as np
+import pytest
+
+from flexrt_imaging_reion_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-3
+
+
+def _print_case(*a):
+    print("CASE", *a, flush=True)
+
+
+def _state(n=16, n_src=8, steps=2):
+    from flexrt_imaging_reion_amd._abi import FIELDS
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    wl = inputs.make_workload("lognormal64", n=n, n_src=n_src, steps=steps)
+    rt = FlexRT(dict(wl["cfg"], precision=0))
+    inputs.setup(rt, wl)
+    for _ in range(steps):
+        rt.step()
+    fields = {k: rt.download_field(k) for k in FIELDS}
+    return wl, rt, fields, rt.rays(), rt.get_time()
+
+
+def test_adjoint_against_finite_differences():
+    import torch
+
+    from flexrt_imaging_reion_amd.adjoint import FindAvgUnu, FrozenStep
+
+    _print_case("adjoint fp64 lognormal n=16 src=8 after 4 steps")
+    wl, rt, fields, rays, t = _state(steps=4)
+    try:
+        step = FrozenStep(rt, fields, rays, t)
+        F = rt.nfreq
+        n3 = 16 ** 3
+        nph0 = np.asarray(rays["nphot"], dtype=np.float64).reshape(-1, F)
+        c0 = fields["clump"].astype(np.float64)
+        s0 = np.zeros(n3)
+        nH0 = fields["nH"].astype(np.float64)
+        rng = np.random.default_rng(3)
+        x0 = step.run(nph0, c0, s0) / nH0
+        ion = (1. - x0) > 0.01  # reached by the radiation: ionized fraction > 1 %
+        assert 10 < ion.sum() < n3 // 2, "the state must hold ionized regions"
+        w = np.where(ion, rng.standard_normal(n3), 0.)
+
+        def loss_np(nph, c, s):
+            return float(np.sum(w * step.run(nph, c, s) / (nH0 * np.exp(s))))
+
+        tn = torch.tensor(nph0, requires_grad=True)
+        tc = torch.tensor(c0, requires_grad=True)
+        ts = torch.tensor(s0, requires_grad=True)
+        nh1 = FindAvgUnu.apply(tn, tc, ts, step)
+        L = (torch.from_numpy(w) * nh1 / (torch.from_numpy(nH0) * torch.exp(ts))).sum()
+        L.backward()
+        g = {"nph": tn.grad.numpy(), "c": tc.grad.numpy(), "s": ts.grad.numpy()}
+        assert all(np.all(np.isfinite(v)) for v in g.values())
+        dirs = {"nph": rng.standard_normal(nph0.shape) * nph0, "c": rng.standard_normal(n3) * c0,
+                "s": rng.standard_normal(n3)}
+        for name, v in dirs.items():
+            args_p = {"nph": nph0, "c": c0, "s": s0}
+            args_m = dict(args_p)
+            args_p[name] = args_p[name] + H * v
+            args_m[name] = args_m[name] - H * v
+            fd = (loss_np(args_p["nph"], args_p["c"], args_p["s"]) -
+                  loss_np(args_m["nph"], args_m["c"], args_m["s"])) / (2 * H)
+            an = float(np.sum(g[name] * v))
+            print("direction", name, "adjoint", an, "finite difference", fd, flush=True)
+            assert abs(an - fd) <= 1e-2 * abs(fd), (name, an, fd)
+    finally:
+        rt.close()
+
+
+def test_adjoint_needs_a_tape():
+    from flexrt_imaging_reion_amd.rt import FlexRT, FlexRTError
+
+    _print_case("adjoint without tape")
+    wl = inputs.make_workload("lognormal64", n=16, n_src=8, steps=1)
+    with FlexRT(dict(wl["cfg"], precision=0)) as rt:
+        inputs.setup(rt, wl)
+        with pytest.raises(FlexRTError):
+            rt.adj_find_avg_unu(np.zeros(16 ** 3))
+    with FlexRT(dict(wl["cfg"], precision=1)) as rt:
+        with pytest.raises(FlexRTError):
+            rt.adj_enable(True)
