@@ -74,3 +74,20 @@ class FindAvgUnu(torch.autograd.Function):
         g_log_rho = sum(g[k] * f[k].astype(np.float64) * s for k in ("nH", "nHe", "nH1", "nHe1"))
         return (torch.from_numpy(np.array(g["nphot"])), torch.from_numpy(g["clump"]),
                 torch.from_numpy(g_log_rho), None)
+
+
+def log_luminosity_gradient(grad_nphot, rays, src_cells):
+    """d/dlog(L) of each source cell from the photon gradient of one step.
+
+    The rays a source cast at the end of the previous step (new_rays,
+    rt_funcs.cc:240-261) have not moved yet: r = 0 and their cell is the
+    source cell, and their photons are proportional to its luminosity
+    (ray_tracing.cc:302-304).  Rays cast earlier depend on it through earlier
+    steps, which this one-step gradient holds fixed."""
+    g = np.asarray(grad_nphot, dtype=np.float64)
+    nph = np.asarray(rays["nphot"], dtype=np.float64).reshape(g.shape)
+    new = np.asarray(rays["r"]) == 0
+    cell = np.asarray(rays["cell"])
+    per_ray = np.sum(g * nph, axis=1)
+    return np.array([float(per_ray[new & (cell == c)].sum()) for c in np.asarray(src_cells)])
+

@@ -80,6 +80,18 @@ def test_adjoint_against_finite_differences():
             an = float(np.sum(g[name] * v))
             print("direction", name, "adjoint", an, "finite difference", fd, flush=True)
             assert abs(an - fd) <= 1e-2 * abs(fd), (name, an, fd)
+        # source luminosities: the photons of each source's freshly cast rays scale with L
+        from flexrt_imaging_reion_amd.adjoint import log_luminosity_gradient
+
+        gl = log_luminosity_gradient(g["nph"], rays, wl["src_cells"])
+        new = np.asarray(rays["r"]) == 0
+        assert new.sum() == 12 * len(wl["src_cells"])  # 12 HEALPix rays per source cell
+        j = int(np.argmax(np.abs(gl)))
+        sel = (new & (np.asarray(rays["cell"]) == wl["src_cells"][j]))[:, None]
+        fd = (loss_np(np.where(sel, nph0 * (1 + H), nph0), c0, s0) -
+              loss_np(np.where(sel, nph0 * (1 - H), nph0), c0, s0)) / (2 * H)
+        print("d/dlogL of source", j, "adjoint", gl[j], "finite difference", fd, flush=True)
+        assert abs(gl[j] - fd) <= 1e-2 * abs(fd)
     finally:
         rt.close()
 
