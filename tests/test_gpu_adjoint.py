@@ -108,3 +108,24 @@ def test_adjoint_needs_a_tape():
     with FlexRT(dict(wl["cfg"], precision=1)) as rt:
         with pytest.raises(FlexRTError):
             rt.adj_enable(True)
+
+
+def test_tape_leaves_forward_unchanged():
+    """Recording the tape changes no bit of the forward."""
+    from flexrt_imaging_reion_amd.adjoint import FrozenStep
+
+    _print_case("adjoint tape on/off forward identity")
+    wl, rt, fields, rays, t = _state(steps=3)
+    try:
+        step = FrozenStep(rt, fields, rays, t)
+        args = (np.asarray(rays["nphot"]).reshape(-1, rt.nfreq), fields["clump"], np.zeros(16 ** 3))
+        on = step.run(*args)
+        rays_on = rt.rays()
+        rt.adj_enable(False)
+        off = step.run(*args)
+        assert np.array_equal(on.view(np.uint64), off.view(np.uint64))
+        from parity_util import compare_rays
+
+        assert compare_rays(rt.rays(), rays_on) is None
+    finally:
+        rt.close()
