@@ -54,6 +54,9 @@ _PROTOS = {
     "frt_adj_enable": (C.c_int, [_CTX, C.c_int32]),
     "frt_adj_ray_count": (C.c_int, [_CTX, C.POINTER(C.c_int64)]),
     "frt_adj_find_avg_unu": (C.c_int, [_CTX] + [C.POINTER(C.c_double)] * 7),
+    "frt_checkpoint_save": (C.c_int, [_CTX, C.POINTER(C.c_void_p)]),
+    "frt_checkpoint_restore": (C.c_int, [_CTX, C.c_void_p]),
+    "frt_checkpoint_free": (C.c_int, [C.c_void_p]),
     "frt_write_ray_file": (C.c_int, [_CTX, C.c_char_p]),
     "frt_read_ray_file": (C.c_int, [_CTX, C.c_char_p]),
     "frt_write_gas_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),

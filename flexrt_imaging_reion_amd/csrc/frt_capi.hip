@@ -748,6 +748,155 @@ int frt_write_otf(frt_ctx* ctx, const char* path, float avg_rho, float temp_0) {
   return FRT_OK;
 }
 
+// ---- device checkpoints -------------------------------------------------------
+}  // extern "C"
+
+// The whole state a later step depends on, copied device to device: every
+// cell field, the packed opacities, the live rays, the sources' accumulated
+// ray_dt, and the scalars (time, step and casting counters, photon bounds,
+// the grid constants that expansion and hydro steps change).
+struct frt_checkpoint {
+  int device = 0;
+  int64_t n3 = 0, nrays = 0;
+  int nfreq = 0, groups = 0;
+  float* cells = nullptr;
+  float2* avg2 = nullptr;
+  float4* pos = nullptr;
+  float4* dir = nullptr;
+  int4* meta = nullptr;
+  float4* phot = nullptr;
+  float* src_ray_dt = nullptr;
+  std::vector<int32_t> src_cell;
+  std::vector<double> src_lum;
+  GridConst g;
+  frt_config cfg;
+  float t = 0.f, nphot_merge = 0.f, gadget_step_time = 0.f;
+  double nphot_min = 0., nphot_max = 0.;
+  int64_t step = 0;
+  uint64_t cast_calls = 0;
+  bool have_rho = false;
+};
+
+static void checkpoint_release(frt_checkpoint* ck) {
+  hipFree(ck->cells); hipFree(ck->avg2); hipFree(ck->pos); hipFree(ck->dir); hipFree(ck->meta);
+  hipFree(ck->phot); hipFree(ck->src_ray_dt);
+  delete ck;
+}
+
+extern "C" {
+
+int frt_checkpoint_save(frt_ctx* ctx, frt_checkpoint** out) {
+  if (!ctx || !out) return FRT_E_ARG;
+  *out = nullptr;
+  int rc = ensure_compact(ctx);
+  if (rc) return rc;
+  frt_checkpoint* ck = new frt_checkpoint();
+  ck->device = ctx->device;
+  ck->n3 = ctx->g.n3;
+  ck->nfreq = ctx->g.nfreq;
+  ck->groups = frt_phot_groups(ctx->g.nfreq);
+  ck->nrays = ctx->nrays;
+  const size_t ncell = (size_t)(C_NSCALAR + 3 * ck->nfreq) * (size_t)ck->n3;
+  const size_t nr = (size_t)std::max<int64_t>(ck->nrays, 1);
+  hipError_t e = hipMalloc((void**)&ck->cells, ncell * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&ck->avg2, (size_t)ck->n3 * sizeof(float2));
+  if (e == hipSuccess) e = hipMalloc((void**)&ck->pos, nr * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc((void**)&ck->dir, nr * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc((void**)&ck->meta, nr * sizeof(int4));
+  if (e == hipSuccess) e = hipMalloc((void**)&ck->phot, nr * ck->groups * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc((void**)&ck->src_ray_dt, (size_t)std::max(ctx->nsrc, 1) * sizeof(float));
+  if (e != hipSuccess) {
+    checkpoint_release(ck);
+    return set_error(ctx, FRT_E_NOMEM, "checkpoint allocation");
+  }
+  const RaySoA& b = ctx->buf[ctx->cur];
+  const size_t n = (size_t)ck->nrays;
+  e = hipMemcpyAsync(ck->cells, ctx->cell_block, ncell * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ck->avg2, ctx->c.avg2, (size_t)ck->n3 * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(ck->pos, b.pos, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(ck->dir, b.dir, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(ck->meta, b.meta, n * sizeof(int4), hipMemcpyDeviceToDevice, ctx->stream);
+  for (int gi = 0; gi < ck->groups && e == hipSuccess && n; ++gi)
+    e = hipMemcpyAsync(ck->phot + (size_t)gi * n, b.phot + (size_t)gi * ctx->g.cap, n * sizeof(float4),
+                       hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && ctx->nsrc > 0)
+    e = hipMemcpyAsync(ck->src_ray_dt, ctx->src_ray_dt, (size_t)ctx->nsrc * sizeof(float), hipMemcpyDeviceToDevice,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    checkpoint_release(ck);
+    return hip_check(ctx, e, "checkpoint");
+  }
+  ck->src_cell = ctx->h_src_cell;
+  ck->src_lum = ctx->h_src_lum;
+  ck->g = ctx->g;
+  ck->cfg = ctx->cfg;
+  ck->t = ctx->t;
+  ck->nphot_merge = ctx->nphot_merge;
+  ck->gadget_step_time = ctx->gadget_step_time;
+  ck->nphot_min = ctx->nphot_min;
+  ck->nphot_max = ctx->nphot_max;
+  ck->step = ctx->step;
+  ck->cast_calls = ctx->cast_calls;
+  ck->have_rho = ctx->have_rho;
+  *out = ck;
+  return FRT_OK;
+}
+
+int frt_checkpoint_restore(frt_ctx* ctx, const frt_checkpoint* ck) {
+  if (!ctx || !ck) return FRT_E_ARG;
+  if (ck->n3 != ctx->g.n3 || ck->nfreq != ctx->g.nfreq || ck->device != ctx->device)
+    return set_error(ctx, FRT_E_ARG, "checkpoint of another grid, bin count or device");
+  if (ck->nrays > ctx->g.cap) return set_error(ctx, FRT_E_CAPACITY, "checkpoint holds more rays than the buffer");
+  int rc = FRT_OK;
+  if (ck->src_cell != ctx->h_src_cell || ck->src_lum != ctx->h_src_lum) {
+    rc = frt_set_sources(ctx, (int32_t)ck->src_cell.size(), ck->src_cell.data(), ck->src_lum.data());
+    if (rc) return rc;
+  }
+  const size_t ncell = (size_t)(C_NSCALAR + 3 * ck->nfreq) * (size_t)ck->n3;
+  const size_t n = (size_t)ck->nrays;
+  const RaySoA& b = ctx->buf[ctx->cur];
+  hipError_t e = hipMemcpyAsync(ctx->cell_block, ck->cells, ncell * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ctx->c.avg2, ck->avg2, (size_t)ck->n3 * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(b.pos, ck->pos, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(b.dir, ck->dir, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(b.meta, ck->meta, n * sizeof(int4), hipMemcpyDeviceToDevice, ctx->stream);
+  for (int gi = 0; gi < ck->groups && e == hipSuccess && n; ++gi)
+    e = hipMemcpyAsync(b.phot + (size_t)gi * ctx->g.cap, ck->phot + (size_t)gi * n, n * sizeof(float4),
+                       hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && ctx->nsrc > 0)
+    e = hipMemcpyAsync(ctx->src_ray_dt, ck->src_ray_dt, (size_t)ctx->nsrc * sizeof(float), hipMemcpyDeviceToDevice,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  rc = hip_check(ctx, e, "checkpoint restore");
+  if (rc) return rc;
+  const int64_t cap = ctx->g.cap;
+  ctx->g = ck->g;
+  ctx->g.cap = cap;
+  ctx->cfg = ck->cfg;
+  ctx->nrays = ck->nrays;
+  ctx->t = ck->t;
+  ctx->nphot_merge = ck->nphot_merge;
+  ctx->gadget_step_time = ck->gadget_step_time;
+  ctx->nphot_min = ck->nphot_min;
+  ctx->nphot_max = ck->nphot_max;
+  ctx->step = ck->step;
+  ctx->cast_calls = ck->cast_calls;
+  ctx->have_rho = ck->have_rho;
+  ctx->have_segments = false;
+  ctx->have_alive = false;
+  ctx->adj_valid = false;
+  return FRT_OK;
+}
+
+int frt_checkpoint_free(frt_checkpoint* ck) {
+  if (!ck) return FRT_E_ARG;
+  checkpoint_release(ck);
+  return FRT_OK;
+}
+
 int frt_get_time(const frt_ctx* ctx, float* t) {
   if (!ctx || !t) return FRT_E_ARG;
   *t = ctx->t;

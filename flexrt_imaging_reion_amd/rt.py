@@ -59,6 +59,26 @@ def vif_path(output_dir, zz, z_rt):
     return str(output_dir) + "vIF_z=" + pad + ("%f" % float(z_rt))[:-2]
 
 
+class Checkpoint:
+    """A device checkpoint (frt_checkpoint_save); freed with free() or when
+    collected."""
+
+    def __init__(self, lib, handle):
+        self._lib = lib
+        self.handle = handle
+
+    def free(self):
+        if self.handle:
+            self._lib.frt_checkpoint_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class FlexRT:
     """One GPU RT context (device state resident in HBM)."""
 
@@ -279,6 +299,16 @@ class FlexRT:
     def write_vif_file(self, path):
         """write_vIF_binary (io_funcs.cc:396-430)."""
         self._check(self._lib.frt_write_vif_file(self._h, os.fsencode(path)), "write_vif_file")
+
+    # -- device checkpoints -----------------------------------------------------------------
+    def checkpoint(self):
+        """The whole state a later step depends on, kept on the device."""
+        h = C.c_void_p()
+        self._check(self._lib.frt_checkpoint_save(self._h, C.byref(h)), "checkpoint_save")
+        return Checkpoint(self._lib, h)
+
+    def restore(self, ck):
+        self._check(self._lib.frt_checkpoint_restore(self._h, ck.handle), "checkpoint_restore")
 
     # -- adjoint of find_avg_unu (FP64 form) ------------------------------------------------
     def adj_enable(self, on=True):

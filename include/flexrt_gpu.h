@@ -348,6 +348,19 @@ int frt_reset_gadget_var(frt_ctx* ctx);
 /* gadget_step_time: the summed time steps since the last reset, s. */
 int frt_get_gadget_step_time(const frt_ctx* ctx, float* step_time);
 
+/* ---- device checkpoints ---------------------------------------------------------
+ * No counterpart in the reference, whose checkpoints are the restart files
+ * (io_funcs.cc:274-393, frt_write_*_file above): the whole state a later step
+ * depends on (cell fields, live rays, the sources' ray_dt, time, step and
+ * casting counters, photon bounds, grid constants), copied device to device
+ * so that a run can return to it, e.g. every K steps of a gradient
+ * computation.  A checkpoint lives on the context's device until freed, and
+ * is restored into a context of the same grid and bin count. */
+typedef struct frt_checkpoint frt_checkpoint;
+int frt_checkpoint_save(frt_ctx* ctx, frt_checkpoint** out);
+int frt_checkpoint_restore(frt_ctx* ctx, const frt_checkpoint* ck);
+int frt_checkpoint_free(frt_checkpoint* ck);
+
 /* ---- adjoint of find_avg_unu (FP64 form) --------------------------------------
  * No counterpart in the reference: the gradient of one find_avg_unu
  * (ray_tracing.cc:623-1216, C2Ray branch) with the ray paths, the dump
