@@ -1,6 +1,8 @@
-// Adjoint of one find_avg_unu (ray_tracing.cc:623-1216, C2Ray branch) in the
-// FP64 form, with the ray paths, the dump decisions and the recombination
-// coefficients held at their forward values.
+// Adjoint of one find_avg_unu (ray_tracing.cc:623-1216, C2Ray branch), with
+// the ray paths, the dump decisions and the recombination coefficients held
+// at their forward values.  The reverse sweep is FP64 in both forms; for the
+// mixed form it differentiates the same expressions the FP32 forward
+// evaluates, at the forward's recorded values.
 //
 // The forward call records a tape (frt_kernels.hip, launch_find_avg_unu): for
 // every C2Ray pass k the opacities n_HI, n_HeI its deposit used and the cell
@@ -251,8 +253,6 @@ extern "C" {
 
 int frt_adj_enable(frt_ctx* ctx, int32_t on) {
   if (!ctx) return FRT_E_ARG;
-  if (on && ctx->cfg.precision != FRT_PREC_FP64)
-    return set_error(ctx, FRT_E_ARG, "the adjoint of find_avg_unu is provided for the FP64 form");
   if (on && !ctx->adj_tape) {
     const size_t nd = (size_t)(4 * ctx->cfg.itercount + 2) * (size_t)ctx->g.n3;
     if (hipMalloc((void**)&ctx->adj_tape, nd * sizeof(double)) != hipSuccess) {

@@ -1863,9 +1863,9 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   const int F = g.nfreq;
   ctx->deposit_bytes_last = 0.;
   ctx->deposit_launches_last = 0;
-  // adjoint tape (FP64 form): per pass [opacity HI, HeI, Gamma_HI, Gamma_HeI][n3],
-  // then the starting f_H1, f_He1, and the rays' starting photons
-  double* tape = (ctx->adj_on && !mixed) ? ctx->adj_tape : nullptr;
+  // adjoint tape: per pass [opacity HI, HeI, Gamma_HI, Gamma_HeI][n3], then
+  // the starting f_H1, f_He1, and the rays' starting photons
+  double* tape = ctx->adj_on ? ctx->adj_tape : nullptr;
   if (tape) {
     FRT_LAUNCH(ctx, "adj_tape", gc, k_tape_pair, ctx->c.f[C_FH1], ctx->c.f[C_FHE1], tape + (size_t)4 * I * g.n3,
                (int64_t)g.n3);

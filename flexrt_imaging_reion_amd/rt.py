@@ -310,7 +310,7 @@ class FlexRT:
     def restore(self, ck):
         self._check(self._lib.frt_checkpoint_restore(self._h, ck.handle), "checkpoint_restore")
 
-    # -- adjoint of find_avg_unu (FP64 form) ------------------------------------------------
+    # -- adjoint of find_avg_unu ------------------------------------------------
     def adj_enable(self, on=True):
         """Record an adjoint tape in every following find_avg_unu."""
         self._check(self._lib.frt_adj_enable(self._h, 1 if on else 0), "adj_enable")

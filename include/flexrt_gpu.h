@@ -361,10 +361,11 @@ int frt_checkpoint_save(frt_ctx* ctx, frt_checkpoint** out);
 int frt_checkpoint_restore(frt_ctx* ctx, const frt_checkpoint* ck);
 int frt_checkpoint_free(frt_checkpoint* ck);
 
-/* ---- adjoint of find_avg_unu (FP64 form) --------------------------------------
+/* ---- adjoint of find_avg_unu ---------------------------------------------------
  * No counterpart in the reference: the gradient of one find_avg_unu
  * (ray_tracing.cc:623-1216, C2Ray branch) with the ray paths, the dump
- * decisions and the recombination coefficients held at their forward values.
+ * decisions and the recombination coefficients held at their forward values
+ * (reverse sweep in FP64 for both precision forms).
  * frt_adj_enable(ctx, 1) makes every following find_avg_unu (also inside
  * frt_step) record a tape; frt_adj_find_avg_unu then takes the adjoint of the
  * n_HI field that call produced (a_nh1, row-major) and returns, for any
