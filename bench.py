@@ -61,6 +61,9 @@ def parse():
                     help="after the mixed run, time the driver's per-step device ops "
                          "(update_cosmo_expansion, update_hydro_step) on its state, and the CPU "
                          "checker's on the same grid")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time one find_avg_unu forward with its adjoint tape and the adjoint "
+                         "(frt_adj_find_avg_unu) on the differentiation workload (64^3, 16 sources)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the runs, copy MIB MiB through the library's plain copy kernel "
                          "(reference dispatch for HBM counter calibration)")
@@ -231,6 +234,51 @@ def time_driver_ops(g, wl, reps=5, sample_rays=2_000_000):
     log("driver ops: cells %.3f ms, rays %.3f ms (GPU); cells %.1f ms (CPU)" % (cell_ms, ray_ms, 1e3 * tc))
     return out
 
+
+
+def time_adjoint(precision=0, steps=5, reps=5):
+    """Forward (update_rays + find_avg_unu) without and with the tape, and the
+    adjoint, on the adjoint64 workload after `steps` steps: wall ms, best of
+    `reps` (the state is restored from a device checkpoint before each)."""
+    from flexrt_imaging_reion_amd import inputs
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    wl = inputs.make_workload("adjoint64", steps=steps)
+    g = FlexRT(dict(wl["cfg"], precision=precision))
+    inputs.setup(g, wl)
+    for _ in range(steps):
+        g.step()
+    ck = g.checkpoint()
+    n3 = int(wl["cfg"]["n"]) ** 3
+    a = np.random.default_rng(0).standard_normal(n3)
+
+    def fwd():
+        g.restore(ck)
+        g.update_rays()
+        g.find_avg_unu()
+        g.synchronize()
+
+    out = {}
+    for tape in (False, True):
+        g.adj_enable(tape)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fwd()
+            ts.append(time.perf_counter() - t0)
+        out["forward_with_tape_ms" if tape else "forward_ms"] = round(1e3 * min(ts), 3)
+    ts = []
+    for _ in range(reps):
+        fwd()
+        t0 = time.perf_counter()
+        g.adj_find_avg_unu(a)
+        ts.append(time.perf_counter() - t0)
+    out["adjoint_ms"] = round(1e3 * min(ts), 3)
+    out["workload"] = "adjoint64: 64^3, 16 sources, C=2, state after %d steps, %d rays" % (steps, g.ray_count())
+    ck.free()
+    g.close()
+    log("adjoint: %s" % out)
+    return out
 
 
 def restart_files(g, wl, d, sample=2_000_000):
@@ -430,6 +478,8 @@ def main():
             "T_max_rel": rel(state_m["temp"], state_f["temp"]),
             "ray_counts_exact": all(all(sm[k] == sf[k] for k in keys) for sm, sf in zip(st_m, st_f)),
         }
+    if a.adjoint:
+        out["adjoint"] = time_adjoint()
     if files and "driver_ops" in files:
         out["driver_ops"] = files.pop("driver_ops")
     if files:

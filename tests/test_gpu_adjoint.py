@@ -127,3 +127,56 @@ def test_tape_leaves_forward_unchanged():
         assert compare_rays(rt.rays(), rays_on) is None
     finally:
         rt.close()
+
+
+def test_adjoint_at_the_differentiation_config():
+    """The differentiation configuration (64^3, lognormal sigma_ln 0.8, 16
+    sources on the densest cells, C = 2): d/dlog rho and d/dlog L of one
+    source from the adjoint against finite differences, FP64 form."""
+    import torch
+
+    from flexrt_imaging_reion_amd._abi import FIELDS
+    from flexrt_imaging_reion_amd.adjoint import FindAvgUnu, FrozenStep, log_luminosity_gradient
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    _print_case("adjoint fp64 adjoint64 (64^3, 16 sources) after 3 steps")
+    wl = inputs.make_workload("adjoint64", steps=3)
+    rt = FlexRT(dict(wl["cfg"], precision=0))
+    try:
+        inputs.setup(rt, wl)
+        for _ in range(3):
+            rt.step()
+        fields = {k: rt.download_field(k) for k in FIELDS}
+        rays = rt.rays()
+        step = FrozenStep(rt, fields, rays, rt.get_time())
+        F, n3 = rt.nfreq, 64 ** 3
+        nph0 = np.asarray(rays["nphot"], dtype=np.float64).reshape(-1, F)
+        c0 = fields["clump"].astype(np.float64)
+        s0 = np.zeros(n3)
+        nH0 = fields["nH"].astype(np.float64)
+        x0 = step.run(nph0, c0, s0) / nH0
+        rng = np.random.default_rng(5)
+        w = np.where((1. - x0) > 0.01, rng.standard_normal(n3), 0.)
+        assert np.count_nonzero(w) > 100
+
+        def loss_np(nph, s):
+            return float(np.sum(w * step.run(nph, c0, s) / (nH0 * np.exp(s))))
+
+        tn = torch.tensor(nph0, requires_grad=True)
+        ts = torch.tensor(s0, requires_grad=True)
+        nh1 = FindAvgUnu.apply(tn, torch.tensor(c0), ts, step)
+        (torch.from_numpy(w) * nh1 / (torch.from_numpy(nH0) * torch.exp(ts))).sum().backward()
+        v = rng.standard_normal(n3)
+        fd = (loss_np(nph0, s0 + H * v) - loss_np(nph0, s0 - H * v)) / (2 * H)
+        an = float(np.sum(ts.grad.numpy() * v))
+        print("d/dlog rho adjoint", an, "finite difference", fd, flush=True)
+        assert abs(an - fd) <= 1e-2 * abs(fd)
+        gl = log_luminosity_gradient(tn.grad.numpy(), rays, wl["src_cells"])
+        j = int(np.argmax(np.abs(gl)))
+        sel = ((np.asarray(rays["r"]) == 0) & (np.asarray(rays["cell"]) == wl["src_cells"][j]))[:, None]
+        fd = (loss_np(np.where(sel, nph0 * (1 + H), nph0), s0) -
+              loss_np(np.where(sel, nph0 * (1 - H), nph0), s0)) / (2 * H)
+        print("d/dlog L adjoint", gl[j], "finite difference", fd, flush=True)
+        assert abs(gl[j] - fd) <= 1e-2 * abs(fd)
+    finally:
+        rt.close()
