@@ -624,10 +624,11 @@ struct SegPipe {
 
 // One segment of the FP64 deposit; returns true if the ray is dumped here.
 // The last pass stores the segment's photon record instead of its deposit.
+template <int NFC>
 __device__ __forceinline__ bool deposit_segment_fp64(const GridConst& g, ContribPtrs& cp, bool last, int slot,
                                                      float dist, float ah1, float ahe1, const double* nph,
                                                      double* tauh, double* tauhe) {
-  const int F = g.nfreq;
+  const int F = NFC > 0 ? NFC : g.nfreq;
   const double distd = (double)dist;
   double dth[FRT_MAX_FREQ], dthe[FRT_MAX_FREQ], nin[FRT_MAX_FREQ];
   double tot = 0.;
@@ -659,7 +660,7 @@ __device__ __forceinline__ bool deposit_segment_fp64(const GridConst& g, Contrib
 
 // FP64 form: the reference's double locals and libm exp (through detmath),
 // with the per-cell accumulation in double.
-template <bool LAST>
+template <bool LAST, int NFC = 0>
 __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, GridConst g,
                                                       const int32_t* __restrict__ path_cell,
                                                       const float* __restrict__ path_dist,
@@ -672,7 +673,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
   for_rays_by_count<DEP_CHUNK_FP64>(seg_cnt, nrays, [&](int64_t i, int c) {
     bool dumped_flag = false, removed = false;
     {
-      const int F = g.nfreq;
+      const int F = NFC > 0 ? NFC : g.nfreq;
       double nph[FRT_MAX_FREQ], tauh[FRT_MAX_FREQ], tauhe[FRT_MAX_FREQ];
       float n0[8];
       phot_load(rb, i, F, g.cap, n0);
@@ -696,7 +697,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
       for (; p < c; ++p) {
         sp.prefetch(p);
         last_cell = sp.cell0;
-        if (deposit_segment_fp64(g, cp, LAST, sp.slot0, sp.dist0, sp.ah0, sp.ahe0, nph, tauh, tauhe)) {
+        if (deposit_segment_fp64<NFC>(g, cp, LAST, sp.slot0, sp.dist0, sp.ah0, sp.ahe0, nph, tauh, tauhe)) {
           dumped = true;
           ++p;
           break;
@@ -742,10 +743,11 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
 
 // One segment of the mixed deposit; returns true if the ray is dumped here.
 // The last pass stores the segment's photon record instead of its deposit.
+template <int NFC>
 __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, ContribPtrs& cp, bool last, int slot,
                                                       float dist, float ah1, float ahe1, const float* nph,
                                                       float* trans, float inv_dtv, float nmin) {
-  const int F = g.nfreq;
+  const int F = NFC > 0 ? NFC : g.nfreq;
   float nin[FRT_MAX_FREQ];
   float tot = 0.f;
   FRT_FOR_BINS(nu) {
@@ -783,7 +785,7 @@ __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, Contri
 
 // Mixed form: FP32 attenuation and photon bookkeeping; per-cell sums are
 // compensated (Kahan) in the cell kernel.
-template <bool LAST>
+template <bool LAST, int NFC = 0>
 __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays, GridConst g,
                                                        const int32_t* __restrict__ path_cell,
                                                        const float* __restrict__ path_dist,
@@ -799,7 +801,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
   for_rays(seg_cnt, nrays, [&](int64_t i, int c) {
     bool dumped_flag = false, removed = false;
     {
-      const int F = g.nfreq;
+      const int F = NFC > 0 ? NFC : g.nfreq;
       float nph[8], trans[FRT_MAX_FREQ];
       phot_load(rb, i, F, g.cap, nph);  // in units of phot_norm
       FRT_FOR_BINS(nu) trans[nu] = 1.f;  // exp(-tau) carried multiplicatively
@@ -818,7 +820,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
       for (; p < c; ++p) {
         sp.prefetch(p);
         last_cell = sp.cell0;
-        if (deposit_segment_mixed(g, cp, LAST, sp.slot0, sp.dist0, sp.ah0, sp.ahe0, nph, trans, inv_dtv, nmin)) {
+        if (deposit_segment_mixed<NFC>(g, cp, LAST, sp.slot0, sp.dist0, sp.ah0, sp.ahe0, nph, trans, inv_dtv, nmin)) {
           dumped = true;
           ++p;
           break;
@@ -939,7 +941,9 @@ __device__ __forceinline__ int thermal_cell(const GridConst& g, const ThermalIn&
 // finalize 1139-1210), one thread per cell.  FIRST also performs the
 // iteration-0 reset of :655-683; LAST finalizes and, with THERMAL, runs
 // update_thermal in the same pass (merge and split touch no cell field).
-template <typename ACC, bool FIRST, bool LAST, bool THERMAL>
+// NFC > 0 fixes the bin count at compile time (the per-bin register arrays
+// then hold NFC values instead of FRT_MAX_FREQ); NFC = 0 reads g.nfreq.
+template <typename ACC, bool FIRST, bool LAST, bool THERMAL, int NFC = 0>
 __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribPtrs cp,
                                               const int32_t* __restrict__ cstart,
                                               const int32_t* __restrict__ cend,
@@ -949,7 +953,7 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
   if (k < g.n3) {
     const int64_t n3 = g.n3;
     const int s0 = cstart[k], s1 = cend[k];
-    const int F = g.nfreq;
+    const int F = NFC > 0 ? NFC : g.nfreq;
     // opacities this pass's deposit used (ray_tracing.cc:656-657,733-743)
     const float ah1 = FIRST ? cf.f[C_NH1][k] : cf.f[C_AVG_NH1][k];
     const float ahe1 = FIRST ? cf.f[C_NHE1][k] : cf.f[C_AVG_NHE1][k];
@@ -1115,6 +1119,15 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       tape_g[k] = GH;
       tape_g[n3 + k] = GHe;
     }
+    if (LAST) {
+      // per-bin rates depend on the sums only: stored here, so they are not
+      // held in registers through the chemistry below
+      FRT_FOR_BINS(nu) {
+        cf.gamma_h1_nu[nu * n3 + k] = (float)GHnu[nu];
+        cf.gamma_he1_nu[nu * n3 + k] = (float)GHenu[nu];
+        cf.ngam_nu[nu * n3 + k] = (float)NGnu[nu];
+      }
+    }
     const float nH = cf.f[C_NH][k];
     const float nHe = cf.f[C_NHE][k];
     const float clump = cf.f[C_CLUMP][k];
@@ -1165,11 +1178,6 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       if (((double)(avg_h1 / nH) < 0.5) && (cf.f[C_TREION_FLAG][k] == 0.f)) {
         cf.f[C_TREION][k] = g.t;
         cf.f[C_TREION_FLAG][k] = 1.f;
-      }
-      for (int nu = 0; nu < g.nfreq; ++nu) {
-        cf.gamma_h1_nu[nu * n3 + k] = (float)GHnu[nu];
-        cf.gamma_he1_nu[nu * n3 + k] = (float)GHenu[nu];
-        cf.ngam_nu[nu * n3 + k] = (float)NGnu[nu];
       }
       cf.f[C_GAMMA_H1_TOT][k] = (float)GH;
       cf.f[C_GAMMA_HE1_TOT][k] = (float)GHe;
@@ -1879,25 +1887,30 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
                  ctx->c.f[first ? C_NHE1 : C_AVG_NHE1], tk, (int64_t)g.n3);
     if (first)  // the first pass attenuates with n_HI, n_HeI themselves
       FRT_LAUNCH(ctx, "pack_avg", gc, k_pack_avg, ctx->c.f[C_NH1], ctx->c.f[C_NHE1], ctx->c.avg2, g.n3);
+    // five-bin specializations for the FP64 form (198 -> 167 and 141 VGPRs,
+    // 2 -> 3 waves per SIMD); the mixed kernels were not faster specialized
+    // (first passes 71 -> 75 VGPRs; last pass 84 -> 69 but 3 % slower)
+#define FRT_DEP(NAME, GRID, KERN, L1, SPEC)                                                         \
+  do {                                                                                              \
+    if ((SPEC) && F == 5)                                                                           \
+      FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1, 5>), rb, n, g, ctx->path_cell, ctx->path_dist,          \
+                 ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt);   \
+    else                                                                                            \
+      FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1>), rb, n, g, ctx->path_cell, ctx->path_dist,             \
+                 ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt);   \
+  } while (0)
     if (mixed) {
       if (last)
-        FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib,
-                   ctx->alive, ctx->dcnt);
+        FRT_DEP("deposit_mixed", grid_for(n), k_deposit_mixed, true, false);
       else
-        FRT_LAUNCH(ctx, "deposit_mixed", grid_for(n), k_deposit_mixed<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib,
-                   ctx->alive, ctx->dcnt);
+        FRT_DEP("deposit_mixed", grid_for(n), k_deposit_mixed, false, false);
     } else {
       if (last)
-        FRT_LAUNCH(ctx, "deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64<true>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib,
-                   ctx->alive, ctx->dcnt);
+        FRT_DEP("deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64, true, true);
       else
-        FRT_LAUNCH(ctx, "deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64<false>, rb, n, g, ctx->path_cell,
-                   ctx->path_dist, ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib,
-                   ctx->alive, ctx->dcnt);
+        FRT_DEP("deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64, false, true);
     }
+#undef FRT_DEP
     // algorithmic bytes of this deposit launch (model used for the roofline):
     // per ray: seg_cnt (4) + nphot read (4F); per segment: path cell + dist (8)
     // + slot (4) + avg_nh1/avg_nhe1 (8) + the 2-value deposit (first
@@ -1911,9 +1924,16 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
       ctx->deposit_bytes_last += b;
       ctx->deposit_launches_last += 1;
     }
-#define FRT_CHEM(ACC, F1, L1, TH)                                                           \
-  FRT_LAUNCH(ctx, "chem", gc, (k_chem<ACC, F1, L1, TH>), ctx->c, g, ctx->contrib, ctx->cell_start, \
-             ctx->cell_end, ctx->dcnt, tk ? tk + 2 * g.n3 : nullptr)
+    // the last pass's record loop is specialized for five bins
+#define FRT_CHEM(ACC, F1, L1, TH)                                                                 \
+  do {                                                                                            \
+    if (L1 && F == 5)                                                                             \
+      FRT_LAUNCH(ctx, "chem", gc, (k_chem<ACC, F1, L1, TH, 5>), ctx->c, g, ctx->contrib,          \
+                 ctx->cell_start, ctx->cell_end, ctx->dcnt, tk ? tk + 2 * g.n3 : nullptr);        \
+    else                                                                                          \
+      FRT_LAUNCH(ctx, "chem", gc, (k_chem<ACC, F1, L1, TH>), ctx->c, g, ctx->contrib,             \
+                 ctx->cell_start, ctx->cell_end, ctx->dcnt, tk ? tk + 2 * g.n3 : nullptr);        \
+  } while (0)
     if (mixed) {
       if (first && last) {
         if (fuse_thermal) FRT_CHEM(float, true, true, true); else FRT_CHEM(float, true, true, false);
