@@ -176,7 +176,8 @@ def time_driver_ops(g, wl, reps=5, sample_rays=2_000_000):
     kc, nc = g.kernel_time("cosmo_cells")
     kr, nr = g.kernel_time("cosmo_rays")
     g.reset_kernel_times()
-    g.update_hydro_step(np.float32(wl["cfg"]["zz"]), np.float32(a0))
+    avg_rho0 = np.float32(g.download_field("rho").astype(np.float64).mean())
+    g.update_hydro_step(np.float32(wl["cfg"]["zz"]), np.float32(a0), avg_rho0, np.float32(3.15e13))
     g.synchronize()
     kh, _ = g.kernel_time("cosmo_cells")
     g.reset_kernel_times()

@@ -28,7 +28,7 @@ FIELDS = [
     "temp", "dne_dt", "clump", "nH1_prev", "avg_nh1", "avg_nhe1", "recomb_H2", "recomb_He2",
     "gamma_H1_tot", "gamma_He1_tot", "recrate_H2", "recrate_He2", "treion", "treion_flag",
     "heat_rate", "cool_rate", "IF_speed", "Inc_Flux", "gamma_H1_nu", "gamma_He1_nu", "ngam_nu",
-    "rho", "f_He3", "mfp", "mfp_912", "avg_gamma_prev", "gadget_gamma", "gadget_heat",
+    "rho", "f_He3", "mfp", "mfp_912", "avg_gamma_prev", "gadget_gamma", "gadget_heat", "ddelta_dt", "rho_prev",
 ]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 PER_BIN = {"gamma_H1_nu", "gamma_He1_nu", "ngam_nu"}

@@ -111,15 +111,18 @@ __global__ void __launch_bounds__(ABLK) k_adj_chem(CellPtrs cf, GridConst g, con
   a_C[k] += a_cH * (double)rH2 + a_cHe * (double)rHe2;
 }
 
-// One ray per thread: replay the pass's attenuation (the same expressions and
-// order as deposit_segment_fp64, so the same dump decision), then walk the
-// processed segments backwards.  R[nu] carries the adjoint of the optical
-// depth accumulated in front of the current segment.
+// One ray per thread: the segments the forward pass processed and whether
+// it dumped the ray there come from the tape (the forward's own decision, in
+// either precision form); the optical depth in front of each segment is
+// summed forward, then the processed segments are walked backwards.  R[nu]
+// carries the adjoint of the optical depth accumulated in front of the
+// current segment.
 __global__ void __launch_bounds__(ABLK) k_adj_deposit(GridConst g, int64_t nrays, const int32_t* __restrict__ seg_cnt,
                                                       const int32_t* __restrict__ path_cell,
                                                       const float* __restrict__ path_dist,
                                                       const int32_t* __restrict__ path_slot,
                                                       const double* __restrict__ tk, const float* __restrict__ nph0,
+                                                      const uint8_t* __restrict__ tape_np,
                                                       const double* __restrict__ aGH, const double* __restrict__ aGHe,
                                                       double2* __restrict__ slot_out, double* __restrict__ a_nph) {
   const int64_t i = (int64_t)blockIdx.x * ABLK + threadIdx.x;
@@ -135,25 +138,16 @@ __global__ void __launch_bounds__(ABLK) k_adj_deposit(GridConst g, int64_t nrays
     nph[nu] = (double)nph0[i * F + nu] * FRT_PHOT_NORM;
     tauh[nu] = tauhe[nu] = R[nu] = anph[nu] = 0.;
   }
-  int np = c;
-  bool dumped = false;
-  for (int p = 0; p < c; ++p) {
+  const int rec = tape_np[i];
+  const int np = rec & 0x7f;
+  const bool dumped = (rec & 0x80) != 0;
+  for (int p = 0; p < np; ++p) {
     const int cell = path_cell[(int64_t)p * cap + i];
     const double dist = (double)path_dist[(int64_t)p * cap + i];
     const double ah = tah[cell], ahe = tahe[cell];
-    double tot = 0.;
-    for (int nu = 0; nu < F; ++nu) {
-      const double nin = nph[nu] * dm_exp(-tauh[nu] - tauhe[nu]);
-      tot += nin / FRT_PHOT_NORM;
-    }
     for (int nu = 0; nu < F; ++nu) {
       tauh[nu] += (ah * g.sigH[nu]) * dist;
       tauhe[nu] += (ahe * g.sigHe[nu]) * dist;
-    }
-    if (tot < g.nphot_min) {
-      dumped = true;
-      np = p + 1;
-      break;
     }
   }
   for (int p = np; p < c; ++p) slot_out[path_slot[(int64_t)p * cap + i]] = make_double2(0., 0.);
@@ -253,6 +247,7 @@ extern "C" {
 
 int frt_adj_enable(frt_ctx* ctx, int32_t on) {
   if (!ctx) return FRT_E_ARG;
+  if (on && ctx->g.P > 127) return set_error(ctx, FRT_E_ARG, "the adjoint tape holds at most 127 segments per ray");
   if (on && !ctx->adj_tape) {
     const size_t nd = (size_t)(4 * ctx->cfg.itercount + 2) * (size_t)ctx->g.n3;
     if (hipMalloc((void**)&ctx->adj_tape, nd * sizeof(double)) != hipSuccess) {
@@ -264,6 +259,14 @@ int frt_adj_enable(frt_ctx* ctx, int32_t on) {
       ctx->adj_tape = nullptr;
       ctx->adj_nph = nullptr;
       return set_error(ctx, FRT_E_NOMEM, "adjoint tape (ray photons)");
+    }
+    if (hipMalloc((void**)&ctx->adj_np, (size_t)ctx->g.cap * ctx->cfg.itercount) != hipSuccess) {
+      hipFree(ctx->adj_tape);
+      hipFree(ctx->adj_nph);
+      ctx->adj_tape = nullptr;
+      ctx->adj_nph = nullptr;
+      ctx->adj_np = nullptr;
+      return set_error(ctx, FRT_E_NOMEM, "adjoint tape (dump decisions)");
     }
   }
   ctx->adj_on = on != 0;
@@ -288,11 +291,16 @@ int frt_adj_find_avg_unu(frt_ctx* ctx, const double* a_nh1, double* g_nphot, dou
   const int64_t R = ctx->adj_nrays;
   std::vector<int32_t> perm((size_t)n3);  // internal index of each row-major cell
   for (int64_t r = 0; r < n3; ++r) perm[(size_t)r] = frt_cell_from_rowmajor(ctx->g.lay, (int)r);
-  // device buffers: 11 cell arrays and the ray photons' adjoint
+  // device buffers: 11 cell arrays, the ray photons' adjoint and one
+  // {HI, HeI} opacity adjoint per segment slot (the forward's slot buffers
+  // hold float pairs in the mixed form, so the adjoint keeps its own)
   double* buf = nullptr;
   const size_t ncell = (size_t)11 * n3, nray = (size_t)std::max<int64_t>(R, 1) * F;
-  if (hipMalloc((void**)&buf, (ncell + nray) * sizeof(double)) != hipSuccess)
+  const size_t nslot = 2 * (size_t)std::max<int64_t>(ctx->nseg, 1);
+  const size_t slot_off = (ncell + nray + 1) & ~(size_t)1;  // 16-byte aligned double2 slots
+  if (hipMalloc((void**)&buf, (slot_off + nslot) * sizeof(double)) != hipSuccess)
     return set_error(ctx, FRT_E_NOMEM, "adjoint buffers");
+  double2* slots = (double2*)(buf + slot_off);
   double *aGH = buf, *aGHe = buf + n3, *ah0 = buf + 2 * n3, *ahe0 = buf + 3 * n3, *ah1 = buf + 4 * n3,
          *ahe1 = buf + 5 * n3, *anh1 = buf + 6 * n3, *aNH = buf + 7 * n3, *aNHe = buf + 8 * n3, *aC = buf + 9 * n3,
          *scratch = buf + 10 * n3, *aNph = buf + ncell;
@@ -321,10 +329,11 @@ int frt_adj_find_avg_unu(frt_ctx* ctx, const double* a_nh1, double* g_nphot, dou
                          ae_cur, aNH, aNHe, aC);
     if (R > 0)
       hipLaunchKernelGGL(k_adj_deposit, dim3(gr), dim3(ABLK), 0, ctx->stream, ctx->g, R, ctx->seg_cnt,
-                         ctx->path_cell, ctx->path_dist, ctx->path_slot, tk, ctx->adj_nph, (const double*)aGH,
-                         (const double*)aGHe, (double2*)ctx->contrib.pair, aNph);
+                         ctx->path_cell, ctx->path_dist, ctx->path_slot, tk, ctx->adj_nph,
+                         (const uint8_t*)(ctx->adj_np + (size_t)k * ctx->g.cap), (const double*)aGH,
+                         (const double*)aGHe, slots, aNph);
     hipLaunchKernelGGL(k_adj_cellsum, dim3(gc), dim3(ABLK), 0, ctx->stream, n3, ctx->cell_start, ctx->cell_end,
-                       (const double2*)ctx->contrib.pair, a_cur, ae_cur);
+                       (const double2*)slots, a_cur, ae_cur);
     e = hipGetLastError();
     std::swap(a_cur, a_next);
     std::swap(ae_cur, ae_next);

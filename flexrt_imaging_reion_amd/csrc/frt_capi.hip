@@ -238,6 +238,8 @@ float* field_ptr(frt_ctx* ctx, int field) {
     case FRT_F_GAMMA_PREV: return ctx->c.f[C_GAMMA_PREV];
     case FRT_F_GADGET_GAMMA: return ctx->c.f[C_GADGET_GAMMA];
     case FRT_F_GADGET_HEAT: return ctx->c.f[C_GADGET_HEAT];
+    case FRT_F_DDELTA_DT: return ctx->c.f[C_DDELTA_DT];
+    case FRT_F_RHO_PREV: return ctx->c.f[C_RHO_PREV];
     default: return nullptr;
   }
 }
@@ -429,6 +431,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->adj_nrays = 0;
   ctx->adj_tape = nullptr;
   ctx->adj_nph = nullptr;
+  ctx->adj_np = nullptr;
 
   int rc = FRT_OK;
   for (int b = 0; b < 2 && rc == FRT_OK; ++b) rc = alloc_rays(ctx, ctx->buf[b], cap, c.nfreq);
@@ -459,6 +462,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e == hipSuccess) e = dalloc(&ctx->rot_fwd, 9 * FRT_NTHETA);
     if (e == hipSuccess) e = dalloc(&ctx->rot_inv, 9 * FRT_NTHETA);
     if (e == hipSuccess) e = dalloc(&ctx->dcnt, FRT_CTR_COPIES);
+    if (e == hipSuccess) e = dalloc(&ctx->dsmall, 64);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hcnt, FRT_CTR_COPIES * sizeof(DevCounters), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hscalar, 64, hipHostMallocDefault);
     if (e != hipSuccess) rc = set_error(ctx, FRT_E_NOMEM, std::string("allocation: ") + hipGetErrorString(e));
@@ -508,8 +512,9 @@ int frt_destroy(frt_ctx* ctx) {
   hipFree(ctx->otf_buf);
   hipFree(ctx->adj_tape);
   hipFree(ctx->adj_nph);
+  hipFree(ctx->adj_np);
   hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
-  hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt);
+  hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt); hipFree(ctx->dsmall);
   if (ctx->hcnt) hipHostFree(ctx->hcnt);
   if (ctx->hscalar) hipHostFree(ctx->hscalar);
   hipFree(ctx->cub_tmp); hipFree(ctx->copy_src); hipFree(ctx->copy_dst);
@@ -635,20 +640,24 @@ int frt_update_cosmo_expansion(frt_ctx* ctx, float a_current, float a_prev) {
   ctx->g.dx = ctx->g.dx * ratio;
   ctx->g.sf = ctx->g.dx / ctx->g.dx0;
   ctx->g.vol = (double)ctx->g.dx * (double)ctx->g.dx * (double)ctx->g.dx;
+  ctx->g.avg_rho = (float)((double)ctx->g.avg_rho * rho_scale);  // avg_rho *= pow(a_prev/a_current, 3.) (:515)
   int rc = launch_cosmo_expansion(ctx, ratio, rho_scale, true);
   if (rc) return rc;
   return hip_check(ctx, hipGetLastError(), "update_cosmo_expansion");
 }
 
-int frt_update_hydro_step(frt_ctx* ctx, float zz, float a_rt) {
+int frt_update_hydro_step(frt_ctx* ctx, float zz, float a_rt, float avg_rho, float dt_cosmic) {
   if (!ctx) return FRT_E_ARG;
   if (!(a_rt > 0.f) || !(zz > -1.f)) return set_error(ctx, FRT_E_ARG, "need a_rt > 0 and zz > -1");
+  if (!(avg_rho > 0.f) || !std::isfinite(dt_cosmic) || dt_cosmic == 0.f)
+    return set_error(ctx, FRT_E_ARG, "need avg_rho > 0 and a finite, nonzero dt_cosmic");
   if (!ctx->have_rho) return set_error(ctx, FRT_E_ARG, "update_hydro_step needs rho: upload FRT_F_RHO first");
   ctx->cfg.zz = zz;
   ctx->cfg.a_rt = a_rt;
   set_redshift(ctx->g, ctx->cfg, zz);
   set_cell_size(ctx->g, ctx->cfg, a_rt);  // Lx = Lx0*a_rt, dx = Lx/Nx*kpc_to_cm
-  int rc = launch_cosmo_expansion(ctx, 1.f, std::pow((double)a_rt, -3.), false);
+  ctx->g.avg_rho = avg_rho;
+  int rc = launch_hydro_reload(ctx, std::pow((double)a_rt, -3.), dt_cosmic);
   if (rc) return rc;
   return hip_check(ctx, hipGetLastError(), "update_hydro_step");
 }

@@ -65,6 +65,7 @@ struct GridConst {
   double compton_coef;  // compton_rate(zz)
   double t_cmb;         // 2.726*(1 + zz)
   float zz;
+  float avg_rho;        // the driver's volume-average rho (thermal ddelta_dt term)
   float nhe_to_nh;
   double sig912;        // sigmapi_H1(13.6/h_eV)
   double sigH[FRT_MAX_FREQ];
@@ -106,7 +107,7 @@ enum CellField {
   C_TEMP, C_DNE_DT, C_CLUMP, C_NH1_PREV, C_AVG_NH1, C_AVG_NHE1, C_RECOMB_H2, C_RECOMB_HE2,
   C_GAMMA_H1_TOT, C_GAMMA_HE1_TOT, C_RECRATE_H2, C_RECRATE_HE2, C_TREION, C_TREION_FLAG,
   C_HEAT, C_COOL, C_IF_SPEED, C_INC_FLUX, C_RHO, C_FHE3, C_MFP, C_MFP912, C_GAMMA_PREV, C_GADGET_GAMMA,
-  C_GADGET_HEAT, C_NSCALAR
+  C_GADGET_HEAT, C_DDELTA_DT, C_RHO_PREV, C_NSCALAR
 };
 
 struct CellPtrs {
@@ -156,6 +157,7 @@ struct frt_ctx {
   int64_t adj_nrays;   // rays on the tape
   double* adj_tape;    // [itercount][4][n3] + [2][n3]
   float* adj_nph;      // [capacity][nfreq] photons at the start of the step
+  uint8_t* adj_np;     // [itercount][capacity] per pass: segments processed, | 0x80 if dumped
   bool have_segments;  // update_rays ran and find_avg_unu has not yet consumed the paths
   bool have_alive;     // find_avg_unu ran in this step (alive flags valid)
   int64_t nseg;
@@ -205,6 +207,7 @@ struct frt_ctx {
   DevCounters* dcnt;       // device, FRT_CTR_COPIES copies
   DevCounters* hcnt;       // pinned host: FRT_CTR_COPIES raw copies, their sum in hcnt[0]
   int64_t* hscalar;        // pinned host scratch
+  double* dsmall;          // device scratch for scalars computed on the device (64 doubles)
   void* cub_tmp;
   size_t cub_tmp_bytes;
 
@@ -253,6 +256,9 @@ int launch_gadget(frt_ctx* ctx, int mode, float* out);  // 0 accumulate, 1 avera
 #define FRT_OTF_BLOCKS 1024
 int launch_otf(frt_ctx* ctx, float avg_rho, float temp_0, double* part, double* out);
 int launch_cosmo_expansion(frt_ctx* ctx, float ratio, double rho_scale, bool rays);
+// update_hydro_step's cell loop: densities from the new rho, then ddelta_dt
+// and rho_prev (gas_funcs.cc:700-738).
+int launch_hydro_reload(frt_ctx* ctx, double rho_scale, float dt_cosmic);
 
 // helpers shared with the C-ABI file
 int ensure_cub_tmp(frt_ctx* ctx, size_t bytes);

@@ -668,7 +668,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
                                                       const int32_t* __restrict__ path_slot,
                                                       const float2* __restrict__ avg2,
                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
-                                                      DevCounters* cnt) {
+                                                      DevCounters* cnt, uint8_t* __restrict__ tape_np) {
   unsigned long long ndumped = 0, nremoved = 0;
   for_rays_by_count<DEP_CHUNK_FP64>(seg_cnt, nrays, [&](int64_t i, int c) {
     bool dumped_flag = false, removed = false;
@@ -705,6 +705,8 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
         sp.advance();
       }
       // segments behind a dump deposit nothing this pass
+      // adjoint tape: segments this pass processed, and whether it dumped
+      if (tape_np) tape_np[i] = (uint8_t)((dumped ? 0x80 : 0) | p);
       for (; p < c; ++p) {
         const int slot = path_slot[(int64_t)p * g.cap + i];
         last_cell = path_cell[(int64_t)p * g.cap + i];
@@ -793,7 +795,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
                                                        const int32_t* __restrict__ path_slot,
                                                        const float2* __restrict__ avg2,
                                                        ContribPtrs cp, uint8_t* __restrict__ alive,
-                                                       DevCounters* cnt) {
+                                                       DevCounters* cnt, uint8_t* __restrict__ tape_np) {
   unsigned long long ndumped = 0, nremoved = 0;
   // 1/(dt vol) in units of phot_norm photons
   const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
@@ -828,6 +830,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
         sp.advance();
       }
       // segments behind a dump deposit nothing this pass
+      if (tape_np) tape_np[i] = (uint8_t)((dumped ? 0x80 : 0) | p);
       for (; p < c; ++p) {
         const int slot = path_slot[(int64_t)p * g.cap + i];
         last_cell = path_cell[(int64_t)p * g.cap + i];
@@ -869,7 +872,15 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
 // :104-129, update_heat_cool :134-242, get_thermal_tstep :353-358), subgrid off.
 struct ThermalIn {
   float nH, nHe, nH1, nH2, nHe2, ne, ntot, dne, fH2, ah1, ahe1, nH1_prev;
+  float ddelta;   // ddelta_dt (0 until a hydro-step reload sets it)
+  float rho_rel;  // rho / avg_rho (float division)
 };
+
+__device__ __forceinline__ void thermal_density_terms(const CellPtrs& cf, const GridConst& g, int64_t k,
+                                                      ThermalIn& in) {
+  in.ddelta = cf.f[C_DDELTA_DT][k];
+  in.rho_rel = in.ddelta != 0.f ? cf.f[C_RHO][k] / g.avg_rho : 1.f;
+}
 
 __device__ __forceinline__ int thermal_cell(const GridConst& g, const ThermalIn& in,
                                             const float* gh1nu, const float* ghe1nu,
@@ -928,8 +939,11 @@ __device__ __forceinline__ int thermal_cell(const GridConst& g, const ThermalIn&
       cool = (float)((double)cool + frt_free_free_rate(Tm) * (double)in.ne * (double)nff);
     }
     if (g.compton) cool = (float)((double)cool - g.compton_coef * (double)in.ne * (g.t_cmb - Tm));
-    temp = (float)(((double)temp + k23 * (double)(heat - cool) * (double)dt_temp) /
-                   (1. + 2. * Hd * (double)dt_temp + 1. / (double)in.ntot * (double)in.dne * (double)dt_temp));
+    double den = 1. + 2. * Hd * (double)dt_temp + 1. / (double)in.ntot * (double)in.dne * (double)dt_temp;
+    // - 2 ddelta_dt/3/(rho/avg_rho) dt_temp: skipped while ddelta_dt is 0, where
+    // it subtracts zero (and where rho may not have been uploaded at all)
+    if (in.ddelta != 0.f) den = den - 2. * (double)in.ddelta / 3. / (double)in.rho_rel * (double)dt_temp;
+    temp = (float)(((double)temp + k23 * (double)(heat - cool) * (double)dt_temp) / den);
     t_temp += dt_temp;
     if (++nsub >= 64) break;
   }
@@ -1218,6 +1232,7 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
           in.ah1 = avg_h1;
           in.ahe1 = avg_he1;
           in.nH1_prev = cf.f[C_NH1_PREV][k];
+          thermal_density_terms(cf, g, k, in);
           float temp = cf.f[C_TEMP][k];
           float heat = cf.f[C_HEAT][k];
           float cool = cf.f[C_COOL][k];
@@ -1259,6 +1274,7 @@ __global__ void __launch_bounds__(BLK) k_thermal(CellPtrs cf, GridConst g, DevCo
       in.ah1 = cf.f[C_AVG_NH1][k];
       in.ahe1 = cf.f[C_AVG_NHE1][k];
       in.nH1_prev = cf.f[C_NH1_PREV][k];
+      thermal_density_terms(cf, g, k, in);
       float temp = cf.f[C_TEMP][k];
       float heat = cf.f[C_HEAT][k];
       float cool = cf.f[C_COOL][k];
@@ -1882,6 +1898,7 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   for (int it = 0; it < I; ++it) {
     const bool first = (it == 0), last = (it == I - 1);
     double* tk = tape ? tape + (size_t)4 * it * g.n3 : nullptr;
+    uint8_t* tnp = tape ? ctx->adj_np + (size_t)it * g.cap : nullptr;
     if (tk)
       FRT_LAUNCH(ctx, "adj_tape", gc, k_tape_pair, ctx->c.f[first ? C_NH1 : C_AVG_NH1],
                  ctx->c.f[first ? C_NHE1 : C_AVG_NHE1], tk, (int64_t)g.n3);
@@ -1894,10 +1911,12 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   do {                                                                                              \
     if ((SPEC) && F == 5)                                                                           \
       FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1, 5>), rb, n, g, ctx->path_cell, ctx->path_dist,          \
-                 ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt);   \
+                 ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt,    \
+                 tnp);                                                                              \
     else                                                                                            \
       FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1>), rb, n, g, ctx->path_cell, ctx->path_dist,             \
-                 ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt);   \
+                 ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt,    \
+                 tnp);                                                                              \
   } while (0)
     if (mixed) {
       if (last)
@@ -2410,6 +2429,56 @@ int launch_cosmo_expansion(frt_ctx* ctx, float ratio, double rho_scale, bool ray
   if (rays && ctx->nrays > 0)
     FRT_LAUNCH(ctx, "cosmo_rays", grid_for(ctx->nrays), k_cosmo_rays, cur_buf(ctx).pos, ctx->nrays, ratio,
                ctx->g.dx, ctx->g.min_dist);
+  return FRT_OK;
+}
+
+// calc_vol_avg (data_funcs.cc:75-92) of a field over the row-major cells in
+// a fixed order: thread t of one block sums rows t, t + VAVG_THREADS, ... in
+// double, then the partial sums are added pairwise (stride halving); the
+// result is (float)((double)(float)sum / n^3).  The CPU checker sums in the
+// same order, so the two agree bit for bit.
+static constexpr int VAVG_THREADS = 1024;
+
+__global__ void __launch_bounds__(VAVG_THREADS) k_vol_avg(const float* __restrict__ f, FrtCellLayout lay,
+                                                          int64_t n3, float* __restrict__ out) {
+  __shared__ double part[VAVG_THREADS];
+  double t = 0.;
+  for (int64_t r = threadIdx.x; r < n3; r += VAVG_THREADS) t += (double)f[frt_cell_from_rowmajor(lay, (int)r)];
+  part[threadIdx.x] = t;
+  __syncthreads();
+  for (int h = VAVG_THREADS / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = (float)((double)(float)part[0] / (double)n3);
+}
+
+// ddelta_dt and rho_prev of update_hydro_step (gas_funcs.cc:732-737), after
+// rho was rescaled: float arithmetic as in the reference.
+__global__ void __launch_bounds__(BLK) k_hydro_ddelta(CellPtrs cf, int64_t n3, float avg_rho,
+                                                      const float* __restrict__ avg_rho_prev, float dt_cosmic) {
+  const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (k >= n3) return;
+  const float rho = cf.f[C_RHO][k];
+  const float arp = *avg_rho_prev;
+  if (arp > 0.f) cf.f[C_DDELTA_DT][k] = (rho / avg_rho - cf.f[C_RHO_PREV][k] / arp) / dt_cosmic;
+  cf.f[C_RHO_PREV][k] = rho;
+}
+
+int launch_hydro_reload(frt_ctx* ctx, double rho_scale, float dt_cosmic) {
+  float* arp = (float*)ctx->dsmall;
+  {
+    ProfRec rec;
+    prof_begin(ctx, "vol_avg", &rec);
+    hipLaunchKernelGGL(k_vol_avg, dim3(1), dim3(VAVG_THREADS), 0, ctx->stream, ctx->c.f[C_RHO_PREV], ctx->g.lay,
+                       (int64_t)ctx->g.n3, arp);
+    prof_end(ctx, &rec);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_check(ctx, e, "vol_avg");
+  }
+  FRT_LAUNCH(ctx, "cosmo_cells", grid_for(ctx->g.n3), k_cosmo_cells, ctx->c, (int64_t)ctx->g.n3, rho_scale);
+  FRT_LAUNCH(ctx, "hydro_ddelta", grid_for(ctx->g.n3), k_hydro_ddelta, ctx->c, (int64_t)ctx->g.n3, ctx->g.avg_rho,
+             (const float*)arp, dt_cosmic);
   return FRT_OK;
 }
 

@@ -174,11 +174,14 @@ class FlexRT:
         self._check(self._lib.frt_update_cosmo_expansion(self._h, float(a_current), float(a_prev)),
                     "update_cosmo_expansion")
 
-    def update_hydro_step(self, zz, a_rt):
+    def update_hydro_step(self, zz, a_rt, avg_rho, dt_cosmic):
         """Device part of update_hydro_step (gas_funcs.cc:554-752): call after
         uploading the next snapshot's comoving density as "rho" (and its
-        sources with set_sources)."""
-        self._check(self._lib.frt_update_hydro_step(self._h, float(zz), float(a_rt)), "update_hydro_step")
+        sources with set_sources).  avg_rho is the driver's volume-average
+        density after its a_rt^-3 scaling, dt_cosmic = cosmic_time(aa_next) -
+        cosmic_time(aa); they set ddelta_dt and rho_prev (:700-738)."""
+        self._check(self._lib.frt_update_hydro_step(self._h, float(zz), float(a_rt), float(avg_rho),
+                                                    float(dt_cosmic)), "update_hydro_step")
 
     def otf_averages(self, avg_rho, temp_0=100.0):
         """write_otf's global averages (io_funcs.cc:500-632), reduced on the GPU."""

@@ -150,6 +150,8 @@ typedef enum {
   FRT_F_GAMMA_PREV,    /* avg_gamma_prev (0 on a fresh start, Gamma_HI of the gas file on a restart) */
   FRT_F_GADGET_GAMMA,  /* gadget_gamma: sum of Gamma_HI dt since the last reset (then the average) */
   FRT_F_GADGET_HEAT,   /* gadget_heat: sum of heat_rate dt / rho since the last reset (then the average) */
+  FRT_F_DDELTA_DT,     /* ddelta_dt: rate of change of rho/avg_rho between hydro steps (gas_funcs.cc:733), 1/s */
+  FRT_F_RHO_PREV,      /* rho_prev: rho after the last hydro-step reload (gas_funcs.cc:737) */
   FRT_F_COUNT
 } frt_field;
 
@@ -215,11 +217,18 @@ int frt_update_cosmo_expansion(frt_ctx* ctx, float a_current, float a_prev);
 /* The device part of update_hydro_step (gas_funcs.cc:554-752) once the driver
  * has read the next snapshot: upload its comoving density into FRT_F_RHO
  * (get_ugas io_funcs.cc:33-62) and its sources (frt_set_sources), then call
- * this with the snapshot's redshift and a_rt = scale_factor(t + tinit): rho
+ * this with the snapshot's redshift, a_rt = scale_factor(t + tinit), the
+ * driver's avg_rho after its a_rt^-3 scaling (gas_funcs.cc:697) and
+ * dt_cosmic = cosmic_time(aa_next) - cosmic_time(aa) (float, :733): rho
  * becomes proper (times a_rt^-3), the cell size Lx0 a_rt / Nx, nH, nHe, the
- * ion densities, avg_nh1, avg_nhe1, ne, n_tot and nH1_prev are recomputed,
- * and the redshift-dependent rates of update_thermal follow zz. */
-int frt_update_hydro_step(frt_ctx* ctx, float zz, float a_rt);
+ * ion densities, avg_nh1, avg_nhe1, ne, n_tot and nH1_prev are recomputed;
+ * when the volume average of rho_prev (calc_vol_avg, data_funcs.cc:75-92) is
+ * positive, ddelta_dt = (rho/avg_rho - rho_prev/avg_rho_prev)/dt_cosmic, and
+ * rho_prev = rho (gas_funcs.cc:700-738); the redshift-dependent rates of
+ * update_thermal follow zz, and its -2/3 ddelta_dt/(rho/avg_rho) dt term
+ * (gas_funcs.cc:399) uses avg_rho, which frt_update_cosmo_expansion then
+ * scales with rho (gas_funcs.cc:515). */
+int frt_update_hydro_step(frt_ctx* ctx, float zz, float a_rt, float avg_rho, float dt_cosmic);
 
 /* Global averages of write_otf (io_funcs.cc:500-657), reduced on the GPU,
  * each in the reference's form: volume average (float)(sum) / N^3 and mass

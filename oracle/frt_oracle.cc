@@ -165,8 +165,10 @@ struct State {
   std::vector<float> nH, nHe, nH1, nHe1, nH2, nHe2, ne, n_tot, f_H1, f_He1, f_H2, f_He2, temp, dne_dt,
       clump, nH1_prev, avg_nh1, avg_nhe1, recomb_H2, recomb_He2, gamma_H1_tot, gamma_He1_tot,
       recrate_H2, recrate_He2, treion, treion_flag, heat_rate, cool_rate, IF_speed, Inc_Flux,
-      gamma_H1_nu, gamma_He1_nu, ngam_nu, rho, f_He3, mfp, mfp_912, gamma_prev, gadget_gamma, gadget_heat;
+      gamma_H1_nu, gamma_He1_nu, ngam_nu, rho, f_He3, mfp, mfp_912, gamma_prev, gadget_gamma, gadget_heat,
+      ddelta_dt, rho_prev;
   float gadget_step_time = 0.f;
+  float avg_rho = 0.f;  // the driver's avg_rho (global_variables.cc:84), set by update_hydro_step
   std::vector<double> avg_gamma, avg_gamma_he1, avg_gamma_nu, avg_gamma_nu_he1, ngam_acc;
   std::vector<Ray> ray;
   std::vector<int> src_cell;
@@ -225,6 +227,8 @@ static std::vector<float>* field_vec(State& s, int f) {
     case FRT_F_GAMMA_PREV: return &s.gamma_prev;
     case FRT_F_GADGET_GAMMA: return &s.gadget_gamma;
     case FRT_F_GADGET_HEAT: return &s.gadget_heat;
+    case FRT_F_DDELTA_DT: return &s.ddelta_dt;
+    case FRT_F_RHO_PREV: return &s.rho_prev;
     default: return nullptr;
   }
 }
@@ -935,8 +939,12 @@ static void update_thermal(State& s) {
       while (t_temp < s.dt) {
         float dt_temp = mind(get_thermal_tstep(s, c), s.dt - t_temp);
         update_heat_cool(s, c);
-        s.temp[c] = (s.temp[c] + 2. / 3. / k_B / s.n_tot[c] * (s.heat_rate[c] - s.cool_rate[c]) * dt_temp) /
-                    (1. + 2. * H_of_a(s, 1. / (1. + s.zz)) * dt_temp + 1. / s.n_tot[c] * s.dne_dt[c] * dt_temp);
+        // gas_funcs.cc:398-399; the ddelta_dt term subtracts zero while
+        // ddelta_dt is 0 (no reload yet) and is skipped then, as rho may not
+        // have been given at all
+        double den = 1. + 2. * H_of_a(s, 1. / (1. + s.zz)) * dt_temp + 1. / s.n_tot[c] * s.dne_dt[c] * dt_temp;
+        if (s.ddelta_dt[c] != 0.f) den = den - 2. * s.ddelta_dt[c] / 3. / (s.rho[c] / s.avg_rho) * dt_temp;
+        s.temp[c] = (s.temp[c] + 2. / 3. / k_B / s.n_tot[c] * (s.heat_rate[c] - s.cool_rate[c]) * dt_temp) / den;
         t_temp += dt_temp;
         if (++nsub >= 64) break;
       }
@@ -1145,6 +1153,12 @@ int orc_download_field(void* h, int f, float* dst) {
 }
 
 void orc_set_time(void* h, float t) { ((State*)h)->t = t; }
+// Completed steps, as frt_set_step: the merge and casting angle streams follow.
+void orc_set_step(void* h, int64_t step) {
+  State& s = *(State*)h;
+  s.step = step;
+  s.cast_calls = (uint64_t)step + 1;
+}
 float orc_get_time(void* h) { return ((State*)h)->t; }
 
 void orc_init_rt(void* h) {
@@ -1309,7 +1323,7 @@ int orc_write_ray_file(void* h, const char* path) {
 // Number densities from rho (scaled by rho_scale first) and the ionized
 // fractions with fion = 1 (rt_funcs.cc:55): the cell loops of
 // update_cosmo_expansion (gas_funcs.cc:483-512) and update_hydro_step
-// (:688-744, without the subgrid-only ddelta_dt).
+// (:688-744; its ddelta_dt and rho_prev follow in orc_update_hydro_step).
 static void densities_from_rho(State& s, double rho_scale) {
   for (int c = 0; c < s.N3; c++) {
     s.rho[c] *= rho_scale;
@@ -1336,6 +1350,7 @@ void orc_update_cosmo_expansion(void* h, float a_current, float a_prev) {
   State& s = *(State*)h;
   s.dx *= a_current / a_prev;
   densities_from_rho(s, std::pow(a_prev / a_current, 3.));
+  s.avg_rho *= std::pow(a_prev / a_current, 3.);  // gas_funcs.cc:515
   for (auto& r : s.ray) {
     if (!r.tag) continue;
     float* x[3] = {&r.x, &r.y, &r.z};
@@ -1347,15 +1362,36 @@ void orc_update_cosmo_expansion(void* h, float a_current, float a_prev) {
   }
 }
 
+// calc_vol_avg (data_funcs.cc:75-92) over the row-major cells, the double
+// sum taken in the library's fixed order (the reference's OpenMP reduction
+// order is unspecified): 1024 strided partial sums, added pairwise.
+static float vol_avg_fixed_order(const State& s, const std::vector<float>& f) {
+  const int T = 1024;
+  std::vector<double> part(T, 0.);
+  for (int t = 0; t < T; t++)
+    for (int64_t r = t; r < s.N3; r += T) part[t] += (double)f[r];
+  for (int hh = T / 2; hh > 0; hh >>= 1)
+    for (int t = 0; t < hh; t++) part[t] += part[t + hh];
+  return (float)part[0] / std::pow((double)s.N, 3.);
+}
+
 // update_hydro_step (gas_funcs.cc:554-752) after a new comoving density was
 // uploaded into rho: redshift zz, Lx = Lx0*a_rt, rho *= pow(a_rt, -3.), the
-// densities recomputed.
-void orc_update_hydro_step(void* h, float zz, float a_rt) {
+// densities recomputed; avg_rho is the driver's (already scaled, :697),
+// dt_cosmic = cosmic_time(aa_next) - cosmic_time(aa); ddelta_dt where the
+// average of rho_prev is positive, then rho_prev = rho (:700-738).
+void orc_update_hydro_step(void* h, float zz, float a_rt, float avg_rho, float dt_cosmic) {
   State& s = *(State*)h;
   s.zz = zz;
   const float Lx = (float)s.cfg.box_ckpc * a_rt;
   s.dx = (float)(Lx / s.N * kpc_to_cm);
   densities_from_rho(s, std::pow(a_rt, -3.));
+  s.avg_rho = avg_rho;
+  const float avg_rho_prev = vol_avg_fixed_order(s, s.rho_prev);
+  for (int c = 0; c < s.N3; c++) {
+    if (avg_rho_prev > 0.) s.ddelta_dt[c] = (s.rho[c] / s.avg_rho - s.rho_prev[c] / avg_rho_prev) / dt_cosmic;
+    s.rho_prev[c] = s.rho[c];
+  }
 }
 
 // write_otf's averages (io_funcs.cc:500-632) with fion = 1, serial over the

@@ -66,7 +66,8 @@ _PROTOS = {
     "orc_reset_gadget_var": (None, [C.c_void_p]),
     "orc_get_gadget_step_time": (C.c_float, [C.c_void_p]),
     "orc_otf_averages": (None, [C.c_void_p, C.c_float, C.c_float, C.POINTER(FrtOtf)]),
-    "orc_update_hydro_step": (None, [C.c_void_p, C.c_float, C.c_float]),
+    "orc_set_step": (None, [C.c_void_p, C.c_int64]),
+    "orc_update_hydro_step": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]),
     "orc_get_cell_size": (C.c_float, [C.c_void_p]),
     "orc_itheta_hash": (C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint64]),
     "orc_derived": (None, [_VP, C.POINTER(C.c_double)]),
@@ -149,6 +150,9 @@ class Oracle:
     def get_time(self):
         return self._lib.orc_get_time(self._h)
 
+    def set_step(self, step):
+        self._lib.orc_set_step(self._h, int(step))
+
     def init_rt(self):
         self._lib.orc_init_rt(self._h)
 
@@ -207,8 +211,8 @@ class Oracle:
     def update_cosmo_expansion(self, a_current, a_prev):
         self._lib.orc_update_cosmo_expansion(self._h, float(a_current), float(a_prev))
 
-    def update_hydro_step(self, zz, a_rt):
-        self._lib.orc_update_hydro_step(self._h, float(zz), float(a_rt))
+    def update_hydro_step(self, zz, a_rt, avg_rho, dt_cosmic):
+        self._lib.orc_update_hydro_step(self._h, float(zz), float(a_rt), float(avg_rho), float(dt_cosmic))
 
     def otf_averages(self, avg_rho, temp_0=100.0):
         o = FrtOtf()

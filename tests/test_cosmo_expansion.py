@@ -121,7 +121,8 @@ def test_hydro_step_reload():
         o.upload_field("rho", rho_c)
         fr = {k: o.download_field(k).astype(np.float64) for k in ("f_H2", "f_He2")}
         zz, a = F32(6.8), F32(1 / 7.8)
-        o.update_hydro_step(zz, a)
+        avg_rho = F32((rho_c.astype(np.float64) * float(a) ** -3).mean())
+        o.update_hydro_step(zz, a, avg_rho, F32(3.0e13))
         kpc = 3.086e21
         dx_want = float(F32(wl["cfg"]["box_ckpc"]) * a) / 16 * kpc
         assert abs(o.cell_size() / dx_want - 1) < 2e-7
@@ -133,6 +134,51 @@ def test_hydro_step_reload():
         nHe = o.download_field("nHe").astype(np.float64)
         assert np.max(np.abs(o.download_field("nHe2") / nHe / fr["f_He2"] - 1)) < 2e-7
         o.step()  # the new state and redshift step on
+        assert np.all(np.isfinite(o.download_field("temp")))
+    finally:
+        o.close()
+
+
+def test_hydro_step_ddelta_dt():
+    """ddelta_dt and rho_prev of update_hydro_step (gas_funcs.cc:700-738): at
+    the first reload rho_prev is still 0, so its average is not positive and
+    ddelta_dt stays 0; rho_prev becomes the proper rho.  At the next reload
+    ddelta_dt = (rho/avg_rho - rho_prev/avg_rho_prev)/dt_cosmic with
+    avg_rho_prev the volume average of rho_prev: positive where the density
+    contrast grew, and zero for a density that only scaled uniformly."""
+    wl, o = stepped()
+    try:
+        n3 = 16 ** 3
+        rng = np.random.default_rng(23)
+        a1, a2 = F32(1 / 7.8), F32(1 / 7.6)
+        dtc = F32(2.5e13)
+        rho1_c = (o.download_field("rho").astype(np.float64) * (1 / 8.0) ** 3).astype(F32)
+        avg1 = F32((rho1_c.astype(np.float64) * float(a1) ** -3).mean())
+        o.upload_field("rho", rho1_c)
+        o.update_hydro_step(F32(6.8), a1, avg1, dtc)
+        assert np.all(o.download_field("ddelta_dt") == 0)
+        rho1 = o.download_field("rho")
+        assert np.array_equal(o.download_field("rho_prev"), rho1)
+        o.step()
+        grow = rng.uniform(0.9, 1.2, n3)
+        rho2_c = (rho1_c.astype(np.float64) * grow).astype(F32)
+        avg2 = F32((rho2_c.astype(np.float64) * float(a2) ** -3).mean())
+        o.upload_field("rho", rho2_c)
+        o.update_hydro_step(F32(6.6), a2, avg2, dtc)
+        rho2 = o.download_field("rho").astype(np.float64)
+        dd = o.download_field("ddelta_dt").astype(np.float64)
+        avgp = rho1.astype(np.float64).mean()
+        want = (rho2 / float(avg2) - rho1.astype(np.float64) / avgp) / float(dtc)
+        assert np.allclose(dd, want, rtol=1e-4, atol=1e-6 / float(dtc))
+        contrast = rho2 / float(avg2) - rho1 / avgp
+        assert np.all(np.sign(dd[np.abs(contrast) > 1e-4]) == np.sign(contrast[np.abs(contrast) > 1e-4]))
+        assert np.array_equal(o.download_field("rho_prev"), o.download_field("rho"))
+        # a uniform rescaling leaves every density contrast, so ddelta_dt ~ 0
+        rho3_c = (rho2_c.astype(np.float64) * 1.0).astype(F32)
+        o.upload_field("rho", rho3_c)
+        o.update_hydro_step(F32(6.6), a2, avg2, dtc)
+        assert np.max(np.abs(o.download_field("ddelta_dt").astype(np.float64))) < 1e-6 / float(dtc)
+        o.step()
         assert np.all(np.isfinite(o.download_field("temp")))
     finally:
         o.close()

@@ -180,3 +180,49 @@ def test_adjoint_at_the_differentiation_config():
         assert abs(gl[j] - fd) <= 1e-2 * abs(fd)
     finally:
         rt.close()
+
+
+def test_mixed_adjoint_after_segment_growth():
+    """Mixed form: the adjoint's per-slot buffer is its own (16-byte opacity
+    pairs), sized by the step's segment count.  A context whose ray capacity
+    is just above the live rays reserves 2 segments per ray, so the 32^3
+    single-source case (about 4 crossings per moved ray) has more segments
+    than half the segment buffer and grows it; its gradient must equal, bit
+    for bit, that of a context with the default (large) capacity, where the
+    capacity changes only array strides."""
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    _print_case("mixed adjoint uniform32 3 steps, small vs default capacity")
+    wl = inputs.make_workload("uniform32", steps=3)
+    cfg = dict(wl["cfg"], precision=1)
+    big = FlexRT(cfg)
+    try:
+        inputs.setup(big, wl)
+        peak = big.ray_count()
+        for _ in range(3):
+            st = big.step()
+            peak = max(peak, st["rays_start"], st["rays_end"])
+        small_cap = int(peak * 1.1) + 12 * len(wl["src_cells"]) * 4 + 64
+        small = FlexRT(dict(cfg, ray_capacity=small_cap))
+        try:
+            inputs.setup(small, wl)
+            for _ in range(3):
+                small.step()
+            out = []
+            for rt in (big, small):
+                rt.adj_enable(True)
+                rt.update_rays()
+                rt.find_avg_unu()
+                out.append(rt)
+            nseg = small.stats()["segments"]
+            print("segments", nseg, "small capacity", small_cap, flush=True)
+            assert nseg > small_cap, "the case must overflow half of the reserved segment buffer"
+            a = np.random.default_rng(2).standard_normal(32 ** 3)
+            ga, gb = big.adj_find_avg_unu(a), small.adj_find_avg_unu(a)
+            for k in ga:
+                assert np.all(np.isfinite(ga[k])), k
+                assert np.array_equal(ga[k].view(np.uint64), gb[k].view(np.uint64)), k
+        finally:
+            small.close()
+    finally:
+        big.close()

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from flexrt_imaging_reion_amd import inputs
+from flexrt_imaging_reion_amd.rt import FlexRT
 from parity_util import STATE_FIELDS, bits_equal, compare_rays, counts, fields, make_pair, max_rel
 
 pytestmark = pytest.mark.gpu
@@ -88,26 +89,36 @@ def test_expansion_needs_rho():
 
 def test_fp64_hydro_step_bit_exact():
     """Three steps, a new comoving density snapshot and redshift
-    (update_hydro_step), three more steps: bit for bit like the checker."""
-    _print_case("fp64 hydro step lognormal n=16 src=16 3+3 steps")
-    wl = inputs.make_workload("lognormal64", n=16, n_src=16, steps=6)
+    (update_hydro_step: rho_prev set, ddelta_dt still 0), two steps, a second
+    snapshot with a changed density (ddelta_dt now nonzero and entering the
+    temperature update), three more steps: bit for bit like the checker."""
+    _print_case("fp64 hydro step lognormal n=16 src=16 3+2+3 steps, two reloads")
+    wl = inputs.make_workload("lognormal64", n=16, n_src=16, steps=8)
     g, o = make_pair(wl, precision=0)
-    a_new = F32(1 / 7.6)
     rng = np.random.default_rng(5)
+    a = float(wl["cfg"]["a_rt"])
     try:
         for s in range(3):
             sg, so = g.step(), o.step()
             assert counts(sg) == counts(so), "step %d" % s
-        rho_c = (g.download_field("rho").astype(np.float64) * float(wl["cfg"]["a_rt"]) ** 3
-                 * rng.uniform(0.9, 1.1, 16 ** 3)).astype(F32)
-        for rt in (g, o):
-            rt.upload_field("rho", rho_c)
-            rt.update_hydro_step(F32(6.6), a_new)
-            rt.get_nphot_min_max()
-        assert F32(g.cell_size()) == F32(o.cell_size())
-        for s in range(3, 6):
-            sg, so = g.step(), o.step()
-            assert counts(sg) == counts(so), "step %d" % s
+        step = 3
+        for zz, a_new, nsteps in ((6.6, 1 / 7.6, 2), (6.5, 1 / 7.5, 3)):
+            rho_c = (g.download_field("rho").astype(np.float64) * a ** 3
+                     * rng.uniform(0.9, 1.1, 16 ** 3)).astype(F32)
+            avg = F32((rho_c.astype(np.float64) * float(F32(a_new)) ** -3).mean())
+            for rt in (g, o):
+                rt.upload_field("rho", rho_c)
+                rt.update_hydro_step(F32(zz), F32(a_new), avg, F32(2.9e14))
+                rt.get_nphot_min_max()
+            a = float(F32(a_new))
+            assert F32(g.cell_size()) == F32(o.cell_size())
+            for k in ("ddelta_dt", "rho_prev", "rho"):
+                assert bits_equal(g.download_field(k), o.download_field(k)), k
+            for _ in range(nsteps):
+                sg, so = g.step(), o.step()
+                assert counts(sg) == counts(so), "step %d" % step
+                step += 1
+        assert np.count_nonzero(g.download_field("ddelta_dt")) > 0.9 * 16 ** 3
         err = compare_rays(g.rays(), o.rays())
         assert err is None, err
         fg, fo = fields(g), fields(o)
@@ -189,3 +200,44 @@ def test_gadget_coupling_bit_exact(tmp_path):
     finally:
         g.close()
         o.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_dropin_order_matches_fused_step(precision):
+    """The drop-in binding's order (main.cc:117-198 with the library's phases
+    and the host's own update_thermal between them, flexrt_imaging_reion_amd
+    .dropin): the host holds only what it pulls after find_avg_unu, runs the
+    checker's update_thermal (the restatement of gas_funcs.cc:362-457) and
+    pushes temp and the rates back.  Over 6 steps it matches a context that
+    runs the fused frt_step, bit for bit, in counts, rays and every state
+    field: the pulled list covers everything the host's thermal update reads."""
+    from flexrt_imaging_reion_amd import dropin
+    from oracle.pyoracle import Oracle
+
+    _print_case("drop-in order precision=%d lognormal n=16 src=16 steps=6" % precision)
+    wl = inputs.make_workload("lognormal64", n=16, n_src=16, steps=6)
+    cfg = dict(wl["cfg"], precision=precision)
+    fused = FlexRT(cfg)
+    lib = FlexRT(cfg)
+    host = Oracle(cfg)
+    try:
+        for rt in (fused, lib, host):
+            inputs.setup(rt, wl)
+        dt = host.derived()["dt"]
+        for s in range(6):
+            sf = fused.step()
+            dropin.step_dropin(lib, host, s, dt)
+            sl = lib.stats()
+            assert sl["rays_end"] == sf["rays_end"], "step %d" % s
+        assert lib.get_time() == fused.get_time()
+        err = compare_rays(lib.rays(), fused.rays())
+        assert err is None, err
+        fl, ff = fields(lib), fields(fused)
+        bad = [k for k in STATE_FIELDS if not bits_equal(fl[k], ff[k])]
+        assert not bad, {k: max_rel(fl[k], ff[k]) for k in bad}
+        for k in ("temp", "heat_rate", "cool_rate", "IF_speed", "Inc_Flux"):
+            assert bits_equal(host.download_field(k), fused.download_field(k)), k
+    finally:
+        fused.close()
+        lib.close()
+        host.close()

@@ -212,7 +212,7 @@ def test_file_errors(tmp_path):
             g.read_gas_file(tmp_path / "gas")
         assert ei.value.code == FRT_E_IO
         with pytest.raises(FlexRTError) as ei:
-            g.update_hydro_step(7.0, -1.0)
+            g.update_hydro_step(7.0, -1.0, 1e-27, 1e14)
         assert ei.value.code == 1
         with pytest.raises(FlexRTError) as ei:
             g.update_cosmo_expansion(0.0, 0.125)

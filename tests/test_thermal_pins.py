@@ -13,10 +13,23 @@ closed forms computed independently in numpy:
   * the ionization-front speed v_IF = dx (f_ion,new - f_ion,old) / dt and the
     incident flux (1 + n_He/n_H) n_H v_IF.
 
-The post-front temperature fit (D'Aloisio et al.) itself is a table of
-coefficients: its values are parity unpinned beyond the mixing rule checked
-here (a cell that does not ionize further keeps its temperature).
+  * the density-change term of a hydro-step reload: one sub-step gives
+    T' = T / (1 + 2 H dt - (2/3) ddelta_dt / (rho/avg_rho) dt), so a cell
+    whose density contrast grows heats (gas_funcs.cc:399);
+  * sub-cycling: when 0.1 T/|dT/dt| falls below 0.1 dt the step is cut into
+    ten sub-steps of 0.1 dt (get_thermal_tstep, gas_funcs.cc:353-358), each
+    the semi-implicit factor, so T' = T (1 + a dt/10)^-10 for
+    dT/dt = -a T -- not T/(1 + a dt) -- and within the scheme's error of the
+    exact T e^{-a dt};
+  * the post-front temperature: the D'Aloisio et al. fit
+    ln T_reion = sum_t Cn[t] ln(v_IF/1e5)^t with the coefficients taken from
+    the reference's source (tests/golden/treion_fit_coefficients.json, made
+    by tests/golden/make_treion_fixture.py) and the mixing rule
+    (f_old T + (f_new - f_old) T_reion)/f_new below f_HII = 0.98.
 """
+import json
+import os
+
 import math
 
 import numpy as np
@@ -141,3 +154,102 @@ def test_front_speed_and_flux():
     assert np.all(T[same] <= T0[same])
     assert np.all(T[~same] > T0[~same] * 0.5)
     o.close()
+
+
+def _reload_same_state(o, cfg):
+    """update_hydro_step on the oracle's own density (comoving = proper a^3),
+    which sets avg_rho and leaves every density as it was."""
+    a = F32(cfg["a_rt"])
+    rho = o.download_field("rho").astype(np.float64)
+    avg = F32(rho.mean())
+    o.upload_field("rho", (rho * float(a) ** 3).astype(F32))
+    o.update_hydro_step(F32(cfg["zz"]), a, avg, F32(3.0e13))
+    return avg
+
+
+def test_density_change_term():
+    o, wl, cfg = thermal_oracle(recomb_cool=0, free_free=0, compton=0)
+    n3 = 4 ** 3
+    try:
+        avg = _reload_same_state(o, cfg)
+        rho = o.download_field("rho")
+        nH = o.download_field("nH")
+        dt = o.derived()["dt"]
+        rng = np.random.default_rng(21)
+        u = rng.uniform(-0.3, 0.3, n3)
+        dd = (u / dt).astype(F32)
+        T0 = np.full(n3, 5000.0, dtype=F32)
+        set_fields(o, n3, temp=T0, nH1=nH, nH1_prev=nH, f_H2=0.0, dne_dt=0.0, ddelta_dt=dd)
+        o.update_thermal()
+        T = o.download_field("temp").astype(np.float64)
+        H = hubble(cfg)
+        assert 0.05 / H > dt and o.stats()["thermal_substeps"] == n3
+        q = rho.astype(np.float64) / float(avg)
+        want = 5000.0 / (1.0 + 2.0 * H * dt - 2.0 / 3.0 * dd.astype(np.float64) / q * dt)
+        assert np.allclose(T, want, rtol=3e-6, atol=0)
+        grow = dd > 0
+        assert np.all(T[grow] > 5000.0 / (1 + 2 * H * dt)) and np.all(T[~grow] < 5000.0 / (1 + 2 * H * dt))
+    finally:
+        o.close()
+
+
+def test_subcycled_update():
+    o, wl, cfg = thermal_oracle(recomb_cool=0, free_free=0, compton=0)
+    n3 = 4 ** 3
+    try:
+        nH = o.download_field("nH")
+        ntot = o.download_field("n_tot").astype(np.float64)
+        dt = o.derived()["dt"]
+        rng = np.random.default_rng(3)
+        k = rng.uniform(1.5, 4.0, n3)                     # dne/n_tot = k/dt: a dt ~ k
+        dne = (k * ntot / dt).astype(F32)
+        T0 = rng.uniform(5e3, 2e4, n3).astype(F32)
+        set_fields(o, n3, temp=T0, nH1=nH, nH1_prev=nH, f_H2=0.0, dne_dt=dne)
+        o.update_thermal()
+        T = o.download_field("temp").astype(np.float64)
+        a = 2.0 * hubble(cfg) + dne.astype(np.float64) / ntot.astype(F32)
+        assert np.all(0.1 / a < 0.1 * dt)                 # the step is clamped to 0.1 dt
+        want = T0.astype(np.float64) * (1.0 + a * dt / 10.0) ** -10
+        assert np.allclose(T, np.maximum(want, 100.0), rtol=3e-5, atol=0)
+        subs = o.stats()["thermal_substeps"]
+        assert 10 * n3 <= subs <= 11 * n3                 # a last sliver of float rounding at most
+        # first-order scheme: ln(T/T_exact) = a dt - 10 ln(1 + a dt/10) <= (a dt)^2/20
+        exact = T0.astype(np.float64) * np.exp(-a * dt)
+        above = exact > 100.0
+        lr = np.log(T[above] / exact[above])
+        assert np.all(lr > 0) and np.all(lr <= (a[above] * dt) ** 2 / 20 + 1e-6)
+    finally:
+        o.close()
+
+
+def test_post_front_temperature_fit():
+    with open(os.path.join(os.path.dirname(__file__), "golden", "treion_fit_coefficients.json")) as f:
+        Cn = json.load(f)["Cn"]
+    o, wl, cfg = thermal_oracle(recomb_cool=0, free_free=0, compton=0)
+    n3 = 4 ** 3
+    try:
+        nH = o.download_field("nH").astype(np.float64)
+        d = o.derived()
+        rng = np.random.default_rng(17)
+        f_old = rng.uniform(0.0, 0.4, n3)
+        f_new = f_old + rng.uniform(0.05, 0.5, n3)        # all below 0.98, all ionizing further
+        T0 = rng.uniform(100.0, 3000.0, n3).astype(F32)
+        nH1 = (nH * (1 - f_new)).astype(F32)
+        nH1p = (nH * (1 - f_old)).astype(F32)
+        set_fields(o, n3, nH1=nH1, nH1_prev=nH1p, temp=T0, f_H2=0.5, dne_dt=0.0)
+        o.update_thermal()
+        nHf = nH.astype(F32)
+        fn = ((nHf - nH1) / nHf).astype(np.float64)
+        fo = ((nHf - nH1p) / nHf).astype(np.float64)
+        v = d["dx"] * (fn - fo) / d["dt"]
+        L = np.log(v / 1e5)
+        T_re = np.exp(sum(c * L ** t for t, c in enumerate(Cn)))
+        mixed = (fo * T0 + (fn - fo) * T_re) / fn
+        H = hubble(cfg)
+        want = mixed / (1.0 + 2.0 * H * d["dt"])
+        T = o.download_field("temp").astype(np.float64)
+        assert np.all(fn < 0.98)
+        assert np.allclose(T, want, rtol=2e-5, atol=0)
+        assert np.all(T_re > 1e4) and np.all(T_re < 1e5)  # D'Aloisio+: 2-4e4 K for fronts of 1e8-1e9 cm/s
+    finally:
+        o.close()
