@@ -30,6 +30,7 @@ _PROTOS = {
     "frt_update_cosmo_expansion": (C.c_int, [_CTX, C.c_float, C.c_float]),
     "frt_update_hydro_step": (C.c_int, [_CTX, C.c_float, C.c_float, C.c_float, C.c_float]),
     "frt_otf_averages": (C.c_int, [_CTX, C.c_float, C.c_float, C.POINTER(FrtOtf)]),
+    "frt_otf_averages_noalloc": (C.c_int, [_CTX, C.c_float, C.c_float, C.POINTER(FrtOtf)]),
     "frt_write_otf": (C.c_int, [_CTX, C.c_char_p, C.c_float, C.c_float]),
     "frt_init_rt": (C.c_int, [_CTX]),
     "frt_get_nphot_min_max": (C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

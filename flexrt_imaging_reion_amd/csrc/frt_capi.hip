@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "frt_internal.h"
 #include "frt_order.h"
@@ -93,7 +94,36 @@ void prof_resolve(frt_ctx* ctx) {
 
 using namespace frt;
 
+// Device scratch of frt_otf_averages_noalloc: the per-block partial sums of
+// the write_otf reductions and their combination, FRT_OTF_SCRATCH doubles
+// (about 400 KiB), held by every device the library runs on until the process
+// ends.  Host code never uses the name `scratch` as an address: the device
+// address of each GPU's copy comes from hipGetSymbolAddress at frt_create and
+// is kept per device.
+#define FRT_OTF_SCRATCH ((size_t)FRT_OTF_BLOCKS * (FRT_OTF_NV + 1) + 64)
+static __device__ __align__(256) double scratch[FRT_OTF_SCRATCH];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_address(frt_ctx* ctx, void** out) {
+  *out = nullptr;
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return set_error(ctx, FRT_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[ctx->device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(ctx, FRT_E_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = ptr;
+  }
+  *out = g_scratch_ptr[ctx->device];
+  return FRT_OK;
+}
 
 template <typename T>
 hipError_t dalloc(T** p, size_t count) {
@@ -284,6 +314,8 @@ T get_le(const uint8_t* p) {
 
 }  // namespace
 
+static void otf_fill(const frt_ctx* ctx, const double* S, frt_otf* out);
+
 extern "C" {
 
 const char* frt_version(void) { return "flexrt-mi355x 0.1 (gfx950)"; }
@@ -425,6 +457,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->have_alive = false;
   ctx->have_rho = false;
   ctx->otf_buf = nullptr;
+  ctx->otf_scratch = nullptr;
   ctx->gadget_step_time = 0.f;
   ctx->adj_on = ctx->adj_valid = false;
   ctx->adj_iters = 0;
@@ -464,7 +497,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e == hipSuccess) e = dalloc(&ctx->dcnt, FRT_CTR_COPIES);
     if (e == hipSuccess) e = dalloc(&ctx->dsmall, 64);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hcnt, FRT_CTR_COPIES * sizeof(DevCounters), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hscalar, 64, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hscalar, 512, hipHostMallocDefault);
     if (e != hipSuccess) rc = set_error(ctx, FRT_E_NOMEM, std::string("allocation: ") + hipGetErrorString(e));
   }
   if (rc == FRT_OK) {
@@ -487,6 +520,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e != hipSuccess) rc = set_error(ctx, FRT_E_HIP, std::string("init: ") + hipGetErrorString(e));
   }
   if (rc == FRT_OK) rc = reserve_workspace(ctx);
+  if (rc == FRT_OK) rc = scratch_address(ctx, &ctx->otf_scratch);
   if (rc != FRT_OK) {
     std::string msg = ctx->err;
     frt_destroy(ctx);
@@ -677,6 +711,44 @@ int frt_otf_averages(frt_ctx* ctx, float avg_rho, float temp_0, frt_otf* out) {
   double S[FRT_OTF_NV + 1];
   rc = hip_check(ctx, stream_copy(ctx, S, dout, sizeof(S), hipMemcpyDeviceToHost), "otf");
   if (rc) return rc;
+  otf_fill(ctx, S, out);
+  return FRT_OK;
+}
+
+int frt_otf_averages_noalloc(frt_ctx* ctx, float avg_rho, float temp_0, frt_otf* out) {
+  if (!ctx || !out) return FRT_E_ARG;
+  // the ray list must hold live rays only: compacting it here would use the
+  // ray buffers' scan workspace, so removals pending since the last
+  // find_avg_unu are refused instead
+  if (ctx->have_alive)
+    return set_error(ctx, FRT_E_STATE,
+                     "rays were removed since the last compaction: call frt_step or frt_split_rays first");
+  if (!ctx->otf_scratch) return set_error(ctx, FRT_E_STATE, "no device scratch on this context");
+  const int64_t nb = std::min<int64_t>((ctx->g.n3 + 255) / 256, FRT_OTF_BLOCKS);
+  const int64_t nbr = std::max<int64_t>(1, std::min<int64_t>((ctx->nrays + 255) / 256, FRT_OTF_BLOCKS));
+  if ((size_t)(nb * FRT_OTF_NV + nbr) > (size_t)FRT_OTF_BLOCKS * (FRT_OTF_NV + 1))
+    return set_error(ctx, FRT_E_CAPACITY, "write_otf partial sums exceed the device scratch");
+  // every partial the final combination reads is written by this call, so
+  // nothing needs clearing; calls that overlap on one GPU share the array and race
+  double* part = (double*)ctx->otf_scratch;
+  double* dout = part + (size_t)FRT_OTF_BLOCKS * (FRT_OTF_NV + 1);
+  int rc = launch_otf(ctx, avg_rho, temp_0, part, dout);
+  if (rc) return rc;
+  double* S = (double*)ctx->hscalar;  // pinned, 512 bytes
+  static_assert((FRT_OTF_NV + 1) * sizeof(double) <= 512, "pinned readback buffer");
+  rc = hip_check(ctx, hipMemcpyAsync(S, dout, (FRT_OTF_NV + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                                     ctx->stream), "otf");
+  if (rc) return rc;
+  rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "otf");
+  if (rc) return rc;
+  otf_fill(ctx, S, out);
+  return FRT_OK;
+}
+
+}  // extern "C"
+
+// calc_vol_avg / calc_mass_avg forms of the reduced sums (io_funcs.cc:500-632).
+static void otf_fill(const frt_ctx* ctx, const double* S, frt_otf* out) {
   // calc_vol_avg: (float)tot / pow(N, 3.); calc_mass_avg: (float)tot / m_tot
   const double N3 = (double)ctx->g.n3;
   auto vol = [&](int i) { return (float)((double)(float)S[i] / N3); };
@@ -712,8 +784,9 @@ int frt_otf_averages(frt_ctx* ctx, float avg_rho, float temp_0, frt_otf* out) {
   o.nsrc = ctx->nsrc;
   o.fion[0] = (float)((double)(float)N3 / N3);
   o.fion[1] = (float)((double)(float)S[7] / S[7]);
-  return FRT_OK;
 }
+
+extern "C" {
 
 int frt_write_otf(frt_ctx* ctx, const char* path, float avg_rho, float temp_0) {
   if (!ctx || !path) return FRT_E_ARG;

@@ -149,6 +149,7 @@ struct frt_ctx {
   uint32_t* cand_key_tmp;  // [cap]
   bool have_rho;       // rho was uploaded or read from a gas file
   double* otf_buf;     // write_otf partial sums (allocated at first use)
+  void* otf_scratch;   // device address of this device's static scratch (frt_otf_averages_noalloc)
   float gadget_step_time;  // MP-Gadget coupling: dt summed since the last reset
   // adjoint of find_avg_unu (FP64 form): tape of the last recorded call
   bool adj_on;         // record a tape in find_avg_unu

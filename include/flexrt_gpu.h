@@ -261,6 +261,17 @@ int frt_otf_averages(frt_ctx* ctx, float avg_rho, float temp_0, frt_otf* out);
 /* write_otf: appends the step's block of text to path, in the reference's
  * format (step, redshift, time and time step in Myr, then the averages). */
 int frt_write_otf(frt_ctx* ctx, const char* path, float avg_rho, float temp_0);
+/* No counterpart in the reference: frt_otf_averages for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels and results; the partial sums live in a
+ * static device array of the library (about 400 KiB per GPU, held until the
+ * process ends) instead of a buffer allocated at the first call, and the
+ * result is read back through pinned memory the context already holds.
+ * Calls on one GPU share that array: calls that overlap on one GPU race.
+ * Returns FRT_E_STATE when rays were removed since the last compaction (call
+ * frt_step or frt_split_rays first) and FRT_E_CAPACITY if the partial sums
+ * would not fit (not reachable with the fixed block counts). */
+int frt_otf_averages_noalloc(frt_ctx* ctx, float avg_rho, float temp_0, frt_otf* out);
 
 /* ---- reference routines on the per-step path ----------------------------- */
 /* init_rt (rt_funcs.cc:36-126), fresh start: empty ray buffer, ray_dt = dt
