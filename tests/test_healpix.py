@@ -103,3 +103,73 @@ def test_children_inside_parent():
             for c in range(4):
                 v = P.pix2vec(order + 1, 4 * p + c)
                 assert P.vec2pix(order, v) == p
+
+
+# Gorski et al. 2005: face f of the 12 base pixels sits on ring jrll[f] N and
+# starts at in-ring position jpll[f] (in units of half a pixel); inside a
+# face the NESTED index interleaves x (even bits) and y (odd bits), with x
+# toward the north-east and y toward the north-west.
+JRLL = [2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4]
+JPLL = [1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7]
+
+
+def _deinterleave(v):
+    x = y = 0
+    b = 0
+    while v:
+        x |= (v & 1) << b
+        y |= ((v >> 1) & 1) << b
+        v >>= 2
+        b += 1
+    return x, y
+
+
+def _ring_position(order, p):
+    """(ring index i, in-ring index j from 1) of NESTED pixel p."""
+    N = 1 << order
+    f, sub = p >> (2 * order), p & (N * N - 1)
+    ix, iy = _deinterleave(sub)
+    i = JRLL[f] * N - ix - iy - 1
+    nr = i if i < N else (4 * N - i if i > 3 * N else N)
+    shift = ((i - N) & 1) if N <= i <= 3 * N else 0
+    j = (JPLL[f] * nr + ix - iy + 1 + shift) // 2
+    j = j - 4 * N if j > 4 * N else (j + 4 * N if j < 1 else j)
+    return i, j
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_longitude_of_every_nested_pixel(order):
+    """phi of each NESTED pixel centre from its ring and in-ring index:
+    polar ring i holds 4i pixels at phi = (pi/2i)(j - 1/2); equatorial ring i
+    holds 4N at phi = (pi/2N)(j - (1 + (i - N) mod 2)/2), every other ring
+    shifted by half a pixel (Gorski et al. 2005, eqs. 8-9, with j counted
+    from 1 as in the HEALPix library).  A swap of the x and y bits inside a
+    face, or a shifted ring, moves phi."""
+    N = 1 << order
+    seen = set()
+    for p in range(12 * N * N):
+        i, j = _ring_position(order, p)
+        seen.add((i, j))
+        if i < N:
+            phi = math.pi / (2 * i) * (j - 0.5)
+        elif i > 3 * N:
+            phi = math.pi / (2 * (4 * N - i)) * (j - 0.5)
+        else:
+            phi = math.pi / (2 * N) * (j - (1 + (i - N) % 2) / 2)
+        v = P.pix2vec(order, p)
+        got = math.atan2(v[1], v[0]) % (2 * math.pi)
+        d = abs(got - phi % (2 * math.pi))
+        assert min(d, 2 * math.pi - d) < 1e-12, (p, i, j, got, phi)
+    assert len(seen) == 12 * N * N  # the ring positions are a bijection
+
+
+def test_children_orientation():
+    """Inside each base pixel the four order-1 children run south (0), east
+    (1, the x direction), west (2, y) and north (3)."""
+    for f in range(12):
+        z = [P.pix2vec(1, 4 * f + c)[2] for c in range(4)]
+        phi = [math.atan2(P.pix2vec(1, 4 * f + c)[1], P.pix2vec(1, 4 * f + c)[0]) for c in range(4)]
+        assert z[0] < z[1] and z[0] < z[2] and z[3] > z[1] and z[3] > z[2]
+        assert abs(z[1] - z[2]) < 1e-12
+        dphi = (phi[1] - phi[2] + math.pi) % (2 * math.pi) - math.pi
+        assert dphi > 0, f

@@ -220,3 +220,67 @@ def test_split_children():
     for l in range(13):
         rays_per_cell = 12 * 4 ** l * d["dx0"] ** 2 / (4 * math.pi * d["rmax"][l] ** 2)
         assert rays_per_cell == pytest.approx(2.0, rel=1e-5)
+
+
+def test_split_child_positions():
+    """split_ray's child positions (ray_tracing.cc:539-572): each child starts
+    at the parent's position moved by (v_child - v_parent) r dx/dx0, walked
+    cell by cell into the periodic cell that holds it.  Parents at orders 0
+    to 3 with r beyond r_max(l), some next to the box corner so that the walk
+    wraps; expected cells and positions from float64 arithmetic."""
+    wl = inputs.make_workload("uniform32")
+    o = Oracle(wl["cfg"])
+    try:
+        inputs.setup(o, wl)
+        d = o.derived()
+        n = int(wl["cfg"]["n"])
+        dx, sf = float(d["dx"]), float(d["dx"]) / float(d["dx0"])
+        base = o.rays()
+        rng = np.random.default_rng(31)
+        m = 24
+        one = {key: np.repeat(val[:1], m, axis=0).copy() for key, val in base.items()}
+        cells = []
+        pixels = {l: rng.choice(12 << (2 * l), size=m // 4, replace=False) for l in range(4)}  # unique per order
+        for q in range(m):
+            l = q % 4
+            pix = int(pixels[l][q // 4])
+            it = int(rng.integers(0, 42875))
+            v = P.rotate(it, 0, P.pix2vec(l, pix).astype(np.float32))
+            i, j, k = ((n - 1, n - 1, 0) if q % 3 == 0 else tuple(int(x) for x in rng.integers(0, n, 3)))
+            one["order"][q], one["pixel"][q], one["itheta"][q] = l, pix, it
+            one["xhat"][q], one["yhat"][q], one["zhat"][q] = v
+            one["r"][q] = np.float32(d["rmax"][l] * rng.uniform(1.05, 1.6))
+            one["x"][q], one["y"][q], one["z"][q] = rng.uniform(0.01, 0.99, 3).astype(np.float32) * np.float32(dx)
+            one["cell"][q] = i * n * n + j * n + k
+            cells.append((i, j, k))
+        o.upload_rays(one)
+        o.split_rays()
+        kids = o.rays()
+        assert len(kids["cell"]) == 4 * m
+        got = {(int(p), int(kids["order"][t])): t for t, p in enumerate(kids["pixel"])}
+        moved = 0
+        for q in range(m):
+            l = int(one["order"][q])
+            pv = np.array([one["xhat"][q], one["yhat"][q], one["zhat"][q]], dtype=np.float64)
+            for c in range(4):
+                t = got[(4 * int(one["pixel"][q]) + c, l + 1)]
+                cv = np.array([kids["xhat"][t], kids["yhat"][t], kids["zhat"][t]], dtype=np.float64)
+                pos = np.array([one["x"][q], one["y"][q], one["z"][q]], dtype=np.float64)
+                pos = pos + (cv - pv) * float(one["r"][q]) * sf
+                ijk = list(cells[q])
+                for a in range(3):
+                    while pos[a] > dx:
+                        pos[a] -= dx
+                        ijk[a] = (ijk[a] + 1) % n
+                    while pos[a] < 0:
+                        pos[a] += dx
+                        ijk[a] = (ijk[a] - 1) % n
+                moved += ijk != list(cells[q])
+                assert int(kids["cell"][t]) == ijk[0] * n * n + ijk[1] * n + ijk[2], (q, c)
+                kp = np.array([kids["x"][t], kids["y"][t], kids["z"][t]], dtype=np.float64)
+                assert np.allclose(kp, pos, rtol=0, atol=2e-6 * dx), (q, c, kp / dx, pos / dx)
+                assert float(kids["r"][t]) == float(one["r"][q])
+                assert float(kids["dr"][t]) == pytest.approx(d["rmax"][l + 1] - float(one["r"][q]), rel=1e-6)
+        assert moved >= 8  # the case holds walks into neighbouring (and wrapped) cells
+    finally:
+        o.close()
