@@ -28,6 +28,7 @@ _PROTOS = {
     "frt_set_dt": (C.c_int, [_CTX, C.c_float]),
     "frt_get_cell_size": (C.c_int, [_CTX, C.POINTER(C.c_float)]),
     "frt_update_cosmo_expansion": (C.c_int, [_CTX, C.c_float, C.c_float]),
+    "frt_set_opacity_dither": (C.c_int, [_CTX, C.c_uint64]),
     "frt_update_hydro_step": (C.c_int, [_CTX, C.c_float, C.c_float, C.c_float, C.c_float]),
     "frt_otf_averages": (C.c_int, [_CTX, C.c_float, C.c_float, C.POINTER(FrtOtf)]),
     "frt_otf_averages_noalloc": (C.c_int, [_CTX, C.c_float, C.c_float, C.POINTER(FrtOtf)]),

@@ -651,6 +651,12 @@ int frt_set_step(frt_ctx* ctx, int64_t step) {
   return FRT_OK;
 }
 
+int frt_set_opacity_dither(frt_ctx* ctx, uint64_t seed) {
+  if (!ctx) return FRT_E_ARG;
+  ctx->g.dither_seed = seed;
+  return FRT_OK;
+}
+
 int frt_set_dt(frt_ctx* ctx, float dt) {
   if (!ctx || !(dt > 0.f)) return FRT_E_ARG;
   ctx->g.dt = dt;

@@ -66,6 +66,9 @@ struct GridConst {
   double t_cmb;         // 2.726*(1 + zz)
   float zz;
   float avg_rho;        // the driver's volume-average rho (thermal ddelta_dt term)
+  uint64_t dither_seed; // opacity dither (frt_rng.h), 0 = off
+  int64_t dither_step;  // completed steps, for the dither hash
+  int pass;             // C2Ray pass of the current cell launch
   float nhe_to_nh;
   double sig912;        // sigmapi_H1(13.6/h_eV)
   double sigH[FRT_MAX_FREQ];

@@ -558,21 +558,41 @@ __device__ __forceinline__ void bin_deposit_mixed(const GridConst& g, float dth,
   const float omh = -mh, omhe = -mhe;
   float fh = 0.f, fhe = 0.f;
   if ((dth > 0.f) || (dthe > 0.f)) {
-    const float a = omh * ehe;
-    const float b = omhe * eh;
-    const float den = a + b;
-    if (den >= 1e-30f) {
-      const float inv = __builtin_amdgcn_rcpf(den);  // 1 ulp
-      fh = a * inv;
-      fhe = b * inv;
-    } else if (den > 0.f) {
-      fh = a / den;
-      fhe = b / den;
-    } else if (g.frac_limit) {
-      fh = 1.f / (1.f + expf(dthe - dth));
-      fhe = 1.f / (1.f + expf(dth - dthe));
+    // HI share fh = a/(a+b), a = (1 - eh) ehe, b = (1 - ehe) eh
+    // (ray_tracing.cc:770-771), with a and b divided by eh ehe:
+    //   fh = omh / (omh + omhe e^-(dth - dthe))                for dth >= dthe,
+    //   fh = omh e^-(dthe - dth) / (omh e^-(dthe - dth) + omhe) otherwise,
+    // exact where eh or ehe leaves the float range (a float eh is 0 beyond a
+    // depth of 87, while the FP64 form's double stays positive to 745).
+    // Beyond 745 the FP64 form's exp underflows too, and its limits are kept:
+    // fh = 1 if only dth is that deep, 0 if only dthe, the logistic
+    // 1/(1 + e^(dthe - dth)) of frac_limit if both.
+    constexpr float DEEP = 745.13322f;  // double exp(-x) is 0 beyond this
+    const bool dh = dth > DEEP, dhe = dthe > DEEP;
+    if (dh && dhe) {
+      if (g.frac_limit) {
+        fh = 1.f / (1.f + expf(dthe - dth));
+        fhe = 1.f / (1.f + expf(dth - dthe));
+      } else {
+        fh = fhe = __int_as_float(0x7fc00000);
+      }
+    } else if (dh || dhe) {
+      fh = dh ? 1.f : 0.f;
+      fhe = dh ? 0.f : 1.f;
     } else {
-      fh = fhe = __int_as_float(0x7fc00000);
+      float w, unused;
+      frt_exp_expm1_neg(fabsf(dth - dthe), w, unused);  // e^-|dth - dthe|
+      const float a = dth >= dthe ? omh : omh * w;
+      const float b = dth >= dthe ? omhe * w : omhe;
+      const float den = a + b;
+      if (den > 0.f) {
+        const float inv = __builtin_amdgcn_rcpf(den);  // 1 ulp
+        fh = a * inv;
+        fhe = b * inv;
+      } else {  // depths so small that 1 - e^-d rounds to 0 in both
+        fh = dth >= dthe ? 1.f : 0.f;
+        fhe = 1.f - fh;
+      }
     }
   }
   // 1 - exp(-(dth + dthe)) = omh + omhe - omh * omhe (exact identity)
@@ -1173,13 +1193,18 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
     const double xeq = GH / pr;
     const double ti = 1. / pr;
     const double e_h = dm_exp(-dtd / ti);
-    const float avg_h1 = (float)((double)nH * (1. - (xeq + ti / dtd * (x0 - xeq) * (1. - e_h))));
+    float avg_h1 = (float)((double)nH * (1. - (xeq + ti / dtd * (x0 - xeq) * (1. - e_h))));
     const double x0he = 1. - (double)cf.f[C_FHE1][k];
     const double prhe = GHe + (double)(clump * rHe2) * nelec;
     const double xeqhe = GHe / prhe;
     const double tihe = 1. / prhe;
     const double e_he = dm_exp(-dtd / tihe);
-    const float avg_he1 = (float)((double)nHe * (1. - (xeqhe + tihe / dtd * (x0he - xeqhe) * (1. - e_he))));
+    float avg_he1 = (float)((double)nHe * (1. - (xeqhe + tihe / dtd * (x0he - xeqhe) * (1. - e_he))));
+    if (g.dither_seed) {  // rounding-sensitivity measurement (frt_rng.h)
+      const uint64_t r = (uint64_t)frt_cell_to_rowmajor(g.lay, (int)k);
+      avg_h1 = frt_ulp_step(avg_h1, frt_dither_up(g.dither_seed, (uint64_t)g.dither_step, g.pass, r, 0));
+      avg_he1 = frt_ulp_step(avg_he1, frt_dither_up(g.dither_seed, (uint64_t)g.dither_step, g.pass, r, 1));
+    }
     cf.f[C_AVG_NH1][k] = avg_h1;
     cf.f[C_AVG_NHE1][k] = avg_he1;
     cf.avg2[k] = make_float2(avg_h1, avg_he1);
@@ -1895,8 +1920,10 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
                (int64_t)g.n3);
     FRT_LAUNCH(ctx, "adj_tape", grid_for(n), k_tape_nph, rb, n, F, g.cap, ctx->adj_nph);
   }
+  g.dither_step = ctx->step;
   for (int it = 0; it < I; ++it) {
     const bool first = (it == 0), last = (it == I - 1);
+    g.pass = it;
     double* tk = tape ? tape + (size_t)4 * it * g.n3 : nullptr;
     uint8_t* tnp = tape ? ctx->adj_np + (size_t)it * g.cap : nullptr;
     if (tk)

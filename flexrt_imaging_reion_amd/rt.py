@@ -158,6 +158,11 @@ class FlexRT:
         """Restart: completed steps (they key the casting and merging angle streams)."""
         self._check(self._lib.frt_set_step(self._h, int(step)), "set_step")
 
+    def set_opacity_dither(self, seed):
+        """Rounding-sensitivity measurement: one-ulp dither of the stored
+        opacities after every C2Ray pass (seed 0 turns it off)."""
+        self._check(self._lib.frt_set_opacity_dither(self._h, int(seed)), "set_opacity_dither")
+
     def set_dt(self, dt):
         """update_dt (rt_funcs.cc:9-12): the driver's new time step."""
         self._check(self._lib.frt_set_dt(self._h, float(dt)), "set_dt")

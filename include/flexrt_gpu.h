@@ -201,6 +201,15 @@ int frt_get_time(const frt_ctx* ctx, float* t);
  * context that ran `step` calls of frt_step after one frt_new_rays; the
  * step is reported in frt_step_stats.step. */
 int frt_set_step(frt_ctx* ctx, int64_t step);
+/* Rounding-sensitivity measurement, no counterpart in the reference (seed 0,
+ * the default, turns it off): after every C2Ray pass each cell's stored
+ * opacities avg_nh1 and avg_nhe1 move one float ulp up or down, chosen by a
+ * hash of (seed, step, pass, row-major cell).  The reference stores these in
+ * float, so the spread between runs with and without the dither is what
+ * rounding at that storage does to the result; it bounds the agreement any
+ * other precision form can be asked for.  The CPU checker applies the same
+ * dither. */
+int frt_set_opacity_dither(frt_ctx* ctx, uint64_t seed);
 /* update_dt (rt_funcs.cc:9-12) computes dt on the host from dx; the driver
  * hands the new value over (dt > 0). */
 int frt_set_dt(frt_ctx* ctx, float dt);

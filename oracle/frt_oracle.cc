@@ -169,6 +169,7 @@ struct State {
       ddelta_dt, rho_prev;
   float gadget_step_time = 0.f;
   float avg_rho = 0.f;  // the driver's avg_rho (global_variables.cc:84), set by update_hydro_step
+  uint64_t dither_seed = 0;  // one-ulp opacity dither after every pass (frt_rng.h), 0 = off
   std::vector<double> avg_gamma, avg_gamma_he1, avg_gamma_nu, avg_gamma_nu_he1, ngam_acc;
   std::vector<Ray> ray;
   std::vector<int> src_cell;
@@ -624,6 +625,10 @@ static void find_avg_unu(State& s) {
                              s.recomb_H2[c], s.recomb_He2[c], s.avg_gamma[c], s.avg_gamma_he1[c], s.dt);
       s.avg_nh1[c] = o.avg_nh1;
       s.avg_nhe1[c] = o.avg_nhe1;
+      if (s.dither_seed) {  // rounding-sensitivity measurement, as the library's
+        s.avg_nh1[c] = frt_ulp_step(s.avg_nh1[c], frt_dither_up(s.dither_seed, (uint64_t)s.step, iter, (uint64_t)c, 0));
+        s.avg_nhe1[c] = frt_ulp_step(s.avg_nhe1[c], frt_dither_up(s.dither_seed, (uint64_t)s.step, iter, (uint64_t)c, 1));
+      }
       if (iter == itercount - 1) {
         s.nH1[c] = o.nH1;
         s.nH2[c] = s.nH[c] - s.nH1[c];
@@ -1153,6 +1158,7 @@ int orc_download_field(void* h, int f, float* dst) {
 }
 
 void orc_set_time(void* h, float t) { ((State*)h)->t = t; }
+void orc_set_opacity_dither(void* h, uint64_t seed) { ((State*)h)->dither_seed = seed; }
 // Completed steps, as frt_set_step: the merge and casting angle streams follow.
 void orc_set_step(void* h, int64_t step) {
   State& s = *(State*)h;

@@ -67,6 +67,7 @@ _PROTOS = {
     "orc_get_gadget_step_time": (C.c_float, [C.c_void_p]),
     "orc_otf_averages": (None, [C.c_void_p, C.c_float, C.c_float, C.POINTER(FrtOtf)]),
     "orc_set_step": (None, [C.c_void_p, C.c_int64]),
+    "orc_set_opacity_dither": (None, [C.c_void_p, C.c_uint64]),
     "orc_update_hydro_step": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]),
     "orc_get_cell_size": (C.c_float, [C.c_void_p]),
     "orc_itheta_hash": (C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint64]),
@@ -152,6 +153,9 @@ class Oracle:
 
     def set_step(self, step):
         self._lib.orc_set_step(self._h, int(step))
+
+    def set_opacity_dither(self, seed):
+        self._lib.orc_set_opacity_dither(self._h, int(seed))
 
     def init_rt(self):
         self._lib.orc_init_rt(self._h)
