@@ -1,9 +1,9 @@
 """The benchmark configuration itself (128^3 cells, 4096 sources, 5 bins),
 checked through properties that hold at any size, since the CPU checker is
-too slow for it: the mixed form against the FP64 form (bit-identical to the
-checker at every size the checker runs) within the mixed-precision targets,
-run-to-run repeatability of the mixed form, and a ray-file round trip of the
-full ray list."""
+too slow for it: run-to-run repeatability of the mixed form over all 20
+steps (merging starts near step 10), and a ray-file round trip of the full
+ray list.  The mixed form against the FP64 form over the 20 steps is in
+test_gpu_precision.py."""
 import numpy as np
 import pytest
 
@@ -12,7 +12,8 @@ from parity_util import STATE_FIELDS, bits_equal, compare_rays, counts, fields
 
 pytestmark = pytest.mark.gpu
 
-STEPS = 4
+STEPS = 20
+FILE_STEPS = 4  # the ray-file round trip (2 GB of file at 20 steps)
 
 
 def _print_case(*a):
@@ -24,6 +25,17 @@ def wl():
     return inputs.make_workload("batch128", steps=STEPS)
 
 
+def test_full_size_merges(wl):
+    """The 20 steps reach the merge stage."""
+    _print_case("full size batch128 mixed, %d steps, merging" % STEPS)
+    g, st = run(wl, 1)
+    try:
+        assert sum(s["rays_merged_away"] for s in st) > 0
+        assert all(np.isfinite(g.download_field(k)).all() for k in ("nH1", "temp", "gamma_H1_tot"))
+    finally:
+        g.close()
+
+
 def run(wl, precision, steps=STEPS):
     from flexrt_imaging_reion_amd.rt import FlexRT
 
@@ -31,27 +43,6 @@ def run(wl, precision, steps=STEPS):
     inputs.setup(g, wl)
     st = [g.step() for _ in range(steps)]
     return g, st
-
-
-def test_full_size_mixed_against_fp64(wl):
-    """x_HI median relative error < 1e-5, fewer than 0.5 % of cells beyond
-    1e-3, ionized volume fraction within 1e-4 (the mixed-precision target)."""
-    _print_case("full size batch128 mixed vs fp64, %d steps" % STEPS)
-    gm, sm = run(wl, 1)
-    gf, sf = run(wl, 0)
-    try:
-        xm = gm.download_field("nH1").astype(np.float64) / gm.download_field("nH")
-        xf = gf.download_field("nH1").astype(np.float64) / gf.download_field("nH")
-        assert np.all(np.isfinite(xm))
-        rel = np.abs(xm - xf) / np.abs(xf)
-        assert np.median(rel) < 1e-5
-        assert np.mean(rel > 1e-3) < 5e-3
-        assert abs(float((xm < 0.5).mean()) - float((xf < 0.5).mean())) < 1e-4
-        # ray counts may differ only by threshold flips: well under 0.1 %
-        assert abs(sm[-1]["rays_end"] - sf[-1]["rays_end"]) <= 1e-3 * sf[-1]["rays_end"]
-    finally:
-        gm.close()
-        gf.close()
 
 
 def test_full_size_mixed_repeatable(wl):
@@ -71,8 +62,8 @@ def test_full_size_mixed_repeatable(wl):
 def test_full_size_ray_file_round_trip(wl, tmp_path):
     from flexrt_imaging_reion_amd.rt import FlexRT
 
-    _print_case("full size batch128 ray file round trip, %d steps" % STEPS)
-    g, _ = run(wl, 1)
+    _print_case("full size batch128 ray file round trip, %d steps" % FILE_STEPS)
+    g, _ = run(wl, 1, FILE_STEPS)
     try:
         path = tmp_path / "ray_full"
         g.write_ray_file(path)

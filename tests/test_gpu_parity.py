@@ -103,41 +103,24 @@ def test_fp64_merging_bit_exact(rpc, steps):
 
 
 def test_mixed_within_tolerance():
-    """Mixed form against the FP64 checker.
-
-    A ray whose photon total sits within FP32 rounding (~1e-7 relative) of the
-    dump or survival threshold can take the other branch than in FP64; its
-    remaining photons then land in a different cell, which moves x_HI there
-    by far more than the rounding.  Such flips are rare but cannot be ruled
-    out with FP32 photon bookkeeping (one flip moved x_HI by >1e-3 in 61 of
-    32768 cells here).  The criteria therefore bound the error away from
-    flips and the damage a flip does: 99 % of cells within 1e-4 in x_HI
-    and Gamma_HI (relative to its max) and within 1e-3 in T (the thermal
-    sub-step count is itself a threshold decision), at most 0.5 % of cells
-    beyond 1e-3 in x_HI, ionized volume fraction within 1e-4, and per-step
-    counters within 0.2 % (at least 2) of the checker's.  The maximum and
-    percentiles are printed."""
+    """Mixed form against the FP64 checker at 32^3 (24 sources, 8 steps):
+    the mixed-precision targets as stated -- identical counts at every step,
+    x_HI within 1e-3 relative in every cell, ionized volume fraction within
+    1e-4 -- with Gamma_HI (relative to its maximum) and T errors printed and
+    bounded at their 99th percentiles."""
     _print_case("mixed lognormal n=32")
     wl = inputs.make_workload("lognormal64", n=32, n_src=24, steps=8)
     g, o = make_pair(wl, precision=1)
     for s in range(wl["steps"]):
         cg = counts(g.step())
         co = counts(o.step())
-        diff = {k: (cg[k], co[k]) for k in cg if cg[k] != co[k]}
-        if diff:
-            print("step", s, "count differences", diff)
-        for k in ("rays_end", "rays_dumped", "rays_terminated", "rays_split", "segments"):
-            assert abs(cg[k] - co[k]) <= max(2, 2e-3 * co[k]), (s, k, cg[k], co[k])
+        assert cg == co, (s, {k: (cg[k], co[k]) for k in cg if cg[k] != co[k]})
     xg = g.download_field("nH1") / g.download_field("nH")
     xo = o.download_field("nH1") / o.download_field("nH")
     rel = np.abs(xg.astype(np.float64) - xo) / np.abs(xo)
-    worst = np.argsort(-rel)[:5]
-    print("worst x_HI cells", [(int(c), float(xo[c]), float(xg[c]), float(rel[c])) for c in worst])
-    print("x_HI rel err: max %.3g p99.9 %.3g p99 %.3g median %.3g cells>1e-3 %d"
-          % (rel.max(), np.percentile(rel, 99.9), np.percentile(rel, 99), np.median(rel),
-             int((rel > 1e-3).sum())))
-    assert np.percentile(rel, 99) <= 1e-4
-    assert (rel > 1e-3).mean() <= 5e-3
+    print("x_HI rel err: max %.3g p99.9 %.3g p99 %.3g median %.3g"
+          % (rel.max(), np.percentile(rel, 99.9), np.percentile(rel, 99), np.median(rel)))
+    assert rel.max() <= 1e-3
     assert abs(float((xg < 0.5).mean()) - float((xo < 0.5).mean())) <= 1e-4
     Tg, To = g.download_field("temp"), o.download_field("temp")
     rT = np.abs(Tg.astype(np.float64) - To) / np.abs(To)
@@ -145,8 +128,8 @@ def test_mixed_within_tolerance():
     rG = np.abs(gg.astype(np.float64) - go) / float(np.abs(go).max())
     print("T rel err: max %.3g p99 %.3g; Gamma_HI err/max: max %.3g p99 %.3g"
           % (rT.max(), np.percentile(rT, 99), rG.max(), np.percentile(rG, 99)))
-    assert np.percentile(rT, 99) <= 1e-3
-    assert np.percentile(rG, 99) <= 1e-4
+    assert np.percentile(rT, 99) <= 1e-4
+    assert np.percentile(rG, 99) <= 1e-5
 
 
 def test_mixed_repeatable_across_contexts():
