@@ -50,7 +50,6 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="batch128")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--restart-files", default=None, metavar="DIR",
@@ -358,8 +357,57 @@ def pmc_traffic(kernel, workload, mixed):
     return None, None
 
 
+# Algorithmic bytes of the step's terms (per unit), as the scope table of the
+# hot path fixes them: a moved ray's record read and written (72 B each way),
+# a stored segment written once by the march (8 B) and read with its cell's two
+# opacities in every C2Ray pass (b_seg), per cell and pass the Gamma and photon
+# outputs, chemistry inputs and opacity outputs (b_ci), and per cell the
+# finalize and thermal update (b_cf).
+B_RAY = 144.0
+B_SEG_MARCH = 8.0
+B_SEG = {True: 16.0, False: 24.0}
+B_CI = {True: 104.0, False: 208.0}
+B_CF = {True: 108.0, False: 216.0}
+
+
+def step_bytes(wl, st, mixed):
+    """B_step = N_ray 144 + N_seg (8 + I b_seg) + N_cell (I b_ci + b_cf), with
+    N_ray the rays moved and N_seg the segments stored in that step."""
+    I = float(wl["cfg"]["itercount"])
+    n3 = float(wl["cfg"]["n"]) ** 3
+    return (float(st["rays_moved"]) * B_RAY + float(st["segments"]) * (B_SEG_MARCH + I * B_SEG[mixed]) +
+            n3 * (I * B_CI[mixed] + B_CF[mixed]))
+
+
+def spec_bytes(name, wl, st, mixed):
+    """Bytes of one launch of `name` by the same per-unit figures: a deposit
+    pass moves N_seg b_seg, a cell pass N_cell b_ci (the last adds b_cf), the
+    march N_ray 144 + N_seg 8; None for the other kernels."""
+    I = float(wl["cfg"]["itercount"])
+    n3 = float(wl["cfg"]["n"]) ** 3
+    if name.startswith("deposit"):
+        return float(st["segments"]) * B_SEG[mixed]
+    if name == "chem":
+        return n3 * B_CI[mixed] + n3 * B_CF[mixed] / I
+    if name == "march":
+        return float(st["rays_moved"]) * B_RAY + float(st["segments"]) * B_SEG_MARCH
+    return None
+
+
+def step_roofline(wl, stats, wall, mixed):
+    """Whole-step figure: the steps' summed B_step over their wall time."""
+    tot = sum(step_bytes(wl, st, mixed) for st in stats)
+    gbs = tot / wall / 1e9
+    return {"bytes_per_step": round(tot / len(stats)), "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5)}
+
+
 def roofline(wl, stats, ktimes, mixed):
-    """Roofline figure of the kernel with the largest share of GPU time."""
+    """Roofline figure of the kernel with the largest share of GPU time:
+    achieved = its algorithmic bytes per launch (the per-unit figures of
+    step_bytes times the units one launch handles) over its average launch
+    time.  The finer per-kernel byte model (kernel_bytes: slot indices, path
+    reads, per-ray photon loads, partial writes) is reported beside it."""
     best = None
     for name, (ms, launches) in ktimes.items():
         if launches == 0:
@@ -380,40 +428,112 @@ def roofline(wl, stats, ktimes, mixed):
         return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": name, "kernel_ms_total": ms, "launches": launches}
     avg_ms = ms / launches
-    avg_bytes = total_bytes / n_launch
+    model_bytes = total_bytes / n_launch
+    spec = [spec_bytes(name, wl, st, mixed) for st in stats]
+    avg_bytes = sum(spec) / len(spec) if spec and spec[0] is not None else model_bytes
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
     traffic, source = pmc_traffic(name, wl["name"], mixed)
     out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 5),
            "traffic": round(traffic) if traffic is not None else None, "kernel": name,
            "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": round(avg_bytes),
-           "launches": launches}
+           "launches": launches, "model_bytes_per_launch": round(model_bytes),
+           "model_frac": round(model_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
     if source:
         out["traffic_source"] = source
         out["traffic_GBps"] = round(traffic / (avg_ms * 1e-3) / 1e9, 2)
     return out
 
 
-def cpu_baseline(wl, seconds):
+CPU_SAMPLE_STEPS = (5, 10, 15, 20)
+
+
+def fp64_snapshots(wl, steps, sample, make=None):
+    """Run the FP64 form (bit-identical to the CPU checker; `make` may supply
+    another backend) and keep its full state before each sampled step: every
+    field, the ray list, the time and the merge threshold carried into the
+    step; also each step's stats."""
+    from flexrt_imaging_reion_amd import inputs
+    from flexrt_imaging_reion_amd._abi import FIELDS
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    snaps, stats = {}, {}
+    g = make() if make else FlexRT(dict(wl["cfg"], precision=0))
+    try:
+        inputs.setup(g, wl)
+        merge = 0.0
+        for s in range(1, steps + 1):
+            if s in sample:
+                snaps[s] = {"fields": {k: g.download_field(k) for k in FIELDS}, "rays": g.rays(),
+                            "t": g.get_time(), "nphot_merge": merge}
+            stats[s] = g.step()
+            merge = stats[s]["nphot_merge"]
+    finally:
+        g.close()
+    return snaps, stats
+
+
+def resume(wl, snap, s, omp):
+    """A CPU checker context holding the state before step s."""
     from flexrt_imaging_reion_amd import inputs
     from oracle.pyoracle import Oracle
 
-    o = Oracle(wl["cfg"])
+    o = Oracle(wl["cfg"], omp=omp)
     inputs.setup(o, wl)
-    t0 = time.perf_counter()
-    k = 0
-    while k < wl["steps"]:
-        o.step()
-        k += 1
-        log("cpu checker step %d done at %.1f s" % (k, time.perf_counter() - t0))
-        if time.perf_counter() - t0 > seconds:
-            break
-    wall = time.perf_counter() - t0
+    for k, v in snap["fields"].items():
+        o.upload_field(k, v)
+    o.upload_rays(snap["rays"])
+    o.set_time(snap["t"])
+    o.set_step(s - 1)
+    o.get_nphot_min_max()
+    o.set_nphot_merge(snap["nphot_merge"])
+    return o
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, steps, sample=CPU_SAMPLE_STEPS):
+    """The CPU checker's OpenMP build (the reference's parallel loops on all
+    host threads, atomic per-cell sums) timed on single steps of the same
+    workload, each started from the FP64 GPU state before it (identical to
+    the checker's own state there); the 20-step wall time is the piecewise-
+    linear interpolation of the sampled step times through (0, 0), summed."""
+    from oracle.pyoracle import set_threads
+
+    sample = tuple(s for s in sample if s <= steps)
+    snaps, gpu_stats = fp64_snapshots(wl, steps, sample)
+    threads = set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    times, rays = {}, {}
+    for s in sample:
+        o = resume(wl, snaps.pop(s), s, omp=True)
+        try:
+            t0 = time.perf_counter()
+            st = o.step()
+            times[s] = time.perf_counter() - t0
+            rays[s] = (int(st["rays_end"]), int(gpu_stats[s]["rays_end"]))
+        finally:
+            o.close()
+        log("cpu checker (OpenMP, %d threads) step %d: %.2f s, %d rays (GPU FP64: %d)"
+            % (threads, s, times[s], rays[s][0], rays[s][1]))
+    xs = [0] + list(sample)
+    ys = [0.0] + [times[s] for s in sample]
+    est = float(sum(np.interp(k, xs, ys) for k in range(1, steps + 1)))
     n3 = float(wl["cfg"]["n"]) ** 3
-    o.close()
-    return {"value": round(n3 * k / wall, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "first %d of %d steps of the same workload, CPU checker (oracle/frt_oracle.cc, "
-                      "g++ -O3, serial), %.1f s" % (k, wl["steps"], wall)}
+    return {"value": round(n3 * steps / est, 1), "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": "steps %s of the %d-step workload, each from the FP64 GPU state before it, CPU checker "
+                      "OpenMP build (oracle/frt_oracle.cc, g++ -O3 -fopenmp, %d threads); %d-step wall "
+                      "interpolated from the sampled steps: %.1f s" % (list(sample), steps, threads, steps, est),
+            "step_seconds": {str(s): round(times[s], 3) for s in sample},
+            "rays_end_cpu_vs_gpu": {str(s): rays[s] for s in sample},
+            "host": {"nproc": os.cpu_count(), "model": cpu_model()}}
 
 
 def main():
@@ -449,6 +569,7 @@ def main():
                                   len(wl["src_cells"]), steps, wl["params"]["dt_myr"], wl["params"]["clump"]),
                    "parallelism": "single GPU"},
         "roofline": roofline(wl, st_m, kt_m, True),
+        "step_roofline": step_roofline(wl, st_m, wall_m, True),
         "ms_per_step_with_launch_events": round(1e3 * wp_m / steps, 3),
         "rays_end": int(st_m[-1]["rays_end"]),
         "segments_per_step": int(np.mean([s["segments"] for s in st_m])),
@@ -462,7 +583,8 @@ def main():
         keys = ("rays_moved", "segments", "rays_dumped", "rays_terminated", "rays_merged_away",
                 "rays_split", "rays_end")
         out["fp64"] = {"value": round(n3 * steps / wall_f, 1), "ms_per_step": round(1e3 * wall_f / steps, 3),
-                       "roofline": roofline(wl, st_f, kt_f, False)}
+                       "roofline": roofline(wl, st_f, kt_f, False),
+                       "step_roofline": step_roofline(wl, st_f, wall_f, False)}
         out["mixed_over_fp64"] = round(wall_f / wall_m, 3)
         rx = np.abs(xm.astype(np.float64) - xf) / np.abs(xf.astype(np.float64))
         first_diff = next((k for k, (sm, sf) in enumerate(zip(st_m, st_f))
@@ -486,7 +608,7 @@ def main():
     if files:
         out["restart_files"] = files
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(wl, steps)
     if a.calibrate_copy:
         from flexrt_imaging_reion_amd.rt import FlexRT
 

@@ -15,6 +15,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libflexrt_gpu.so")
 ORACLE_SRC = os.path.join(REPO_DIR, "oracle", "frt_oracle.cc")
 ORACLE_LIB = os.path.join(REPO_DIR, "oracle", "liboracle.so")
+ORACLE_OMP_LIB = os.path.join(REPO_DIR, "oracle", "liboracle_omp.so")
 
 HIP_SOURCES = ["frt_kernels.hip", "frt_capi.hip", "frt_adjoint.hip"]
 HIPCC_FLAGS = [
@@ -47,10 +48,16 @@ def build_library(out=LIB_PATH):
     return out
 
 
-def build_oracle(out=ORACLE_LIB):
+def build_oracle(out=ORACLE_LIB, openmp=False):
+    """The serial build is the checker; the OpenMP build (same source, the
+    reference's parallel loops over all threads) is the timed CPU baseline."""
     tmp = out + ".tmp"
     cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-mfma", "-ffp-contract=off",
-           "-fno-fast-math", "-o", tmp, ORACLE_SRC]
+           "-fno-fast-math", "-Wno-unknown-pragmas"] + (["-fopenmp"] if openmp else []) + ["-o", tmp, ORACLE_SRC]
     _run(cmd, os.path.dirname(ORACLE_SRC))
     os.replace(tmp, out)
     return out
+
+
+def build_oracle_omp(out=ORACLE_OMP_LIB):
+    return build_oracle(out, openmp=True)

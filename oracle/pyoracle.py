@@ -68,6 +68,8 @@ _PROTOS = {
     "orc_otf_averages": (None, [C.c_void_p, C.c_float, C.c_float, C.POINTER(FrtOtf)]),
     "orc_set_step": (None, [C.c_void_p, C.c_int64]),
     "orc_set_opacity_dither": (None, [C.c_void_p, C.c_uint64]),
+    "orc_set_threads": (C.c_int, [C.c_int]),
+    "orc_set_nphot_merge": (None, [C.c_void_p, C.c_float]),
     "orc_update_hydro_step": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]),
     "orc_get_cell_size": (C.c_float, [C.c_void_p]),
     "orc_itheta_hash": (C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint64]),
@@ -79,25 +81,47 @@ RATES = {"sigmapi_H1": 0, "sigmapi_He1": 1, "alphaB_H2": 2, "alphaB_He2": 3, "al
          "free_free_rate": 10, "compton_rate": 11}
 
 
-def load():
-    global _lib
+_omp_lib = None
+OMP_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle_omp.so")
+
+
+def _bind(path):
+    lib = C.CDLL(path)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def load(omp=False):
+    """The serial checker, or with omp=True its OpenMP build (the timed CPU
+    baseline: the reference's parallel loops, sums in scheduling order)."""
+    global _lib, _omp_lib
+    if omp:
+        if _omp_lib is None:
+            if not os.path.exists(OMP_LIB):
+                raise FileNotFoundError("oracle/liboracle_omp.so is not built; run __graft_entry__.build()")
+            _omp_lib = _bind(OMP_LIB)
+        return _omp_lib
     if _lib is None:
         if not os.path.exists(LIB):
             raise FileNotFoundError("oracle/liboracle.so is not built; run __graft_entry__.build()")
-        lib = C.CDLL(LIB)
-        for name, (res, args) in _PROTOS.items():
-            fn = getattr(lib, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = lib
+        _lib = _bind(LIB)
     return _lib
 
 
-class Oracle:
-    """Serial CPU restatement of the RT step with the FlexRT method names."""
+def set_threads(n):
+    """Threads of the OpenMP build; returns the count in use."""
+    return load(omp=True).orc_set_threads(int(n))
 
-    def __init__(self, cfg):
-        self._lib = load()
+
+class Oracle:
+    """CPU restatement of the RT step with the FlexRT method names (serial
+    unless omp=True)."""
+
+    def __init__(self, cfg, omp=False):
+        self._lib = load(omp)
         self.cfg = dict(cfg)
         s = config_struct(self.cfg)
         h = self._lib.orc_create(C.byref(s))
@@ -156,6 +180,9 @@ class Oracle:
 
     def set_opacity_dither(self, seed):
         self._lib.orc_set_opacity_dither(self._h, int(seed))
+
+    def set_nphot_merge(self, v):
+        self._lib.orc_set_nphot_merge(self._h, float(v))
 
     def init_rt(self):
         self._lib.orc_init_rt(self._h)
