@@ -490,6 +490,46 @@ def resume(wl, snap, s, omp):
     return o
 
 
+def final_state(wl, precision, steps, dither=0):
+    """Fields and per-step stats after `steps` untimed steps (opacity dither
+    seed `dither`, 0 = off)."""
+    from flexrt_imaging_reion_amd import inputs
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    with FlexRT(dict(wl["cfg"], precision=precision)) as g:
+        inputs.setup(g, wl)
+        g.set_opacity_dither(dither)
+        st = [g.step() for _ in range(steps)]
+        return {f: g.download_field(f) for f in ("nH1", "nH", "temp", "gamma_H1_tot")}, st
+
+
+COUNT_KEYS = ("rays_moved", "segments", "rays_dumped", "rays_terminated", "rays_merged_away", "rays_split",
+              "rays_end")
+
+
+def errors_against(state_a, st_a, state_f, st_f):
+    """Errors of a run (state_a, st_a) against the FP64 run (state_f, st_f)."""
+    xa = state_a["nH1"] / state_a["nH"]
+    xf = state_f["nH1"] / state_f["nH"]
+    rel = lambda u, v: float(np.max(np.abs(u.astype(np.float64) - v) / np.maximum(np.abs(v), 1e-300)))
+    gmax = float(np.abs(state_f["gamma_H1_tot"]).max()) or 1.0
+    rx = np.abs(xa.astype(np.float64) - xf) / np.abs(xf.astype(np.float64))
+    first_diff = next((k + 1 for k, (sa, sf) in enumerate(zip(st_a, st_f))
+                       if any(sa[q] != sf[q] for q in COUNT_KEYS)), None)
+    return {
+        "x_HI_max_rel": rel(xa, xf),
+        "x_HI_rel_p99_9": float(np.percentile(rx, 99.9)),
+        "x_HI_rel_median": float(np.median(rx)),
+        "cells_x_HI_rel_over_1e-3": int((rx > 1e-3).sum()),
+        "first_step_with_count_difference": first_diff,
+        "ionized_volume_fraction_abs": abs(float((xa < 0.5).mean()) - float((xf < 0.5).mean())),
+        "Gamma_HI_max_abs_over_max": float(np.abs(state_a["gamma_H1_tot"].astype(np.float64) -
+                                                  state_f["gamma_H1_tot"]).max()) / gmax,
+        "T_max_rel": rel(state_a["temp"], state_f["temp"]),
+        "ray_counts_exact": first_diff is None,
+    }
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -576,31 +616,18 @@ def main():
     }
     if not a.no_fp64:
         wall_f, st_f, kt_f, state_f, wp_f, _ = run_form(wl, 0, steps, a.warmup, True)
-        xm = state_m["nH1"] / state_m["nH"]
-        xf = state_f["nH1"] / state_f["nH"]
-        rel = lambda u, v: float(np.max(np.abs(u.astype(np.float64) - v) / np.maximum(np.abs(v), 1e-300)))
-        gmax = float(np.abs(state_f["gamma_H1_tot"]).max()) or 1.0
-        keys = ("rays_moved", "segments", "rays_dumped", "rays_terminated", "rays_merged_away",
-                "rays_split", "rays_end")
         out["fp64"] = {"value": round(n3 * steps / wall_f, 1), "ms_per_step": round(1e3 * wall_f / steps, 3),
                        "roofline": roofline(wl, st_f, kt_f, False),
                        "step_roofline": step_roofline(wl, st_f, wall_f, False)}
         out["mixed_over_fp64"] = round(wall_f / wall_m, 3)
-        rx = np.abs(xm.astype(np.float64) - xf) / np.abs(xf.astype(np.float64))
-        first_diff = next((k for k, (sm, sf) in enumerate(zip(st_m, st_f))
-                           if any(sm[q] != sf[q] for q in keys)), None)
-        out["errors_vs_fp64"] = {
-            "x_HI_max_rel": rel(xm, xf),
-            "x_HI_rel_p99_9": float(np.percentile(rx, 99.9)),
-            "x_HI_rel_median": float(np.median(rx)),
-            "cells_x_HI_rel_over_1e-3": int((rx > 1e-3).sum()),
-            "first_step_with_count_difference": first_diff,
-            "ionized_volume_fraction_abs": abs(float((xm < 0.5).mean()) - float((xf < 0.5).mean())),
-            "Gamma_HI_max_abs_over_max": float(np.abs(state_m["gamma_H1_tot"].astype(np.float64) -
-                                                      state_f["gamma_H1_tot"]).max()) / gmax,
-            "T_max_rel": rel(state_m["temp"], state_f["temp"]),
-            "ray_counts_exact": all(all(sm[k] == sf[k] for k in keys) for sm, sf in zip(st_m, st_f)),
-        }
+        out["errors_vs_fp64"] = errors_against(state_m, st_m, state_f, st_f)
+        # the same measures for the FP64 form with every stored opacity moved by
+        # one float ulp after each pass: the spread rounding at the reference's
+        # float storage causes by itself
+        state_d, st_d = final_state(wl, 0, steps, dither=1)
+        out["fp64_rounding_floor"] = dict(errors_against(state_d, st_d, state_f, st_f),
+                                          perturbation="one float ulp on every stored opacity after each pass")
+        log("fp64 rounding floor: %s" % out["fp64_rounding_floor"])
     if a.adjoint:
         out["adjoint"] = time_adjoint()
     if files and "driver_ops" in files:
