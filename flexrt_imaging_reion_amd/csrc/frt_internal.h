@@ -150,6 +150,8 @@ struct frt_ctx {
   int32_t* cand_flag;  // [cap+1]
   int32_t* cand_off;   // [cap+1]
   uint32_t* cand_key_tmp;  // [cap]
+  int32_t* moved_idx;  // [cap] rays that moved this step (seg_cnt > 0), in list order
+  int64_t nmoved;
   bool have_rho;       // rho was uploaded or read from a gas file
   double* otf_buf;     // write_otf partial sums (allocated at first use)
   void* otf_scratch;   // device address of this device's static scratch (frt_otf_averages_noalloc)

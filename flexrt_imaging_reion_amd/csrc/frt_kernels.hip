@@ -95,21 +95,24 @@ static inline unsigned chunk_grid(int64_t n, int chunk) {
   return (unsigned)(g < (int64_t)MAX_GRID ? g : (int64_t)MAX_GRID);
 }
 
-// Calls body(i, seg_cnt[i]) for every ray i < nrays with seg_cnt[i] > 0, in
-// list order (grid-stride).
+// Calls body(i, seg_cnt[i]) for every moved ray i = moved[t], t < nmoved,
+// in list order (grid-stride).  The moved rays (seg_cnt > 0) are listed once
+// per step, so no lane waits on a ray that did not move.
 template <typename Body>
-__device__ __forceinline__ void for_rays(const int32_t* __restrict__ seg_cnt, int64_t nrays, Body&& body) {
-  for (int64_t i = xcd_block() * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
-    const int c = seg_cnt[i];
-    if (c > 0) body(i, c);
+__device__ __forceinline__ void for_rays(const int32_t* __restrict__ seg_cnt, const int32_t* __restrict__ moved,
+                                         int64_t nmoved, Body&& body) {
+  for (int64_t t = xcd_block() * BLK + threadIdx.x; t < nmoved; t += (int64_t)gridDim.x * BLK) {
+    const int64_t i = moved[t];
+    body(i, seg_cnt[i]);
   }
 }
 
-// Calls body(i, seg_cnt[i]) for every ray i < nrays with seg_cnt[i] > 0, in
-// decreasing segment count within each chunk.  Every thread of the block
-// must reach the call.
+// Calls body(i, seg_cnt[i]) for every moved ray i = moved[t], t < nmoved, in
+// decreasing segment count within each chunk of the moved list.  Every
+// thread of the block must reach the call.
 template <int CHUNK, typename Body>
-__device__ __forceinline__ void for_rays_by_count(const int32_t* __restrict__ seg_cnt, int64_t nrays,
+__device__ __forceinline__ void for_rays_by_count(const int32_t* __restrict__ seg_cnt,
+                                                  const int32_t* __restrict__ moved, int64_t nrays,
                                                   Body&& body) {
   static_assert(CHUNK % BLK == 0 && CHUNK <= 32768, "chunk of whole blocks, int16 local indices");
   constexpr int PER = CHUNK / BLK;
@@ -122,8 +125,8 @@ __device__ __forceinline__ void for_rays_by_count(const int32_t* __restrict__ se
     int bkt[PER], rank[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int64_t i = base + k * BLK + threadIdx.x;
-      const int c = i < nrays ? seg_cnt[i] : 0;
+      const int64_t t = base + k * BLK + threadIdx.x;
+      const int c = t < nrays ? seg_cnt[moved[t]] : 0;
       bkt[k] = c <= 0 ? DEP_BUCKETS - 1 : (c >= DEP_BUCKETS - 1 ? 0 : DEP_BUCKETS - 1 - c);
       rank[k] = atomicAdd(&s_cnt[bkt[k]], 1);
     }
@@ -147,7 +150,7 @@ __device__ __forceinline__ void for_rays_by_count(const int32_t* __restrict__ se
     for (int k = 0; k < PER; ++k) {
       const int pos = k * BLK + threadIdx.x;
       if (pos < nact) {
-        const int64_t i = base + s_idx[pos];
+        const int64_t i = moved[base + s_idx[pos]];
         body(i, seg_cnt[i]);
       }
     }
@@ -416,6 +419,22 @@ __global__ void __launch_bounds__(BLK) k_tape_pair(const float* __restrict__ a, 
   }
 }
 
+// Moved-ray flags (seg_cnt > 0) and their compaction into the list of moved
+// rays, in list order.
+__global__ void __launch_bounds__(BLK) k_moved_flags(const int32_t* __restrict__ seg_cnt, int64_t n,
+                                                     int32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i < n) flag[i] = seg_cnt[i] > 0 ? 1 : 0;
+  if (i == n) flag[n] = 0;
+}
+
+__global__ void __launch_bounds__(BLK) k_moved_scatter(const int32_t* __restrict__ flag,
+                                                       const int32_t* __restrict__ off, int64_t n,
+                                                       int32_t* __restrict__ moved) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i < n && flag[i]) moved[off[i]] = (int32_t)i;
+}
+
 __global__ void __launch_bounds__(BLK) k_fill_i32(int32_t* a, int64_t n, int32_t v) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i < n) a[i] = v;
@@ -580,10 +599,18 @@ __device__ __forceinline__ void bin_deposit_mixed(const GridConst& g, float dth,
       fh = dh ? 1.f : 0.f;
       fhe = dh ? 0.f : 1.f;
     } else {
-      float w, unused;
-      frt_exp_expm1_neg(fabsf(dth - dthe), w, unused);  // e^-|dth - dthe|
-      const float a = dth >= dthe ? omh : omh * w;
-      const float b = dth >= dthe ? omhe * w : omhe;
+      // shallower than 64 both exponentials and the products are normal
+      // floats and a = omh ehe, b = omhe eh directly; deeper, the ratio form
+      float a, b;
+      if (fmaxf(dth, dthe) < 64.f) {
+        a = omh * ehe;
+        b = omhe * eh;
+      } else {
+        float w, unused;
+        frt_exp_expm1_neg(fabsf(dth - dthe), w, unused);  // e^-|dth - dthe|
+        a = dth >= dthe ? omh : omh * w;
+        b = dth >= dthe ? omhe * w : omhe;
+      }
       const float den = a + b;
       if (den > 0.f) {
         const float inv = __builtin_amdgcn_rcpf(den);  // 1 ulp
@@ -681,7 +708,8 @@ __device__ __forceinline__ bool deposit_segment_fp64(const GridConst& g, Contrib
 // FP64 form: the reference's double locals and libm exp (through detmath),
 // with the per-cell accumulation in double.
 template <bool LAST, int NFC = 0>
-__global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, GridConst g,
+__global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, const int32_t* __restrict__ moved,
+                                                      int64_t nmoved, GridConst g,
                                                       const int32_t* __restrict__ path_cell,
                                                       const float* __restrict__ path_dist,
                                                       const int32_t* __restrict__ seg_cnt,
@@ -690,7 +718,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, int64_t nrays, 
                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
                                                       DevCounters* cnt, uint8_t* __restrict__ tape_np) {
   unsigned long long ndumped = 0, nremoved = 0;
-  for_rays_by_count<DEP_CHUNK_FP64>(seg_cnt, nrays, [&](int64_t i, int c) {
+  for_rays_by_count<DEP_CHUNK_FP64>(seg_cnt, moved, nmoved, [&](int64_t i, int c) {
     bool dumped_flag = false, removed = false;
     {
       const int F = NFC > 0 ? NFC : g.nfreq;
@@ -808,7 +836,8 @@ __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, Contri
 // Mixed form: FP32 attenuation and photon bookkeeping; per-cell sums are
 // compensated (Kahan) in the cell kernel.
 template <bool LAST, int NFC = 0>
-__global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays, GridConst g,
+__global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, const int32_t* __restrict__ moved,
+                                                       int64_t nmoved, GridConst g,
                                                        const int32_t* __restrict__ path_cell,
                                                        const float* __restrict__ path_dist,
                                                        const int32_t* __restrict__ seg_cnt,
@@ -820,7 +849,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, int64_t nrays,
   // 1/(dt vol) in units of phot_norm photons
   const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
   const float nmin = (float)g.nphot_min;
-  for_rays(seg_cnt, nrays, [&](int64_t i, int c) {
+  for_rays(seg_cnt, moved, nmoved, [&](int64_t i, int c) {
     bool dumped_flag = false, removed = false;
     {
       const int F = NFC > 0 ? NFC : g.nfreq;
@@ -1881,6 +1910,14 @@ int launch_segments(frt_ctx* ctx) {
   ctx->nseg = nseg;
   rc = grow_segments(ctx, nseg > 0 ? nseg : 1);
   if (rc) return rc;
+  // the moved rays, for the deposit passes (cand_flag / cand_off are free
+  // until the merge)
+  FRT_LAUNCH(ctx, "moved_list", grid_for(n + 1), k_moved_flags, ctx->seg_cnt, n, ctx->cand_flag);
+  rc = exclusive_scan(ctx, ctx->cand_flag, ctx->cand_off, n + 1, "scan_moved");
+  if (rc) return rc;
+  rc = read_i32(ctx, ctx->cand_off + n, &ctx->nmoved);
+  if (rc) return rc;
+  FRT_LAUNCH(ctx, "moved_list", grid_for(n), k_moved_scatter, ctx->cand_flag, ctx->cand_off, n, ctx->moved_idx);
   FRT_LAUNCH(ctx, "fill", grid_for(ctx->g.n3), k_fill_i32, ctx->cell_start, (int64_t)ctx->g.n3, 0);
   FRT_LAUNCH(ctx, "fill", grid_for(ctx->g.n3), k_fill_i32, ctx->cell_end, (int64_t)ctx->g.n3, 0);
   if (nseg == 0) return FRT_OK;
@@ -1905,6 +1942,7 @@ __global__ void __launch_bounds__(BLK) k_tape_nph(RaySoA rb, int64_t n, int F, i
 int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   const int I = ctx->cfg.itercount;
   const int64_t n = ctx->nrays;
+  const int64_t nm = ctx->nmoved;
   const bool mixed = ctx->cfg.precision == FRT_PREC_MIXED;
   GridConst g = ctx->g;
   RaySoA rb = cur_buf(ctx);
@@ -1937,24 +1975,25 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
 #define FRT_DEP(NAME, GRID, KERN, L1, SPEC)                                                         \
   do {                                                                                              \
     if ((SPEC) && F == 5)                                                                           \
-      FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1, 5>), rb, n, g, ctx->path_cell, ctx->path_dist,          \
+      FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1, 5>), rb, ctx->moved_idx, nm, g, ctx->path_cell, ctx->path_dist, \
                  ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt,    \
                  tnp);                                                                              \
     else                                                                                            \
-      FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1>), rb, n, g, ctx->path_cell, ctx->path_dist,             \
+      FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1>), rb, ctx->moved_idx, nm, g, ctx->path_cell, ctx->path_dist, \
                  ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt,    \
                  tnp);                                                                              \
   } while (0)
     if (mixed) {
-      if (last)
-        FRT_DEP("deposit_mixed", grid_for(n), k_deposit_mixed, true, false);
-      else
-        FRT_DEP("deposit_mixed", grid_for(n), k_deposit_mixed, false, false);
+      if (last) {
+        FRT_DEP("deposit_mixed", grid_for(nm), k_deposit_mixed, true, false);
+      } else {
+        FRT_DEP("deposit_mixed", grid_for(nm), k_deposit_mixed, false, false);
+      }
     } else {
       if (last)
-        FRT_DEP("deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64, true, true);
+        FRT_DEP("deposit_fp64", chunk_grid(nm, DEP_CHUNK_FP64), k_deposit_fp64, true, true);
       else
-        FRT_DEP("deposit_fp64", chunk_grid(n, DEP_CHUNK_FP64), k_deposit_fp64, false, true);
+        FRT_DEP("deposit_fp64", chunk_grid(nm, DEP_CHUNK_FP64), k_deposit_fp64, false, true);
     }
 #undef FRT_DEP
     // algorithmic bytes of this deposit launch (model used for the roofline):
