@@ -119,9 +119,13 @@ def run_form(wl, precision, steps, warmup, profile, files_dir=None, driver_ops=F
 
     cfg = dict(wl["cfg"])
     cfg["precision"] = precision
+    free0, total = torch.cuda.mem_get_info()
     g = FlexRT(cfg)
     inputs.setup(g, wl)
-    log("precision %d: context ready, %d warmup steps" % (precision, warmup))
+    free1, _ = torch.cuda.mem_get_info()
+    run_form.device_memory_gb = round((free0 - free1) / 1e9, 2)
+    log("precision %d: context ready (%.1f GB of device memory), %d warmup steps"
+        % (precision, (free0 - free1) / 1e9, warmup))
     for _ in range(warmup):
         g.step()
 
@@ -613,6 +617,8 @@ def main():
         "ms_per_step_with_launch_events": round(1e3 * wp_m / steps, 3),
         "rays_end": int(st_m[-1]["rays_end"]),
         "segments_per_step": int(np.mean([s["segments"] for s in st_m])),
+        "ray_segments_per_s": round(float(np.sum([s["segments"] for s in st_m])) / wall_m, 1),
+        "device_memory_GB": getattr(run_form, "device_memory_gb", None),
     }
     if not a.no_fp64:
         wall_f, st_f, kt_f, state_f, wp_f, _ = run_form(wl, 0, steps, a.warmup, True)

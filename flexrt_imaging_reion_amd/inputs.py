@@ -240,6 +240,11 @@ def make_workload(name, n=None, n_src=None, steps=None, precision=0, seed_offset
                zz=w["z"], dt=w["dt_myr"] * 1e6 * YR_TO_S, hh=h, omega_m=COSMO["omega_m"],
                omega_l=1.0 - COSMO["omega_m"], seed=1000 + seed)
     cfg["path_length"] = path_length_for(cfg)
+    # path entries are indexed in int32: capacity x path_length stays below 2^31
+    # (at 256^3 the reference's 30 n^3 slots would not)
+    limit = (2 ** 31 - 1) // cfg["path_length"] - 1
+    if 30 * n3 > limit:
+        cfg["ray_capacity"] = limit
     if w["sigma_ln"] is None:
         opd = np.ones((nn, nn, nn))
     else:
