@@ -65,6 +65,11 @@ _PROTOS = {
     "frt_read_gas_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),
     "frt_write_treion_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),
     "frt_read_treion_file": (C.c_int, [_CTX, C.c_char_p, C.POINTER(FrtFileExtras)]),
+    "frt_adj_run_begin": (C.c_int, [_CTX]),
+    "frt_adj_seed": (C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "frt_adj_step_reverse": (C.c_int, [_CTX]),
+    "frt_adj_run_finish": (C.c_int, [_CTX, C.c_float, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "frt_set_profiling": (C.c_int, [_CTX, C.c_int32]),
     "frt_kernel_time": (C.c_int, [_CTX, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "frt_reset_kernel_times": (C.c_int, [_CTX]),

@@ -91,3 +91,108 @@ def log_luminosity_gradient(grad_nphot, rays, src_cells):
     per_ray = np.sum(g * nph, axis=1)
     return np.array([float(per_ray[new & (cell == c)].sum()) for c in np.asarray(src_cells)])
 
+
+
+# Fields of the initial gas state (inputs.init_gas) that scale with the density.
+INITIAL_DENSITY_FIELDS = DENSITY_FIELDS + ("nH1_prev",)
+
+
+class Trajectory:
+    """A run of `steps` RT steps from a workload's initial state, as a function
+    of the source luminosities, a uniform clumping factor and the per-cell
+    density, with its gradient from the library's adjoint across steps.
+
+    The forward keeps a device checkpoint every K steps.  The reverse sweep
+    walks the blocks backwards; for each step it returns to the block's
+    checkpoint, recomputes the march and the steps up to it, runs the step
+    with the tape on and reverses it (frt_adj_step_reverse): the thermal
+    update, find_avg_unu's five passes (chemistry and attenuation march with
+    their paths and dump decisions held), split and compaction, and the cast,
+    which hands the adjoint of each new ray's photons to its source.  The
+    split, dump, survival and thermal sub-step decisions are held at their
+    forward values; a run whose steps merge rays is refused."""
+
+    def __init__(self, wl, precision=0, K=5):
+        from .rt import FlexRT
+
+        self.wl = wl
+        self.K = int(K)
+        self.steps = int(wl["steps"])
+        self.rt = FlexRT(dict(wl["cfg"], precision=precision))
+        self.n3 = int(wl["cfg"]["n"]) ** 3
+        self.ck = {}
+
+    def close(self):
+        for c in self.ck.values():
+            c.free()
+        self.ck = {}
+        self.rt.close()
+
+    def _setup(self, log_L, log_C, log_rho):
+        from . import inputs
+
+        wl = dict(self.wl)
+        scale = np.exp(np.asarray(log_rho, dtype=np.float64))
+        f = {}
+        for k, v in wl["fields"].items():
+            if k in INITIAL_DENSITY_FIELDS:
+                v = (np.asarray(v, dtype=np.float64) * scale).astype(np.float32)
+            f[k] = v
+        f["clump"] = np.full(self.n3, np.float32(np.exp(float(log_C))), dtype=np.float32)
+        wl["fields"] = f
+        wl["src_lum"] = np.exp(np.asarray(log_L, dtype=np.float64))
+        inputs.setup(self.rt, wl)
+
+    def run(self, log_L, log_C, log_rho, record=False):
+        """x_HI = nH1/nH after the steps (float64 of the stored floats) and nH."""
+        rt = self.rt
+        for c in self.ck.values():
+            c.free()
+        self.ck = {}
+        self._setup(log_L, log_C, log_rho)
+        rt.adj_enable(record)
+        for s in range(self.steps):
+            if record and s % self.K == 0:
+                self.ck[s] = rt.checkpoint()
+            rt.step()
+        nH = rt.download_field("nH").astype(np.float64)
+        return rt.download_field("nH1").astype(np.float64) / nH, nH
+
+    def gradient(self, a_nH1, a_nH):
+        """Adjoint of the recorded run for the seed (a_nH1, a_nH) of its final
+        state: d/dL per source, d/dC per cell, d/dlog rho per cell."""
+        rt = self.rt
+        rt.adj_run_begin()
+        rt.adj_seed(a_nH1, a_nH)
+        for b in sorted(self.ck, reverse=True):
+            for s in reversed(range(b, min(b + self.K, self.steps))):
+                rt.restore(self.ck[b])
+                for _ in range(b, s):
+                    rt.step()
+                rt.step()
+                rt.adj_step_reverse()
+        rt.restore(self.ck[0])
+        return rt.adj_run_finish(np.float32(self.wl["cfg"]["dt"]))
+
+
+class XHIAfterRun(torch.autograd.Function):
+    """x_HI after a Trajectory's steps as a function of (log L per source,
+    log C, log rho per cell)."""
+
+    @staticmethod
+    def forward(ctx, log_L, log_C, log_rho, traj):
+        x, nH = traj.run(log_L.detach().cpu().numpy(), float(log_C.detach()), log_rho.detach().cpu().numpy(),
+                         record=True)
+        ctx.traj = traj
+        ctx.x, ctx.nH = x, nH
+        ctx.L = np.exp(log_L.detach().cpu().numpy().astype(np.float64))
+        ctx.C = float(np.exp(float(log_C.detach())))
+        return torch.from_numpy(x)
+
+    @staticmethod
+    def backward(ctx, grad_x):
+        gx = grad_x.detach().cpu().numpy().astype(np.float64)
+        g = ctx.traj.gradient(gx / ctx.nH, -gx * ctx.x / ctx.nH)
+        return (torch.from_numpy(g["lum"] * ctx.L), torch.tensor(float(np.sum(g["clump"])) * ctx.C,
+                                                                 dtype=torch.float64),
+                torch.from_numpy(g["log_rho"]), None)

@@ -465,6 +465,18 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->adj_tape = nullptr;
   ctx->adj_nph = nullptr;
   ctx->adj_np = nullptr;
+  ctx->adj_tstate = nullptr;
+  ctx->adj_raydt = nullptr;
+  ctx->adj_raydt_cap = 0;
+  ctx->split_gen = 0;
+  ctx->adj_split_gen = -1;
+  ctx->adj_cast_base = -1;
+  ctx->adj_split_n = ctx->adj_split_total = -1;
+  ctx->adjr_cell = ctx->adjr_ray = ctx->adjr_ray2 = ctx->adjr_lum = ctx->adjr_work = nullptr;
+  ctx->adjr_inv = nullptr;
+  ctx->adjr_nray = 0;
+  ctx->adjr_step = 0;
+  ctx->adjr_on = false;
 
   int rc = FRT_OK;
   for (int b = 0; b < 2 && rc == FRT_OK; ++b) rc = alloc_rays(ctx, ctx->buf[b], cap, c.nfreq);
@@ -548,6 +560,9 @@ int frt_destroy(frt_ctx* ctx) {
   hipFree(ctx->adj_tape);
   hipFree(ctx->adj_nph);
   hipFree(ctx->adj_np);
+  hipFree(ctx->adj_tstate); hipFree(ctx->adj_raydt);
+  hipFree(ctx->adjr_cell); hipFree(ctx->adjr_ray); hipFree(ctx->adjr_ray2); hipFree(ctx->adjr_lum);
+  hipFree(ctx->adjr_work); hipFree(ctx->adjr_inv);
   hipFree(ctx->src_cell); hipFree(ctx->src_lum); hipFree(ctx->src_ray_dt);
   hipFree(ctx->rot_fwd); hipFree(ctx->rot_inv); hipFree(ctx->dcnt); hipFree(ctx->dsmall);
   if (ctx->hcnt) hipHostFree(ctx->hcnt);

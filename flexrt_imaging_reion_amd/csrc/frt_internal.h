@@ -164,6 +164,25 @@ struct frt_ctx {
   double* adj_tape;    // [itercount][4][n3] + [2][n3]
   float* adj_nph;      // [capacity][nfreq] photons at the start of the step
   uint8_t* adj_np;     // [itercount][capacity] per pass: segments processed, | 0x80 if dumped
+  // multi-step adjoint: what the taped step read before overwriting it, and
+  // the adjoint carried from step to step (frt_adj_run_*)
+  float* adj_tstate;        // [5][n3] temp, heat, cool, nH1_prev, ne at the start of the taped step
+  float* adj_raydt;         // [adj_raydt_cap] ray_dt of the taped step's cast
+  int32_t adj_raydt_cap;
+  int64_t split_gen;        // compactions so far (the taped step's split mapping is live until the next)
+  int64_t adj_split_gen;
+  int64_t adj_cast_base;    // first ray of that cast (-1: no cast taped)
+  int64_t adj_split_n;      // rays before and after the taped step's split (-1: no split taped)
+  int64_t adj_split_total;
+  double* adjr_cell;        // [ADJR_NCELL][n3] carried state and parameter adjoints (internal cell order)
+  double* adjr_ray;         // [capacity][nfreq] adjoint of the ray photons (stored units)
+  double* adjr_ray2;        // [capacity][nfreq]
+  double* adjr_lum;         // [nsrc] adjoint of the source photon rates
+  double* adjr_work;        // per-step work arrays ([ADJR_NWORK][n3] + [2 nfreq][n3])
+  int32_t* adjr_inv;        // [capacity] inverse of the compaction order
+  int64_t adjr_nray;        // rays the ray adjoint refers to
+  int64_t adjr_step;        // step count of the state the carried adjoint refers to
+  bool adjr_on;
   bool have_segments;  // update_rays ran and find_avg_unu has not yet consumed the paths
   bool have_alive;     // find_avg_unu ran in this step (alive flags valid)
   int64_t nseg;

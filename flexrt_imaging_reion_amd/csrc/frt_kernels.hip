@@ -916,89 +916,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, const int32_t*
 }
 
 
-// ---------------------------------------------------------------------------
-// update_thermal for one cell (gas_funcs.cc:362-419 with set_Treion_default
-// :104-129, update_heat_cool :134-242, get_thermal_tstep :353-358), subgrid off.
-struct ThermalIn {
-  float nH, nHe, nH1, nH2, nHe2, ne, ntot, dne, fH2, ah1, ahe1, nH1_prev;
-  float ddelta;   // ddelta_dt (0 until a hydro-step reload sets it)
-  float rho_rel;  // rho / avg_rho (float division)
-};
-
-__device__ __forceinline__ void thermal_density_terms(const CellPtrs& cf, const GridConst& g, int64_t k,
-                                                      ThermalIn& in) {
-  in.ddelta = cf.f[C_DDELTA_DT][k];
-  in.rho_rel = in.ddelta != 0.f ? cf.f[C_RHO][k] / g.avg_rho : 1.f;
-}
-
-__device__ __forceinline__ int thermal_cell(const GridConst& g, const ThermalIn& in,
-                                            const float* gh1nu, const float* ghe1nu,
-                                            int64_t n3, int64_t k, float& temp, float& heat,
-                                            float& cool, float& if_speed, float& inc_flux) {
-  // set_Treion_default
-  const float fion_new = (in.nH - in.nH1) / in.nH;
-  const float fion_old = (in.nH - in.nH1_prev) / in.nH;
-  const float deltax = g.dx * (fion_new - fion_old);
-  const float vIF = deltax / g.dt;
-  const float Cn[5] = {9.5432f, -1.6441e-2f, -9.8010e-3f, 4.1664e-3f, -2.2710e-4f};
-  const double L = dm_log((double)vIF / 1e5);
-  float lnT = 0.f;
-  for (int t = 0; t < 5; ++t) lnT = (float)((double)lnT + (double)Cn[t] * dm_pow(L, (double)t));
-  if ((fion_new > fion_old) && ((double)fion_new < 0.98))
-    temp = (fion_old * temp + (fion_new - fion_old) * dm_expf(lnT)) / fion_new;
-  const float vpos = frt_maxd(0.f, vIF);
-  if_speed = vpos;
-  inc_flux = (float)((1. + (double)g.nhe_to_nh) * (double)in.nH * (double)vpos);
-
-  const double k23 = 2. / 3. / FRT_K_B / (double)in.ntot;
-  const double Hd = (double)g.hubble;
-  const float dne_over_n = in.dne / in.ntot;
-  const float min_dt = (float)(0.1 * (double)g.dt);
-  const bool photoheat = (double)in.fH2 > 0.98;
-  float t_temp = 0.f;
-  int nsub = 0;
-  while (t_temp < g.dt) {
-    // get_thermal_tstep uses the heating/cooling of the previous sub-step
-    const float dTdt = (float)(k23 * (double)(heat - cool) - (2. * Hd + (double)dne_over_n) * (double)temp);
-    const float tstep = frt_maxd(min_dt, frt_mind((float)(0.1 * (double)temp / (double)frt_absd(dTdt)), g.dt));
-    const float dt_temp = frt_mind(tstep, g.dt - t_temp);
-    // update_heat_cool
-    heat = 0.f;
-    cool = 0.f;
-    if (photoheat) {
-      for (int nu = 0; nu < g.nfreq; ++nu) {
-        heat = (float)((double)heat + (double)(in.ah1 * gh1nu[nu * n3 + k]) * FRT_H_PLANCK *
-                                          (FRT_H_EV * g.nu[nu] - FRT_ETH_H1) / FRT_H_EV);
-        heat = (float)((double)heat + (double)(in.ahe1 * ghe1nu[nu * n3 + k]) * FRT_H_PLANCK *
-                                          (FRT_H_EV * g.nu[nu] - FRT_ETH_HE1) / FRT_H_EV);
-      }
-    }
-    const double Tm = (double)frt_maxd(g.temp_min, temp);
-    if (g.recomb_cool) {
-      if (g.rec_case_b) {
-        cool = (float)((double)cool + frt_rcrB_H2(Tm) * (double)in.ne * (double)in.nH2);
-        cool = (float)((double)cool + frt_rcrB_He2(Tm) * (double)in.ne * (double)in.nHe2);
-      } else {
-        cool = (float)((double)cool + frt_rcrA_H2(Tm) * (double)in.ne * (double)in.nH2);
-        cool = (float)((double)cool + frt_rcrA_He2(Tm) * (double)in.ne * (double)in.nHe2);
-      }
-    }
-    if (g.free_free) {
-      const float nff = in.nH2 + in.nHe2;  // + 4 nHe3, nHe3 = 0 in C2Ray mode
-      cool = (float)((double)cool + frt_free_free_rate(Tm) * (double)in.ne * (double)nff);
-    }
-    if (g.compton) cool = (float)((double)cool - g.compton_coef * (double)in.ne * (g.t_cmb - Tm));
-    double den = 1. + 2. * Hd * (double)dt_temp + 1. / (double)in.ntot * (double)in.dne * (double)dt_temp;
-    // - 2 ddelta_dt/3/(rho/avg_rho) dt_temp: skipped while ddelta_dt is 0, where
-    // it subtracts zero (and where rho may not have been uploaded at all)
-    if (in.ddelta != 0.f) den = den - 2. * (double)in.ddelta / 3. / (double)in.rho_rel * (double)dt_temp;
-    temp = (float)(((double)temp + k23 * (double)(heat - cool) * (double)dt_temp) / den);
-    t_temp += dt_temp;
-    if (++nsub >= 64) break;
-  }
-  temp = frt_maxd(temp, g.temp_min);
-  return nsub;
-}
+#include "frt_thermal.h"
 
 // find_avg_unu cell loop (ray_tracing.cc:941-1214, C2Ray branch 1077-1136,
 // finalize 1139-1210), one thread per cell.  FIRST also performs the
@@ -1957,6 +1875,15 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     FRT_LAUNCH(ctx, "adj_tape", gc, k_tape_pair, ctx->c.f[C_FH1], ctx->c.f[C_FHE1], tape + (size_t)4 * I * g.n3,
                (int64_t)g.n3);
     FRT_LAUNCH(ctx, "adj_tape", grid_for(n), k_tape_nph, rb, n, F, g.cap, ctx->adj_nph);
+    // the cell state the fused thermal update reads and then overwrites
+    const int tf[5] = {C_TEMP, C_HEAT, C_COOL, C_NH1_PREV, C_NE};
+    for (int q = 0; q < 5; ++q) {
+      hipError_t e = hipMemcpyAsync(ctx->adj_tstate + (size_t)q * g.n3, ctx->c.f[tf[q]], 4 * (size_t)g.n3,
+                                    hipMemcpyDeviceToDevice, ctx->stream);
+      if (e != hipSuccess) return hip_check(ctx, e, "adjoint tape");
+    }
+    ctx->adj_cast_base = -1;
+    ctx->adj_split_n = ctx->adj_split_total = -1;
   }
   g.dither_step = ctx->step;
   for (int it = 0; it < I; ++it) {
@@ -2156,6 +2083,12 @@ int launch_split_compact(frt_ctx* ctx, int64_t* n_after) {
                ctx->n_out, ctx->m_val[1], ctx->cand_key_tmp, ctx->rot_fwd);
   }
   ctx->cur = 1 - ctx->cur;
+  ctx->split_gen += 1;
+  if (ctx->adj_on && ctx->nphot_merge < INFINITY) {  // adjoint tape: the split's input and output counts
+    ctx->adj_split_n = n;
+    ctx->adj_split_total = total;
+    ctx->adj_split_gen = ctx->split_gen;
+  }
   ctx->nrays = total;
   *n_after = total;
   return FRT_OK;
@@ -2169,6 +2102,12 @@ int launch_cast(frt_ctx* ctx) {
   FRT_LAUNCH(ctx, "cast", grid_for(nnew), k_cast, cur_buf(ctx), ctx->nrays, ctx->nsrc, ctx->g,
              ctx->src_cell, ctx->src_lum, ctx->src_ray_dt, ctx->rot_fwd, ctx->cfg.seed,
              (uint64_t)ctx->cast_calls);
+  if (ctx->adj_on && ctx->adj_raydt && ctx->adj_raydt_cap >= ctx->nsrc) {  // adjoint tape: the ray_dt the cast used
+    hipError_t e = hipMemcpyAsync(ctx->adj_raydt, ctx->src_ray_dt, 4 * (size_t)ctx->nsrc, hipMemcpyDeviceToDevice,
+                                  ctx->stream);
+    if (e != hipSuccess) return hip_check(ctx, e, "adjoint tape");
+    ctx->adj_cast_base = ctx->nrays;
+  }
   FRT_LAUNCH(ctx, "zero_ray_dt", grid_for(ctx->nsrc), k_zero_ray_dt, ctx->src_ray_dt, ctx->nsrc);
   ctx->nrays += nnew;
   ctx->cast_calls += 1;

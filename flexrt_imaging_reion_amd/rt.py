@@ -131,6 +131,7 @@ class FlexRT:
         c = np.ascontiguousarray(cells, dtype=np.int32)
         lu = np.ascontiguousarray(lum, dtype=np.float64)
         assert c.size == lu.size
+        self._srcs = c.copy()
         self._check(self._lib.frt_set_sources(
             self._h, int(c.size), c.ctypes.data_as(C.POINTER(C.c_int32)),
             lu.ctypes.data_as(C.POINTER(C.c_double))), "set_sources")
@@ -340,6 +341,37 @@ class FlexRT:
                     "adj_find_avg_unu")
         out["nphot"] = nph[: nr.value * self.nfreq].reshape(nr.value, self.nfreq)
         return out
+
+    # -- adjoint across steps ---------------------------------------------------------
+    def adj_run_begin(self):
+        """Start a reverse sweep from the current state (needs adj_enable)."""
+        self._check(self._lib.frt_adj_run_begin(self._h), "adj_run_begin")
+
+    def adj_seed(self, a_nH1=None, a_nH=None):
+        """Add the adjoint of the current n_HI and nH (row-major cells)."""
+        n3 = self.n ** 3
+        P = lambda x: None if x is None else np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(n3))
+        a, b = P(a_nH1), P(a_nH)
+        ptr = lambda x: x.ctypes.data_as(C.POINTER(C.c_double)) if x is not None else None
+        self._check(self._lib.frt_adj_seed(self._h, ptr(a), ptr(b)), "adj_seed")
+
+    def adj_step_reverse(self):
+        """Reverse the step just run with the tape on."""
+        self._check(self._lib.frt_adj_step_reverse(self._h), "adj_step_reverse")
+
+    def adj_run_finish(self, init_ray_dt):
+        """Parameter adjoints once every step is reversed and the context is
+        back in its initial state: d/dL per source, d/dC and d/dlog rho per
+        row-major cell."""
+        n3 = self.n ** 3
+        nsrc = len(self._srcs) if hasattr(self, "_srcs") else 0
+        gl = np.zeros(max(nsrc, 1))
+        gc = np.zeros(n3)
+        gr = np.zeros(n3)
+        P = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        self._check(self._lib.frt_adj_run_finish(self._h, float(init_ray_dt), P(gl), P(gc), P(gr)),
+                    "adj_run_finish")
+        return {"lum": gl[:nsrc], "clump": gc, "log_rho": gr}
 
     # -- MP-Gadget coupling (io_funcs.cc:434-496) ----------------------------------------
     def update_gadget_var(self):

@@ -409,6 +409,29 @@ int frt_adj_ray_count(const frt_ctx* ctx, int64_t* n);
 int frt_adj_find_avg_unu(frt_ctx* ctx, const double* a_nh1, double* g_nphot, double* g_clump, double* g_nH,
                          double* g_nHe, double* g_nH1, double* g_nHe1);
 
+/* ---- adjoint across steps ------------------------------------------------------
+ * No counterpart in the reference: the gradient of a function of the state
+ * after several frt_step calls (the timestep loop of main.cc:108-198 with the
+ * C2Ray chemistry, thermal update and the split of every step) with respect to
+ * the source rates, the clumping factor and the density of the initial state.
+ * The split, dump, ray-survival and thermal sub-step decisions are held at
+ * their forward values; a step that merged rays is refused (FRT_E_STATE).
+ * Protocol, for a run of S steps from a saved initial state with the tape on
+ * (frt_adj_enable(ctx, 1)):
+ *   frt_adj_run_begin(ctx)            after the forward run, on its final state
+ *   frt_adj_seed(ctx, a_nH1, a_nH)    adjoint of the final nH1 and nH (row-major)
+ *   for s = S-1 .. 0: return to the state before step s (a checkpoint and
+ *     recomputation), run frt_step once, then frt_adj_step_reverse(ctx)
+ *   return to the initial state, then frt_adj_run_finish(ctx, ray_dt of the
+ *     initial cast, g_lum[nsrc], g_clump[n3], g_log_rho[n3])
+ * g_lum is per unit of each source's rate; g_log_rho scales every density of
+ * the initial state together (nH, nHe, n_HI, n_HeI, n_e, nH1_prev).  The
+ * reverse sweep runs in FP64 for both precision forms. */
+int frt_adj_run_begin(frt_ctx* ctx);
+int frt_adj_seed(frt_ctx* ctx, const double* a_nH1, const double* a_nH);
+int frt_adj_step_reverse(frt_ctx* ctx);
+int frt_adj_run_finish(frt_ctx* ctx, float init_ray_dt, double* g_lum, double* g_clump, double* g_log_rho);
+
 /* ---- measurement ----------------------------------------------------------- */
 /* When enabled, every kernel launch is bracketed by HIP events on the
  * context stream and its duration accumulated per kernel name. */
