@@ -17,7 +17,7 @@ ORACLE_SRC = os.path.join(REPO_DIR, "oracle", "frt_oracle.cc")
 ORACLE_LIB = os.path.join(REPO_DIR, "oracle", "liboracle.so")
 ORACLE_OMP_LIB = os.path.join(REPO_DIR, "oracle", "liboracle_omp.so")
 
-HIP_SOURCES = ["frt_kernels.hip", "frt_capi.hip", "frt_adjoint.hip"]
+HIP_SOURCES = ["frt_kernels.hip", "frt_capi.hip", "frt_adjoint.hip", "frt_scan_sort.hip"]
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
     # no implicit a*b+c fusion: the path must reproduce the CPU arithmetic

@@ -43,6 +43,12 @@ _PROTOS = {
     "frt_update_thermal": (C.c_int, [_CTX]),
     "frt_step": (C.c_int, [_CTX, C.POINTER(FrtStepStats)]),
     "frt_get_stats": (C.c_int, [_CTX, C.POINTER(FrtStepStats)]),
+    "frt_set_step_graph": (C.c_int, [_CTX, C.c_int32]),
+    "frt_set_tuning": (C.c_int, [_CTX, C.c_char_p, C.c_int64]),
+    "frt_get_tuning": (C.c_int, [_CTX, C.c_char_p, C.POINTER(C.c_int64)]),
+    "frt_selftest_scan_sort": (C.c_int, [_CTX, C.c_int64, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int64)]),
     "frt_ray_count": (C.c_int, [_CTX, C.POINTER(C.c_int64)]),
     "frt_download_rays": (C.c_int, [_CTX, C.POINTER(FrtRayView)]),
     "frt_upload_rays": (C.c_int, [_CTX, C.POINTER(FrtRayView)]),

@@ -783,7 +783,7 @@ int frt_adj_step_reverse(frt_ctx* ctx) {
                        (const float*)ctx->adj_raydt, (const double*)ctx->adjr_ray, ctx->adjr_lum);
   if (total > 0)
     hipLaunchKernelGGL(k_adjr_inv, dim3(gr(total)), dim3(ABLK), 0, ctx->stream, total,
-                       (const int32_t*)ctx->m_val[1], ctx->adjr_inv);
+                       (const int32_t*)ctx->split_sorted, ctx->adjr_inv);
   hipLaunchKernelGGL(k_adjr_split, dim3(gr(n_pre)), dim3(ABLK), 0, ctx->stream, n_pre, (int)F,
                      (const int32_t*)ctx->n_out, (const int32_t*)ctx->out_off, (const int32_t*)ctx->adjr_inv,
                      (const double*)ctx->adjr_ray, ctx->adjr_ray2);

@@ -90,6 +90,14 @@ void prof_resolve(frt_ctx* ctx) {
   ctx->prof_pending.clear();
 }
 
+// Drops the captured step graphs (they are captured again on the next step).
+void graph_invalidate(frt_ctx* ctx) {
+  for (int c = 0; c < 2; ++c) {
+    if (ctx->graph_valid[c]) hipGraphExecDestroy(ctx->graph_exec[c]);
+    ctx->graph_valid[c] = false;
+  }
+}
+
 }  // namespace frt
 
 using namespace frt;
@@ -148,27 +156,51 @@ int read_counters(frt_ctx* ctx) {
   return read_device_counters(ctx);
 }
 
-int reset_counters(frt_ctx* ctx) {
-  return hip_check(ctx, hipMemsetAsync(ctx->dcnt, 0, FRT_CTR_COPIES * sizeof(DevCounters), ctx->stream),
-                   "reset counters");
+// Clears the counters, queue heads and error bits on the device.
+int reset_counters(frt_ctx* ctx) { return launch_step_begin(ctx, false); }
+
+// Takes the device step scalars into the host mirror after a pull, resolves
+// the adjoint tape's pending counts, and turns the device error bits into an
+// error code.  A split that would overflow the buffer leaves the rays as
+// they were before it (the compaction and everything after it are skipped).
+int apply_pull(frt_ctx* ctx, bool ran_march) {
+  const StepDev* h = ctx->hsd;
+  ctx->nrays = h->nrays;
+  ctx->nseg = h->nseg;
+  ctx->nmoved = h->nmoved;
+  ctx->nphot_merge = h->nphot_merge;
+  ctx->t = h->t;
+  ctx->step = h->step;
+  ctx->cast_calls = h->cast_calls;
+  if (ctx->adj_split_total == -2) ctx->adj_split_total = h->nsplit;
+  if (ctx->adj_cast_base == -2) ctx->adj_cast_base = h->nrays - (int64_t)ctx->nsrc * 12;
+  if (h->err & FRT_SDERR_SPLIT_CAPACITY) {
+    ctx->cur = 1 - ctx->cur;  // the compaction did not run
+    ctx->split_gen -= 1;
+    ctx->adj_split_n = ctx->adj_split_total = -1;
+    return set_error(ctx, FRT_E_CAPACITY,
+                     "ray buffer full: " + std::to_string(h->nsplit + (int64_t)ctx->nsrc * 12) +
+                         " rays needed, capacity " + std::to_string(ctx->g.cap));
+  }
+  if (h->err & FRT_SDERR_CAST_CAPACITY) {
+    ctx->adj_cast_base = -1;
+    return set_error(ctx, FRT_E_CAPACITY, "ray buffer full in new_rays");
+  }
+  if (ran_march && ctx->hcnt->path_overflow)
+    return set_error(ctx, FRT_E_PATH_OVERFLOW,
+                     "a ray crossed more than path_length cells in one step; raise path_length");
+  return FRT_OK;
 }
 
-// Rays held after removals: drop dead rays (no splits) so the buffer holds live rays only.
+// Rays held after removals: drop dead rays (no splits) so the buffer holds
+// live rays only.  Ends with a pull, so the host counts are current.
 int ensure_compact(frt_ctx* ctx) {
   if (!ctx->have_alive) return FRT_OK;
-  const int64_t n = ctx->nrays;
-  std::vector<int32_t> dummy;
-  // reuse the split path with splitting disabled: threshold +inf
-  const float saved_merge = ctx->nphot_merge;
-  const double saved_min = ctx->nphot_min;
-  ctx->nphot_merge = INFINITY;
-  int64_t after = 0;
-  int rc = launch_split_compact(ctx, &after);
-  ctx->nphot_merge = saved_merge;
-  ctx->nphot_min = saved_min;
+  int rc = launch_split_compact(ctx, true);
   ctx->have_alive = false;
-  (void)n;
-  return rc;
+  if (rc) return rc;
+  if ((rc = read_counters(ctx))) return rc;
+  return apply_pull(ctx, false);
 }
 
 int check_ready(frt_ctx* ctx) {
@@ -177,20 +209,13 @@ int check_ready(frt_ctx* ctx) {
   return FRT_OK;
 }
 
+// update_rays: ray_dt, march, segment bookkeeping (launches only).
 int do_update_rays(frt_ctx* ctx) {
-  int rc = ensure_compact(ctx);
-  if (rc) return rc;
-  ctx->g.t = ctx->t;
   ctx->stats.rays_start = ctx->nrays;
+  int rc;
   if ((rc = launch_add_ray_dt(ctx))) return rc;
   if ((rc = launch_march(ctx))) return rc;
   if ((rc = launch_segments(ctx))) return rc;
-  if ((rc = read_counters(ctx))) return rc;
-  ctx->stats.rays_moved = (int64_t)ctx->hcnt->moved;
-  ctx->stats.segments = ctx->nseg;
-  if (ctx->hcnt->path_overflow)
-    return set_error(ctx, FRT_E_PATH_OVERFLOW,
-                     "a ray crossed more than path_length cells in one step; raise path_length");
   ctx->have_segments = true;
   ctx->have_alive = true;
   return FRT_OK;
@@ -198,7 +223,6 @@ int do_update_rays(frt_ctx* ctx) {
 
 int do_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   if (!ctx->have_segments) return set_error(ctx, FRT_E_STATE, "find_avg_unu needs update_rays first");
-  ctx->g.t = ctx->t;
   ctx->g.nphot_min = ctx->nphot_min;
   int rc = launch_find_avg_unu(ctx, fuse_thermal);
   if (rc) return rc;
@@ -209,10 +233,8 @@ int do_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
 
 int fill_alive(frt_ctx* ctx) {
   if (ctx->have_alive) return FRT_OK;
-  if (ctx->nrays > 0) {
-    int rc = hip_check(ctx, hipMemsetAsync(ctx->alive, 1, (size_t)ctx->nrays, ctx->stream), "alive");
-    if (rc) return rc;
-  }
+  int rc = launch_fill_alive(ctx);
+  if (rc) return rc;
   ctx->have_alive = true;
   return FRT_OK;
 }
@@ -227,18 +249,44 @@ int do_merge(frt_ctx* ctx) {
 int do_split(frt_ctx* ctx) {
   int rc = fill_alive(ctx);
   if (rc) return rc;
-  int64_t after = 0;
-  rc = launch_split_compact(ctx, &after);
+  rc = launch_split_compact(ctx, false);
   ctx->have_alive = false;
   return rc;
 }
 
-int finish_stats(frt_ctx* ctx) {
+// Algorithmic bytes of the last find_avg_unu's deposit launches (the
+// roofline model): per ray seg_cnt (4) + nphot read (4F); per segment path
+// cell + dist (8) + slot (4) + avg_nh1/avg_nhe1 (8) + the 2-value deposit
+// (first passes) or the (F+1)-value photon record (last pass); the last
+// pass adds the ray write-back (4F + 4 + 1).
+static void deposit_model(frt_ctx* ctx) {
+  const int64_t I = ctx->deposit_launches_last;
+  ctx->deposit_bytes_last = 0.;
+  if (I <= 0) return;
+  const int F = ctx->g.nfreq;
+  const double acc = ctx->cfg.precision == FRT_PREC_MIXED ? 4. : 8.;
+  const double n = (double)ctx->stats.rays_start, moved = (double)ctx->stats.rays_moved, nseg = (double)ctx->nseg;
+  for (int64_t it = 0; it < I; ++it) {
+    const bool last = it == I - 1;
+    double b = n * 4. + moved * (4. * F) + nseg * 20.;
+    b += nseg * (last ? (F + 1.) : 2.) * acc;
+    if (last) b += moved * (4. * F + 5.);
+    ctx->deposit_bytes_last += b;
+  }
+}
+
+// Pull after an entry point: host mirror, errors, statistics.
+int finish_stats(frt_ctx* ctx, bool ran_march = false) {
   int rc = read_counters(ctx);
   if (rc) return rc;
+  rc = apply_pull(ctx, ran_march);
   DevCounters* h = ctx->hcnt;
+  ctx->stats.rays_moved = (int64_t)h->moved;
+  ctx->stats.segments = ctx->nseg;
   ctx->stats.rays_dumped = (int64_t)h->dumped;
   ctx->stats.rays_terminated = (int64_t)h->terminated;
+  ctx->stats.rays_counted = (int64_t)h->alive;
+  ctx->stats.merge_candidates = ctx->hsd->ncand;
   ctx->stats.merge_groups = (int64_t)h->groups;
   ctx->stats.rays_merged_away = (int64_t)h->merged_away;
   ctx->stats.rays_split = (int64_t)h->splits;
@@ -249,7 +297,101 @@ int finish_stats(frt_ctx* ctx) {
   ctx->stats.nphot_merge = ctx->nphot_merge;
   ctx->stats.t = ctx->t;
   ctx->stats.step = ctx->step;
+  deposit_model(ctx);
   if (ctx->prof) prof_resolve(ctx);
+  return rc;
+}
+
+// ---- step graph -----------------------------------------------------------------
+static void graph_key(const frt_ctx* ctx, GraphKey* k) {
+  memset(k, 0, sizeof(*k));
+  memcpy(&k->g, &ctx->g, sizeof(GridConst));
+  k->nsrc = ctx->nsrc;
+  k->ptrs[0] = ctx->src_cell;
+  k->ptrs[1] = ctx->src_lum;
+  k->ptrs[2] = ctx->src_ray_dt;
+  k->ptrs[3] = ctx->seg_key[0];
+  k->ptrs[4] = ctx->seg_val[0];
+  k->ptrs[5] = ctx->contrib.rec;
+  k->seg_cap = ctx->seg_cap;
+  k->tune[0] = ctx->tune_deposit_queue;
+  k->tune[1] = ctx->tune_march_queue;
+  k->tune[2] = ctx->tune_queue_chunk;
+  k->tune[3] = ctx->tune_blocks_per_cu;
+  k->tune[4] = ctx->tune_grid_cap;
+  k->tune[5] = ctx->tune_fp64_chunk;
+  k->itercount = ctx->cfg.itercount;
+  // ray-count bucket: the smallest power of two (at least 4096) holding the
+  // rays at the start of the step, within the capacity
+  int64_t b = 4096;
+  while (b < ctx->nrays) b *= 2;
+  k->capture_rays = std::min<int64_t>(b, ctx->g.cap);
+}
+
+// The launches of one frt_step after its step_begin.
+static int step_body(frt_ctx* ctx) {
+  int rc;
+  if ((rc = do_update_rays(ctx))) return rc;
+  if ((rc = do_find_avg_unu(ctx, true))) return rc;
+  if ((rc = do_merge(ctx))) return rc;
+  if ((rc = do_split(ctx))) return rc;
+  if ((rc = launch_cast(ctx))) return rc;
+  ctx->stats.rays_new = (int64_t)ctx->nsrc * 12;
+  return launch_step_end(ctx);
+}
+
+static bool graph_eligible(const frt_ctx* ctx) {
+  return ctx->graph_on && !ctx->prof && !ctx->adj_on && !ctx->have_alive && !ctx->have_segments;
+}
+
+// One frt_step as a replay of the graph captured for this ray-buffer parity
+// (captured first if absent or stale).  Capture records the launches without
+// running them; the host-side effects of the recorded calls are undone and
+// then applied once per replay.
+static int graph_step(frt_ctx* ctx) {
+  const int c = ctx->cur;
+  GraphKey key;
+  graph_key(ctx, &key);
+  if (ctx->graph_valid[c] && memcmp(&key, &ctx->graph_key[c], sizeof(GraphKey)) != 0) {
+    hipGraphExecDestroy(ctx->graph_exec[c]);
+    ctx->graph_valid[c] = false;
+  }
+  if (!ctx->graph_valid[c]) {
+    const int64_t split_gen = ctx->split_gen;
+    const frt_step_stats stats = ctx->stats;
+    hipGraph_t graph = nullptr;
+    ctx->capturing = true;
+    ctx->capture_rays = key.capture_rays;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    int rc = (e == hipSuccess) ? launch_step_begin(ctx, true) : hip_check(ctx, e, "begin capture");
+    if (rc == FRT_OK) rc = step_body(ctx);
+    if (e == hipSuccess) {
+      const hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+      if (rc == FRT_OK && e2 != hipSuccess) rc = hip_check(ctx, e2, "end capture");
+    }
+    ctx->capturing = false;
+    ctx->cur = c;
+    ctx->split_gen = split_gen;
+    ctx->stats = stats;
+    ctx->have_alive = ctx->have_segments = false;
+    if (rc == FRT_OK) {
+      e = hipGraphInstantiate(&ctx->graph_exec[c], graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = hip_check(ctx, e, "graph instantiate");
+    }
+    if (graph) hipGraphDestroy(graph);
+    if (rc) return rc;
+    ctx->graph_valid[c] = true;
+    ctx->graph_key[c] = key;
+  }
+  int rc = push_step_scalars(ctx);
+  if (rc) return rc;
+  ctx->stats.rays_start = ctx->nrays;
+  rc = hip_check(ctx, hipGraphLaunch(ctx->graph_exec[c], ctx->stream), "graph launch");
+  if (rc) return rc;
+  ctx->cur = 1 - c;
+  ctx->split_gen += 1;
+  ctx->deposit_launches_last = ctx->cfg.itercount;
+  ctx->stats.rays_new = (int64_t)ctx->nsrc * 12;
   return FRT_OK;
 }
 
@@ -477,6 +619,23 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->adjr_nray = 0;
   ctx->adjr_step = 0;
   ctx->adjr_on = false;
+  ctx->split_sorted = nullptr;
+  ctx->graph_on = 0;
+  ctx->capturing = false;
+  ctx->graph_valid[0] = ctx->graph_valid[1] = false;
+  // launch tuning defaults: one thread per ray, no queues
+  ctx->tune_deposit_queue = 0;
+  ctx->tune_march_queue = 0;
+  ctx->tune_queue_chunk = 256;
+  ctx->tune_blocks_per_cu = 8;
+  ctx->tune_grid_cap = 1 << 20;
+  ctx->tune_fp64_chunk = 4096;
+  ctx->num_cu = 256;
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && ncu > 0)
+      ctx->num_cu = ncu;
+  }
 
   int rc = FRT_OK;
   for (int b = 0; b < 2 && rc == FRT_OK; ++b) rc = alloc_rays(ctx, ctx->buf[b], cap, c.nfreq);
@@ -511,6 +670,8 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e == hipSuccess) e = dalloc(&ctx->dsmall, 64);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hcnt, FRT_CTR_COPIES * sizeof(DevCounters), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hscalar, 512, hipHostMallocDefault);
+    if (e == hipSuccess) e = dalloc(&ctx->sd, 1);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->hsd, sizeof(StepDev), hipHostMallocDefault);
     if (e != hipSuccess) rc = set_error(ctx, FRT_E_NOMEM, std::string("allocation: ") + hipGetErrorString(e));
   }
   if (rc == FRT_OK) {
@@ -529,6 +690,8 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e == hipSuccess) e = stream_copy(ctx, ctx->rot_fwd, fwd.data(), 4 * fwd.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = stream_copy(ctx, ctx->rot_inv, inv.data(), 4 * inv.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemsetAsync(ctx->dcnt, 0, FRT_CTR_COPIES * sizeof(DevCounters), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->sd, 0, sizeof(StepDev), ctx->stream);
+    if (e == hipSuccess) memset(ctx->hsd, 0, sizeof(StepDev));
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = set_error(ctx, FRT_E_HIP, std::string("init: ") + hipGetErrorString(e));
   }
@@ -547,6 +710,10 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
 int frt_destroy(frt_ctx* ctx) {
   if (!ctx) return FRT_E_ARG;
   hipStreamSynchronize(ctx->stream);
+  graph_invalidate(ctx);
+  free_scan_sort(ctx);
+  hipFree(ctx->sd);
+  if (ctx->hsd) hipHostFree(ctx->hsd);
   for (int b = 0; b < 2; ++b) free_rays(ctx->buf[b]);
   hipFree(ctx->path_cell); hipFree(ctx->path_dist); hipFree(ctx->seg_cnt); hipFree(ctx->seg_off);
   hipFree(ctx->alive); hipFree(ctx->n_out); hipFree(ctx->out_off); hipFree(ctx->cand_flag);
@@ -1016,7 +1183,6 @@ int frt_init_rt(frt_ctx* ctx) {
   ctx->step = 0;
   ctx->cast_calls = 0;
   ctx->nphot_merge = 0.f;
-  ctx->g.nphot_merge = 0.f;
   ctx->have_alive = false;
   ctx->have_segments = false;
   if (ctx->nsrc > 0) {
@@ -1052,26 +1218,36 @@ int frt_get_nphot_min_max(frt_ctx* ctx, double* nphot_min, double* nphot_max) {
   return FRT_OK;
 }
 
+// Entry points on the step path write the host mirror of the step scalars to
+// the device first (the host may have changed time, step or rays since) and
+// pull them back at the end; nothing between reads the device.
 int frt_new_rays(frt_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = push_step_scalars(ctx))) return rc;
   if ((rc = ensure_compact(ctx))) return rc;
   const int64_t before = ctx->nrays;
   if ((rc = launch_cast(ctx))) return rc;
+  rc = finish_stats(ctx);
   ctx->stats.rays_new = ctx->nrays - before;
-  return hip_check(ctx, hipStreamSynchronize(ctx->stream), "new_rays");
+  return rc;
 }
 
 int frt_update_rays(frt_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = push_step_scalars(ctx))) return rc;
+  if ((rc = ensure_compact(ctx))) return rc;
+  if ((rc = ensure_segment_capacity(ctx))) return rc;
   if ((rc = reset_counters(ctx))) return rc;
-  return do_update_rays(ctx);
+  if ((rc = do_update_rays(ctx))) return rc;
+  return finish_stats(ctx, true);
 }
 
 int frt_find_avg_unu(frt_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = push_step_scalars(ctx))) return rc;
   if ((rc = do_find_avg_unu(ctx, false))) return rc;
   return finish_stats(ctx);
 }
@@ -1079,6 +1255,7 @@ int frt_find_avg_unu(frt_ctx* ctx) {
 int frt_merge_rays(frt_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = push_step_scalars(ctx))) return rc;
   if ((rc = do_merge(ctx))) return rc;
   return finish_stats(ctx);
 }
@@ -1086,6 +1263,7 @@ int frt_merge_rays(frt_ctx* ctx) {
 int frt_split_rays(frt_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = push_step_scalars(ctx))) return rc;
   if ((rc = do_split(ctx))) return rc;
   return finish_stats(ctx);
 }
@@ -1100,18 +1278,65 @@ int frt_update_thermal(frt_ctx* ctx) {
 int frt_step(frt_ctx* ctx, frt_step_stats* stats) {
   int rc = check_ready(ctx);
   if (rc) return rc;
-  if ((rc = reset_counters(ctx))) return rc;
-  ctx->t += ctx->g.dt;  // main.cc:123
-  if ((rc = do_update_rays(ctx))) return rc;
-  if ((rc = do_find_avg_unu(ctx, true))) return rc;
-  if ((rc = do_merge(ctx))) return rc;
-  if ((rc = do_split(ctx))) return rc;
-  const int64_t before = ctx->nrays;
-  if ((rc = launch_cast(ctx))) return rc;
-  ctx->stats.rays_new = ctx->nrays - before;
-  ctx->step += 1;
-  if ((rc = finish_stats(ctx))) return rc;
+  ctx->g.nphot_min = ctx->nphot_min;
+  if ((rc = ensure_segment_capacity(ctx))) return rc;
+  if (graph_eligible(ctx)) {
+    rc = graph_step(ctx);
+  } else {
+    rc = push_step_scalars(ctx);
+    if (rc == FRT_OK) rc = ensure_compact(ctx);
+    if (rc == FRT_OK) rc = launch_step_begin(ctx, true);  // t += dt (main.cc:123)
+    if (rc == FRT_OK) rc = step_body(ctx);
+  }
+  if (rc) return rc;
+  if ((rc = finish_stats(ctx, true))) return rc;
   if (stats) *stats = ctx->stats;
+  return FRT_OK;
+}
+
+int frt_set_step_graph(frt_ctx* ctx, int32_t on) {
+  if (!ctx) return FRT_E_ARG;
+  ctx->graph_on = on ? 1 : 0;
+  if (!on) graph_invalidate(ctx);
+  return FRT_OK;
+}
+
+int frt_set_tuning(frt_ctx* ctx, const char* key, int64_t value) {
+  if (!ctx || !key) return FRT_E_ARG;
+  const std::string k(key);
+  int32_t* slot = nullptr;
+  if (k == "deposit_queue") slot = &ctx->tune_deposit_queue;
+  else if (k == "march_queue") slot = &ctx->tune_march_queue;
+  else if (k == "queue_chunk") slot = &ctx->tune_queue_chunk;
+  else if (k == "blocks_per_cu") slot = &ctx->tune_blocks_per_cu;
+  else if (k == "grid_cap") slot = &ctx->tune_grid_cap;
+  else if (k == "fp64_chunk") slot = &ctx->tune_fp64_chunk;
+  else return set_error(ctx, FRT_E_ARG, "unknown tuning key: " + k);
+  if ((k == "deposit_queue" || k == "march_queue") && value != 0 && value != 1)
+    return set_error(ctx, FRT_E_ARG, k + " is 0 or 1");
+  if (k == "queue_chunk" && (value < 64 || value > 65536 || value % 64 != 0))
+    return set_error(ctx, FRT_E_ARG, "queue_chunk is a multiple of 64 in [64, 65536]");
+  if (k == "blocks_per_cu" && (value < 1 || value > 64))
+    return set_error(ctx, FRT_E_ARG, "blocks_per_cu is in [1, 64]");
+  if (k == "grid_cap" && (value < 64 || value > (1 << 20)))
+    return set_error(ctx, FRT_E_ARG, "grid_cap is in [64, 2^20]");
+  if (k == "fp64_chunk" && value != 2048 && value != 4096)
+    return set_error(ctx, FRT_E_ARG, "fp64_chunk is 2048 or 4096");
+  *slot = (int32_t)value;
+  return FRT_OK;
+}
+
+int frt_get_tuning(const frt_ctx* ctx, const char* key, int64_t* value) {
+  if (!ctx || !key || !value) return FRT_E_ARG;
+  const std::string k(key);
+  if (k == "deposit_queue") *value = ctx->tune_deposit_queue;
+  else if (k == "march_queue") *value = ctx->tune_march_queue;
+  else if (k == "queue_chunk") *value = ctx->tune_queue_chunk;
+  else if (k == "blocks_per_cu") *value = ctx->tune_blocks_per_cu;
+  else if (k == "grid_cap") *value = ctx->tune_grid_cap;
+  else if (k == "fp64_chunk") *value = ctx->tune_fp64_chunk;
+  else if (k == "num_cu") *value = ctx->num_cu;
+  else return FRT_E_ARG;
   return FRT_OK;
 }
 

@@ -52,10 +52,8 @@ struct GridConst {
   float dt;         // time step, s
   float min_dist;
   float temp_min;
-  float t;          // simulation time after t += dt
   double vol;       // dx*dy*dz as double
   double nphot_min;
-  float nphot_merge;
   int l_max;
   int hpx_lvl;
   int frac_limit;
@@ -67,7 +65,6 @@ struct GridConst {
   float zz;
   float avg_rho;        // the driver's volume-average rho (thermal ddelta_dt term)
   uint64_t dither_seed; // opacity dither (frt_rng.h), 0 = off
-  int64_t dither_step;  // completed steps, for the dither hash
   int pass;             // C2Ray pass of the current cell launch
   float nhe_to_nh;
   double sig912;        // sigmapi_H1(13.6/h_eV)
@@ -99,6 +96,38 @@ struct DevCounters {
 // atomic address.  The host sums the copies (read_device_counters).
 #define FRT_CTR_COPIES 16
 
+// Counts and scalars that kernels of a step produce and later kernels of the
+// same step consume, kept in device memory: no launch of a step needs a host
+// read, so a step runs without stopping the stream and can be captured as
+// one graph.  The host holds a mirror, written before and read after each
+// entry point (push_step_scalars / pull_step_scalars).
+struct StepDev {
+  int64_t nrays;       // rays in the current list
+  int64_t nseg;        // path segments of the current paths
+  int64_t nmoved;      // rays with segments
+  int64_t ncand;       // merge candidates
+  int64_t nsplit;      // rays after the split (before new_rays)
+  int64_t mcount;      // entries of the merge-candidate scan (0: no merge this step)
+  int64_t ncompact;    // rays the compaction writes (0 after a capacity error)
+  int64_t step;        // completed steps
+  uint64_t cast_calls; // new_rays calls so far
+  float t;             // simulation time
+  float nphot_merge;   // merge threshold in photons / phot_norm
+  int32_t pdf_on;      // count_rays exceeded the threshold: recompute nphot_merge
+  int32_t merge_on;    // merge runs this step
+  uint32_t itm;        // rotation index of this step's merge
+  uint32_t err;        // FRT_SDERR_* bits
+  int64_t queue[16];   // work-queue heads of the ray-queue kernels (FRT_Q_*)
+};
+#define FRT_Q_MARCH 0
+#define FRT_Q_DEPOSIT 1  // + C2Ray pass, passes 0..14
+#define FRT_SDERR_SPLIT_CAPACITY 1u  // rays after the split + new_rays exceed the buffer
+#define FRT_SDERR_CAST_CAPACITY 2u   // new_rays would exceed the buffer
+
+// Fixed grids of the device-count scan and sort (frt_scan_sort.hip).
+#define FRT_SCAN_MAX_GRID 2048
+#define FRT_SORT_MAX_GRID 4096
+
 struct ProfRec {
   const char* name;
   hipEvent_t a, b;
@@ -127,6 +156,18 @@ struct ContribPtrs {
   void* pair;          // [slot] {H, He} deposit summed over bins (2 values)
   void* rec;           // [slot] last pass: photons in per bin, then the signed length
   int rec_len;         // values per record (nfreq + 1 rounded up to a multiple of 4)
+};
+
+// Host values a captured step depends on (kernel arguments and launch
+// shapes): a replay is valid while they are unchanged.
+struct GraphKey {
+  GridConst g;
+  int32_t nsrc;
+  const void* ptrs[8];
+  int64_t seg_cap;
+  int32_t tune[6];
+  int32_t itercount;
+  int64_t capture_rays;
 };
 
 struct frt_ctx {
@@ -186,6 +227,28 @@ struct frt_ctx {
   bool have_segments;  // update_rays ran and find_avg_unu has not yet consumed the paths
   bool have_alive;     // find_avg_unu ran in this step (alive flags valid)
   int64_t nseg;
+  StepDev* sd;          // device: this context's step scalars
+  StepDev* hsd;         // pinned host mirror
+  int32_t* split_sorted;  // compaction order of the last split ([total] positions)
+  // scan/sort workspace (frt_scan_sort.hip)
+  int64_t* ss_part;
+  int32_t* ss_hist;
+  int32_t* ss_hoff;
+  // step graph: one executable graph per ray-buffer parity, captured on first use
+  int graph_on;         // frt_set_step_graph
+  bool capturing;       // launches size their grids for capture_rays rays
+  int64_t capture_rays; // ray-count bucket of the graph being captured
+  bool graph_valid[2];
+  hipGraphExec_t graph_exec[2];
+  GraphKey graph_key[2];
+  // launch tuning (frt_set_tuning)
+  int32_t tune_deposit_queue;   // 1: deposit passes pull chunks of the moved list from a queue
+  int32_t tune_march_queue;     // 1: the march pulls chunks of the ray list from a queue
+  int32_t tune_queue_chunk;     // rays per queue pull (a multiple of 64)
+  int32_t tune_blocks_per_cu;   // persistent grid of the queue kernels, blocks per CU
+  int32_t tune_grid_cap;        // block cap of grid-stride launches
+  int32_t tune_fp64_chunk;      // rays per block chunk of the FP64 deposit (1024, 2048, 4096, 8192)
+  int32_t num_cu;
 
   // segment sort
   int64_t seg_cap;
@@ -260,7 +323,11 @@ int read_device_counters(frt_ctx* ctx);
 int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal);
 int launch_thermal(frt_ctx* ctx);
 int launch_merge(frt_ctx* ctx);
-int launch_split_compact(frt_ctx* ctx, int64_t* n_after);
+int launch_split_compact(frt_ctx* ctx, bool nosplit);
+int launch_step_begin(frt_ctx* ctx, bool advance);
+int launch_step_end(frt_ctx* ctx);
+int launch_fill_alive(frt_ctx* ctx);
+int ensure_segment_capacity(frt_ctx* ctx);
 int launch_cast(frt_ctx* ctx);
 int launch_add_ray_dt(frt_ctx* ctx);
 int launch_copy(frt_ctx* ctx, int64_t nbytes, int repeats);
@@ -284,6 +351,21 @@ int launch_cosmo_expansion(frt_ctx* ctx, float ratio, double rho_scale, bool ray
 // update_hydro_step's cell loop: densities from the new rho, then ddelta_dt
 // and rho_prev (gas_funcs.cc:700-738).
 int launch_hydro_reload(frt_ctx* ctx, double rho_scale, float dt_cosmic);
+
+// device-count scan and stable radix sort (frt_scan_sort.hip): the count is
+// *dcount + add (dcount may be NULL), bound a host-side upper bound that sizes
+// the grid; dtotal (may be NULL) receives the scan's total; the sort leaves
+// its result in key[*which], val[*which]
+int dev_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* dcount, int64_t add, int64_t bound,
+             int64_t* dtotal, const char* name);
+int dev_sort_pairs(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t add,
+                   int64_t bound, int bits, int* which, const char* name);
+int alloc_scan_sort(frt_ctx* ctx);
+void free_scan_sort(frt_ctx* ctx);
+// host mirror of the step scalars <-> device
+int push_step_scalars(frt_ctx* ctx);
+int pull_step_scalars(frt_ctx* ctx);
+void graph_invalidate(frt_ctx* ctx);
 
 // helpers shared with the C-ABI file
 int ensure_cub_tmp(frt_ctx* ctx, size_t bytes);

@@ -78,6 +78,29 @@ __device__ __forceinline__ void block_add(unsigned long long* ctr, unsigned long
   __syncthreads();
 }
 
+// Work distribution over n items.  Without a queue head: grid-stride over
+// the XCD-remapped blocks.  With one (reset to 0 before the launch): every
+// wavefront pulls `chunk` consecutive items at a time from *head until they
+// run out, so a persistent grid of long-running wavefronts ends within one
+// chunk of each other instead of on a fixed share.  The order of the calls
+// changes no result: each item writes only its own outputs.
+template <typename Body>
+__device__ __forceinline__ void for_items(int64_t n, int64_t* head, int chunk, Body&& body) {
+  if (!head) {
+    for (int64_t i = xcd_block() * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) body(i);
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd((unsigned long long*)head, (unsigned long long)chunk);
+    base = __shfl(base, 0, 64);
+    if ((int64_t)base >= n) break;
+    const int64_t end = min((int64_t)base + chunk, n);
+    for (int64_t i = (int64_t)base + lane; i < end; i += 64) body(i);
+  }
+}
+
 // Ray order of the deposit kernels.  A wavefront runs as long as its longest
 // path.  The FP64 deposit is bound by FP64 issue, so each block takes CHUNK
 // consecutive rays at a time and reorders them by decreasing segment count
@@ -88,7 +111,6 @@ __device__ __forceinline__ void block_add(unsigned long long* ctr, unsigned long
 // rays over a chunk costs it more in memory requests than it saves in idle
 // lanes.  The order changes no result: each ray writes only its own slots,
 // and the counters are sums.
-static constexpr int DEP_CHUNK_FP64 = 4096;
 static constexpr int DEP_BUCKETS = 32;  // counts >= 31 share the first bucket
 static inline unsigned chunk_grid(int64_t n, int chunk) {
   const int64_t g = (n + chunk - 1) / chunk;
@@ -100,11 +122,11 @@ static inline unsigned chunk_grid(int64_t n, int chunk) {
 // per step, so no lane waits on a ray that did not move.
 template <typename Body>
 __device__ __forceinline__ void for_rays(const int32_t* __restrict__ seg_cnt, const int32_t* __restrict__ moved,
-                                         int64_t nmoved, Body&& body) {
-  for (int64_t t = xcd_block() * BLK + threadIdx.x; t < nmoved; t += (int64_t)gridDim.x * BLK) {
+                                         int64_t nmoved, int64_t* head, int chunk, Body&& body) {
+  for_items(nmoved, head, chunk, [&](int64_t t) {
     const int64_t i = moved[t];
     body(i, seg_cnt[i]);
-  }
+  });
 }
 
 // Calls body(i, seg_cnt[i]) for every moved ray i = moved[t], t < nmoved, in
@@ -113,13 +135,23 @@ __device__ __forceinline__ void for_rays(const int32_t* __restrict__ seg_cnt, co
 template <int CHUNK, typename Body>
 __device__ __forceinline__ void for_rays_by_count(const int32_t* __restrict__ seg_cnt,
                                                   const int32_t* __restrict__ moved, int64_t nrays,
-                                                  Body&& body) {
+                                                  int64_t* head, Body&& body) {
   static_assert(CHUNK % BLK == 0 && CHUNK <= 32768, "chunk of whole blocks, int16 local indices");
   constexpr int PER = CHUNK / BLK;
   __shared__ int s_cnt[DEP_BUCKETS];
   __shared__ int s_off[DEP_BUCKETS];
   __shared__ int16_t s_idx[CHUNK];
-  for (int64_t base = (int64_t)blockIdx.x * CHUNK; base < nrays; base += (int64_t)gridDim.x * CHUNK) {
+  __shared__ int64_t s_base;
+  // chunks dealt round-robin to the blocks, or pulled from the queue head
+  auto next = [&](int64_t prev) -> int64_t {
+    if (!head) return prev < 0 ? (int64_t)blockIdx.x * CHUNK : prev + (int64_t)gridDim.x * CHUNK;
+    if (threadIdx.x == 0) s_base = (int64_t)atomicAdd((unsigned long long*)head, (unsigned long long)CHUNK);
+    __syncthreads();
+    const int64_t b = s_base;
+    __syncthreads();
+    return b;
+  };
+  for (int64_t base = next(-1); base < nrays; base = next(base)) {
     if (threadIdx.x < DEP_BUCKETS) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     int bkt[PER], rank[PER];
@@ -252,15 +284,17 @@ __device__ __forceinline__ float face_time(float hat, float pos, float dx, float
 
 // ---------------------------------------------------------------------------
 // advance_ray (ray_tracing.cc:1222-1512), C2Ray mode, fion = 1.
-__global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridConst g,
+__global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst g,
                                                int32_t* __restrict__ path_cell,
                                                float* __restrict__ path_dist,
                                                int32_t* __restrict__ seg_cnt,
                                                uint8_t* __restrict__ alive,
-                                               DevCounters* cnt) {
+                                               DevCounters* cnt, int chunk) {
   unsigned long long nmoved = 0;
   bool overflow = false;
-  for (int64_t i = xcd_block() * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+  const int64_t nrays = sd->nrays;
+  if (blockIdx.x == 0 && threadIdx.x == 0) seg_cnt[nrays] = 0;  // scanned with the rays
+  for_items(nrays, chunk > 0 ? &sd->queue[FRT_Q_MARCH] : nullptr, chunk, [&](int64_t i) {
     bool moved = false;
     alive[i] = 1;
     const int n = g.n;
@@ -379,25 +413,26 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, int64_t nrays, GridCon
       ray_store(rb, i, ry);
     }
     nmoved += moved ? 1 : 0;
-  }
+  });
   block_add(&cnt->moved, nmoved);
   if (overflow) atomicOr(&cnt->path_overflow, 1u);
 }
 
 // Segment keys for the per-step cell sort.
-__global__ void __launch_bounds__(BLK) k_seg_keys(int64_t nrays, int64_t cap,
+__global__ void __launch_bounds__(BLK) k_seg_keys(const StepDev* __restrict__ sd, int64_t cap,
                                                   const int32_t* __restrict__ seg_cnt,
                                                   const int32_t* __restrict__ seg_off,
                                                   const int32_t* __restrict__ path_cell,
                                                   uint32_t* __restrict__ key,
                                                   int32_t* __restrict__ val) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= nrays) return;
-  const int c = seg_cnt[i];
-  const int base = seg_off[i];
-  for (int p = 0; p < c; ++p) {
-    key[base + p] = (uint32_t)path_cell[(int64_t)p * cap + i];
-    val[base + p] = (int32_t)((int64_t)p * cap + i);  // path-array index (< cap * path_length < 2^31)
+  const int64_t nrays = sd->nrays;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+    const int c = seg_cnt[i];
+    const int base = seg_off[i];
+    for (int p = 0; p < c; ++p) {
+      key[base + p] = (uint32_t)path_cell[(int64_t)p * cap + i];
+      val[base + p] = (int32_t)((int64_t)p * cap + i);  // path-array index (< cap * path_length < 2^31)
+    }
   }
 }
 
@@ -421,38 +456,46 @@ __global__ void __launch_bounds__(BLK) k_tape_pair(const float* __restrict__ a, 
 
 // Moved-ray flags (seg_cnt > 0) and their compaction into the list of moved
 // rays, in list order.
-__global__ void __launch_bounds__(BLK) k_moved_flags(const int32_t* __restrict__ seg_cnt, int64_t n,
+__global__ void __launch_bounds__(BLK) k_moved_flags(const int32_t* __restrict__ seg_cnt, const StepDev* __restrict__ sd,
                                                      int32_t* __restrict__ flag) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i < n) flag[i] = seg_cnt[i] > 0 ? 1 : 0;
-  if (i == n) flag[n] = 0;
+  const int64_t n = sd->nrays;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i <= n; i += (int64_t)gridDim.x * BLK)
+    flag[i] = (i < n && seg_cnt[i] > 0) ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(BLK) k_moved_scatter(const int32_t* __restrict__ flag,
-                                                       const int32_t* __restrict__ off, int64_t n,
+                                                       const int32_t* __restrict__ off, const StepDev* __restrict__ sd,
                                                        int32_t* __restrict__ moved) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i < n && flag[i]) moved[off[i]] = (int32_t)i;
+  const int64_t n = sd->nrays;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK)
+    if (flag[i]) moved[off[i]] = (int32_t)i;
 }
 
 __global__ void __launch_bounds__(BLK) k_fill_i32(int32_t* a, int64_t n, int32_t v) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i < n) a[i] = v;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) a[i] = v;
+}
+
+// Fill of the first *dn + add bytes (device count) with v.
+__global__ void __launch_bounds__(BLK) k_fill_u8_dev(uint8_t* a, const int64_t* __restrict__ dn, int64_t add, uint8_t v) {
+  const int64_t n = *dn + add;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) a[i] = v;
 }
 
 // Per-cell slot ranges of the sorted segments, and the slot of every path
 // entry (the inverse of the sort permutation, laid out like path_cell).
 __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict__ key,
-                                                     const int32_t* __restrict__ val, int64_t nseg,
+                                                     const int32_t* __restrict__ val,
+                                                     const StepDev* __restrict__ sd,
                                                      int32_t* __restrict__ start,
                                                      int32_t* __restrict__ end,
                                                      int32_t* __restrict__ inv) {
-  const int64_t s = xcd_block() * BLK + threadIdx.x;
-  if (s >= nseg) return;
-  const uint32_t k = key[s];
-  if (s == 0 || key[s - 1] != k) start[k] = (int32_t)s;
-  if (s == nseg - 1 || key[s + 1] != k) end[k] = (int32_t)(s + 1);
-  inv[val[s]] = (int32_t)s;
+  const int64_t nseg = sd->nseg;
+  for (int64_t s = xcd_block() * BLK + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * BLK) {
+    const uint32_t k = key[s];
+    if (s == 0 || key[s - 1] != k) start[k] = (int32_t)s;
+    if (s == nseg - 1 || key[s + 1] != k) end[k] = (int32_t)(s + 1);
+    inv[val[s]] = (int32_t)s;
+  }
 }
 
 
@@ -707,9 +750,9 @@ __device__ __forceinline__ bool deposit_segment_fp64(const GridConst& g, Contrib
 
 // FP64 form: the reference's double locals and libm exp (through detmath),
 // with the per-cell accumulation in double.
-template <bool LAST, int NFC = 0>
+template <bool LAST, int NFC = 0, int CHUNK = 4096>
 __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, const int32_t* __restrict__ moved,
-                                                      int64_t nmoved, GridConst g,
+                                                      StepDev* sd, int64_t* head, int chunk, GridConst g,
                                                       const int32_t* __restrict__ path_cell,
                                                       const float* __restrict__ path_dist,
                                                       const int32_t* __restrict__ seg_cnt,
@@ -718,7 +761,9 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, const int32_t* 
                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
                                                       DevCounters* cnt, uint8_t* __restrict__ tape_np) {
   unsigned long long ndumped = 0, nremoved = 0;
-  for_rays_by_count<DEP_CHUNK_FP64>(seg_cnt, moved, nmoved, [&](int64_t i, int c) {
+  (void)chunk;
+  const int64_t nmoved = sd->nmoved;
+  for_rays_by_count<CHUNK>(seg_cnt, moved, nmoved, head, [&](int64_t i, int c) {
     bool dumped_flag = false, removed = false;
     {
       const int F = NFC > 0 ? NFC : g.nfreq;
@@ -837,7 +882,7 @@ __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, Contri
 // compensated (Kahan) in the cell kernel.
 template <bool LAST, int NFC = 0>
 __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, const int32_t* __restrict__ moved,
-                                                       int64_t nmoved, GridConst g,
+                                                       StepDev* sd, int64_t* head, int chunk, GridConst g,
                                                        const int32_t* __restrict__ path_cell,
                                                        const float* __restrict__ path_dist,
                                                        const int32_t* __restrict__ seg_cnt,
@@ -849,7 +894,8 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, const int32_t*
   // 1/(dt vol) in units of phot_norm photons
   const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
   const float nmin = (float)g.nphot_min;
-  for_rays(seg_cnt, moved, nmoved, [&](int64_t i, int c) {
+  const int64_t nmoved = sd->nmoved;
+  for_rays(seg_cnt, moved, nmoved, head, chunk, [&](int64_t i, int c) {
     bool dumped_flag = false, removed = false;
     {
       const int F = NFC > 0 ? NFC : g.nfreq;
@@ -928,7 +974,8 @@ template <typename ACC, bool FIRST, bool LAST, bool THERMAL, int NFC = 0>
 __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribPtrs cp,
                                               const int32_t* __restrict__ cstart,
                                               const int32_t* __restrict__ cend,
-                                              DevCounters* cnt, double* __restrict__ tape_g) {
+                                              DevCounters* cnt, double* __restrict__ tape_g,
+                                              const StepDev* __restrict__ sd) {
   const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
   int nsub = 0;
   if (k < g.n3) {
@@ -1149,8 +1196,9 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
     float avg_he1 = (float)((double)nHe * (1. - (xeqhe + tihe / dtd * (x0he - xeqhe) * (1. - e_he))));
     if (g.dither_seed) {  // rounding-sensitivity measurement (frt_rng.h)
       const uint64_t r = (uint64_t)frt_cell_to_rowmajor(g.lay, (int)k);
-      avg_h1 = frt_ulp_step(avg_h1, frt_dither_up(g.dither_seed, (uint64_t)g.dither_step, g.pass, r, 0));
-      avg_he1 = frt_ulp_step(avg_he1, frt_dither_up(g.dither_seed, (uint64_t)g.dither_step, g.pass, r, 1));
+      const uint64_t st = (uint64_t)sd->step;  // completed steps
+      avg_h1 = frt_ulp_step(avg_h1, frt_dither_up(g.dither_seed, st, g.pass, r, 0));
+      avg_he1 = frt_ulp_step(avg_he1, frt_dither_up(g.dither_seed, st, g.pass, r, 1));
     }
     cf.f[C_AVG_NH1][k] = avg_h1;
     cf.f[C_AVG_NHE1][k] = avg_he1;
@@ -1162,7 +1210,7 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       const float nHe1n = (float)((double)nHe * (1. - (xeqhe + (x0he - xeqhe) * e_he)));
       const float nHe2n = nHe - nHe1n;
       if (((double)(avg_h1 / nH) < 0.5) && (cf.f[C_TREION_FLAG][k] == 0.f)) {
-        cf.f[C_TREION][k] = g.t;
+        cf.f[C_TREION][k] = sd->t;
         cf.f[C_TREION_FLAG][k] = 1.f;
       }
       cf.f[C_GAMMA_H1_TOT][k] = (float)GH;
@@ -1269,12 +1317,36 @@ __global__ void __launch_bounds__(BLK) k_thermal(CellPtrs cf, GridConst g, DevCo
 
 // ---------------------------------------------------------------------------
 // Merging (ray_tracing.cc:176-227 calc_nphot_merge, :324-477 merge).
-__global__ void __launch_bounds__(BLK) k_count_alive(const uint8_t* __restrict__ alive, int64_t n,
+__global__ void __launch_bounds__(BLK) k_count_alive(const uint8_t* __restrict__ alive, const StepDev* __restrict__ sd,
                                                      DevCounters* cnt) {
+  const int64_t n = sd->nrays;
   unsigned long long c = 0;
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK)
     c += alive[i] ? 1 : 0;
   block_add(&cnt->alive, c);
+}
+
+// Sum of one counter over the copies (the host sums them the same way).
+__device__ __forceinline__ unsigned long long counter_total(const DevCounters* cnt, const unsigned long long* field0) {
+  constexpr unsigned stride = sizeof(DevCounters) / sizeof(unsigned long long);
+  unsigned long long t = 0;
+  for (int r = 0; r < FRT_CTR_COPIES; ++r) t += field0[r * stride];
+  return t;
+}
+
+// count_rays against the merge threshold (ray_tracing.cc:157-170): whether
+// calc_nphot_merge runs, and the cleared histogram it fills.
+__global__ void __launch_bounds__(BLK) k_merge_setup(StepDev* sd, const DevCounters* cnt, int64_t num_thresh,
+                                                     int32_t* __restrict__ pdf) {
+  __shared__ int on;
+  if (threadIdx.x == 0) {
+    const int64_t count = (int64_t)counter_total(cnt, &cnt->alive);
+    on = count > num_thresh ? 1 : 0;
+    sd->pdf_on = on;
+  }
+  __syncthreads();
+  if (on)
+    for (int b = threadIdx.x; b < FRT_PDF_BINS; b += BLK) pdf[b] = 0;
 }
 
 // Photon-count histogram of calc_nphot_merge: privatized per block in LDS,
@@ -1282,9 +1354,11 @@ __global__ void __launch_bounds__(BLK) k_count_alive(const uint8_t* __restrict__
 // result does not depend on order).
 static constexpr unsigned PDF_GRID = 1024;
 
-__global__ void __launch_bounds__(BLK) k_pdf(RaySoA rb, int64_t nrays, GridConst g,
+__global__ void __launch_bounds__(BLK) k_pdf(RaySoA rb, const StepDev* __restrict__ sd, GridConst g,
                                              const uint8_t* __restrict__ alive, double lg_min,
                                              double del, int32_t* __restrict__ pdf) {
+  if (!sd->pdf_on) return;
+  const int64_t nrays = sd->nrays;
   __shared__ int32_t h[FRT_PDF_BINS];
   for (int b = threadIdx.x; b < FRT_PDF_BINS; b += BLK) h[b] = 0;
   __syncthreads();
@@ -1302,45 +1376,89 @@ __global__ void __launch_bounds__(BLK) k_pdf(RaySoA rb, int64_t nrays, GridConst
     if (h[b]) atomicAdd(&pdf[b], h[b]);
 }
 
-struct Mat9 {
-  float m[9];
-};
-
-__global__ void __launch_bounds__(BLK) k_merge_flag(RaySoA rb, int64_t nrays, GridConst g,
-                                                    const uint8_t* __restrict__ alive, Mat9 inv,
-                                                    int32_t* __restrict__ cand,
-                                                    uint32_t* __restrict__ key) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i > nrays) return;
-  if (i == nrays) {
-    cand[i] = 0;
-    return;
-  }
-  int c = 0;
-  if (alive[i]) {
-    const float s = sum_nphot(rb, i, g.nfreq, g.cap);
-    if (s < g.nphot_merge) {
-      c = 1;
-      const float4 d = rb.dir[i];
-      float vx, vy, vz;
-      rotate3(inv.m, d.x, d.y, d.z, &vx, &vy, &vz);  // get_unit_vector(.., 6)
-      const int bin = hpx_vec2pix_nest(g.hpx_lvl, (double)vx, (double)vy, (double)vz);
-      const int nb = 12 << (2 * g.hpx_lvl);
-      key[i] = (uint32_t)ray_cell(rb, i) * (uint32_t)nb + (uint32_t)bin;
+// calc_nphot_merge's top-down walk (ray_tracing.cc:205-219): nphot_merge =
+// 10^(b del + lg_min) for the highest bin b whose cumulative count from the
+// top reaches num_thresh (kept from the last time when count_rays did not
+// exceed the threshold, 0 if no bin reaches it).  One block: per-thread sums
+// of consecutive bins, a suffix scan of those, then the last qualifying bin
+// of each thread's range; the largest wins.  Then this step's merge switch
+// and rotation index.
+__global__ void __launch_bounds__(BLK) k_pdf_walk(StepDev* sd, const int32_t* __restrict__ pdf, int64_t num_thresh,
+                                                  double lg_min, double del, uint64_t seed) {
+  constexpr int PER = (FRT_PDF_BINS + BLK - 1) / BLK;
+  __shared__ int64_t tsum[BLK];
+  __shared__ int best;
+  const int t = threadIdx.x;
+  if (sd->pdf_on) {
+    const int b0 = t * PER, b1 = min(b0 + PER, FRT_PDF_BINS);
+    int64_t own = 0;
+    for (int b = b0; b < b1; ++b) own += pdf[b];
+    tsum[t] = own;
+    if (t == 0) best = -1;
+    __syncthreads();
+    // counts in the bins above this thread's range
+    int64_t above = 0;
+    for (int q = t + 1; q < BLK; ++q) above += tsum[q];
+    int64_t cum = above;
+    int found = -1;
+    for (int b = b1 - 1; b >= b0; --b) {
+      cum += pdf[b];
+      if (cum >= num_thresh) {
+        found = b;
+        break;
+      }
     }
+    if (found >= 0) atomicMax(&best, found);
+    __syncthreads();
+    if (t == 0) sd->nphot_merge = best >= 0 ? (float)dm_pow(10., best * del + lg_min) : 0.f;
   }
-  cand[i] = c;
+  if (t == 0) {
+    const int on = (sd->nphot_merge > 0.f) && (sd->nrays > 0);
+    sd->merge_on = on;
+    sd->mcount = on ? sd->nrays + 1 : 0;
+    sd->itm = frt_itheta_hash(seed, FRT_STREAM_MERGE + (uint64_t)sd->step, 0);
+  }
 }
 
-__global__ void __launch_bounds__(BLK) k_merge_scatter(int64_t nrays, const int32_t* __restrict__ cand,
+__global__ void __launch_bounds__(BLK) k_merge_flag(RaySoA rb, const StepDev* __restrict__ sd, GridConst g,
+                                                    const uint8_t* __restrict__ alive,
+                                                    const float* __restrict__ rot_inv,
+                                                    int32_t* __restrict__ cand,
+                                                    uint32_t* __restrict__ key) {
+  if (!sd->merge_on) return;
+  const int64_t nrays = sd->nrays;
+  const float nphot_merge = sd->nphot_merge;
+  const float* inv = rot_inv + 9 * (int64_t)sd->itm;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i <= nrays; i += (int64_t)gridDim.x * BLK) {
+    int c = 0;
+    if (i < nrays && alive[i]) {
+      const float s = sum_nphot(rb, i, g.nfreq, g.cap);
+      if (s < nphot_merge) {
+        c = 1;
+        const float4 d = rb.dir[i];
+        float vx, vy, vz;
+        rotate3(inv, d.x, d.y, d.z, &vx, &vy, &vz);  // get_unit_vector(.., 6)
+        const int bin = hpx_vec2pix_nest(g.hpx_lvl, (double)vx, (double)vy, (double)vz);
+        const int nb = 12 << (2 * g.hpx_lvl);
+        key[i] = (uint32_t)ray_cell(rb, i) * (uint32_t)nb + (uint32_t)bin;
+      }
+    }
+    cand[i] = c;
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_merge_scatter(const StepDev* __restrict__ sd, const int32_t* __restrict__ cand,
                                                        const int32_t* __restrict__ off,
                                                        const uint32_t* __restrict__ key_tmp,
                                                        uint32_t* __restrict__ key,
                                                        int32_t* __restrict__ val) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= nrays || !cand[i]) return;
-  key[off[i]] = key_tmp[i];
-  val[off[i]] = (int32_t)i;
+  if (!sd->merge_on) return;
+  const int64_t nrays = sd->nrays;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
+    if (!cand[i]) continue;
+    key[off[i]] = key_tmp[i];
+    val[off[i]] = (int32_t)i;
+  }
 }
 
 // l_from_r (ray_tracing.cc:87-95).
@@ -1356,16 +1474,18 @@ __device__ __forceinline__ int l_from_r_dev(const GridConst& g, float r, float s
 
 // One thread per occupied (cell, direction) bin: accumulates its members in
 // list order exactly as the serial phase 1, then normalizes (phase 2).
-__global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, GridConst g,
+__global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, const StepDev* __restrict__ sd, GridConst g,
                                                       const uint32_t* __restrict__ key,
                                                       const int32_t* __restrict__ val,
-                                                      uint8_t* __restrict__ alive, Mat9 fwd,
-                                                      uint32_t itheta_m, float safety,
-                                                      DevCounters* cnt) {
-  const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  bool head = false;
-  int64_t merged = 0;
-  if (s < ncand) head = (s == 0) || (key[s - 1] != key[s]);
+                                                      uint8_t* __restrict__ alive, const float* __restrict__ rot_fwd,
+                                                      float safety, DevCounters* cnt) {
+  const int64_t ncand = sd->ncand;
+  const uint32_t itheta_m = sd->itm;
+  const float* fwd = rot_fwd + 9 * (int64_t)itheta_m;
+  unsigned long long nheads = 0, merged = 0;
+  for (int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x; s < ncand; s += (int64_t)gridDim.x * BLK) {
+  const bool head = (s == 0) || (key[s - 1] != key[s]);
+  nheads += head ? 1 : 0;
   if (head) {
     const int F = g.nfreq;
     const int64_t cap = g.cap;
@@ -1424,7 +1544,7 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, 
       const int pixel = hpx_vec2pix_nest(order, vx, vy, vz);
       hpx_pix2vec_nest(order, pixel, &vx, &vy, &vz);
       float ox, oy, oz;
-      rotate3(fwd.m, (float)vx, (float)vy, (float)vz, &ox, &oy, &oz);
+      rotate3(fwd, (float)vx, (float)vy, (float)vz, &ox, &oy, &oz);
       rbase.order = order;
       rbase.pixel = pixel;
       rbase.itheta = (int)itheta_m;
@@ -1441,16 +1561,21 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, int64_t ncand, 
     rbase.time = T;
     ray_store(rb, b, rbase);
   }
-  block_add(&cnt->groups, head ? 1ull : 0ull);
-  block_add(&cnt->merged_away, (unsigned long long)merged);
+  }
+  block_add(&cnt->groups, nheads);
+  block_add(&cnt->merged_away, merged);
 }
 
 // ---------------------------------------------------------------------------
 // split_rays decision (rt_funcs.cc:211-237): rays out per input ray.
-__global__ void __launch_bounds__(BLK) k_split_flag(RaySoA rb, int64_t nrays, GridConst g,
+__global__ void __launch_bounds__(BLK) k_split_flag(RaySoA rb, const StepDev* __restrict__ sd, GridConst g,
                                                     const uint8_t* __restrict__ alive,
-                                                    float split_thresh, int32_t* __restrict__ n_out,
+                                                    int nosplit, int32_t* __restrict__ n_out,
                                                     DevCounters* cnt) {
+  const int64_t nrays = sd->nrays;
+  // maxd(4.*nphot_merge, 4.*nphot_min): both narrowed to float
+  const float split_thresh =
+      nosplit ? INFINITY : frt_maxd((float)(4. * (double)sd->nphot_merge), (float)(4. * g.nphot_min));
   unsigned long long nsplit = 0;
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i <= nrays; i += (int64_t)gridDim.x * BLK) {
     int out = 0;
@@ -1519,16 +1644,41 @@ __device__ __forceinline__ ChildGeo child_geometry(const Ray& par, int ch, const
 // split is rare, and its child geometry costs hundreds of instructions), so
 // a wavefront with k splits computes 4k children in ceil(4k / 64) rounds
 // instead of four rounds per split lane.
-__global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, int64_t nrays, GridConst g,
+// Capacity check of the split's output (ray_tracing.cc:275-287): the rays
+// after the split plus the next new_rays must fit; otherwise nothing more of
+// the step is written and the host reports the error.
+__global__ void k_split_check(StepDev* sd, int64_t cap, int64_t nnew) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (sd->err) {
+    sd->ncompact = 0;
+    return;
+  }
+  if (sd->nsplit + nnew > cap) {
+    sd->err |= FRT_SDERR_SPLIT_CAPACITY;
+    sd->ncompact = 0;
+  } else {
+    sd->ncompact = sd->nsplit;
+  }
+}
+
+__global__ void k_split_done(StepDev* sd) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && !sd->err) sd->nrays = sd->ncompact;
+}
+
+__global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, const StepDev* __restrict__ sd, GridConst g,
                                                   const int32_t* __restrict__ n_out,
                                                   const int32_t* __restrict__ out_off,
                                                   const float* __restrict__ rot_fwd,
                                                   uint32_t* __restrict__ key, int32_t* __restrict__ val,
                                                   uint32_t* __restrict__ src_of) {
   __shared__ int s_par[BLK];  // split rays of each wavefront, in lane order
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (sd->ncompact == 0) return;
+  const int64_t nrays = sd->nrays;
   const int lane = threadIdx.x & 63;
   const int wbase = threadIdx.x & ~63;
+  // block-uniform trip count: the ballot and the barrier see every lane
+  for (int64_t base = (int64_t)blockIdx.x * BLK; base < nrays; base += (int64_t)gridDim.x * BLK) {
+  const int64_t i = base + threadIdx.x;
   const int no = i < nrays ? n_out[i] : 0;
   if (no == 1) {
     const int o = out_off[i];
@@ -1550,15 +1700,18 @@ __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, int64_t nrays, Gri
     val[o + ch] = o + ch;
     src_of[o + ch] = ((uint32_t)ip << 2) | (uint32_t)ch;
   }
+  __syncthreads();  // s_par is rewritten by the next round
+  }
 }
 
 // Compaction into the other buffer in stable tile order, as a gather: output q
 // takes the ray (or split child) that sorted to position q.
-__global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, int64_t total, GridConst g,
+__global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, const StepDev* __restrict__ sd, GridConst g,
                                                  const int32_t* __restrict__ n_out,
                                                  const int32_t* __restrict__ sorted_pos,
                                                  const uint32_t* __restrict__ src_of,
                                                  const float* __restrict__ rot_fwd) {
+  const int64_t total = sd->ncompact;
   const int F = g.nfreq;
   const int nph = frt_phot_groups(F);
   const int64_t cap = g.cap;
@@ -1603,12 +1756,18 @@ __global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, int64_t
 
 // new_rays/init_rays (rt_funcs.cc:240-261, ray_tracing.cc:230-321), hpx_lvl 0:
 // 12 rays per source cell, appended in source order.
-__global__ void __launch_bounds__(BLK) k_cast(RaySoA dst, int64_t base, int nsrc, GridConst g,
+__global__ void __launch_bounds__(BLK) k_cast(RaySoA dst, StepDev* sd, int nsrc, GridConst g,
                                               const int32_t* __restrict__ src_cell,
                                               const double* __restrict__ src_lum,
                                               float* __restrict__ src_ray_dt,
-                                              const float* __restrict__ rot_fwd, uint64_t seed,
-                                              uint64_t stream) {
+                                              const float* __restrict__ rot_fwd, uint64_t seed) {
+  if (sd->err) return;
+  const int64_t base = sd->nrays;
+  const uint64_t stream = sd->cast_calls;
+  if (base + (int64_t)nsrc * 12 > g.cap) {  // new_rays would overflow the buffer
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&sd->err, FRT_SDERR_CAST_CAPACITY);
+    return;
+  }
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (t >= (int64_t)nsrc * 12) return;
   const int s = (int)(t / 12);
@@ -1644,9 +1803,41 @@ __global__ void __launch_bounds__(BLK) k_cast(RaySoA dst, int64_t base, int nsrc
   ray_store(dst, q, ry);
 }
 
-__global__ void __launch_bounds__(BLK) k_zero_ray_dt(float* __restrict__ ray_dt, int nsrc) {
+// After a cast: the sources' ray_dt restart from 0 and the list grows by
+// the new rays (skipped after a capacity error).
+__global__ void __launch_bounds__(BLK) k_cast_done(StepDev* sd, float* __restrict__ ray_dt, int nsrc) {
+  if (sd->err) return;
   const int s = blockIdx.x * BLK + threadIdx.x;
   if (s < nsrc) ray_dt[s] = 0.f;
+  if (s == 0) {
+    sd->nrays += (int64_t)nsrc * 12;
+    sd->cast_calls += 1;
+  }
+}
+
+// Start of frt_step: time advanced (main.cc:123), counters and queue heads
+// cleared, error bits reset.
+__global__ void __launch_bounds__(BLK) k_step_begin(StepDev* sd, DevCounters* cnt, float dt, int advance) {
+  constexpr int nw = FRT_CTR_COPIES * sizeof(DevCounters) / sizeof(unsigned long long);
+  unsigned long long* w = (unsigned long long*)cnt;
+  for (int q = threadIdx.x; q < nw; q += BLK) w[q] = 0ull;
+  if (threadIdx.x < 16) sd->queue[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    if (advance) sd->t = sd->t + dt;
+    sd->err = 0u;
+    sd->pdf_on = 0;
+    sd->merge_on = 0;
+    sd->mcount = 0;
+    sd->ncand = 0;
+  }
+}
+
+__global__ void k_queue_reset(StepDev* sd) {
+  if (threadIdx.x < 16 && blockIdx.x == 0) sd->queue[threadIdx.x] = 0;
+}
+
+__global__ void k_step_end(StepDev* sd) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && !sd->err) sd->step += 1;
 }
 
 __global__ void __launch_bounds__(BLK) k_add_ray_dt(float* __restrict__ ray_dt, int nsrc, float dt) {
@@ -1677,62 +1868,68 @@ __global__ void __launch_bounds__(BLK) k_copy(const int4* __restrict__ a, int4* 
 
 static RaySoA& cur_buf(frt_ctx* ctx) { return ctx->buf[ctx->cur]; }
 
+// Host-side sizes of the launches: the ray count known at the start of the
+// entry point, or, while a step graph is captured, the ray-count bucket the
+// graph serves (a power of two; replays with more rays recapture).  Every
+// kernel reads its actual count from the step scalars and strides over it,
+// so a size only shapes the grid.
+static int64_t bound_rays(const frt_ctx* ctx) { return ctx->capturing ? ctx->capture_rays : ctx->nrays; }
+static int64_t bound_segs(const frt_ctx* ctx) {
+  return std::min<int64_t>(ctx->seg_cap, bound_rays(ctx) * (int64_t)ctx->g.P);
+}
+
+// Blocks of a grid-stride launch: one thread per item up to the size, at
+// most grid_cap blocks.
+static unsigned ray_grid(const frt_ctx* ctx, int64_t bound) {
+  const int64_t g = (std::max<int64_t>(bound, 1) + BLK - 1) / BLK;
+  return (unsigned)std::min<int64_t>(g, std::max(ctx->tune_grid_cap, 1));
+}
+
+// Persistent grid of the queue kernels.
+static unsigned queue_grid(const frt_ctx* ctx) {
+  return (unsigned)std::max(1, ctx->num_cu * std::max(ctx->tune_blocks_per_cu, 1));
+}
+
 int launch_add_ray_dt(frt_ctx* ctx) {
   if (ctx->nsrc > 0)
     FRT_LAUNCH(ctx, "add_ray_dt", grid_for(ctx->nsrc), k_add_ray_dt, ctx->src_ray_dt, ctx->nsrc, ctx->g.dt);
   return FRT_OK;
 }
 
-int launch_march(frt_ctx* ctx) {
-  ctx->adj_valid = false;  // new paths: a recorded adjoint tape no longer matches
-  FRT_LAUNCH(ctx, "march", grid_for(ctx->nrays), k_march, cur_buf(ctx), ctx->nrays, ctx->g,
-             ctx->path_cell, ctx->path_dist, ctx->seg_cnt, ctx->alive, ctx->dcnt);
+// Clears the counters, queue heads and error bits; with advance, also the
+// time step of frt_step.
+int launch_step_begin(frt_ctx* ctx, bool advance) {
+  FRT_LAUNCH(ctx, "step_begin", 1, k_step_begin, ctx->sd, ctx->dcnt, ctx->g.dt, advance ? 1 : 0);
   return FRT_OK;
 }
 
-static int exclusive_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, int64_t n, const char* name) {
-  size_t bytes = 0;
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream);
-  if (e != hipSuccess) return hip_check(ctx, e, name);
-  int rc = ensure_cub_tmp(ctx, bytes);
-  if (rc) return rc;
-  ProfRec rec;
-  prof_begin(ctx, name, &rec);
-  e = hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp, bytes, in, out, (int)n, ctx->stream);
-  prof_end(ctx, &rec);
-  return hip_check(ctx, e, name);
+int launch_step_end(frt_ctx* ctx) {
+  FRT_LAUNCH(ctx, "step_end", 1, k_step_end, ctx->sd);
+  return FRT_OK;
 }
 
-static int sort_pairs(frt_ctx* ctx, const uint32_t* kin, uint32_t* kout, const int32_t* vin,
-                      int32_t* vout, int64_t n, int bits, const char* name) {
-  size_t bytes = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0,
-                                                     bits, ctx->stream);
-  if (e != hipSuccess) return hip_check(ctx, e, name);
-  int rc = ensure_cub_tmp(ctx, bytes);
-  if (rc) return rc;
-  ProfRec rec;
-  prof_begin(ctx, name, &rec);
-  e = hipcub::DeviceRadixSort::SortPairs(ctx->cub_tmp, bytes, kin, kout, vin, vout, (int)n, 0, bits,
-                                         ctx->stream);
-  prof_end(ctx, &rec);
-  return hip_check(ctx, e, name);
+int launch_march(frt_ctx* ctx) {
+  ctx->adj_valid = false;  // new paths: a recorded adjoint tape no longer matches
+  const bool q = ctx->tune_march_queue != 0;
+  if (q) FRT_LAUNCH(ctx, "queue_reset", 1, k_queue_reset, ctx->sd);
+  FRT_LAUNCH(ctx, "march", q ? queue_grid(ctx) : ray_grid(ctx, bound_rays(ctx)), k_march, cur_buf(ctx), ctx->sd,
+             ctx->g, ctx->path_cell, ctx->path_dist, ctx->seg_cnt, ctx->alive, ctx->dcnt,
+             q ? ctx->tune_queue_chunk : 0);
+  return FRT_OK;
 }
 
-static int read_i32(frt_ctx* ctx, const int32_t* dptr, int64_t* out) {
-  int rc = hip_check(ctx, hipMemcpyAsync(ctx->hscalar, dptr, 4, hipMemcpyDeviceToHost, ctx->stream), "read scalar");
-  if (rc) return rc;
-  rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
-  if (rc) return rc;
-  *out = (int64_t)(*(int32_t*)ctx->hscalar);
+int launch_fill_alive(frt_ctx* ctx) {
+  FRT_LAUNCH(ctx, "fill_alive", ray_grid(ctx, bound_rays(ctx)), k_fill_u8_dev, ctx->alive,
+             (const int64_t*)&ctx->sd->nrays, (int64_t)0, (uint8_t)1);
   return FRT_OK;
 }
 
 static int grow_segments(frt_ctx* ctx, int64_t need) {
   if (need <= ctx->seg_cap) return FRT_OK;
   int64_t cap = ctx->seg_cap ? ctx->seg_cap : 1024;
-  while (cap < need) cap *= 2;
+  while (cap < need) cap += std::max<int64_t>(cap / 4, 1024);
   hipStreamSynchronize(ctx->stream);  // the stream may still use the old buffers
+  graph_invalidate(ctx);              // captured graphs hold the old addresses
   for (int b = 0; b < 2; ++b) {
     hipFree(ctx->seg_key[b]);
     hipFree(ctx->seg_val[b]);
@@ -1767,10 +1964,22 @@ static int grow_segments(frt_ctx* ctx, int64_t need) {
   return FRT_OK;
 }
 
-// Copy the counter copies to the host and sum them into hcnt[0].
+// Segment buffers for the paths of the current rays: at most path_length
+// segments per ray, so a step never meets more than nrays * path_length
+// (while capturing, the bound must hold for every replay: the graph is
+// recaptured whenever the buffers grow).
+int ensure_segment_capacity(frt_ctx* ctx) {
+  return grow_segments(ctx, std::max<int64_t>(ctx->nrays * (int64_t)ctx->g.P, 1));
+}
+
+// Copy the counter copies and the step scalars to the host and sum the
+// counters into hcnt[0].
 int read_device_counters(frt_ctx* ctx) {
   int rc = hip_check(ctx, hipMemcpyAsync(ctx->hcnt, ctx->dcnt, FRT_CTR_COPIES * sizeof(DevCounters),
                                          hipMemcpyDeviceToHost, ctx->stream), "counters");
+  if (rc) return rc;
+  rc = hip_check(ctx, hipMemcpyAsync(ctx->hsd, ctx->sd, sizeof(StepDev), hipMemcpyDeviceToHost, ctx->stream),
+                 "step scalars");
   if (rc) return rc;
   rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
   if (rc) return rc;
@@ -1791,61 +2000,42 @@ int read_device_counters(frt_ctx* ctx) {
 
 // Up-front sizing of the segment buffers and of the scan/sort workspace, so
 // that the step loop does not reallocate (each reallocation waits for the
-// stream and costs milliseconds) while the ray and segment counts grow:
-// two segments per ray at full ray capacity, capped at 2^27 segments, and
-// the workspace of the largest scan and sorts.  Counts beyond that still
-// grow the buffers on demand.
+// stream) while the ray and segment counts grow: two segments per ray at full
+// ray capacity, capped at 2^27 segments.  Larger counts grow the buffers
+// before the step that could need them.
 int reserve_workspace(frt_ctx* ctx) {
   const int64_t cap = ctx->g.cap;
   const int64_t segs = std::min<int64_t>(cap * std::min(ctx->cfg.path_length, 2), (int64_t)1 << 27);
   int rc = grow_segments(ctx, segs);
   if (rc) return rc;
-  size_t need = 0, b = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, ctx->seg_key[0], ctx->seg_key[1], ctx->seg_val[0],
-                                                    ctx->seg_val[1], (int)segs, 0, 32, ctx->stream);
-  if (e != hipSuccess) return hip_check(ctx, e, "workspace size");
-  need = std::max(need, b);
-  e = hipcub::DeviceRadixSort::SortPairs(nullptr, b, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1],
-                                         (int)cap, 0, 32, ctx->stream);
-  if (e != hipSuccess) return hip_check(ctx, e, "workspace size");
-  need = std::max(need, b);
-  e = hipcub::DeviceScan::ExclusiveSum(nullptr, b, ctx->seg_cnt, ctx->seg_off, (int)(cap + 1), ctx->stream);
-  if (e != hipSuccess) return hip_check(ctx, e, "workspace size");
-  need = std::max(need, b);
-  return ensure_cub_tmp(ctx, need);
+  return alloc_scan_sort(ctx);
 }
 
-// Segment offsets, cell sort and cell ranges of the current paths.
+// Segment offsets, moved-ray list, cell sort and cell ranges of the current paths.
 int launch_segments(frt_ctx* ctx) {
-  const int64_t n = ctx->nrays;
-  int rc;
-  FRT_LAUNCH(ctx, "fill", grid_for(1), k_fill_i32, ctx->seg_cnt + n, (int64_t)1, 0);
-  rc = exclusive_scan(ctx, ctx->seg_cnt, ctx->seg_off, n + 1, "scan_segments");
-  if (rc) return rc;
-  int64_t nseg = 0;
-  rc = read_i32(ctx, ctx->seg_off + n, &nseg);
-  if (rc) return rc;
-  ctx->nseg = nseg;
-  rc = grow_segments(ctx, nseg > 0 ? nseg : 1);
+  const int64_t nb = bound_rays(ctx);
+  StepDev* sd = ctx->sd;
+  int rc = dev_scan(ctx, ctx->seg_cnt, ctx->seg_off, &sd->nrays, 1, nb + 1, &sd->nseg, "scan_segments");
   if (rc) return rc;
   // the moved rays, for the deposit passes (cand_flag / cand_off are free
   // until the merge)
-  FRT_LAUNCH(ctx, "moved_list", grid_for(n + 1), k_moved_flags, ctx->seg_cnt, n, ctx->cand_flag);
-  rc = exclusive_scan(ctx, ctx->cand_flag, ctx->cand_off, n + 1, "scan_moved");
+  FRT_LAUNCH(ctx, "moved_list", ray_grid(ctx, nb + 1), k_moved_flags, ctx->seg_cnt, sd, ctx->cand_flag);
+  rc = dev_scan(ctx, ctx->cand_flag, ctx->cand_off, &sd->nrays, 1, nb + 1, &sd->nmoved, "scan_moved");
   if (rc) return rc;
-  rc = read_i32(ctx, ctx->cand_off + n, &ctx->nmoved);
-  if (rc) return rc;
-  FRT_LAUNCH(ctx, "moved_list", grid_for(n), k_moved_scatter, ctx->cand_flag, ctx->cand_off, n, ctx->moved_idx);
+  FRT_LAUNCH(ctx, "moved_list", ray_grid(ctx, nb), k_moved_scatter, ctx->cand_flag, ctx->cand_off, sd,
+             ctx->moved_idx);
   FRT_LAUNCH(ctx, "fill", grid_for(ctx->g.n3), k_fill_i32, ctx->cell_start, (int64_t)ctx->g.n3, 0);
   FRT_LAUNCH(ctx, "fill", grid_for(ctx->g.n3), k_fill_i32, ctx->cell_end, (int64_t)ctx->g.n3, 0);
-  if (nseg == 0) return FRT_OK;
-  FRT_LAUNCH(ctx, "seg_keys", grid_for(n), k_seg_keys, n, ctx->g.cap, ctx->seg_cnt, ctx->seg_off,
+  FRT_LAUNCH(ctx, "seg_keys", ray_grid(ctx, nb), k_seg_keys, sd, ctx->g.cap, ctx->seg_cnt, ctx->seg_off,
              ctx->path_cell, ctx->seg_key[0], ctx->seg_val[0]);
-  rc = sort_pairs(ctx, ctx->seg_key[0], ctx->seg_key[1], ctx->seg_val[0], ctx->seg_val[1], nseg,
-                  ctx->key_bits, "sort_segments");
+  int which = 0;
+  rc = dev_sort_pairs(ctx, ctx->seg_key, ctx->seg_val, &sd->nseg, 0, bound_segs(ctx), ctx->key_bits, &which,
+                      "sort_segments");
   if (rc) return rc;
-  FRT_LAUNCH(ctx, "cell_ranges", grid_for(nseg), k_cell_ranges, ctx->seg_key[1], ctx->seg_val[1], nseg,
-             ctx->cell_start, ctx->cell_end, ctx->path_slot);
+  // light per-segment work: a capped grid strides over the segments (a
+  // grid sized for the bound would mostly run empty blocks)
+  FRT_LAUNCH(ctx, "cell_ranges", std::min(ray_grid(ctx, bound_segs(ctx)), 4 * MAX_GRID), k_cell_ranges, ctx->seg_key[which],
+             ctx->seg_val[which], sd, ctx->cell_start, ctx->cell_end, ctx->path_slot);
   return FRT_OK;
 }
 
@@ -1860,14 +2050,15 @@ __global__ void __launch_bounds__(BLK) k_tape_nph(RaySoA rb, int64_t n, int F, i
 int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   const int I = ctx->cfg.itercount;
   const int64_t n = ctx->nrays;
-  const int64_t nm = ctx->nmoved;
   const bool mixed = ctx->cfg.precision == FRT_PREC_MIXED;
+  const bool q = ctx->tune_deposit_queue != 0 && I <= 16 - FRT_Q_DEPOSIT;
   GridConst g = ctx->g;
   RaySoA rb = cur_buf(ctx);
   const unsigned gc = grid_for(g.n3);
   const int F = g.nfreq;
-  ctx->deposit_bytes_last = 0.;
+  StepDev* sd = ctx->sd;
   ctx->deposit_launches_last = 0;
+  if (q) FRT_LAUNCH(ctx, "queue_reset", 1, k_queue_reset, sd);
   // adjoint tape: per pass [opacity HI, HeI, Gamma_HI, Gamma_HeI][n3], then
   // the starting f_H1, f_He1, and the rays' starting photons
   double* tape = ctx->adj_on ? ctx->adj_tape : nullptr;
@@ -1877,20 +2068,25 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     FRT_LAUNCH(ctx, "adj_tape", grid_for(n), k_tape_nph, rb, n, F, g.cap, ctx->adj_nph);
     // the cell state the fused thermal update reads and then overwrites
     const int tf[5] = {C_TEMP, C_HEAT, C_COOL, C_NH1_PREV, C_NE};
-    for (int q = 0; q < 5; ++q) {
-      hipError_t e = hipMemcpyAsync(ctx->adj_tstate + (size_t)q * g.n3, ctx->c.f[tf[q]], 4 * (size_t)g.n3,
+    for (int qq = 0; qq < 5; ++qq) {
+      hipError_t e = hipMemcpyAsync(ctx->adj_tstate + (size_t)qq * g.n3, ctx->c.f[tf[qq]], 4 * (size_t)g.n3,
                                     hipMemcpyDeviceToDevice, ctx->stream);
       if (e != hipSuccess) return hip_check(ctx, e, "adjoint tape");
     }
     ctx->adj_cast_base = -1;
     ctx->adj_split_n = ctx->adj_split_total = -1;
   }
-  g.dither_step = ctx->step;
+  const unsigned dgrid_mixed = q ? queue_grid(ctx) : ray_grid(ctx, bound_rays(ctx));
+  const int fchunk = ctx->tune_fp64_chunk == 2048 ? 2048 : 4096;
+  const unsigned dgrid_fp64 =
+      q ? queue_grid(ctx) : (unsigned)std::max<int64_t>(1, (bound_rays(ctx) + fchunk - 1) / fchunk);
   for (int it = 0; it < I; ++it) {
     const bool first = (it == 0), last = (it == I - 1);
     g.pass = it;
     double* tk = tape ? tape + (size_t)4 * it * g.n3 : nullptr;
     uint8_t* tnp = tape ? ctx->adj_np + (size_t)it * g.cap : nullptr;
+    int64_t* head = q ? &sd->queue[FRT_Q_DEPOSIT + it] : nullptr;
+    const int chunk = ctx->tune_queue_chunk;
     if (tk)
       FRT_LAUNCH(ctx, "adj_tape", gc, k_tape_pair, ctx->c.f[first ? C_NH1 : C_AVG_NH1],
                  ctx->c.f[first ? C_NHE1 : C_AVG_NHE1], tk, (int64_t)g.n3);
@@ -1899,52 +2095,42 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     // five-bin specializations for the FP64 form (198 -> 167 and 141 VGPRs,
     // 2 -> 3 waves per SIMD); the mixed kernels were not faster specialized
     // (first passes 71 -> 75 VGPRs; last pass 84 -> 69 but 3 % slower)
-#define FRT_DEP(NAME, GRID, KERN, L1, SPEC)                                                         \
-  do {                                                                                              \
-    if ((SPEC) && F == 5)                                                                           \
-      FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1, 5>), rb, ctx->moved_idx, nm, g, ctx->path_cell, ctx->path_dist, \
-                 ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt,    \
-                 tnp);                                                                              \
-    else                                                                                            \
-      FRT_LAUNCH(ctx, NAME, GRID, (KERN<L1>), rb, ctx->moved_idx, nm, g, ctx->path_cell, ctx->path_dist, \
-                 ctx->seg_cnt, ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt,    \
-                 tnp);                                                                              \
-  } while (0)
+#define FRT_DEP_ARGS rb, ctx->moved_idx, sd, head, chunk, g, ctx->path_cell, ctx->path_dist, ctx->seg_cnt, \
+                     ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt, tnp
     if (mixed) {
-      if (last) {
-        FRT_DEP("deposit_mixed", grid_for(nm), k_deposit_mixed, true, false);
+      if (last)
+        FRT_LAUNCH(ctx, "deposit_mixed", dgrid_mixed, (k_deposit_mixed<true>), FRT_DEP_ARGS);
+      else
+        FRT_LAUNCH(ctx, "deposit_mixed", dgrid_mixed, (k_deposit_mixed<false>), FRT_DEP_ARGS);
+    } else if (F == 5) {
+      if (fchunk == 2048) {
+        if (last)
+          FRT_LAUNCH(ctx, "deposit_fp64", dgrid_fp64, (k_deposit_fp64<true, 5, 2048>), FRT_DEP_ARGS);
+        else
+          FRT_LAUNCH(ctx, "deposit_fp64", dgrid_fp64, (k_deposit_fp64<false, 5, 2048>), FRT_DEP_ARGS);
       } else {
-        FRT_DEP("deposit_mixed", grid_for(nm), k_deposit_mixed, false, false);
+        if (last)
+          FRT_LAUNCH(ctx, "deposit_fp64", dgrid_fp64, (k_deposit_fp64<true, 5, 4096>), FRT_DEP_ARGS);
+        else
+          FRT_LAUNCH(ctx, "deposit_fp64", dgrid_fp64, (k_deposit_fp64<false, 5, 4096>), FRT_DEP_ARGS);
       }
     } else {
       if (last)
-        FRT_DEP("deposit_fp64", chunk_grid(nm, DEP_CHUNK_FP64), k_deposit_fp64, true, true);
+        FRT_LAUNCH(ctx, "deposit_fp64", dgrid_fp64, (k_deposit_fp64<true, 0, 4096>), FRT_DEP_ARGS);
       else
-        FRT_DEP("deposit_fp64", chunk_grid(nm, DEP_CHUNK_FP64), k_deposit_fp64, false, true);
+        FRT_LAUNCH(ctx, "deposit_fp64", dgrid_fp64, (k_deposit_fp64<false, 0, 4096>), FRT_DEP_ARGS);
     }
-#undef FRT_DEP
-    // algorithmic bytes of this deposit launch (model used for the roofline):
-    // per ray: seg_cnt (4) + nphot read (4F); per segment: path cell + dist (8)
-    // + slot (4) + avg_nh1/avg_nhe1 (8) + the 2-value deposit (first
-    // passes) or the (F+1)-value photon record (last pass); the last pass adds
-    // the ray write-back (4F + 4 + 1).
-    {
-      const double acc = mixed ? 4. : 8.;
-      double b = (double)n * 4. + (double)ctx->stats.rays_moved * (4. * F) + (double)ctx->nseg * 20.;
-      b += (double)ctx->nseg * (last ? (F + 1.) : 2.) * acc;
-      if (last) b += (double)ctx->stats.rays_moved * (4. * F + 5.);
-      ctx->deposit_bytes_last += b;
-      ctx->deposit_launches_last += 1;
-    }
+#undef FRT_DEP_ARGS
+    ctx->deposit_launches_last += 1;
     // the last pass's record loop is specialized for five bins
 #define FRT_CHEM(ACC, F1, L1, TH)                                                                 \
   do {                                                                                            \
     if (L1 && F == 5)                                                                             \
       FRT_LAUNCH(ctx, "chem", gc, (k_chem<ACC, F1, L1, TH, 5>), ctx->c, g, ctx->contrib,          \
-                 ctx->cell_start, ctx->cell_end, ctx->dcnt, tk ? tk + 2 * g.n3 : nullptr);        \
+                 ctx->cell_start, ctx->cell_end, ctx->dcnt, tk ? tk + 2 * g.n3 : nullptr, sd);    \
     else                                                                                          \
       FRT_LAUNCH(ctx, "chem", gc, (k_chem<ACC, F1, L1, TH>), ctx->c, g, ctx->contrib,             \
-                 ctx->cell_start, ctx->cell_end, ctx->dcnt, tk ? tk + 2 * g.n3 : nullptr);        \
+                 ctx->cell_start, ctx->cell_end, ctx->dcnt, tk ? tk + 2 * g.n3 : nullptr, sd);    \
   } while (0)
     if (mixed) {
       if (first && last) {
@@ -1982,135 +2168,107 @@ int launch_thermal(frt_ctx* ctx) {
   return FRT_OK;
 }
 
-// count_rays, calc_nphot_merge (if needed) and merge.
+// count_rays, calc_nphot_merge (if needed) and merge, with every decision
+// taken on the device.
 int launch_merge(frt_ctx* ctx) {
-  const int64_t n = ctx->nrays;
+  const int64_t nb = bound_rays(ctx);
   GridConst g = ctx->g;
   RaySoA rb = cur_buf(ctx);
+  StepDev* sd = ctx->sd;
   int rc;
-  // count_rays
-  FRT_LAUNCH(ctx, "count_alive", sgrid(n), k_count_alive, ctx->alive, n, ctx->dcnt);
-  rc = read_device_counters(ctx);
-  if (rc) return rc;
-  const int64_t count = (int64_t)ctx->hcnt->alive;
-  const int64_t nb = 12ll << (2 * ctx->cfg.hpx_lvl);
-  const int64_t num_thresh = (int64_t)g.n3 * (int64_t)(ctx->cfg.ray_per_cell_max - nb);
-  ctx->stats.rays_counted = count;
+  FRT_LAUNCH(ctx, "count_alive", std::min(ray_grid(ctx, nb), MAX_GRID), k_count_alive, ctx->alive, sd, ctx->dcnt);
+  const int64_t nbins = 12ll << (2 * ctx->cfg.hpx_lvl);
+  const int64_t num_thresh = (int64_t)g.n3 * (int64_t)(ctx->cfg.ray_per_cell_max - nbins);
   ctx->stats.num_thresh = num_thresh;
-  if (count > num_thresh) {
-    // calc_nphot_merge: log-spaced photon-count histogram, top-down cumulative count
-    const double lg_max = dm_log10(ctx->nphot_max);
-    const double lg_min = dm_log10(ctx->nphot_min);
-    const double del = (lg_max - lg_min) / (10000 - 1);
-    FRT_LAUNCH(ctx, "fill", grid_for(FRT_PDF_BINS), k_fill_i32, ctx->pdf, (int64_t)FRT_PDF_BINS, 0);
-    FRT_LAUNCH(ctx, "pdf", std::min(sgrid(n), PDF_GRID), k_pdf, rb, n, g, ctx->alive, lg_min, del, ctx->pdf);
-    std::vector<int32_t> pdf(FRT_PDF_BINS);
-    rc = hip_check(ctx, hipMemcpyAsync(pdf.data(), ctx->pdf, 4 * FRT_PDF_BINS, hipMemcpyDeviceToHost, ctx->stream), "pdf");
-    if (rc) return rc;
-    rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
-    if (rc) return rc;
-    ctx->nphot_merge = 0.f;
-    int64_t cum = 0;
-    for (int b = FRT_PDF_BINS - 1; b >= 0; --b) {
-      cum += pdf[b];
-      if (cum >= num_thresh) {
-        ctx->nphot_merge = (float)dm_pow(10., b * del + lg_min);
-        break;
-      }
-    }
-  }
-  ctx->g.nphot_merge = ctx->nphot_merge;
-  g.nphot_merge = ctx->nphot_merge;
+  // calc_nphot_merge's log-spaced photon-count bins (ray_tracing.cc:181-186)
+  const double lg_max = dm_log10(ctx->nphot_max);
+  const double lg_min = dm_log10(ctx->nphot_min);
+  const double del = (lg_max - lg_min) / (10000 - 1);
+  FRT_LAUNCH(ctx, "merge_setup", 1, k_merge_setup, sd, (const DevCounters*)ctx->dcnt, num_thresh, ctx->pdf);
+  FRT_LAUNCH(ctx, "pdf", std::min(ray_grid(ctx, nb), PDF_GRID), k_pdf, rb, sd, g, ctx->alive, lg_min, del, ctx->pdf);
+  FRT_LAUNCH(ctx, "pdf_walk", 1, k_pdf_walk, sd, (const int32_t*)ctx->pdf, num_thresh, lg_min, del, ctx->cfg.seed);
   // merge(): rotation index drawn every pass (ray_tracing.cc:327)
-  const uint32_t itm = frt_itheta_hash(ctx->cfg.seed, FRT_STREAM_MERGE + (uint64_t)ctx->step, 0);
-  if (!(ctx->nphot_merge > 0.f) || n == 0) {
-    ctx->stats.merge_candidates = 0;
-    return FRT_OK;
-  }
-  Mat9 inv, fwd;
-  rc = hip_check(ctx, hipMemcpyAsync(inv.m, ctx->rot_inv + 9 * (int64_t)itm, 36, hipMemcpyDeviceToHost, ctx->stream), "rot");
+  FRT_LAUNCH(ctx, "merge_flag", ray_grid(ctx, nb + 1), k_merge_flag, rb, sd, g, ctx->alive, ctx->rot_inv,
+             ctx->cand_flag, ctx->cand_key_tmp);
+  rc = dev_scan(ctx, ctx->cand_flag, ctx->cand_off, &sd->mcount, 0, nb + 1, &sd->ncand, "scan_merge");
   if (rc) return rc;
-  rc = hip_check(ctx, hipMemcpyAsync(fwd.m, ctx->rot_fwd + 9 * (int64_t)itm, 36, hipMemcpyDeviceToHost, ctx->stream), "rot");
-  if (rc) return rc;
-  rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
-  if (rc) return rc;
-  FRT_LAUNCH(ctx, "merge_flag", grid_for(n + 1), k_merge_flag, rb, n, g, ctx->alive, inv, ctx->cand_flag,
-             ctx->cand_key_tmp);
-  rc = exclusive_scan(ctx, ctx->cand_flag, ctx->cand_off, n + 1, "scan_merge");
-  if (rc) return rc;
-  int64_t ncand = 0;
-  rc = read_i32(ctx, ctx->cand_off + n, &ncand);
-  if (rc) return rc;
-  ctx->stats.merge_candidates = ncand;
-  if (ncand == 0) return FRT_OK;
-  FRT_LAUNCH(ctx, "merge_scatter", grid_for(n), k_merge_scatter, n, ctx->cand_flag, ctx->cand_off,
+  FRT_LAUNCH(ctx, "merge_scatter", ray_grid(ctx, nb), k_merge_scatter, sd, ctx->cand_flag, ctx->cand_off,
              ctx->cand_key_tmp, ctx->m_key[0], ctx->m_val[0]);
   int bits = 1;
-  while ((1ll << bits) < (int64_t)g.n3 * nb) ++bits;
-  rc = sort_pairs(ctx, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1], ncand, bits,
-                  "sort_merge");
+  while ((1ll << bits) < (int64_t)g.n3 * nbins) ++bits;
+  int which = 0;
+  rc = dev_sort_pairs(ctx, ctx->m_key, ctx->m_val, &sd->ncand, 0, nb, bits, &which, "sort_merge");
   if (rc) return rc;
-  FRT_LAUNCH(ctx, "merge_groups", grid_for(ncand), k_merge_groups, rb, ncand, g, ctx->m_key[1],
-             ctx->m_val[1], ctx->alive, fwd, itm, (float)ctx->cfg.safety_factor, ctx->dcnt);
+  FRT_LAUNCH(ctx, "merge_groups", std::min(ray_grid(ctx, nb), 4 * MAX_GRID), k_merge_groups, rb, sd, g,
+             ctx->m_key[which], ctx->m_val[which],
+             ctx->alive, ctx->rot_fwd, (float)ctx->cfg.safety_factor, ctx->dcnt);
   return FRT_OK;
 }
 
-// split_rays into the other buffer; returns the ray count after the split.
-int launch_split_compact(frt_ctx* ctx, int64_t* n_after) {
-  const int64_t n = ctx->nrays;
+// split_rays into the other buffer (nosplit: compaction only).
+int launch_split_compact(frt_ctx* ctx, bool nosplit) {
+  const int64_t nb = bound_rays(ctx);
   GridConst g = ctx->g;
-  // maxd(4.*nphot_merge, 4.*nphot_min): both narrowed to float
-  const float split_thresh = frt_maxd((float)(4. * (double)ctx->nphot_merge), (float)(4. * ctx->nphot_min));
-  FRT_LAUNCH(ctx, "split_flag", sgrid(n + 1), k_split_flag, cur_buf(ctx), n, g, ctx->alive,
-             split_thresh, ctx->n_out, ctx->dcnt);
-  int rc = exclusive_scan(ctx, ctx->n_out, ctx->out_off, n + 1, "scan_split");
+  StepDev* sd = ctx->sd;
+  FRT_LAUNCH(ctx, "split_flag", std::min(ray_grid(ctx, nb + 1), MAX_GRID), k_split_flag, cur_buf(ctx), sd, g,
+             ctx->alive, nosplit ? 1 : 0, ctx->n_out, ctx->dcnt);
+  int rc = dev_scan(ctx, ctx->n_out, ctx->out_off, &sd->nrays, 1, nb + 1, &sd->nsplit, "scan_split");
   if (rc) return rc;
-  int64_t total = 0;
-  rc = read_i32(ctx, ctx->out_off + n, &total);
+  FRT_LAUNCH(ctx, "split_check", 1, k_split_check, sd, g.cap, (int64_t)ctx->nsrc * 12);
+  // stable sort of the output rays by cell tile: the list order of every step
+  FRT_LAUNCH(ctx, "out_keys", ray_grid(ctx, nb), k_out_keys, cur_buf(ctx), sd, g, ctx->n_out, ctx->out_off,
+             ctx->rot_fwd, ctx->m_key[0], ctx->m_val[0], ctx->cand_key_tmp);
+  int which = 0;
+  const int64_t nout = std::min<int64_t>(g.cap, 4 * nb);
+  rc = dev_sort_pairs(ctx, ctx->m_key, ctx->m_val, &sd->ncompact, 0, nout, ctx->tile_bits, &which, "sort_rays");
   if (rc) return rc;
-  if (total + (int64_t)ctx->nsrc * 12 > ctx->g.cap)
-    return set_error(ctx, FRT_E_CAPACITY,
-                     "ray buffer full: " + std::to_string(total + (int64_t)ctx->nsrc * 12) +
-                         " rays needed, capacity " + std::to_string(ctx->g.cap));
-  if (total > 0) {
-    // stable sort of the output rays by cell tile: the list order of every step
-    FRT_LAUNCH(ctx, "out_keys", grid_for(n), k_out_keys, cur_buf(ctx), n, g, ctx->n_out, ctx->out_off,
-               ctx->rot_fwd, ctx->m_key[0], ctx->m_val[0], ctx->cand_key_tmp);
-    rc = sort_pairs(ctx, ctx->m_key[0], ctx->m_key[1], ctx->m_val[0], ctx->m_val[1], total, ctx->tile_bits,
-                    "sort_rays");
-    if (rc) return rc;
-    FRT_LAUNCH(ctx, "compact", grid_for(total), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], total, g,
-               ctx->n_out, ctx->m_val[1], ctx->cand_key_tmp, ctx->rot_fwd);
-  }
+  // one thread per input ray; the grid-stride loop takes the split children beyond
+  FRT_LAUNCH(ctx, "compact", ray_grid(ctx, nb), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], sd, g, ctx->n_out,
+             ctx->m_val[which], ctx->cand_key_tmp, ctx->rot_fwd);
+  FRT_LAUNCH(ctx, "split_done", 1, k_split_done, sd);
+  ctx->split_sorted = ctx->m_val[which];
   ctx->cur = 1 - ctx->cur;
   ctx->split_gen += 1;
-  if (ctx->adj_on && ctx->nphot_merge < INFINITY) {  // adjoint tape: the split's input and output counts
-    ctx->adj_split_n = n;
-    ctx->adj_split_total = total;
+  if (ctx->adj_on && !nosplit) {  // adjoint tape: the split's input count (its output count after the pull)
+    ctx->adj_split_n = ctx->nrays;
+    ctx->adj_split_total = -2;
     ctx->adj_split_gen = ctx->split_gen;
   }
-  ctx->nrays = total;
-  *n_after = total;
   return FRT_OK;
 }
 
 int launch_cast(frt_ctx* ctx) {
   if (ctx->nsrc == 0) return FRT_OK;
   const int64_t nnew = (int64_t)ctx->nsrc * 12;
-  if (ctx->nrays + nnew > ctx->g.cap)
-    return set_error(ctx, FRT_E_CAPACITY, "ray buffer full in new_rays");
-  FRT_LAUNCH(ctx, "cast", grid_for(nnew), k_cast, cur_buf(ctx), ctx->nrays, ctx->nsrc, ctx->g,
-             ctx->src_cell, ctx->src_lum, ctx->src_ray_dt, ctx->rot_fwd, ctx->cfg.seed,
-             (uint64_t)ctx->cast_calls);
+  FRT_LAUNCH(ctx, "cast", grid_for(nnew), k_cast, cur_buf(ctx), ctx->sd, ctx->nsrc, ctx->g, ctx->src_cell,
+             ctx->src_lum, ctx->src_ray_dt, ctx->rot_fwd, ctx->cfg.seed);
   if (ctx->adj_on && ctx->adj_raydt && ctx->adj_raydt_cap >= ctx->nsrc) {  // adjoint tape: the ray_dt the cast used
     hipError_t e = hipMemcpyAsync(ctx->adj_raydt, ctx->src_ray_dt, 4 * (size_t)ctx->nsrc, hipMemcpyDeviceToDevice,
                                   ctx->stream);
     if (e != hipSuccess) return hip_check(ctx, e, "adjoint tape");
-    ctx->adj_cast_base = ctx->nrays;
+    ctx->adj_cast_base = -2;  // the list length before the cast, after the pull
   }
-  FRT_LAUNCH(ctx, "zero_ray_dt", grid_for(ctx->nsrc), k_zero_ray_dt, ctx->src_ray_dt, ctx->nsrc);
-  ctx->nrays += nnew;
-  ctx->cast_calls += 1;
+  FRT_LAUNCH(ctx, "cast_done", grid_for(ctx->nsrc), k_cast_done, ctx->sd, ctx->src_ray_dt, ctx->nsrc);
+  return FRT_OK;
+}
+
+// Writes the host mirror of the step scalars to the device (a kernel with the
+// values as arguments, so no host buffer stays in flight) and clears the
+// error bits.
+__global__ void k_set_scalars(StepDev* sd, int64_t nrays, float t, int64_t step, uint64_t cast_calls,
+                              float nphot_merge) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  sd->nrays = nrays;
+  sd->t = t;
+  sd->step = step;
+  sd->cast_calls = cast_calls;
+  sd->nphot_merge = nphot_merge;
+  sd->err = 0u;
+}
+
+int push_step_scalars(frt_ctx* ctx) {
+  FRT_LAUNCH(ctx, "set_scalars", 1, k_set_scalars, ctx->sd, ctx->nrays, ctx->t, ctx->step,
+             (uint64_t)ctx->cast_calls, ctx->nphot_merge);
   return FRT_OK;
 }
 

@@ -241,6 +241,46 @@ class FlexRT:
         self._check(self._lib.frt_get_stats(self._h, C.byref(st)), "get_stats")
         return st.as_dict()
 
+    # -- step graph and launch tuning ---------------------------------------------------
+    def set_step_graph(self, on=True):
+        """Capture frt_step into a HIP graph on first use and replay it."""
+        self._check(self._lib.frt_set_step_graph(self._h, 1 if on else 0), "set_step_graph")
+
+    TUNING_KEYS = ("deposit_queue", "march_queue", "queue_chunk", "blocks_per_cu", "grid_cap", "fp64_chunk")
+
+    def set_tuning(self, **kw):
+        for k, v in kw.items():
+            self._check(self._lib.frt_set_tuning(self._h, k.encode(), int(v)), "set_tuning(%s)" % k)
+
+    def tuning(self):
+        out = {}
+        for k in self.TUNING_KEYS + ("num_cu",):
+            v = C.c_int64()
+            self._check(self._lib.frt_get_tuning(self._h, k.encode(), C.byref(v)), "get_tuning(%s)" % k)
+            out[k] = v.value
+        return out
+
+    def selftest_scan_sort(self, keys=None, bits=32, counts=None):
+        """The library's device-count scan and stable sort on host arrays:
+        (sorted keys, permutation, exclusive scan, total)."""
+        n = len(keys) if keys is not None else len(counts)
+        P32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        PU = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+        k = ko = perm = c = so = None
+        if keys is not None:
+            k = np.ascontiguousarray(keys, dtype=np.uint32)
+            ko = np.zeros(n, dtype=np.uint32)
+            perm = np.zeros(n, dtype=np.int32)
+        if counts is not None:
+            c = np.ascontiguousarray(counts, dtype=np.int32)
+            so = np.zeros(n, dtype=np.int32)
+        tot = C.c_int64(-1)
+        self._check(self._lib.frt_selftest_scan_sort(
+            self._h, n, int(bits), PU(k) if k is not None else None, PU(ko) if ko is not None else None,
+            P32(perm) if perm is not None else None, P32(c) if c is not None else None,
+            P32(so) if so is not None else None, C.byref(tot)), "selftest_scan_sort")
+        return ko, perm, so, tot.value
+
     # -- rays --------------------------------------------------------------------------
     def ray_count(self):
         n = C.c_int64()

@@ -311,6 +311,36 @@ int frt_update_thermal(frt_ctx* ctx);
 int frt_step(frt_ctx* ctx, frt_step_stats* stats);
 int frt_get_stats(const frt_ctx* ctx, frt_step_stats* stats);
 
+/* ---- step graph and launch tuning ---------------------------------------------
+ * No counterpart in the reference, whose timestep loop (main.cc:108-198) runs
+ * the phases as OpenMP loops over per-thread ray partitions
+ * (mem_funcs.cc:186-299).  Every launch of a step sizes itself from counts
+ * in device memory, so a step never stops the stream to read one back.
+ * frt_set_step_graph(ctx, 1): frt_step captures its launches into a HIP graph
+ * on first use and replays it (one graph per ray-buffer parity; its grids
+ * are shaped for the power-of-two bucket of the ray count at the start of
+ * the step, and a step in another bucket, or a change of the grid, sources,
+ * segment buffers or tuning, recaptures).  Results are the same bits as
+ * without the graph.  Not used while profiling or recording an adjoint tape
+ * (those steps run uncaptured). */
+int frt_set_step_graph(frt_ctx* ctx, int32_t on);
+/* Launch parameters, by name:
+ *   "deposit_queue"  0/1: deposit passes pull chunks of the moved-ray list
+ *                    from a device work queue with a persistent grid
+ *   "march_queue"    0/1: the same for the march
+ *   "queue_chunk"    rays per pull (multiple of 64)
+ *   "blocks_per_cu"  persistent grid of the queue kernels, blocks per CU
+ *   "grid_cap"       block cap of grid-stride launches
+ *   "fp64_chunk"     2048 or 4096: rays per block chunk of the FP64 deposit
+ * frt_get_tuning also reads "num_cu".  Results do not depend on them. */
+int frt_set_tuning(frt_ctx* ctx, const char* key, int64_t value);
+int frt_get_tuning(const frt_ctx* ctx, const char* key, int64_t* value);
+/* Verification hook: the step's device-count exclusive scan of counts[0..n)
+ * and stable radix sort of (keys[i], i) on the low `bits` key bits, results
+ * to host arrays (any may be NULL). */
+int frt_selftest_scan_sort(frt_ctx* ctx, int64_t n, int32_t bits, const uint32_t* keys, uint32_t* keys_out,
+                           int32_t* perm_out, const int32_t* counts, int32_t* scan_out, int64_t* total);
+
 /* ---- ray buffer ----------------------------------------------------------- */
 int frt_ray_count(const frt_ctx* ctx, int64_t* count);
 /* Copy the ray buffer to caller-owned arrays sized for view->count >= the
