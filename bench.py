@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--adjoint", action="store_true",
                     help="time one find_avg_unu forward with its adjoint tape and the adjoint "
                          "(frt_adj_find_avg_unu) on the differentiation workload (64^3, 16 sources)")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="run with the library's default launches instead of the tuning table's "
+                         "choice for this workload (flexrt_imaging_reion_amd/tuning_table.json)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the runs, copy MIB MiB through the library's plain copy kernel "
                          "(reference dispatch for HBM counter calibration)")
@@ -122,6 +125,13 @@ def run_form(wl, precision, steps, warmup, profile, files_dir=None, driver_ops=F
     free0, total = torch.cuda.mem_get_info()
     g = FlexRT(cfg)
     inputs.setup(g, wl)
+    run_form.launch = None
+    if getattr(run_form, "use_tuning", True):
+        from flexrt_imaging_reion_amd import autotune
+
+        key, settings = autotune.apply(g, wl["cfg"], precision, len(wl["src_lum"]))
+        run_form.launch = {"table_key": key, "settings": settings} if key else None
+        log("precision %d: launch settings %s" % (precision, settings if key else "defaults (no table entry)"))
     free1, _ = torch.cuda.mem_get_info()
     run_form.device_memory_gb = round((free0 - free1) / 1e9, 2)
     log("precision %d: context ready (%.1f GB of device memory), %d warmup steps"
@@ -588,6 +598,7 @@ def main():
         sys.exit(2)
     from flexrt_imaging_reion_amd import inputs
 
+    run_form.use_tuning = not a.no_tuning
     wl = inputs.make_workload(a.workload, steps=max(a.steps, 1))
     log("workload %s built: %d source cells" % (a.workload, len(wl["src_cells"])))
     steps = a.steps
@@ -619,10 +630,12 @@ def main():
         "segments_per_step": int(np.mean([s["segments"] for s in st_m])),
         "ray_segments_per_s": round(float(np.sum([s["segments"] for s in st_m])) / wall_m, 1),
         "device_memory_GB": getattr(run_form, "device_memory_gb", None),
+        "launch": run_form.launch,
     }
     if not a.no_fp64:
         wall_f, st_f, kt_f, state_f, wp_f, _ = run_form(wl, 0, steps, a.warmup, True)
         out["fp64"] = {"value": round(n3 * steps / wall_f, 1), "ms_per_step": round(1e3 * wall_f / steps, 3),
+                       "launch": run_form.launch,
                        "roofline": roofline(wl, st_f, kt_f, False),
                        "step_roofline": step_roofline(wl, st_f, wall_f, False)}
         out["mixed_over_fp64"] = round(wall_f / wall_m, 3)

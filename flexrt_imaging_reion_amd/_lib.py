@@ -46,6 +46,7 @@ _PROTOS = {
     "frt_set_step_graph": (C.c_int, [_CTX, C.c_int32]),
     "frt_set_tuning": (C.c_int, [_CTX, C.c_char_p, C.c_int64]),
     "frt_get_tuning": (C.c_int, [_CTX, C.c_char_p, C.POINTER(C.c_int64)]),
+    "frt_device_arch": (C.c_int, [C.c_char_p, C.c_int32]),
     "frt_selftest_scan_sort": (C.c_int, [_CTX, C.c_int64, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int64)]),

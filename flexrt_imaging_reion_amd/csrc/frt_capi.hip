@@ -1294,6 +1294,17 @@ int frt_step(frt_ctx* ctx, frt_step_stats* stats) {
   return FRT_OK;
 }
 
+int frt_device_arch(char* buf, int32_t len) {
+  if (!buf || len <= 0) return FRT_E_ARG;
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return FRT_E_HIP;
+  std::string a(p.gcnArchName);
+  a = a.substr(0, a.find(':'));
+  snprintf(buf, (size_t)len, "%s", a.c_str());
+  return FRT_OK;
+}
+
 int frt_set_step_graph(frt_ctx* ctx, int32_t on) {
   if (!ctx) return FRT_E_ARG;
   ctx->graph_on = on ? 1 : 0;

@@ -334,6 +334,9 @@ int frt_set_step_graph(frt_ctx* ctx, int32_t on);
  *   "fp64_chunk"     2048 or 4096: rays per block chunk of the FP64 deposit
  * frt_get_tuning also reads "num_cu".  Results do not depend on them. */
 int frt_set_tuning(frt_ctx* ctx, const char* key, int64_t value);
+/* GPU architecture of the current device (e.g. "gfx950"), the key of the
+ * tuning table's entries. */
+int frt_device_arch(char* buf, int32_t len);
 int frt_get_tuning(const frt_ctx* ctx, const char* key, int64_t* value);
 /* Verification hook: the step's device-count exclusive scan of counts[0..n)
  * and stable radix sort of (keys[i], i) on the low `bits` key bits, results
