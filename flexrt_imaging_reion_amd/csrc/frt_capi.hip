@@ -142,7 +142,7 @@ int alloc_rays(frt_ctx* ctx, RaySoA& b, int64_t cap, int nfreq) {
   hipError_t e = dalloc(&b.pos, cap);
   if (e == hipSuccess) e = dalloc(&b.dir, cap);
   if (e == hipSuccess) e = dalloc(&b.meta, cap);
-  if (e == hipSuccess) e = dalloc(&b.phot, cap * frt_phot_groups(nfreq));
+  if (e == hipSuccess) e = hipMalloc((void**)&b.phot, std::max<size_t>(frt_phot_bytes(nfreq, cap), 16));
   if (e != hipSuccess) return set_error(ctx, FRT_E_NOMEM, std::string("ray buffer: ") + hipGetErrorString(e));
   return FRT_OK;
 }
@@ -1065,7 +1065,7 @@ int frt_checkpoint_save(frt_ctx* ctx, frt_checkpoint** out) {
   ck->device = ctx->device;
   ck->n3 = ctx->g.n3;
   ck->nfreq = ctx->g.nfreq;
-  ck->groups = frt_phot_groups(ctx->g.nfreq);
+  ck->groups = frt_phot_planes(ctx->g.nfreq);
   ck->nrays = ctx->nrays;
   const size_t ncell = (size_t)(C_NSCALAR + 3 * ck->nfreq) * (size_t)ck->n3;
   const size_t nr = (size_t)std::max<int64_t>(ck->nrays, 1);
@@ -1074,7 +1074,7 @@ int frt_checkpoint_save(frt_ctx* ctx, frt_checkpoint** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&ck->pos, nr * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc((void**)&ck->dir, nr * sizeof(float4));
   if (e == hipSuccess) e = hipMalloc((void**)&ck->meta, nr * sizeof(int4));
-  if (e == hipSuccess) e = hipMalloc((void**)&ck->phot, nr * ck->groups * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc((void**)&ck->phot, frt_phot_bytes(ck->nfreq, (int64_t)nr));
   if (e == hipSuccess) e = hipMalloc((void**)&ck->src_ray_dt, (size_t)std::max(ctx->nsrc, 1) * sizeof(float));
   if (e != hipSuccess) {
     checkpoint_release(ck);
@@ -1088,9 +1088,11 @@ int frt_checkpoint_save(frt_ctx* ctx, frt_checkpoint** out) {
   if (e == hipSuccess && n) e = hipMemcpyAsync(ck->pos, b.pos, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
   if (e == hipSuccess && n) e = hipMemcpyAsync(ck->dir, b.dir, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
   if (e == hipSuccess && n) e = hipMemcpyAsync(ck->meta, b.meta, n * sizeof(int4), hipMemcpyDeviceToDevice, ctx->stream);
-  for (int gi = 0; gi < ck->groups && e == hipSuccess && n; ++gi)
-    e = hipMemcpyAsync(ck->phot + (size_t)gi * n, b.phot + (size_t)gi * ctx->g.cap, n * sizeof(float4),
-                       hipMemcpyDeviceToDevice, ctx->stream);
+  // photons: the 4-bin vectors, then each further bin's plane (ray-buffer layout with cap = n)
+  if (e == hipSuccess && n) e = hipMemcpyAsync(ck->phot, b.phot, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
+  for (int k = 0; k < ck->groups && e == hipSuccess && n; ++k)
+    e = hipMemcpyAsync((float*)(ck->phot + n) + (size_t)k * n, (const float*)(b.phot + ctx->g.cap) + (size_t)k * ctx->g.cap,
+                       n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
   if (e == hipSuccess && ctx->nsrc > 0)
     e = hipMemcpyAsync(ck->src_ray_dt, ctx->src_ray_dt, (size_t)ctx->nsrc * sizeof(float), hipMemcpyDeviceToDevice,
                        ctx->stream);
@@ -1134,9 +1136,10 @@ int frt_checkpoint_restore(frt_ctx* ctx, const frt_checkpoint* ck) {
   if (e == hipSuccess && n) e = hipMemcpyAsync(b.pos, ck->pos, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
   if (e == hipSuccess && n) e = hipMemcpyAsync(b.dir, ck->dir, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
   if (e == hipSuccess && n) e = hipMemcpyAsync(b.meta, ck->meta, n * sizeof(int4), hipMemcpyDeviceToDevice, ctx->stream);
-  for (int gi = 0; gi < ck->groups && e == hipSuccess && n; ++gi)
-    e = hipMemcpyAsync(b.phot + (size_t)gi * ctx->g.cap, ck->phot + (size_t)gi * n, n * sizeof(float4),
-                       hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(b.phot, ck->phot, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream);
+  for (int k = 0; k < ck->groups && e == hipSuccess && n; ++k)
+    e = hipMemcpyAsync((float*)(b.phot + ctx->g.cap) + (size_t)k * ctx->g.cap, (const float*)(ck->phot + n) + (size_t)k * n,
+                       n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
   if (e == hipSuccess && ctx->nsrc > 0)
     e = hipMemcpyAsync(ctx->src_ray_dt, ck->src_ray_dt, (size_t)ctx->nsrc * sizeof(float), hipMemcpyDeviceToDevice,
                        ctx->stream);
@@ -1377,15 +1380,17 @@ int frt_download_rays(frt_ctx* ctx, frt_ray_view* v) {
   if (n == 0) return FRT_OK;
   RaySoA& b = ctx->buf[ctx->cur];
   const int F = ctx->g.nfreq;
-  const int G = frt_phot_groups(F);
-  std::vector<float4> pos(n), dir(n), phot((size_t)G * n);
+  const int PL = frt_phot_planes(F);
+  std::vector<float4> pos(n), dir(n), phot(n);
+  std::vector<float> phot_hi((size_t)PL * n);
   std::vector<int4> meta(n);
   hipError_t e = stream_copy(ctx, pos.data(), b.pos, 16 * (size_t)n, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = stream_copy(ctx, dir.data(), b.dir, 16 * (size_t)n, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = stream_copy(ctx, meta.data(), b.meta, 16 * (size_t)n, hipMemcpyDeviceToHost);
-  for (int v = 0; v < G && e == hipSuccess; ++v)
-    e = stream_copy(ctx, phot.data() + (size_t)v * n, b.phot + (size_t)v * ctx->g.cap, 16 * (size_t)n,
-                    hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = stream_copy(ctx, phot.data(), b.phot, 16 * (size_t)n, hipMemcpyDeviceToHost);
+  for (int k = 0; k < PL && e == hipSuccess; ++k)
+    e = stream_copy(ctx, phot_hi.data() + (size_t)k * n, (const float*)(b.phot + ctx->g.cap) + (size_t)k * ctx->g.cap,
+                    4 * (size_t)n, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_check(ctx, e, "download rays");
   for (int64_t i = 0; i < n; ++i) {
     if (v->x) v->x[i] = pos[i].x;
@@ -1405,9 +1410,9 @@ int frt_download_rays(frt_ctx* ctx, frt_ray_view* v) {
     if (v->itheta) v->itheta[i] = (uint16_t)((uint32_t)meta[i].w >> 16);
     if (v->nphot)
       for (int nu = 0; nu < F; ++nu) {
-        const float4& q = phot[(size_t)(nu >> 2) * n + i];
-        const int k = nu & 3;
-        v->nphot[i * F + nu] = k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w;
+        const float4& q = phot[i];
+        v->nphot[i * F + nu] = nu == 0 ? q.x : nu == 1 ? q.y : nu == 2 ? q.z : nu == 3 ? q.w
+                                                                              : phot_hi[(size_t)(nu - 4) * n + i];
       }
   }
   return hip_check(ctx, e, "download rays");
@@ -1431,8 +1436,9 @@ int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* v) {
   hipError_t e = hipSuccess;
   if (n > 0) {
     const int F = ctx->g.nfreq;
-    const int G = frt_phot_groups(F);
-    std::vector<float4> pos(n), dir(n), phot((size_t)G * n);
+    const int PL = frt_phot_planes(F);
+    std::vector<float4> pos(n), dir(n), phot(n);
+    std::vector<float> phot_hi((size_t)PL * n);
     std::vector<int4> meta(n);
     for (int64_t i = 0; i < n; ++i) {
       pos[i] = make_float4(v->x[i], v->y[i], v->z[i], v->time[i]);
@@ -1443,14 +1449,16 @@ int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* v) {
                           (v->order[i] & 0xffff) | ((int)v->itheta[i] << 16));
       float q[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       for (int nu = 0; nu < F; ++nu) q[nu] = v->nphot[i * F + nu];
-      for (int g = 0; g < G; ++g) phot[(size_t)g * n + i] = make_float4(q[4 * g], q[4 * g + 1], q[4 * g + 2], q[4 * g + 3]);
+      phot[i] = make_float4(q[0], q[1], q[2], q[3]);
+      for (int k = 0; k < PL; ++k) phot_hi[(size_t)k * n + i] = q[4 + k];
     }
     e = stream_copy(ctx, b.pos, pos.data(), 16 * (size_t)n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = stream_copy(ctx, b.dir, dir.data(), 16 * (size_t)n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = stream_copy(ctx, b.meta, meta.data(), 16 * (size_t)n, hipMemcpyHostToDevice);
-    for (int g = 0; g < G && e == hipSuccess; ++g)
-      e = stream_copy(ctx, b.phot + (size_t)g * ctx->g.cap, phot.data() + (size_t)g * n, 16 * (size_t)n,
-                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = stream_copy(ctx, b.phot, phot.data(), 16 * (size_t)n, hipMemcpyHostToDevice);
+    for (int k = 0; k < PL && e == hipSuccess; ++k)
+      e = stream_copy(ctx, (float*)(b.phot + ctx->g.cap) + (size_t)k * ctx->g.cap, phot_hi.data() + (size_t)k * n,
+                      4 * (size_t)n, hipMemcpyHostToDevice);
   }
   ctx->nrays = n;
   ctx->have_alive = false;

@@ -24,8 +24,10 @@
 //   pos  = {x, y, z, time}
 //   dir  = {xhat, yhat, zhat, r}
 //   meta = {dr (float bits), cell, pixel, order | itheta << 16}
-//   phot = photon counts in groups of four bins: phot[v * cap + i] holds bins
-//          4v..4v+3 of ray i (unused lanes are zero).
+//   phot = photon counts: bins 0..3 of ray i in phot[i] (unused lanes zero),
+//          bins 4..nfreq-1 in float planes after the cap vectors:
+//          ((float*)(phot + cap))[(nu - 4) * cap + i], so a ray's photons
+//          are 16 + 4 (nfreq - 4) bytes with no padding.
 struct RaySoA {
   float4* pos;
   float4* dir;
@@ -33,8 +35,12 @@ struct RaySoA {
   float4* phot;
 };
 
-// Number of 4-bin photon groups for nfreq bins.
-static inline __host__ __device__ int frt_phot_groups(int nfreq) { return (nfreq + 3) >> 2; }
+// Float planes of the photon bins beyond the first four.
+static inline __host__ __device__ int frt_phot_planes(int nfreq) { return nfreq > 4 ? nfreq - 4 : 0; }
+// Bytes of the photon arrays for cap rays.
+static inline __host__ __device__ size_t frt_phot_bytes(int nfreq, int64_t cap) {
+  return (size_t)cap * (16 + 4 * (size_t)frt_phot_planes(nfreq));
+}
 
 // Scalars every kernel may need, passed by value.
 struct GridConst {

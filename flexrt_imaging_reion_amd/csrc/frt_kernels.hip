@@ -241,17 +241,20 @@ __device__ __forceinline__ void ray_set_cell(const RaySoA& b, int64_t i, int cel
   reinterpret_cast<int*>(b.meta + i)[1] = cell;
 }
 
-// Photon counts of ray i in n[0..7] (bins >= F are zero).
+// Photon counts of ray i in n[0..7] (bins >= F are zero).  Per-bin loads
+// (a 16-byte vector load into the predicated per-bin array is a code form
+// avoided here).
 __device__ __forceinline__ void phot_load(const RaySoA& b, int64_t i, int F, int64_t cap, float* n) {
   const float* p0 = reinterpret_cast<const float*>(b.phot + i);
-  const float* p1 = reinterpret_cast<const float*>(b.phot + cap + i);
-  FRT_FOR_BINS(nu) n[nu] = (nu < 4) ? p0[nu] : p1[nu - 4];
+  const float* hi = reinterpret_cast<const float*>(b.phot + cap) + i;
+  FRT_FOR_BINS(nu) n[nu] = (nu < 4) ? p0[nu] : hi[(int64_t)(nu - 4) * cap];
   for (int nu = F; nu < 8; ++nu) n[nu] = 0.f;
 }
 
 __device__ __forceinline__ void phot_store(const RaySoA& b, int64_t i, int F, int64_t cap, const float* n) {
   b.phot[i] = make_float4(n[0], F > 1 ? n[1] : 0.f, F > 2 ? n[2] : 0.f, F > 3 ? n[3] : 0.f);
-  if (F > 4) b.phot[cap + i] = make_float4(n[4], F > 5 ? n[5] : 0.f, F > 6 ? n[6] : 0.f, F > 7 ? n[7] : 0.f);
+  float* hi = reinterpret_cast<float*>(b.phot + cap) + i;
+  for (int nu = 4; nu < F; ++nu) hi[(int64_t)(nu - 4) * cap] = n[nu];
 }
 
 // sum_nphot (io_funcs.cc:10-16): float sum over bins in bin order.
@@ -1713,7 +1716,7 @@ __global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, const S
                                                  const float* __restrict__ rot_fwd) {
   const int64_t total = sd->ncompact;
   const int F = g.nfreq;
-  const int nph = frt_phot_groups(F);
+  const int nplanes = frt_phot_planes(F);
   const int64_t cap = g.cap;
   for (int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x; q < total; q += (int64_t)gridDim.x * BLK) {
     const uint32_t packed = src_of[sorted_pos[q]];
@@ -1724,7 +1727,9 @@ __global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, const S
       dst.dir[q] = src.dir[i];
       dst.meta[q] = src.meta[i];
       dst.phot[q] = src.phot[i];
-      if (nph > 1) dst.phot[cap + q] = src.phot[cap + i];
+      const float* shi = reinterpret_cast<const float*>(src.phot + cap);
+      float* dhi = reinterpret_cast<float*>(dst.phot + cap);
+      for (int k = 0; k < nplanes; ++k) dhi[(int64_t)k * cap + q] = shi[(int64_t)k * cap + i];
       continue;
     }
     Ray par;
