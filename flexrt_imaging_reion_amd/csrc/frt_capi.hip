@@ -176,6 +176,7 @@ int apply_pull(frt_ctx* ctx, bool ran_march) {
   if (ctx->adj_cast_base == -2) ctx->adj_cast_base = h->nrays - (int64_t)ctx->nsrc * 12;
   if (h->err & FRT_SDERR_SPLIT_CAPACITY) {
     ctx->cur = 1 - ctx->cur;  // the compaction did not run
+    ctx->deferred = false;
     ctx->split_gen -= 1;
     ctx->adj_split_n = ctx->adj_split_total = -1;
     return set_error(ctx, FRT_E_CAPACITY,
@@ -195,6 +196,7 @@ int apply_pull(frt_ctx* ctx, bool ran_march) {
 // Rays held after removals: drop dead rays (no splits) so the buffer holds
 // live rays only.  Ends with a pull, so the host counts are current.
 int ensure_compact(frt_ctx* ctx) {
+  if (int rc0 = materialize_rays(ctx)) return rc0;
   if (!ctx->have_alive) return FRT_OK;
   int rc = launch_split_compact(ctx, true);
   ctx->have_alive = false;
@@ -326,6 +328,7 @@ static void graph_key(const frt_ctx* ctx, GraphKey* k) {
   int64_t b = 4096;
   while (b < ctx->nrays) b *= 2;
   k->capture_rays = std::min<int64_t>(b, ctx->g.cap);
+  k->deferred = ctx->deferred ? 1 : 0;
 }
 
 // The launches of one frt_step after its step_begin.
@@ -359,6 +362,8 @@ static int graph_step(frt_ctx* ctx) {
   if (!ctx->graph_valid[c]) {
     const int64_t split_gen = ctx->split_gen;
     const frt_step_stats stats = ctx->stats;
+    const bool deferred = ctx->deferred;
+    int32_t* const split_sorted = ctx->split_sorted;
     hipGraph_t graph = nullptr;
     ctx->capturing = true;
     ctx->capture_rays = key.capture_rays;
@@ -373,6 +378,10 @@ static int graph_step(frt_ctx* ctx) {
     ctx->cur = c;
     ctx->split_gen = split_gen;
     ctx->stats = stats;
+    ctx->deferred_after_graph[c] = ctx->deferred;  // the captured step's effect, applied per replay
+    ctx->sorted_after_graph[c] = ctx->split_sorted;
+    ctx->deferred = deferred;
+    ctx->split_sorted = split_sorted;
     ctx->have_alive = ctx->have_segments = false;
     if (rc == FRT_OK) {
       e = hipGraphInstantiate(&ctx->graph_exec[c], graph, nullptr, nullptr, 0);
@@ -390,6 +399,8 @@ static int graph_step(frt_ctx* ctx) {
   if (rc) return rc;
   ctx->cur = 1 - c;
   ctx->split_gen += 1;
+  ctx->deferred = ctx->deferred_after_graph[c];
+  ctx->split_sorted = ctx->sorted_after_graph[c];
   ctx->deposit_launches_last = ctx->cfg.itercount;
   ctx->stats.rays_new = (int64_t)ctx->nsrc * 12;
   return FRT_OK;
@@ -620,6 +631,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->adjr_step = 0;
   ctx->adjr_on = false;
   ctx->split_sorted = nullptr;
+  ctx->deferred = false;
   ctx->graph_on = 0;
   ctx->capturing = false;
   ctx->graph_valid[0] = ctx->graph_valid[1] = false;
@@ -856,6 +868,9 @@ int frt_update_cosmo_expansion(frt_ctx* ctx, float a_current, float a_prev) {
   if (!ctx) return FRT_E_ARG;
   if (!(a_current > 0.f) || !(a_prev > 0.f)) return set_error(ctx, FRT_E_ARG, "scale factors must be positive");
   if (!ctx->have_rho) return set_error(ctx, FRT_E_ARG, "update_cosmo_expansion needs rho: upload FRT_F_RHO first");
+  // a compaction left to the next march runs first: split children take the
+  // cell size of the step that split them
+  if (int rc0 = materialize_rays(ctx)) return rc0;
   // dx[i] *= a_current/a_prev (float); rho *= pow(a_prev/a_current, 3.), the
   // scalar computed on the host with the C library's pow, as the reference does
   const float ratio = a_current / a_prev;
@@ -875,6 +890,7 @@ int frt_update_hydro_step(frt_ctx* ctx, float zz, float a_rt, float avg_rho, flo
   if (!(avg_rho > 0.f) || !std::isfinite(dt_cosmic) || dt_cosmic == 0.f)
     return set_error(ctx, FRT_E_ARG, "need avg_rho > 0 and a finite, nonzero dt_cosmic");
   if (!ctx->have_rho) return set_error(ctx, FRT_E_ARG, "update_hydro_step needs rho: upload FRT_F_RHO first");
+  if (int rc0 = materialize_rays(ctx)) return rc0;  // children with the cell size of their split
   ctx->cfg.zz = zz;
   ctx->cfg.a_rt = a_rt;
   set_redshift(ctx->g, ctx->cfg, zz);
@@ -1151,6 +1167,7 @@ int frt_checkpoint_restore(frt_ctx* ctx, const frt_checkpoint* ck) {
   ctx->g.cap = cap;
   ctx->cfg = ck->cfg;
   ctx->nrays = ck->nrays;
+  ctx->deferred = false;
   ctx->t = ck->t;
   ctx->nphot_merge = ck->nphot_merge;
   ctx->gadget_step_time = ck->gadget_step_time;
@@ -1182,6 +1199,7 @@ int frt_init_rt(frt_ctx* ctx) {
   // fresh start: empty buffer, run clock and merge threshold at their initial values
   ctx->nrays = 0;
   ctx->cur = 0;
+  ctx->deferred = false;
   ctx->t = 0.f;
   ctx->step = 0;
   ctx->cast_calls = 0;
@@ -1240,7 +1258,7 @@ int frt_update_rays(frt_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
   if ((rc = push_step_scalars(ctx))) return rc;
-  if ((rc = ensure_compact(ctx))) return rc;
+  if (ctx->have_alive && (rc = ensure_compact(ctx))) return rc;  // a deferred compaction stays for the march
   if ((rc = ensure_segment_capacity(ctx))) return rc;
   if ((rc = reset_counters(ctx))) return rc;
   if ((rc = do_update_rays(ctx))) return rc;
@@ -1287,7 +1305,7 @@ int frt_step(frt_ctx* ctx, frt_step_stats* stats) {
     rc = graph_step(ctx);
   } else {
     rc = push_step_scalars(ctx);
-    if (rc == FRT_OK) rc = ensure_compact(ctx);
+    if (rc == FRT_OK && ctx->have_alive) rc = ensure_compact(ctx);  // a deferred compaction stays for the march
     if (rc == FRT_OK) rc = launch_step_begin(ctx, true);  // t += dt (main.cc:123)
     if (rc == FRT_OK) rc = step_body(ctx);
   }
@@ -1461,6 +1479,7 @@ int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* v) {
                       4 * (size_t)n, hipMemcpyHostToDevice);
   }
   ctx->nrays = n;
+  ctx->deferred = false;
   ctx->have_alive = false;
   ctx->have_segments = false;
   return hip_check(ctx, e, "upload rays");

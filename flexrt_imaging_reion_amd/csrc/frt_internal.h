@@ -174,6 +174,7 @@ struct GraphKey {
   int32_t tune[6];
   int32_t itercount;
   int64_t capture_rays;
+  int32_t deferred;
 };
 
 struct frt_ctx {
@@ -236,6 +237,7 @@ struct frt_ctx {
   StepDev* sd;          // device: this context's step scalars
   StepDev* hsd;         // pinned host mirror
   int32_t* split_sorted;  // compaction order of the last split ([total] positions)
+  bool deferred;          // the last split's compaction is left to the next march (or materialize_rays)
   // scan/sort workspace (frt_scan_sort.hip)
   int64_t* ss_part;
   int32_t* ss_hist;
@@ -247,6 +249,8 @@ struct frt_ctx {
   bool graph_valid[2];
   hipGraphExec_t graph_exec[2];
   GraphKey graph_key[2];
+  bool deferred_after_graph[2];   // host-side effects of a replay
+  int32_t* sorted_after_graph[2];
   // launch tuning (frt_set_tuning)
   int32_t tune_deposit_queue;   // 1: deposit passes pull chunks of the moved list from a queue
   int32_t tune_march_queue;     // 1: the march pulls chunks of the ray list from a queue
@@ -332,6 +336,7 @@ int launch_merge(frt_ctx* ctx);
 int launch_split_compact(frt_ctx* ctx, bool nosplit);
 int launch_step_begin(frt_ctx* ctx, bool advance);
 int launch_step_end(frt_ctx* ctx);
+int materialize_rays(frt_ctx* ctx);
 int launch_fill_alive(frt_ctx* ctx);
 int ensure_segment_capacity(frt_ctx* ctx);
 int launch_cast(frt_ctx* ctx);

@@ -285,24 +285,116 @@ __device__ __forceinline__ float face_time(float hat, float pos, float dx, float
   return INFINITY;                                                       // 9e99 stored in float
 }
 
+// Geometry of child ch (0..3) of split parent i (split_ray, ray_tracing.cc:482-619):
+// pixel 4p+ch at order l+1, direction rotated like the parent, position moved by
+// (v_child - v_parent) r and walked into the containing periodic cell.
+struct ChildGeo {
+  int cell;
+  int pix;
+  float vx, vy, vz;
+  float x, y, z;
+};
+
+__device__ __forceinline__ ChildGeo child_geometry(const Ray& par, int ch, const GridConst& g,
+                                                   const float* __restrict__ rot_fwd) {
+  const int l = par.order;
+  const int pn = par.pixel;
+  const float r0 = par.r;
+  const float* m = rot_fwd + 9 * (int64_t)par.itheta;
+  const double vox = (double)par.xhat, voy = (double)par.yhat, voz = (double)par.zhat;
+  const int n = g.n;
+  int ic, jc, kc;
+  frt_cell_ijk(g.lay, par.cell, ic, jc, kc);
+  const float rsf = r0 * g.sf;
+  ChildGeo c;
+  c.pix = 4 * pn + ch;
+  double px, py, pz;
+  hpx_pix2vec_nest(l + 1, c.pix, &px, &py, &pz);
+  rotate3(m, (float)px, (float)py, (float)pz, &c.vx, &c.vy, &c.vz);
+  float xn = (float)((double)par.x + ((double)c.vx - vox) * (double)rsf);
+  float yn = (float)((double)par.y + ((double)c.vy - voy) * (double)rsf);
+  float zn = (float)((double)par.z + ((double)c.vz - voz) * (double)rsf);
+  int icn = ic, jcn = jc, kcn = kc;
+  // periodic cell walk (ray_tracing.cc:549-572); the guard only bounds non-finite input
+  int w = 0;
+  while (xn > g.dx && w < 4096) { xn -= g.dx; icn = wrap_mod(icn + 1, n); ++w; }
+  while ((double)xn < 0. && w < 4096) { xn += g.dx; icn = wrap_mod(icn - 1, n); ++w; }
+  while (yn > g.dx && w < 4096) { yn -= g.dx; jcn = wrap_mod(jcn + 1, n); ++w; }
+  while ((double)yn < 0. && w < 4096) { yn += g.dx; jcn = wrap_mod(jcn - 1, n); ++w; }
+  while (zn > g.dx && w < 4096) { zn -= g.dx; kcn = wrap_mod(kcn + 1, n); ++w; }
+  while ((double)zn < 0. && w < 4096) { zn += g.dx; kcn = wrap_mod(kcn - 1, n); ++w; }
+  c.cell = frt_cell_index(g.lay, icn, jcn, kcn);
+  c.x = xn;
+  c.y = yn;
+  c.z = zn;
+  return c;
+}
+
+// The ray at position q of the compacted list after a split: ray i copied,
+// or child ch of split ray i (split_ray, ray_tracing.cc:482-619), with its
+// photons in n (a quarter of the parent's per bin).
+__device__ __forceinline__ void output_ray(const RaySoA& src, int64_t q, const GridConst& g,
+                                           const int32_t* __restrict__ n_out,
+                                           const int32_t* __restrict__ sorted_pos,
+                                           const uint32_t* __restrict__ src_of,
+                                           const float* __restrict__ rot_fwd, Ray& out, float* n) {
+  const uint32_t packed = src_of[sorted_pos[q]];
+  const int64_t i = (int64_t)(packed >> 2);
+  const int ch = (int)(packed & 3u);
+  ray_load(src, i, out);
+  phot_load(src, i, g.nfreq, g.cap, n);
+  if (n_out[i] == 1) return;
+  const Ray par = out;
+  const int l = par.order;
+  const ChildGeo c = child_geometry(par, ch, g, rot_fwd);
+  out.cell = c.cell;
+  out.pixel = c.pix;
+  out.order = l + 1;
+  out.xhat = c.vx;
+  out.yhat = c.vy;
+  out.zhat = c.vz;
+  out.dr = g.rmax_tab[l + 1] - par.r;
+  out.x = c.x;
+  out.y = c.y;
+  out.z = c.z;
+  const int F = g.nfreq;
+  FRT_FOR_BINS(nu) n[nu] = (float)((double)n[nu] / 4.);
+}
+
 // ---------------------------------------------------------------------------
 // advance_ray (ray_tracing.cc:1222-1512), C2Ray mode, fion = 1.
+// With GATHER, the list is still the previous split's permutation of the
+// other buffer (src): the first sd->ncompact rays are fetched through it
+// (output_ray, the compaction's gather) and written whole to rb after the
+// march, so the compaction and the march share one pass over the rays; the
+// rays after them (new_rays's) are in rb already.
+template <bool GATHER>
 __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst g,
                                                int32_t* __restrict__ path_cell,
                                                float* __restrict__ path_dist,
                                                int32_t* __restrict__ seg_cnt,
                                                uint8_t* __restrict__ alive,
-                                               DevCounters* cnt, int chunk) {
+                                               DevCounters* cnt, int chunk, RaySoA src,
+                                               const int32_t* __restrict__ n_out,
+                                               const int32_t* __restrict__ sorted_pos,
+                                               const uint32_t* __restrict__ src_of,
+                                               const float* __restrict__ rot_fwd) {
   unsigned long long nmoved = 0;
   bool overflow = false;
   const int64_t nrays = sd->nrays;
+  const int64_t ngather = GATHER ? sd->ncompact : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) seg_cnt[nrays] = 0;  // scanned with the rays
   for_items(nrays, chunk > 0 ? &sd->queue[FRT_Q_MARCH] : nullptr, chunk, [&](int64_t i) {
     bool moved = false;
     alive[i] = 1;
     const int n = g.n;
     Ray ry;
-    ray_load(rb, i, ry);
+    float nph[8];
+    const bool whole = GATHER && i < ngather;  // write every field of the ray
+    if (whole)
+      output_ray(src, i, g, n_out, sorted_pos, src_of, rot_fwd, ry, nph);
+    else
+      ray_load(rb, i, ry);
     int ci, cj, ck;
     frt_cell_ijk(g.lay, ry.cell, ci, cj, ck);
     const float xhat = ry.xhat, yhat = ry.yhat, zhat = ry.zhat;
@@ -315,7 +407,13 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst
     moved = (ax > hi) || (ay > hi) || (az > hi) || ((double)ax < lo) || ((double)ay < lo) ||
             ((double)az < lo);
     if (!moved) {
-      rb.pos[i] = make_float4(ry.x, ry.y, ry.z, deltat);
+      if (whole) {
+        ry.time = deltat;
+        ray_store(rb, i, ry);
+        phot_store(rb, i, g.nfreq, g.cap, nph);
+      } else {
+        rb.pos[i] = make_float4(ry.x, ry.y, ry.z, deltat);
+      }
       seg_cnt[i] = 0;
     } else {
       float r0 = ry.r;
@@ -414,6 +512,7 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst
       ry.z = z0;
       ry.time = 0.f;
       ray_store(rb, i, ry);
+      if (whole) phot_store(rb, i, g.nfreq, g.cap, nph);
     }
     nmoved += moved ? 1 : 0;
   });
@@ -1595,51 +1694,6 @@ __global__ void __launch_bounds__(BLK) k_split_flag(RaySoA rb, const StepDev* __
   block_add(&cnt->splits, nsplit);
 }
 
-// Geometry of child ch (0..3) of split parent i (split_ray, ray_tracing.cc:482-619):
-// pixel 4p+ch at order l+1, direction rotated like the parent, position moved by
-// (v_child - v_parent) r and walked into the containing periodic cell.
-struct ChildGeo {
-  int cell;
-  int pix;
-  float vx, vy, vz;
-  float x, y, z;
-};
-
-__device__ __forceinline__ ChildGeo child_geometry(const Ray& par, int ch, const GridConst& g,
-                                                   const float* __restrict__ rot_fwd) {
-  const int l = par.order;
-  const int pn = par.pixel;
-  const float r0 = par.r;
-  const float* m = rot_fwd + 9 * (int64_t)par.itheta;
-  const double vox = (double)par.xhat, voy = (double)par.yhat, voz = (double)par.zhat;
-  const int n = g.n;
-  int ic, jc, kc;
-  frt_cell_ijk(g.lay, par.cell, ic, jc, kc);
-  const float rsf = r0 * g.sf;
-  ChildGeo c;
-  c.pix = 4 * pn + ch;
-  double px, py, pz;
-  hpx_pix2vec_nest(l + 1, c.pix, &px, &py, &pz);
-  rotate3(m, (float)px, (float)py, (float)pz, &c.vx, &c.vy, &c.vz);
-  float xn = (float)((double)par.x + ((double)c.vx - vox) * (double)rsf);
-  float yn = (float)((double)par.y + ((double)c.vy - voy) * (double)rsf);
-  float zn = (float)((double)par.z + ((double)c.vz - voz) * (double)rsf);
-  int icn = ic, jcn = jc, kcn = kc;
-  // periodic cell walk (ray_tracing.cc:549-572); the guard only bounds non-finite input
-  int w = 0;
-  while (xn > g.dx && w < 4096) { xn -= g.dx; icn = wrap_mod(icn + 1, n); ++w; }
-  while ((double)xn < 0. && w < 4096) { xn += g.dx; icn = wrap_mod(icn - 1, n); ++w; }
-  while (yn > g.dx && w < 4096) { yn -= g.dx; jcn = wrap_mod(jcn + 1, n); ++w; }
-  while ((double)yn < 0. && w < 4096) { yn += g.dx; jcn = wrap_mod(jcn - 1, n); ++w; }
-  while (zn > g.dx && w < 4096) { zn -= g.dx; kcn = wrap_mod(kcn + 1, n); ++w; }
-  while ((double)zn < 0. && w < 4096) { zn += g.dx; kcn = wrap_mod(kcn - 1, n); ++w; }
-  c.cell = frt_cell_index(g.lay, icn, jcn, kcn);
-  c.x = xn;
-  c.y = yn;
-  c.z = zn;
-  return c;
-}
-
 // Tile of the cell of every output ray of the compaction (key of the stable
 // sort that fixes the list order); values are the compacted positions, src
 // records (input ray << 2 | child).  One thread per input ray.  The four
@@ -1917,9 +1971,29 @@ int launch_march(frt_ctx* ctx) {
   ctx->adj_valid = false;  // new paths: a recorded adjoint tape no longer matches
   const bool q = ctx->tune_march_queue != 0;
   if (q) FRT_LAUNCH(ctx, "queue_reset", 1, k_queue_reset, ctx->sd);
-  FRT_LAUNCH(ctx, "march", q ? queue_grid(ctx) : ray_grid(ctx, bound_rays(ctx)), k_march, cur_buf(ctx), ctx->sd,
-             ctx->g, ctx->path_cell, ctx->path_dist, ctx->seg_cnt, ctx->alive, ctx->dcnt,
-             q ? ctx->tune_queue_chunk : 0);
+  const unsigned grid = q ? queue_grid(ctx) : ray_grid(ctx, bound_rays(ctx));
+  const int chunk = q ? ctx->tune_queue_chunk : 0;
+  if (ctx->deferred) {  // the previous split's compaction, gathered by the march
+    FRT_LAUNCH(ctx, "march", grid, k_march<true>, cur_buf(ctx), ctx->sd, ctx->g, ctx->path_cell, ctx->path_dist,
+               ctx->seg_cnt, ctx->alive, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)ctx->n_out,
+               (const int32_t*)ctx->split_sorted, (const uint32_t*)ctx->cand_key_tmp, (const float*)ctx->rot_fwd);
+    ctx->deferred = false;
+  } else {
+    FRT_LAUNCH(ctx, "march", grid, k_march<false>, cur_buf(ctx), ctx->sd, ctx->g, ctx->path_cell, ctx->path_dist,
+               ctx->seg_cnt, ctx->alive, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)nullptr,
+               (const int32_t*)nullptr, (const uint32_t*)nullptr, (const float*)ctx->rot_fwd);
+  }
+  return FRT_OK;
+}
+
+// The compaction a split left to the next march, run on its own: whatever
+// reads or rewrites the rays before that march sees the compacted list.
+int materialize_rays(frt_ctx* ctx) {
+  if (!ctx->deferred) return FRT_OK;
+  const int64_t nb = std::min<int64_t>(ctx->g.cap, 4 * ctx->nrays + 1);
+  FRT_LAUNCH(ctx, "compact", ray_grid(ctx, nb), k_compact, ctx->buf[1 - ctx->cur], cur_buf(ctx), ctx->sd, ctx->g,
+             ctx->n_out, ctx->split_sorted, ctx->cand_key_tmp, ctx->rot_fwd);
+  ctx->deferred = false;
   return FRT_OK;
 }
 
@@ -2053,6 +2127,7 @@ __global__ void __launch_bounds__(BLK) k_tape_nph(RaySoA rb, int64_t n, int F, i
 }
 
 int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
+  if (int rc0 = materialize_rays(ctx)) return rc0;
   const int I = ctx->cfg.itercount;
   const int64_t n = ctx->nrays;
   const bool mixed = ctx->cfg.precision == FRT_PREC_MIXED;
@@ -2176,6 +2251,7 @@ int launch_thermal(frt_ctx* ctx) {
 // count_rays, calc_nphot_merge (if needed) and merge, with every decision
 // taken on the device.
 int launch_merge(frt_ctx* ctx) {
+  if (int rc0 = materialize_rays(ctx)) return rc0;
   const int64_t nb = bound_rays(ctx);
   GridConst g = ctx->g;
   RaySoA rb = cur_buf(ctx);
@@ -2212,6 +2288,7 @@ int launch_merge(frt_ctx* ctx) {
 
 // split_rays into the other buffer (nosplit: compaction only).
 int launch_split_compact(frt_ctx* ctx, bool nosplit) {
+  if (int rc0 = materialize_rays(ctx)) return rc0;
   const int64_t nb = bound_rays(ctx);
   GridConst g = ctx->g;
   StepDev* sd = ctx->sd;
@@ -2227,11 +2304,15 @@ int launch_split_compact(frt_ctx* ctx, bool nosplit) {
   const int64_t nout = std::min<int64_t>(g.cap, 4 * nb);
   rc = dev_sort_pairs(ctx, ctx->m_key, ctx->m_val, &sd->ncompact, 0, nout, ctx->tile_bits, &which, "sort_rays");
   if (rc) return rc;
-  // one thread per input ray; the grid-stride loop takes the split children beyond
-  FRT_LAUNCH(ctx, "compact", ray_grid(ctx, nb), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], sd, g, ctx->n_out,
-             ctx->m_val[which], ctx->cand_key_tmp, ctx->rot_fwd);
-  FRT_LAUNCH(ctx, "split_done", 1, k_split_done, sd);
   ctx->split_sorted = ctx->m_val[which];
+  if (nosplit) {
+    // one thread per input ray; the grid-stride loop takes the split children beyond
+    FRT_LAUNCH(ctx, "compact", ray_grid(ctx, nb), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], sd, g, ctx->n_out,
+               ctx->m_val[which], ctx->cand_key_tmp, ctx->rot_fwd);
+  } else {
+    ctx->deferred = true;  // the next march gathers the list through the permutation
+  }
+  FRT_LAUNCH(ctx, "split_done", 1, k_split_done, sd);
   ctx->cur = 1 - ctx->cur;
   ctx->split_gen += 1;
   if (ctx->adj_on && !nosplit) {  // adjoint tape: the split's input count (its output count after the pull)
@@ -2499,6 +2580,7 @@ __global__ void __launch_bounds__(BLK) k_otf_final(const double* __restrict__ pa
 }
 
 int launch_otf(frt_ctx* ctx, float avg_rho, float temp_0, double* part, double* out) {
+  if (int rc0 = materialize_rays(ctx)) return rc0;
   const unsigned nb = (unsigned)std::min<int64_t>((ctx->g.n3 + BLK - 1) / BLK, FRT_OTF_BLOCKS);
   const unsigned nbr = (unsigned)std::max<int64_t>(1, std::min<int64_t>((ctx->nrays + BLK - 1) / BLK, FRT_OTF_BLOCKS));
   double* rpart = part + (size_t)FRT_OTF_BLOCKS * FRT_OTF_NV;
@@ -2593,6 +2675,8 @@ __global__ void __launch_bounds__(BLK) k_cosmo_rays(float4* __restrict__ pos, in
 }
 
 int launch_cosmo_expansion(frt_ctx* ctx, float ratio, double rho_scale, bool rays) {
+  if (rays)
+    if (int rc0 = materialize_rays(ctx)) return rc0;
   FRT_LAUNCH(ctx, "cosmo_cells", grid_for(ctx->g.n3), k_cosmo_cells, ctx->c, (int64_t)ctx->g.n3, rho_scale);
   if (rays && ctx->nrays > 0)
     FRT_LAUNCH(ctx, "cosmo_rays", grid_for(ctx->nrays), k_cosmo_rays, cur_buf(ctx).pos, ctx->nrays, ratio,
@@ -2651,6 +2735,7 @@ int launch_hydro_reload(frt_ctx* ctx, double rho_scale, float dt_cosmic) {
 }
 
 int launch_pack_ray_file(frt_ctx* ctx, uint8_t* out) {
+  if (int rc0 = materialize_rays(ctx)) return rc0;
   FRT_LAUNCH(ctx, "pack_ray_file", grid_for(ctx->nrays), k_pack_ray_file, cur_buf(ctx), ctx->nrays, ctx->g, out);
   return FRT_OK;
 }
