@@ -177,6 +177,7 @@ int apply_pull(frt_ctx* ctx, bool ran_march) {
   if (h->err & FRT_SDERR_SPLIT_CAPACITY) {
     ctx->cur = 1 - ctx->cur;  // the compaction did not run
     ctx->deferred = false;
+    ctx->psum_state = FRT_PSUM_NONE;
     ctx->split_gen -= 1;
     ctx->adj_split_n = ctx->adj_split_total = -1;
     return set_error(ctx, FRT_E_CAPACITY,
@@ -401,6 +402,7 @@ static int graph_step(frt_ctx* ctx) {
   ctx->split_gen += 1;
   ctx->deferred = ctx->deferred_after_graph[c];
   ctx->split_sorted = ctx->sorted_after_graph[c];
+  ctx->psum_state = FRT_PSUM_NONE;  // a step ends with the cast
   ctx->deposit_launches_last = ctx->cfg.itercount;
   ctx->stats.rays_new = (int64_t)ctx->nsrc * 12;
   return FRT_OK;
@@ -632,6 +634,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   ctx->adjr_on = false;
   ctx->split_sorted = nullptr;
   ctx->deferred = false;
+  ctx->psum_state = FRT_PSUM_NONE;
   ctx->graph_on = 0;
   ctx->capturing = false;
   ctx->graph_valid[0] = ctx->graph_valid[1] = false;
@@ -660,6 +663,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e == hipSuccess) e = dalloc(&ctx->seg_cnt, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->seg_off, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->alive, cap + 1);
+    if (e == hipSuccess) e = dalloc(&ctx->psum, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->n_out, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->out_off, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->cand_flag, cap + 1);
@@ -728,7 +732,7 @@ int frt_destroy(frt_ctx* ctx) {
   if (ctx->hsd) hipHostFree(ctx->hsd);
   for (int b = 0; b < 2; ++b) free_rays(ctx->buf[b]);
   hipFree(ctx->path_cell); hipFree(ctx->path_dist); hipFree(ctx->seg_cnt); hipFree(ctx->seg_off);
-  hipFree(ctx->alive); hipFree(ctx->n_out); hipFree(ctx->out_off); hipFree(ctx->cand_flag);
+  hipFree(ctx->alive); hipFree(ctx->psum); hipFree(ctx->n_out); hipFree(ctx->out_off); hipFree(ctx->cand_flag);
   hipFree(ctx->cand_off); hipFree(ctx->cand_key_tmp); hipFree(ctx->moved_idx);
   for (int b = 0; b < 2; ++b) {
     hipFree(ctx->seg_key[b]); hipFree(ctx->seg_val[b]); hipFree(ctx->m_key[b]); hipFree(ctx->m_val[b]);
@@ -1168,6 +1172,7 @@ int frt_checkpoint_restore(frt_ctx* ctx, const frt_checkpoint* ck) {
   ctx->cfg = ck->cfg;
   ctx->nrays = ck->nrays;
   ctx->deferred = false;
+  ctx->psum_state = FRT_PSUM_NONE;
   ctx->t = ck->t;
   ctx->nphot_merge = ck->nphot_merge;
   ctx->gadget_step_time = ck->gadget_step_time;
@@ -1200,6 +1205,7 @@ int frt_init_rt(frt_ctx* ctx) {
   ctx->nrays = 0;
   ctx->cur = 0;
   ctx->deferred = false;
+  ctx->psum_state = FRT_PSUM_NONE;
   ctx->t = 0.f;
   ctx->step = 0;
   ctx->cast_calls = 0;
@@ -1480,6 +1486,7 @@ int frt_upload_rays(frt_ctx* ctx, const frt_ray_view* v) {
   }
   ctx->nrays = n;
   ctx->deferred = false;
+  ctx->psum_state = FRT_PSUM_NONE;
   ctx->have_alive = false;
   ctx->have_segments = false;
   return hip_check(ctx, e, "upload rays");

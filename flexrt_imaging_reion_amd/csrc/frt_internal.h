@@ -166,6 +166,9 @@ struct ContribPtrs {
 
 // Host values a captured step depends on (kernel arguments and launch
 // shapes): a replay is valid while they are unchanged.
+// psum_state: nothing known, the march's rays that did not move, every ray.
+enum : int { FRT_PSUM_NONE = 0, FRT_PSUM_UNMOVED = 1, FRT_PSUM_VALID = 2 };
+
 struct GraphKey {
   GridConst g;
   int32_t nsrc;
@@ -238,6 +241,13 @@ struct frt_ctx {
   StepDev* hsd;         // pinned host mirror
   int32_t* split_sorted;  // compaction order of the last split ([total] positions)
   bool deferred;          // the last split's compaction is left to the next march (or materialize_rays)
+  // psum[i]: the float photon sum of ray i (sum_nphot, in bin order), the
+  // value the merge histogram, the merge flag and the split flag compare.
+  // The march writes it for the rays it does not move and for the rays it
+  // gathers, the last deposit pass for the moved rays, merge_groups for the
+  // merged rays; psum_state says which of them hold (FRT_PSUM_*).
+  float* psum;            // [cap + 1]
+  int psum_state;
   // scan/sort workspace (frt_scan_sort.hip)
   int64_t* ss_part;
   int32_t* ss_hist;
@@ -337,6 +347,7 @@ int launch_split_compact(frt_ctx* ctx, bool nosplit);
 int launch_step_begin(frt_ctx* ctx, bool advance);
 int launch_step_end(frt_ctx* ctx);
 int materialize_rays(frt_ctx* ctx);
+int ensure_psum(frt_ctx* ctx);
 int launch_fill_alive(frt_ctx* ctx);
 int ensure_segment_capacity(frt_ctx* ctx);
 int launch_cast(frt_ctx* ctx);

@@ -374,6 +374,7 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst
                                                float* __restrict__ path_dist,
                                                int32_t* __restrict__ seg_cnt,
                                                uint8_t* __restrict__ alive,
+                                               float* __restrict__ psum,
                                                DevCounters* cnt, int chunk, RaySoA src,
                                                const int32_t* __restrict__ n_out,
                                                const int32_t* __restrict__ sorted_pos,
@@ -407,12 +408,15 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst
     moved = (ax > hi) || (ay > hi) || (az > hi) || ((double)ax < lo) || ((double)ay < lo) ||
             ((double)az < lo);
     if (!moved) {
+      // photon sum of a ray the deposit passes do not visit
       if (whole) {
         ry.time = deltat;
         ray_store(rb, i, ry);
         phot_store(rb, i, g.nfreq, g.cap, nph);
+        psum[i] = phot_sum(nph, g.nfreq);
       } else {
         rb.pos[i] = make_float4(ry.x, ry.y, ry.z, deltat);
+        psum[i] = sum_nphot(rb, i, g.nfreq, g.cap);
       }
       seg_cnt[i] = 0;
     } else {
@@ -615,6 +619,10 @@ __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict_
 
 // Photon record of a segment: nin[0..F-1], the signed length at F, zeros up
 // to the record length (a multiple of 4 values), written as 16-byte stores.
+// (A record of F + 1 values rounded to even, 24 instead of 32 bytes at five
+// bins, measured no faster: the cell kernel's walk then takes three 8-byte
+// loads per record instead of two 16-byte ones, and each load of a wavefront
+// touches one cache line per lane.)
 #define FRT_MAX_REC 12  // record values at most: FRT_MAX_FREQ + 1 rounded up to 4
 static_assert(FRT_MAX_FREQ + 1 <= FRT_MAX_REC, "record too short");
 
@@ -861,7 +869,8 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, const int32_t* 
                                                       const int32_t* __restrict__ path_slot,
                                                       const float2* __restrict__ avg2,
                                                       ContribPtrs cp, uint8_t* __restrict__ alive,
-                                                      DevCounters* cnt, uint8_t* __restrict__ tape_np) {
+                                                      DevCounters* cnt, uint8_t* __restrict__ tape_np,
+                                                      float* __restrict__ psum) {
   unsigned long long ndumped = 0, nremoved = 0;
   (void)chunk;
   const int64_t nmoved = sd->nmoved;
@@ -920,6 +929,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_fp64(RaySoA rb, const int32_t* 
           sum += v;
         }
         phot_store(rb, i, F, g.cap, n1);
+        psum[i] = sum;  // = phot_sum(n1): the same float sum in bin order
         dumped_flag = dumped;
         if ((double)sum > g.nphot_min) {
           ray_set_cell(rb, i, last_cell);
@@ -991,7 +1001,8 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, const int32_t*
                                                        const int32_t* __restrict__ path_slot,
                                                        const float2* __restrict__ avg2,
                                                        ContribPtrs cp, uint8_t* __restrict__ alive,
-                                                       DevCounters* cnt, uint8_t* __restrict__ tape_np) {
+                                                       DevCounters* cnt, uint8_t* __restrict__ tape_np,
+                                                       float* __restrict__ psum) {
   unsigned long long ndumped = 0, nremoved = 0;
   // 1/(dt vol) in units of phot_norm photons
   const float inv_dtv = (float)(FRT_PHOT_NORM / ((double)g.dt * g.vol));
@@ -1045,6 +1056,7 @@ __global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, const int32_t*
           sum += v;
         }
         phot_store(rb, i, F, g.cap, n1);
+        psum[i] = sum;  // = phot_sum(n1): the same float sum in bin order
         dumped_flag = dumped;
         if ((double)sum > g.nphot_min) {
           ray_set_cell(rb, i, last_cell);
@@ -1456,7 +1468,7 @@ __global__ void __launch_bounds__(BLK) k_merge_setup(StepDev* sd, const DevCount
 // result does not depend on order).
 static constexpr unsigned PDF_GRID = 1024;
 
-__global__ void __launch_bounds__(BLK) k_pdf(RaySoA rb, const StepDev* __restrict__ sd, GridConst g,
+__global__ void __launch_bounds__(BLK) k_pdf(const float* __restrict__ psum, const StepDev* __restrict__ sd,
                                              const uint8_t* __restrict__ alive, double lg_min,
                                              double del, int32_t* __restrict__ pdf) {
   if (!sd->pdf_on) return;
@@ -1466,7 +1478,7 @@ __global__ void __launch_bounds__(BLK) k_pdf(RaySoA rb, const StepDev* __restric
   __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
     if (!alive[i]) continue;
-    const float s = sum_nphot(rb, i, g.nfreq, g.cap);
+    const float s = psum[i];
     // (int) maxd((log10(sum) - lg_min)/del, 0.): maxd narrows to float
     int ind = (int)frt_maxd((float)((dm_log10((double)s) - lg_min) / del), 0.f);
     if (ind > FRT_PDF_BINS - 1) ind = FRT_PDF_BINS - 1;
@@ -1524,6 +1536,7 @@ __global__ void __launch_bounds__(BLK) k_pdf_walk(StepDev* sd, const int32_t* __
 
 __global__ void __launch_bounds__(BLK) k_merge_flag(RaySoA rb, const StepDev* __restrict__ sd, GridConst g,
                                                     const uint8_t* __restrict__ alive,
+                                                    const float* __restrict__ psum,
                                                     const float* __restrict__ rot_inv,
                                                     int32_t* __restrict__ cand,
                                                     uint32_t* __restrict__ key) {
@@ -1534,7 +1547,7 @@ __global__ void __launch_bounds__(BLK) k_merge_flag(RaySoA rb, const StepDev* __
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i <= nrays; i += (int64_t)gridDim.x * BLK) {
     int c = 0;
     if (i < nrays && alive[i]) {
-      const float s = sum_nphot(rb, i, g.nfreq, g.cap);
+      const float s = psum[i];
       if (s < nphot_merge) {
         c = 1;
         const float4 d = rb.dir[i];
@@ -1580,7 +1593,7 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, const StepDev* 
                                                       const uint32_t* __restrict__ key,
                                                       const int32_t* __restrict__ val,
                                                       uint8_t* __restrict__ alive, const float* __restrict__ rot_fwd,
-                                                      float safety, DevCounters* cnt) {
+                                                      float safety, DevCounters* cnt, float* __restrict__ psum) {
   const int64_t ncand = sd->ncand;
   const uint32_t itheta_m = sd->itm;
   const float* fwd = rot_fwd + 9 * (int64_t)itheta_m;
@@ -1655,6 +1668,7 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, const StepDev* 
       rbase.zhat = oz;
     }
     phot_store(rb, b, F, cap, NP);
+    psum[b] = ntot;  // = phot_sum(NP)
     rbase.r = R;
     rbase.dr = DR;
     rbase.x = X;
@@ -1672,6 +1686,7 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, const StepDev* 
 // split_rays decision (rt_funcs.cc:211-237): rays out per input ray.
 __global__ void __launch_bounds__(BLK) k_split_flag(RaySoA rb, const StepDev* __restrict__ sd, GridConst g,
                                                     const uint8_t* __restrict__ alive,
+                                                    const float* __restrict__ psum,
                                                     int nosplit, int32_t* __restrict__ n_out,
                                                     DevCounters* cnt) {
   const int64_t nrays = sd->nrays;
@@ -1684,8 +1699,8 @@ __global__ void __launch_bounds__(BLK) k_split_flag(RaySoA rb, const StepDev* __
     if (i < nrays && alive[i]) {
       const int l = rb.meta[i].w & 0xffff;
       const float r = rb.dir[i].w;
-      const bool split = (r > g.rmax_tab[l]) && (l < g.l_max) &&
-                         (sum_nphot(rb, i, g.nfreq, g.cap) > split_thresh);
+      // (a compaction alone splits nothing and reads no photon sum)
+      const bool split = !nosplit && (r > g.rmax_tab[l]) && (l < g.l_max) && (psum[i] > split_thresh);
       out = split ? 4 : 1;
       nsplit += split ? 1 : 0;
     }
@@ -1975,14 +1990,15 @@ int launch_march(frt_ctx* ctx) {
   const int chunk = q ? ctx->tune_queue_chunk : 0;
   if (ctx->deferred) {  // the previous split's compaction, gathered by the march
     FRT_LAUNCH(ctx, "march", grid, k_march<true>, cur_buf(ctx), ctx->sd, ctx->g, ctx->path_cell, ctx->path_dist,
-               ctx->seg_cnt, ctx->alive, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)ctx->n_out,
+               ctx->seg_cnt, ctx->alive, ctx->psum, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)ctx->n_out,
                (const int32_t*)ctx->split_sorted, (const uint32_t*)ctx->cand_key_tmp, (const float*)ctx->rot_fwd);
     ctx->deferred = false;
   } else {
     FRT_LAUNCH(ctx, "march", grid, k_march<false>, cur_buf(ctx), ctx->sd, ctx->g, ctx->path_cell, ctx->path_dist,
-               ctx->seg_cnt, ctx->alive, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)nullptr,
+               ctx->seg_cnt, ctx->alive, ctx->psum, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)nullptr,
                (const int32_t*)nullptr, (const uint32_t*)nullptr, (const float*)ctx->rot_fwd);
   }
+  ctx->psum_state = FRT_PSUM_UNMOVED;
   return FRT_OK;
 }
 
@@ -1994,6 +2010,24 @@ int materialize_rays(frt_ctx* ctx) {
   FRT_LAUNCH(ctx, "compact", ray_grid(ctx, nb), k_compact, ctx->buf[1 - ctx->cur], cur_buf(ctx), ctx->sd, ctx->g,
              ctx->n_out, ctx->split_sorted, ctx->cand_key_tmp, ctx->rot_fwd);
   ctx->deferred = false;
+  ctx->psum_state = FRT_PSUM_NONE;
+  return FRT_OK;
+}
+
+// Photon sums of all rays from their photons, where the march and the
+// deposit passes have not left them (entry points called out of step order).
+__global__ void __launch_bounds__(BLK) k_psum(RaySoA rb, const StepDev* __restrict__ sd, GridConst g,
+                                              float* __restrict__ psum) {
+  const int64_t nrays = sd->nrays;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK)
+    psum[i] = sum_nphot(rb, i, g.nfreq, g.cap);
+}
+
+int ensure_psum(frt_ctx* ctx) {
+  if (ctx->psum_state == FRT_PSUM_VALID) return FRT_OK;
+  FRT_LAUNCH(ctx, "psum", std::min(ray_grid(ctx, bound_rays(ctx)), MAX_GRID), k_psum, cur_buf(ctx), ctx->sd,
+             ctx->g, ctx->psum);
+  ctx->psum_state = FRT_PSUM_VALID;
   return FRT_OK;
 }
 
@@ -2176,7 +2210,7 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     // 2 -> 3 waves per SIMD); the mixed kernels were not faster specialized
     // (first passes 71 -> 75 VGPRs; last pass 84 -> 69 but 3 % slower)
 #define FRT_DEP_ARGS rb, ctx->moved_idx, sd, head, chunk, g, ctx->path_cell, ctx->path_dist, ctx->seg_cnt, \
-                     ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt, tnp
+                     ctx->path_slot, ctx->c.avg2, ctx->contrib, ctx->alive, ctx->dcnt, tnp, ctx->psum
     if (mixed) {
       if (last)
         FRT_LAUNCH(ctx, "deposit_mixed", dgrid_mixed, (k_deposit_mixed<true>), FRT_DEP_ARGS);
@@ -2240,6 +2274,9 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
     ctx->adj_iters = I;
     ctx->adj_nrays = n;
   }
+  // the last pass wrote the photon sums of the moved rays; the march's hold
+  // for the others
+  if (I > 0 && ctx->psum_state != FRT_PSUM_NONE) ctx->psum_state = FRT_PSUM_VALID;
   return FRT_OK;
 }
 
@@ -2266,10 +2303,13 @@ int launch_merge(frt_ctx* ctx) {
   const double lg_min = dm_log10(ctx->nphot_min);
   const double del = (lg_max - lg_min) / (10000 - 1);
   FRT_LAUNCH(ctx, "merge_setup", 1, k_merge_setup, sd, (const DevCounters*)ctx->dcnt, num_thresh, ctx->pdf);
-  FRT_LAUNCH(ctx, "pdf", std::min(ray_grid(ctx, nb), PDF_GRID), k_pdf, rb, sd, g, ctx->alive, lg_min, del, ctx->pdf);
+  if ((rc = ensure_psum(ctx))) return rc;
+  FRT_LAUNCH(ctx, "pdf", std::min(ray_grid(ctx, nb), PDF_GRID), k_pdf, (const float*)ctx->psum, sd, ctx->alive, lg_min,
+             del, ctx->pdf);
   FRT_LAUNCH(ctx, "pdf_walk", 1, k_pdf_walk, sd, (const int32_t*)ctx->pdf, num_thresh, lg_min, del, ctx->cfg.seed);
   // merge(): rotation index drawn every pass (ray_tracing.cc:327)
-  FRT_LAUNCH(ctx, "merge_flag", ray_grid(ctx, nb + 1), k_merge_flag, rb, sd, g, ctx->alive, ctx->rot_inv,
+  FRT_LAUNCH(ctx, "merge_flag", ray_grid(ctx, nb + 1), k_merge_flag, rb, sd, g, ctx->alive, (const float*)ctx->psum,
+             ctx->rot_inv,
              ctx->cand_flag, ctx->cand_key_tmp);
   rc = dev_scan(ctx, ctx->cand_flag, ctx->cand_off, &sd->mcount, 0, nb + 1, &sd->ncand, "scan_merge");
   if (rc) return rc;
@@ -2282,7 +2322,7 @@ int launch_merge(frt_ctx* ctx) {
   if (rc) return rc;
   FRT_LAUNCH(ctx, "merge_groups", std::min(ray_grid(ctx, nb), 4 * MAX_GRID), k_merge_groups, rb, sd, g,
              ctx->m_key[which], ctx->m_val[which],
-             ctx->alive, ctx->rot_fwd, (float)ctx->cfg.safety_factor, ctx->dcnt);
+             ctx->alive, ctx->rot_fwd, (float)ctx->cfg.safety_factor, ctx->dcnt, ctx->psum);
   return FRT_OK;
 }
 
@@ -2292,8 +2332,11 @@ int launch_split_compact(frt_ctx* ctx, bool nosplit) {
   const int64_t nb = bound_rays(ctx);
   GridConst g = ctx->g;
   StepDev* sd = ctx->sd;
+  if (!nosplit) {
+    if (int rc0 = ensure_psum(ctx)) return rc0;
+  }
   FRT_LAUNCH(ctx, "split_flag", std::min(ray_grid(ctx, nb + 1), MAX_GRID), k_split_flag, cur_buf(ctx), sd, g,
-             ctx->alive, nosplit ? 1 : 0, ctx->n_out, ctx->dcnt);
+             ctx->alive, (const float*)ctx->psum, nosplit ? 1 : 0, ctx->n_out, ctx->dcnt);
   int rc = dev_scan(ctx, ctx->n_out, ctx->out_off, &sd->nrays, 1, nb + 1, &sd->nsplit, "scan_split");
   if (rc) return rc;
   FRT_LAUNCH(ctx, "split_check", 1, k_split_check, sd, g.cap, (int64_t)ctx->nsrc * 12);
@@ -2313,6 +2356,7 @@ int launch_split_compact(frt_ctx* ctx, bool nosplit) {
     ctx->deferred = true;  // the next march gathers the list through the permutation
   }
   FRT_LAUNCH(ctx, "split_done", 1, k_split_done, sd);
+  ctx->psum_state = FRT_PSUM_NONE;  // a new list order
   ctx->cur = 1 - ctx->cur;
   ctx->split_gen += 1;
   if (ctx->adj_on && !nosplit) {  // adjoint tape: the split's input count (its output count after the pull)
@@ -2328,6 +2372,7 @@ int launch_cast(frt_ctx* ctx) {
   const int64_t nnew = (int64_t)ctx->nsrc * 12;
   FRT_LAUNCH(ctx, "cast", grid_for(nnew), k_cast, cur_buf(ctx), ctx->sd, ctx->nsrc, ctx->g, ctx->src_cell,
              ctx->src_lum, ctx->src_ray_dt, ctx->rot_fwd, ctx->cfg.seed);
+  ctx->psum_state = FRT_PSUM_NONE;  // new rays without photon sums
   if (ctx->adj_on && ctx->adj_raydt && ctx->adj_raydt_cap >= ctx->nsrc) {  // adjoint tape: the ray_dt the cast used
     hipError_t e = hipMemcpyAsync(ctx->adj_raydt, ctx->src_ray_dt, 4 * (size_t)ctx->nsrc, hipMemcpyDeviceToDevice,
                                   ctx->stream);
