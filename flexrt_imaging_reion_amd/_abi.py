@@ -40,7 +40,7 @@ class FrtConfig(C.Structure):
         ("precision", C.c_int32), ("hpx_lvl", C.c_int32), ("l_max", C.c_int32),
         ("ray_per_cell_max", C.c_int32), ("rec_case_b", C.c_int32), ("temp_ev", C.c_int32),
         ("recomb_cool", C.c_int32), ("free_free", C.c_int32), ("compton", C.c_int32),
-        ("frac_limit", C.c_int32), ("reserved0", C.c_int32), ("reserved1", C.c_int32),
+        ("frac_limit", C.c_int32), ("subgrid", C.c_int32), ("reserved1", C.c_int32),
         ("ray_capacity", C.c_int64), ("seed", C.c_uint64),
         ("box_ckpc", C.c_double), ("a_rt", C.c_double), ("zz", C.c_double), ("dt", C.c_double),
         ("cl_factor", C.c_double), ("nl_const", C.c_double), ("safety_factor", C.c_double),
@@ -162,3 +162,53 @@ def config_struct(cfg):
         if name in cfg:
             setattr(s, name, cfg[name])
     return s
+
+
+# ---- sub-grid I-front model (frt_subgrid, frt_subgrid_state) ----------------------------
+NTCROSS = 10
+SUBGRID_STATE = [("fion", 1), ("fion_max", 1), ("tcross", 1), ("tcross_flag", 1), ("treion_out", 1),
+                 ("delta_fion", 1), ("tcross_new", NTCROSS), ("tcross_check", NTCROSS), ("fh1_nodes", 5)]
+
+
+class FrtSubgrid(C.Structure):
+    _fields_ = [("nt", C.c_int32), ("reserved", C.c_int32), ("time_rh", C.POINTER(C.c_float)),
+                ("fh1_rh", C.POINTER(C.c_float)), ("tre_dom", C.c_float * 3), ("t_z12", C.c_float),
+                ("t_z8", C.c_float), ("tinit", C.c_float), ("avg_rho", C.c_float), ("temp_0", C.c_float),
+                ("fh1_0", C.c_float), ("delta_nl", C.c_float * 3)]
+
+
+class FrtSubgridState(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_float)) for n, _ in SUBGRID_STATE]
+
+
+def make_subgrid(sg):
+    """An frt_subgrid over a dict (see subgrid.make_params); returns (struct, arrays kept alive)."""
+    keep = {k: np.ascontiguousarray(sg[k], dtype=np.float32).reshape(-1) for k in ("time_rh", "fh1_rh")}
+    nt = int(sg["nt"])
+    assert keep["time_rh"].size == 27 * nt and keep["fh1_rh"].size == 27 * nt
+    s = FrtSubgrid()
+    s.nt = nt
+    s.time_rh = keep["time_rh"].ctypes.data_as(C.POINTER(C.c_float))
+    s.fh1_rh = keep["fh1_rh"].ctypes.data_as(C.POINTER(C.c_float))
+    for i in range(3):
+        s.tre_dom[i] = float(sg["tre_dom"][i])
+        s.delta_nl[i] = float(sg["delta_nl"][i])
+    for k in ("t_z12", "t_z8", "tinit", "avg_rho", "temp_0", "fh1_0"):
+        setattr(s, k, float(sg[k]))
+    return s, keep
+
+
+def make_subgrid_state(n3, arrays=None):
+    """An frt_subgrid_state over numpy arrays: the given ones (upload), or
+    fresh arrays for every member (download).  Returns (struct, arrays)."""
+    if arrays is None:
+        arrays = {n: np.zeros(n3 * w, dtype=np.float32) for n, w in SUBGRID_STATE}
+    else:
+        arrays = {n: np.ascontiguousarray(arrays[n], dtype=np.float32).reshape(-1) for n, _ in SUBGRID_STATE
+                  if n in arrays}
+    st = FrtSubgridState()
+    for n, w in SUBGRID_STATE:
+        if n in arrays:
+            assert arrays[n].size == n3 * w, n
+            setattr(st, n, arrays[n].ctypes.data_as(C.POINTER(C.c_float)))
+    return st, arrays

@@ -6,7 +6,7 @@ import of any GPU entry point raises: there is no CPU fallback.
 import ctypes as C
 import os
 
-from ._abi import FrtConfig, FrtFileExtras, FrtOtf, FrtRayView, FrtStepStats
+from ._abi import FrtConfig, FrtFileExtras, FrtOtf, FrtRayView, FrtStepStats, FrtSubgrid, FrtSubgridState
 from ._build import LIB_PATH
 
 _lib = None
@@ -23,6 +23,10 @@ _PROTOS = {
     "frt_upload_field": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_float)]),
     "frt_download_field": (C.c_int, [_CTX, C.c_int32, C.POINTER(C.c_float)]),
     "frt_set_time": (C.c_int, [_CTX, C.c_float]),
+    "frt_set_subgrid": (C.c_int, [_CTX, C.POINTER(FrtSubgrid)]),
+    "frt_set_subgrid_delta_nl": (C.c_int, [_CTX, C.POINTER(C.c_float)]),
+    "frt_download_subgrid_state": (C.c_int, [_CTX, C.POINTER(FrtSubgridState)]),
+    "frt_upload_subgrid_state": (C.c_int, [_CTX, C.POINTER(FrtSubgridState)]),
     "frt_get_time": (C.c_int, [_CTX, C.POINTER(C.c_float)]),
     "frt_set_step": (C.c_int, [_CTX, C.c_int64]),
     "frt_set_dt": (C.c_int, [_CTX, C.c_float]),

@@ -345,7 +345,8 @@ static int step_body(frt_ctx* ctx) {
 }
 
 static bool graph_eligible(const frt_ctx* ctx) {
-  return ctx->graph_on && !ctx->prof && !ctx->adj_on && !ctx->have_alive && !ctx->have_segments;
+  // (the subgrid mode's I-front rounds are sized by the host as they run)
+  return ctx->graph_on && !ctx->prof && !ctx->adj_on && !ctx->have_alive && !ctx->have_segments && !ctx->subgrid;
 }
 
 // One frt_step as a replay of the graph captured for this ray-buffer parity
@@ -535,6 +536,9 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
       (c.precision != FRT_PREC_FP64 && c.precision != FRT_PREC_MIXED) || !(c.a_rt > 0.) ||
       !(c.box_ckpc > 0.))
     return FRT_E_ARG;
+  // the sub-grid model is built in the FP64 form with temperature evolution
+  // (temp_ev = FALSE would take interp_temp, subgrid.cc:637-649)
+  if (c.subgrid != 0 && (c.subgrid != 1 || c.precision != FRT_PREC_FP64 || !c.temp_ev)) return FRT_E_ARG;
   const int64_t n3 = (int64_t)c.n * c.n * c.n;
   const int64_t cap = c.ray_capacity > 0 ? c.ray_capacity : 30 * n3;
   // segment indices are int32; split bookkeeping packs (ray << 2 | child) in 32 bits
@@ -713,6 +717,8 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   }
   if (rc == FRT_OK) rc = reserve_workspace(ctx);
   if (rc == FRT_OK) rc = scratch_address(ctx, &ctx->otf_scratch);
+  ctx->subgrid = c.subgrid == 1;
+  if (rc == FRT_OK && ctx->subgrid) rc = alloc_subgrid(ctx);
   if (rc != FRT_OK) {
     std::string msg = ctx->err;
     frt_destroy(ctx);
@@ -728,6 +734,7 @@ int frt_destroy(frt_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   graph_invalidate(ctx);
   free_scan_sort(ctx);
+  free_subgrid(ctx);
   hipFree(ctx->sd);
   if (ctx->hsd) hipHostFree(ctx->hsd);
   for (int b = 0; b < 2; ++b) free_rays(ctx->buf[b]);
@@ -1079,6 +1086,7 @@ extern "C" {
 int frt_checkpoint_save(frt_ctx* ctx, frt_checkpoint** out) {
   if (!ctx || !out) return FRT_E_ARG;
   *out = nullptr;
+  if (ctx->subgrid) return set_error(ctx, FRT_E_STATE, "device checkpoints do not hold the subgrid state");
   int rc = ensure_compact(ctx);
   if (rc) return rc;
   frt_checkpoint* ck = new frt_checkpoint();
@@ -1212,6 +1220,15 @@ int frt_init_rt(frt_ctx* ctx) {
   ctx->nphot_merge = 0.f;
   ctx->have_alive = false;
   ctx->have_segments = false;
+  if (ctx->subgrid) {
+    // rt_funcs.cc:50-68 for a fresh start with I-front tracking: fion,
+    // fion_max, tcross and the crossing bins 0, flags FALSE
+    SgPtrs& sp = ctx->sg;
+    const size_t n3 = (size_t)ctx->g.n3;
+    hipError_t e = hipMemsetAsync(sp.fion, 0, 4 * n3 * (6 + 2 * FRT_SG_NTCROSS), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->c.f[C_TREION_FLAG], 0, 4 * n3, ctx->stream);
+    if (e != hipSuccess) return hip_check(ctx, e, "init_rt");
+  }
   if (ctx->nsrc > 0) {
     std::vector<float> rdt((size_t)ctx->nsrc, ctx->g.dt);
     hipError_t e = stream_copy(ctx, ctx->src_ray_dt, rdt.data(), 4 * (size_t)ctx->nsrc, hipMemcpyHostToDevice);
@@ -1828,4 +1845,100 @@ int frt_read_treion_file(frt_ctx* ctx, const char* path, frt_file_extras* extras
   return rc;
 }
 
+}  // extern "C"
+
+// ---- sub-grid I-front model ------------------------------------------------------
+extern "C" {
+int frt_set_subgrid(frt_ctx* ctx, const frt_subgrid* sg) {
+  if (!ctx || !sg) return FRT_E_ARG;
+  if (!ctx->subgrid) return set_error(ctx, FRT_E_STATE, "frt_set_subgrid: the context was created with subgrid = 0");
+  if (sg->nt < 2 || !sg->time_rh || !sg->fh1_rh) return set_error(ctx, FRT_E_ARG, "frt_set_subgrid: tables");
+  const size_t n = (size_t)27 * sg->nt;
+  hipStreamSynchronize(ctx->stream);
+  hipFree(ctx->sg_tables);
+  ctx->sg_tables = nullptr;
+  hipError_t e = hipMalloc((void**)&ctx->sg_tables, 8 * n);
+  if (e == hipSuccess) e = stream_copy(ctx, ctx->sg_tables, sg->time_rh, 4 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = stream_copy(ctx, ctx->sg_tables + n, sg->fh1_rh, 4 * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_check(ctx, e, "frt_set_subgrid");
+  ctx->sg.time_rh = ctx->sg_tables;
+  ctx->sg.fh1_rh = ctx->sg_tables + n;
+  SgConst& c = ctx->sgc;
+  c.nt = sg->nt;
+  for (int i = 0; i < 3; ++i) {
+    c.tre_dom[i] = sg->tre_dom[i];
+    c.delta_nl[i] = sg->delta_nl[i];
+  }
+  c.t_z12 = sg->t_z12;
+  c.t_z8 = sg->t_z8;
+  c.tinit = sg->tinit;
+  c.avg_rho = sg->avg_rho;
+  c.temp_0 = sg->temp_0;
+  c.fh1_0 = sg->fh1_0;
+  // time_RH[Nt-1][0][0][0], read by get_tradhydro_index (subgrid.cc:188): with
+  // the one-element shift of the gamma index it holds row Nt-2 of the
+  // (zre 6, delta +sqrt3, gamma 3e-12) table
+  c.t_norm = sg->time_rh[(size_t)((2 * 3 + 2) * 3 + 2) * sg->nt + (sg->nt - 2)];
+  c.iter = 0;
+  c.last = 0;
+  ctx->g.avg_rho = sg->avg_rho;  // also the thermal update's density term
+  ctx->subgrid_set = true;
+  return FRT_OK;
+}
+
+int frt_set_subgrid_delta_nl(frt_ctx* ctx, const float delta_nl[3]) {
+  if (!ctx || !delta_nl) return FRT_E_ARG;
+  if (!ctx->subgrid) return set_error(ctx, FRT_E_STATE, "subgrid = 0");
+  for (int i = 0; i < 3; ++i) ctx->sgc.delta_nl[i] = delta_nl[i];
+  return FRT_OK;
+}
+}  // extern "C"
+
+namespace frt {
+// Host <-> device copy of the I-front state, row-major cells on the host,
+// [member][cell] planes in the internal order on the device.
+int subgrid_state_copy(frt_ctx* ctx, const frt_subgrid_state* st, bool down) {
+  if (!ctx || !st) return FRT_E_ARG;
+  if (!ctx->subgrid) return set_error(ctx, FRT_E_STATE, "subgrid = 0");
+  const size_t n3 = (size_t)ctx->g.n3;
+  std::vector<int32_t> perm;
+  cell_permutation(ctx, perm);
+  SgPtrs& sp = ctx->sg;
+  float* dev[9] = {sp.fion, sp.fion_max, sp.tcross, sp.tcross_flag, sp.treion_out,
+                   sp.delta_fion, sp.tcross_new, sp.tcross_check, sp.fh1_nodes};
+  float* host[9] = {st->fion, st->fion_max, st->tcross, st->tcross_flag, st->treion_out,
+                    st->delta_fion, st->tcross_new, st->tcross_check, st->fh1_nodes};
+  const int w[9] = {1, 1, 1, 1, 1, 1, FRT_SG_NTCROSS, FRT_SG_NTCROSS, 5};
+  if (down) {
+    int rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (rc) return rc;
+  }
+  for (int m = 0; m < 9; ++m) {
+    if (!host[m]) continue;
+    std::vector<float> t((size_t)w[m] * n3);
+    hipError_t e;
+    if (down) {
+      e = stream_copy(ctx, t.data(), dev[m], 4 * t.size(), hipMemcpyDeviceToHost);
+      if (e == hipSuccess)
+        for (size_t r = 0; r < n3; ++r)
+          for (int q = 0; q < w[m]; ++q) host[m][r * w[m] + q] = t[(size_t)q * n3 + (size_t)perm[r]];
+    } else {
+      for (size_t r = 0; r < n3; ++r)
+        for (int q = 0; q < w[m]; ++q) t[(size_t)q * n3 + (size_t)perm[r]] = host[m][r * w[m] + q];
+      e = stream_copy(ctx, dev[m], t.data(), 4 * t.size(), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
+    if (e != hipSuccess) return hip_check(ctx, e, "subgrid state");
+  }
+  return FRT_OK;
+}
+}  // namespace frt
+
+extern "C" {
+int frt_download_subgrid_state(frt_ctx* ctx, const frt_subgrid_state* out) {
+  return subgrid_state_copy(ctx, out, true);
+}
+int frt_upload_subgrid_state(frt_ctx* ctx, const frt_subgrid_state* in) {
+  return subgrid_state_copy(ctx, in, false);
+}
 }  // extern "C"

@@ -164,6 +164,49 @@ struct ContribPtrs {
   int rec_len;         // values per record (nfreq + 1 rounded up to a multiple of 4)
 };
 
+// Sub-grid I-front model (subgrid.cc, cfg.subgrid): constants by value and
+// the per-cell state, in the internal cell order.
+#define FRT_SG_NTCROSS 10  // Ntcross, global_variables.cc:224
+struct SgConst {
+  int nt;            // rows per RadHydro table
+  float tre_dom[3];  // cosmic time at z_re = 12, 8, 6
+  float t_z12, t_z8; // cosmic_time(1/13), cosmic_time(1/9)
+  float tinit;
+  float avg_rho;
+  float temp_0;
+  float fh1_0;
+  float delta_nl[3]; // deltaNL_dom of the current step
+  float t_norm;      // time_RH[Nt-1][0][0][0] as get_tradhydro_index reads it
+  int iter;          // C2Ray pass (0 .. itercount-1)
+  int last;          // the last pass
+};
+struct SgPtrs {
+  float* fion;
+  float* fion_max;
+  float* tcross;
+  float* tcross_flag;
+  float* treion_out;
+  float* delta_fion;
+  float* tcross_new;    // [FRT_SG_NTCROSS][n3]
+  float* tcross_check;  // [FRT_SG_NTCROSS][n3], 0 / 1
+  float* fh1_nodes;     // [5][n3] fh1_gamma_interp
+  double* ngam3;        // [3][n3] ngam of the current pass
+  int32_t* rpos;        // [n3] first slot of the cell not yet resolved by the I-front count
+  const float* time_rh; // [27][nt], [zre][delta][gamma] tables
+  const float* fh1_rh;
+};
+// Ray state of the sub-grid attenuation walk between I-front rounds.
+struct SgWalk {
+  double* tau;          // [4][nfreq][cap]: tau_h1, tau_he1, dtau_h1, dtau_he1
+  int32_t* p;           // [cap] next segment (99 after a dump or absorption)
+  int32_t* np;          // [cap] segments processed
+  int32_t* wait;        // [cap] slot of the pending I-front count, -1 if none
+  uint8_t* flags;       // [cap] bit 0 dumped, bit 1 done
+  double* ifr;          // [nfreq][seg_cap] photons into the neutral part, then left after it
+  uint8_t* slot;        // [seg_cap] 0 not reached, 1 waiting, 2 absorbed, 3 no room, 4 skipped, 5 emptied
+  int64_t* pending;     // device counter of rays still waiting
+};
+
 // Host values a captured step depends on (kernel arguments and launch
 // shapes): a replay is valid while they are unchanged.
 // psum_state: nothing known, the march's rays that did not move, every ray.
@@ -248,6 +291,17 @@ struct frt_ctx {
   // merged rays; psum_state says which of them hold (FRT_PSUM_*).
   float* psum;            // [cap + 1]
   int psum_state;
+  // sub-grid I-front model
+  bool subgrid;
+  bool subgrid_set;       // frt_set_subgrid has been called
+  SgConst sgc;
+  SgPtrs sg;
+  SgWalk sgw;
+  void* sg_block;         // base of the per-cell state allocation
+  float* sg_tables;       // time_rh then fh1_rh
+  int64_t sg_slot_cap;    // segments the slot buffers hold
+  int64_t* sg_pending_h;  // pinned host copy of sgw.pending
+  int64_t sg_rounds;      // I-front rounds of the last find_avg_unu
   // scan/sort workspace (frt_scan_sort.hip)
   int64_t* ss_part;
   int32_t* ss_hist;
@@ -348,6 +402,11 @@ int launch_step_begin(frt_ctx* ctx, bool advance);
 int launch_step_end(frt_ctx* ctx);
 int materialize_rays(frt_ctx* ctx);
 int ensure_psum(frt_ctx* ctx);
+int launch_find_avg_unu_subgrid(frt_ctx* ctx, bool fuse_thermal);
+int launch_thermal_subgrid(frt_ctx* ctx);
+int alloc_subgrid(frt_ctx* ctx);
+void free_subgrid(frt_ctx* ctx);
+int subgrid_state_copy(frt_ctx* ctx, const frt_subgrid_state* st, bool down);
 int launch_fill_alive(frt_ctx* ctx);
 int ensure_segment_capacity(frt_ctx* ctx);
 int launch_cast(frt_ctx* ctx);

@@ -379,7 +379,8 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst
                                                const int32_t* __restrict__ n_out,
                                                const int32_t* __restrict__ sorted_pos,
                                                const uint32_t* __restrict__ src_of,
-                                               const float* __restrict__ rot_fwd) {
+                                               const float* __restrict__ rot_fwd,
+                                               const float* __restrict__ fion) {
   unsigned long long nmoved = 0;
   bool overflow = false;
   const int64_t nrays = sd->nrays;
@@ -493,8 +494,12 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst
           break;
         }
       }
-      // last segment: remaining time in the final cell (fion = 1 branch, :1434-1437)
-      dist = (deltat - tlast + tmin) * g.clight;
+      // last segment: remaining time in the final cell (:1434-1437), or with
+      // the I-front model in a partly ionized cell its whole crossing (:1430-1433)
+      if (fion && fion[frt_cell_index(g.lay, inew, jnew, knew)] < 1.f)
+        dist = tmin * g.clight;
+      else
+        dist = (deltat - tlast + tmin) * g.clight;
       dist = frt_maxd(dist, g.min_dist * g.dx);
       if (p < g.P) {
         path_cell[(int64_t)p * g.cap + i] = frt_cell_index(g.lay, inew, jnew, knew);
@@ -1991,12 +1996,14 @@ int launch_march(frt_ctx* ctx) {
   if (ctx->deferred) {  // the previous split's compaction, gathered by the march
     FRT_LAUNCH(ctx, "march", grid, k_march<true>, cur_buf(ctx), ctx->sd, ctx->g, ctx->path_cell, ctx->path_dist,
                ctx->seg_cnt, ctx->alive, ctx->psum, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)ctx->n_out,
-               (const int32_t*)ctx->split_sorted, (const uint32_t*)ctx->cand_key_tmp, (const float*)ctx->rot_fwd);
+               (const int32_t*)ctx->split_sorted, (const uint32_t*)ctx->cand_key_tmp, (const float*)ctx->rot_fwd,
+               (const float*)(ctx->subgrid ? ctx->sg.fion : nullptr));
     ctx->deferred = false;
   } else {
     FRT_LAUNCH(ctx, "march", grid, k_march<false>, cur_buf(ctx), ctx->sd, ctx->g, ctx->path_cell, ctx->path_dist,
                ctx->seg_cnt, ctx->alive, ctx->psum, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)nullptr,
-               (const int32_t*)nullptr, (const uint32_t*)nullptr, (const float*)ctx->rot_fwd);
+               (const int32_t*)nullptr, (const uint32_t*)nullptr, (const float*)ctx->rot_fwd,
+               (const float*)(ctx->subgrid ? ctx->sg.fion : nullptr));
   }
   ctx->psum_state = FRT_PSUM_UNMOVED;
   return FRT_OK;
@@ -2161,6 +2168,7 @@ __global__ void __launch_bounds__(BLK) k_tape_nph(RaySoA rb, int64_t n, int F, i
 }
 
 int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
+  if (ctx->subgrid) return launch_find_avg_unu_subgrid(ctx, fuse_thermal);
   if (int rc0 = materialize_rays(ctx)) return rc0;
   const int I = ctx->cfg.itercount;
   const int64_t n = ctx->nrays;
@@ -2281,6 +2289,7 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
 }
 
 int launch_thermal(frt_ctx* ctx) {
+  if (ctx->subgrid) return launch_thermal_subgrid(ctx);
   FRT_LAUNCH(ctx, "thermal", grid_for(ctx->g.n3), k_thermal, ctx->c, ctx->g, ctx->dcnt);
   return FRT_OK;
 }
@@ -2833,5 +2842,7 @@ void build_rotation_tables(float* fwd, float* inv) {
         v[8] = cb * ca;
       }
 }
+
+#include "frt_subgrid.h"
 
 }  // namespace frt

@@ -21,7 +21,12 @@ Workloads (grid, box, redshift, density, sources, steps):
   full256        256^3, 80 cMpc/h, z=6.5, lognormal (sigma_ln 1.2, seed 4),
                  T0 uniform in [50, 200] K, 32768 sources ~(1+delta)^2,
                  Ndot in [1e51, 1e56], C = 1 + 2 (1+delta)^0.5, 50 steps of 0.5 Myr
-All use 5 log-spaced bins over 13.6-54.4 eV with L_nu ~ nu^-1.5.
+  subgrid32      32^3, 5 cMpc/h, z=8, lognormal (sigma_ln 0.8, seed 5), 4
+                 sources on the densest cells, Ndot in [1e53, 1e54], C=2, 10
+                 steps of 1 Myr, the sub-grid I-front model (FP64 form) with
+                 placeholder RadHydro tables, one bin at 19 eV as the
+                 reference's production setting
+All but subgrid32 use 5 log-spaced bins over 13.6-54.4 eV with L_nu ~ nu^-1.5.
 """
 import math
 
@@ -46,7 +51,7 @@ def reference_defaults(n):
     return dict(
         n=n, nfreq=1, path_length=4, itercount=5, precision=0, hpx_lvl=0, l_max=12,
         ray_per_cell_max=22, rec_case_b=1, temp_ev=1, recomb_cool=1, free_free=1, compton=1,
-        frac_limit=1, reserved0=0, reserved1=0, ray_capacity=0, seed=1,
+        frac_limit=1, subgrid=0, reserved1=0, ray_capacity=0, seed=1,
         box_ckpc=40 * 1e3 / 0.68, a_rt=1.0 / (1.0 + 12.0038), zz=12.0038, dt=0.0,
         cl_factor=0.3, nl_const=0.5, safety_factor=2.0, min_dist=1e-4, temp_min=1e2,
         hh=0.68, omega_m=0.305147, omega_l=0.6948,
@@ -211,6 +216,8 @@ WORKLOADS = {
                       lum=(53.0, 55.0), steps=20, dt_myr=1.0, clump=2.0, temp=(100.0, 100.0)),
     "full256": dict(n=256, box_cmpc_h=80.0, z=6.5, sigma_ln=1.2, seed=4, n_src=32768, src="weighted",
                     lum=(51.0, 56.0), steps=50, dt_myr=0.5, clump="subgrid", temp=(50.0, 200.0)),
+    "subgrid32": dict(n=32, box_cmpc_h=5.0, z=8.0, sigma_ln=0.8, seed=5, n_src=4, src="densest",
+                      lum=(53.0, 54.0), steps=10, dt_myr=1.0, clump=2.0, temp=(100.0, 100.0), subgrid=True),
 }
 COSMO = dict(h=0.7, omega_m=0.3, omega_b=0.048)
 
@@ -270,8 +277,21 @@ def make_workload(name, n=None, n_src=None, steps=None, precision=0, seed_offset
         clump = np.float32(w["clump"])
     fields = init_gas(rho, temp0, clump)
     nu, spec = powerlaw_bins()
-    return dict(name=name, cfg=cfg, fields=fields, src_cells=src_cells, src_lum=src_lum,
-                nu_phot=nu, spectrum=spec, steps=w["steps"], params=w)
+    out = dict(name=name, cfg=cfg, fields=fields, src_cells=src_cells, src_lum=src_lum,
+               nu_phot=nu, spectrum=spec, steps=w["steps"], params=w)
+    if w.get("subgrid"):
+        # the reference's sub-grid production setting is monochromatic,
+        # Nfreq = 1 at 19 eV (user_inputs.h:79,95): with several bins the
+        # I-front count's per-bin min() can empty one bin while others
+        # survive, and the next front cell's tau update (ray_tracing.cc:911)
+        # is then 0/0 for it
+        from . import subgrid
+        cfg["subgrid"] = 1
+        cfg["nfreq"] = 1
+        out["nu_phot"] = np.array([19.0 / H_EV])
+        out["spectrum"] = np.array([1.0], dtype=np.float32)
+        out["subgrid"] = subgrid.make_params(cfg, rho, temp_0=w["temp"][0])
+    return out
 
 
 def setup(rt, wl):
@@ -290,6 +310,8 @@ def setup(rt, wl):
             rt.upload_field(k, np.zeros(n3 * nf if k in PER_BIN else n3, dtype=np.float32))
     rt.set_spectrum(wl["nu_phot"], wl["spectrum"])
     rt.set_sources(wl["src_cells"], wl["src_lum"])
+    if "subgrid" in wl:
+        rt.set_subgrid(wl["subgrid"])
     rt.init_rt()
     rt.new_rays()
     return rt.get_nphot_min_max()

@@ -13,7 +13,8 @@ import numpy as np
 
 from . import _lib
 from ._abi import (EXTRA_FLAG_FIELDS, EXTRA_FLOAT_FIELDS, FIELD_ID, PER_BIN, PREC_FP64, PREC_MIXED, FrtOtf,
-                   FrtConfig, FrtStepStats, config_struct, make_extras, make_ray_view)
+                   FrtConfig, FrtStepStats, config_struct, make_extras, make_ray_view, make_subgrid,
+                   make_subgrid_state)
 
 
 class FlexRTError(RuntimeError):
@@ -154,6 +155,28 @@ class FlexRT:
 
     def set_time(self, t):
         self._check(self._lib.frt_set_time(self._h, float(t)), "set_time")
+
+    # ---- sub-grid I-front model (cfg["subgrid"] = 1) -------------------------------
+    def set_subgrid(self, sg):
+        """RadHydro tables and constants (a dict as subgrid.make_params returns)."""
+        st, keep = make_subgrid(sg)
+        self._check(self._lib.frt_set_subgrid(self._h, C.byref(st)), "set_subgrid")
+
+    def set_subgrid_delta_nl(self, d):
+        a = np.ascontiguousarray(d, dtype=np.float32)
+        self._check(self._lib.frt_set_subgrid_delta_nl(self._h, a.ctypes.data_as(C.POINTER(C.c_float))),
+                    "set_subgrid_delta_nl")
+
+    def download_subgrid_state(self):
+        """The per-cell I-front state, row-major (tcross_new, tcross_check and
+        fh1_nodes with 10, 10 and 5 values per cell)."""
+        st, arrays = make_subgrid_state(self.n ** 3)
+        self._check(self._lib.frt_download_subgrid_state(self._h, C.byref(st)), "download_subgrid_state")
+        return arrays
+
+    def upload_subgrid_state(self, arrays):
+        st, keep = make_subgrid_state(self.n ** 3, arrays)
+        self._check(self._lib.frt_upload_subgrid_state(self._h, C.byref(st)), "upload_subgrid_state")
 
     def set_step(self, step):
         """Restart: completed steps (they key the casting and merging angle streams)."""

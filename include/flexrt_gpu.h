@@ -65,7 +65,9 @@ typedef struct frt_config {
   int32_t frac_limit;       /* 1: where both HI and HeI segment depths underflow exp()
                                the HI/HeI split of ray_tracing.cc:770-771 is 0/0; use
                                its limit 1/(1+exp(dtau_he - dtau_h)) instead of NaN */
-  int32_t reserved0;
+  int32_t subgrid;          /* 1: the sub-grid I-front model in its production setting,
+                               subgrid = ifront = equilibrium = TRUE, relaxed = FALSE
+                               (user_inputs.h:11-15); FP64 form; needs frt_set_subgrid */
   int32_t reserved1;
   int64_t ray_capacity;     /* rays held per buffer; 0 selects 30 n^3 (global_variables.cc:40-43) */
   uint64_t seed;            /* seed of the rotation-index stream that replaces rand() */
@@ -396,6 +398,55 @@ int frt_read_treion_file(frt_ctx* ctx, const char* path, frt_file_extras* extras
 /* write_vIF_binary (io_funcs.cc:396-430): per row-major cell IF_speed,
  * Inc_Flux, fion (1 without I-front tracking) and Delta_fion (0), float. */
 int frt_write_vif_file(frt_ctx* ctx, const char* path);
+
+/* ---- sub-grid I-front model (cfg.subgrid = 1) ---------------------------------
+ * The RadHydro tables of init_radhydro_data / import_subgrid_file
+ * (subgrid.cc:20-44,132-184) as read: nt rows per (z_re, delta, Gamma) node,
+ * in [zre][delta][gamma][row] order (zre 12, 8, 6; delta -sqrt3, 0, +sqrt3 in
+ * linear delta/sigma; Gamma 3e-14, 3e-13, 3e-12), the time since ionization
+ * already in s and the total HI fraction.  With equilibrium = TRUE the MFP
+ * columns are not read (the MFP is the equilibrium value of
+ * ray_tracing.cc:981-986) and with temp_ev = TRUE neither is the temperature
+ * column (interp_temp, subgrid.cc:637-649, is not called).  The cosmic times
+ * are the driver's cosmic_time() values (cosmo_funcs.cc:43-51). */
+typedef struct frt_subgrid {
+  int32_t nt;              /* rows per table (Nt, global_variables.cc:266), >= 2 */
+  int32_t reserved;
+  const float* time_rh;    /* [3][3][3][nt] time_RH, s */
+  const float* fh1_rh;     /* [3][3][3][nt] fh1_RH */
+  float tre_dom[3];        /* cosmic_time at z_re = 12, 8, 6 (subgrid.cc:180-183) */
+  float t_z12;             /* cosmic_time(1/(1+12)) (subgrid.cc:262) */
+  float t_z8;              /* cosmic_time(1/(1+8)) (subgrid.cc:265) */
+  float tinit;             /* cosmic_time(1/(1+zinit)) (init_funcs.cc:11) */
+  float avg_rho;           /* mean density avg_rho (global_variables.cc:84) */
+  float temp_0;            /* neutral temperature temp_0 (user_inputs.h:96) */
+  float fh1_0;             /* fH1_0, the cap of interp_nh1 (user_inputs.h:98) */
+  float delta_nl[3];       /* deltaNL_dom of the first step (delta_dom_to_deltaNL_dom) */
+} frt_subgrid;
+/* Tables and constants of the sub-grid model (init_radhydro_data,
+ * subgrid.cc:132-184, and main.cc:86-90); the I-front state starts as
+ * init_rt leaves it (rt_funcs.cc:50-68: fion = fion_max = tcross = 0, no
+ * crossing bins, flags FALSE). */
+int frt_set_subgrid(frt_ctx* ctx, const frt_subgrid* sg);
+/* deltaNL_dom for the next step, as main.cc:126-128 sets it each step from
+ * delta_dom_to_deltaNL_dom (subgrid.cc:103-130; deltaNL_dom[1] = 0). */
+int frt_set_subgrid_delta_nl(frt_ctx* ctx, const float delta_nl[3]);
+/* Per-cell I-front state (global_variables.cc:206-233), row-major cells;
+ * NULL members are skipped.  tcross_new and tcross_check hold Ntcross = 10
+ * values per cell, fh1_nodes the 5 Gamma nodes of fh1_gamma_interp. */
+typedef struct frt_subgrid_state {
+  float* fion;
+  float* fion_max;
+  float* tcross;
+  float* tcross_flag;      /* 0 / 1 */
+  float* treion_out;
+  float* delta_fion;
+  float* tcross_new;       /* [cell][10] */
+  float* tcross_check;     /* [cell][10], 0 / 1 */
+  float* fh1_nodes;        /* [cell][5] */
+} frt_subgrid_state;
+int frt_download_subgrid_state(frt_ctx* ctx, const frt_subgrid_state* out);
+int frt_upload_subgrid_state(frt_ctx* ctx, const frt_subgrid_state* in);
 
 /* MP-Gadget coupling (io_funcs.cc:434-496), main.cc:229-240 with
  * MP_GADGET_HEAT: after each step update_gadget_var adds Gamma_HI dt and

@@ -198,6 +198,16 @@ struct State {
   bool dirty;  // rays may have been removed since the last compaction
   frt_step_stats st;
   bool path_overflow;
+  // sub-grid I-front model (cfg.subgrid): global_variables.cc:194-304
+  bool subgrid = false;
+  int nt = 0;
+  std::vector<float> time_rh, fh1_rh;  // [zre][delta][gamma][nt]
+  float tre_dom[3] = {0, 0, 0}, t_z12 = 0, t_z8 = 0, tinit = 0, temp_0 = 0, fh1_0 = 0;
+  float deltaNL_dom[3] = {0, 0, 0};
+  std::vector<float> fion, fion_max, tcross, tcross_flag, treion_out, delta_fion;
+  std::vector<float> tcross_new, tcross_check;  // [cell][Ntcross]
+  std::vector<float> fh1_nodes;                  // [cell][Ngamma + 2] fh1_gamma_interp
+  std::vector<double> ngam3;                     // [cell][3] ngam (global_variables.cc:202)
 };
 
 static std::vector<float>* field_vec(State& s, int f) {
@@ -427,7 +437,12 @@ static void advance_ray(State& s, Ray& ray) {
         break;
       }
     }
-    dist = (deltat - tlast + tmin) * s.clight;
+    // ray_tracing.cc:1430-1437: the whole crossing of a partly ionized last
+    // cell (fion < 1, the I-front model), else the rest of the time step
+    if (s.subgrid && s.fion[i_new * s.N * s.N + j_new * s.N + k_new] < 1.)
+      dist = tmin * s.clight;
+    else
+      dist = (deltat - tlast + tmin) * s.clight;
     dist = maxd(dist, s.min_dist * dxf);
     if (p < s.P) {
       ray.path_ind[p] = i_new * s.N * s.N + j_new * s.N + k_new;
@@ -510,8 +525,476 @@ static CellOut c2ray_cell(float nH, float nHe, float avg_nh1, float avg_nhe1, fl
   return o;
 }
 
+// ---- sub-grid I-front model (subgrid = ifront = equilibrium = TRUE) ----------------------
+// subgrid.cc:20-811 and the subgrid branches of ray_tracing.cc:623-1216 and
+// gas_funcs.cc:134-419, in the reference's production setting
+// (user_inputs.h:11-15: relaxed = FALSE, temp_ev = TRUE).  With equilibrium
+// the MFP is the equilibrium value (ray_tracing.cc:981-986), so of the
+// RadHydro tables only the time and total HI fraction columns are read.
+// Deviations, each applied identically on the GPU:
+//  * The I-front photon count of a cell (ray_tracing.cc:867-919) is consumed
+//    by the rays in list order (the reference: whichever thread takes
+//    ray_lock first).
+//  * log10 of float arguments (std::log10(float) in subgrid.cc, which is
+//    compiled with <math.h> and `using namespace std`) is taken in double
+//    and rounded to float.
+//  * Per-cell Gamma sums in double, as the C2Ray branch here.
+namespace sg {
+const int Ntcross = 10;            // global_variables.cc:224
+const float sigma_eff_rh = 2.517139346618065e-18f;  // global_variables.cc:241
+const float max_subgrid_frac = 0.49f;               // global_variables.cc:245
+const float gamma_min = 1e-16f;                     // global_variables.cc:303
+// global_variables.cc:302 initializes mfp_min = 1e-2*dx[0] while dx[] is
+// still zero (dx is filled in init()), so the global mfp_min is 0.
+const float mfp_min = 0.f;
+const float gamma_dom[5] = {3e-15f, 3e-14f, 3e-13f, 3e-12f, 3e-11f};  // subgrid.cc:146-150
+}  // namespace sg
+
+static float lg10f(float x) { return (float)dm_log10((double)x); }
+
+// Table (zre im, delta ip, gamma node iq = 1..3) as get_gamma_interp indexes
+// it: import_subgrid_file stores file gamma k at index k + 1 (subgrid.cc:22).
+static float sg_time(const State& s, int m, int im, int ip, int iq) {
+  return s.time_rh[((size_t)(im * 3 + ip) * 3 + (iq - 1)) * s.nt + m];
+}
+static float sg_fh1(const State& s, int m, int im, int ip, int iq) {
+  return s.fh1_rh[((size_t)(im * 3 + ip) * 3 + (iq - 1)) * s.nt + m];
+}
+
+// get_tradhydro_index (subgrid.cc:187-190).  time_RH[Nt-1][0][0][0] is the
+// gamma-0 slot that no file fills: with the one-element shift of the
+// [..][Ngamma] rows above it holds the time of row Nt-2 of the (zre 6,
+// delta +sqrt3, gamma 3e-12) table.
+static int get_tradhydro_index(const State& s, float tsince) {
+  const float tn = s.time_rh[((size_t)(2 * 3 + 2) * 3 + 2) * s.nt + (s.nt - 2)];
+  int idx = (int)std::floor(tsince / tn * (float)(s.nt - 1));
+  return std::max(0, std::min(idx, s.nt - 2));
+}
+
+// deltaNL_to_deltasig (subgrid.cc:193-204), Mo & White 1996.
+static float deltaNL_to_deltasig(float deltaNL, float redshift) {
+  float dcmode = 5.10655f;
+  float sigma0 = (float)(dcmode / dm_pow(3., 0.5));
+  float d_of_z = (float)(1. / (1 + redshift) / 0.78205);
+  float delta = (float)(-1.35 * dm_pow(1. + deltaNL, -2. / 3.) + 0.78785 * dm_pow(1. + deltaNL, -0.58661) -
+                        1.12431 * dm_pow(1. + deltaNL, -0.5) + 1.68647);
+  return delta / d_of_z / sigma0;
+}
+
+// Table value at time ts by linear interpolation from row it (subgrid.cc:345-348).
+static float sg_lin_fh1(const State& s, int it, int im, int ip, int iq, float ts) {
+  float a = sg_fh1(s, it, im, ip, iq), b = sg_fh1(s, it + 1, im, ip, iq);
+  float ta = sg_time(s, it, im, ip, iq), tb = sg_time(s, it + 1, im, ip, iq);
+  float v = a + (b - a) / (tb - ta) * (ts - ta);
+  return mind(maxd(v, 0.f), 1.f);  // :357
+}
+
+// get_interpolate_gamma_f with fractype TRUE (subgrid.cc:210-251): log-log in
+// t_re + t_since, then in 1 + deltaNL, at the three gamma nodes; the nodes
+// below and above follow from the fixed slopes of :445-446.
+static void interp_gamma_fh1(const State& s, float tsince, float tre, float deltaNL, const int ind_tre[2],
+                             const int ind_delta[2], const float f[2][2][3], float out[5]) {
+  const float lt = lg10f(tre + tsince);
+  const float lt0 = lg10f(s.tre_dom[ind_tre[0]] + tsince);
+  const float lt1 = lg10f(s.tre_dom[ind_tre[1]] + tsince);
+  const double ld = dm_log10(1. + (double)deltaNL);
+  const double ld0 = dm_log10(1. + (double)s.deltaNL_dom[ind_delta[0]]);
+  const double ld1 = dm_log10(1. + (double)s.deltaNL_dom[ind_delta[1]]);
+  for (int q = 0; q < 3; q++) {
+    float ft[2];
+    for (int p = 0; p < 2; p++) {
+      const float a = lg10f(f[0][p][q]), b = lg10f(f[1][p][q]);
+      ft[p] = a + (b - a) * (lt - lt0) / (lt1 - lt0);
+    }
+    const float v = (float)((double)ft[0] + (double)(ft[1] - ft[0]) * (ld - ld0) / (ld1 - ld0));
+    out[q + 1] = mind((float)dm_pow(10., (double)v), sg::max_subgrid_frac);
+  }
+  out[0] = (float)(dm_pow((double)(sg::gamma_dom[0] / sg::gamma_dom[1]), 0.) * (double)out[1]);
+  out[4] = (float)(dm_pow((double)(sg::gamma_dom[4] / sg::gamma_dom[3]), -1.) * (double)out[3]);
+}
+
+// get_gamma_interp (subgrid.cc:257-550), the fh1_gamma_interp part (the MFP
+// and temperature nodes feed interp_mfp and interp_temp only, which the
+// production setting does not call).
+static void get_gamma_interp_fh1(State& s, int c, float tre, float deltaNL, float tsince) {
+  const int NT = sg::Ntcross;
+  float* tcn = &s.tcross_new[(size_t)c * NT];
+  tre = maxd(s.t_z12, tre);
+  int ind_tre[2], ind_delta[2], ind_tre_tc[NT][2];
+  if (tre > s.t_z8) { ind_tre[1] = 2; ind_tre[0] = 1; } else { ind_tre[1] = 1; ind_tre[0] = 0; }
+  for (int tt = 0; tt < NT; tt++) {
+    if (tre + tcn[tt] > s.t_z8) { ind_tre_tc[tt][1] = 2; ind_tre_tc[tt][0] = 1; }
+    else { ind_tre_tc[tt][1] = 1; ind_tre_tc[tt][0] = 0; }
+  }
+  if (deltaNL_to_deltasig(deltaNL, s.zz) < 0.) { ind_delta[1] = 1; ind_delta[0] = 0; }
+  else { ind_delta[1] = 2; ind_delta[0] = 1; }
+  const bool tcflag = s.tcross_flag[c] != 0.f;
+  const int itf = get_tradhydro_index(s, tsince);
+  float tc_mid[NT];
+  int itftc[NT];
+  for (int tt = 0; tt < NT; tt++) {
+    tc_mid[tt] = (tt == 0) ? (float)(tcn[0] / 2.) : (float)(tcn[tt - 1] + (tcn[tt] - tcn[tt - 1]) / 2.);
+    itftc[tt] = tcflag ? get_tradhydro_index(s, tsince - tc_mid[tt]) : 0;
+  }
+  float f[2][2][3];
+  for (int m = 0; m < 2; m++)
+    for (int p = 0; p < 2; p++)
+      for (int q = 0; q < 3; q++) f[m][p][q] = sg_lin_fh1(s, itf, ind_tre[m], ind_delta[p], q + 1, tsince);
+  float* nodes = &s.fh1_nodes[(size_t)c * 5];
+  interp_gamma_fh1(s, tsince, tre, deltaNL, ind_tre, ind_delta, f, nodes);
+  if (!tcflag) return;
+  // crossing-time bins (:481-547): the nodes averaged over the bins the
+  // I-front has crossed, weighted by the fraction of the cell in each
+  float nt_nodes[NT][5];
+  for (int tt = 0; tt < NT; tt++) {
+    float ftc[2][2][3];
+    for (int m = 0; m < 2; m++)
+      for (int p = 0; p < 2; p++)
+        for (int q = 0; q < 3; q++)
+          ftc[m][p][q] = sg_lin_fh1(s, itftc[tt], ind_tre_tc[tt][m], ind_delta[p], q + 1, tsince - tc_mid[tt]);
+    interp_gamma_fh1(s, tsince - tc_mid[tt], tre + tc_mid[tt], deltaNL, ind_tre_tc[tt], ind_delta, ftc,
+                     nt_nodes[tt]);
+  }
+  const float fmax = s.fion_max[c];
+  const float tc_frac = (float)(1. / ((float)NT));
+  for (int q = 0; q < 5; q++) {
+    float acc = 0.f;
+    if (fmax == 1.) {
+      for (int tt = 0; tt < NT; tt++) acc += tc_frac * nt_nodes[tt][q];
+    } else {
+      for (int tt = 0; tt < NT; tt++) {
+        if ((fmax >= tt * tc_frac) && (fmax < (tt + 1.) * tc_frac)) {
+          for (int ttt = 0; ttt < tt; ttt++) acc += tc_frac * nt_nodes[ttt][q];
+          acc += (maxd(0.01f, fmax) - tt * tc_frac) * nt_nodes[tt][q];
+        }
+      }
+    }
+    acc = acc / maxd(0.01f, fmax);
+    nodes[q] = mind(acc, sg::max_subgrid_frac);
+  }
+}
+
+// interp_nh1 / interp_nhe1 (subgrid.cc:616-634): log-log in Gamma over the nodes.
+static void interp_nh1(State& s, int c, float gammaH1) {
+  const float* fh = &s.fh1_nodes[(size_t)c * 5];
+  const float gamma = maxd(gammaH1, sg::gamma_min);
+  int g = (int)std::floor(((float)3 + 2 - 1.) * (lg10f(gamma) - lg10f(sg::gamma_dom[0])) /
+                          (lg10f(sg::gamma_dom[4]) - lg10f(sg::gamma_dom[0])));
+  g = std::max(0, std::min(g, 3));
+  float v = lg10f(fh[g]) + (lg10f(fh[g + 1]) - lg10f(fh[g])) / (lg10f(sg::gamma_dom[g + 1]) - lg10f(sg::gamma_dom[g])) *
+                               (lg10f(gamma) - lg10f(sg::gamma_dom[g]));
+  v = maxd(mind((float)dm_pow(10., (double)v), s.fh1_0), 0.f);
+  s.nH1[c] = v * s.nH[c];
+  s.nHe1[c] = s.nH1[c] * s.nHe[c] / s.nH[c];
+  s.avg_nhe1[c] = s.nHe1[c];
+}
+
+// Equilibrium MFPs of an ionized cell (ray_tracing.cc:981-986).
+static void mfp_equilibrium(State& s, int c, float ag) {
+  const double ab = alphaB_H2(maxd(s.temp_min, s.temp[c]));
+  const double n2 = dm_pow((double)s.nH[c], 2.);
+  const float g = maxd(sg::gamma_min, ag) / s.clump[c];
+  s.mfp[c] = (float)((double)(g / sg::sigma_eff_rh) / ab / n2 / 1.082);
+  s.mfp_912[c] = (float)((double)g / sigmapi_H1(13.6 / h_eV) / ab / n2 / 1.082);
+  s.mfp[c] = maxd(sg::mfp_min, s.mfp[c]);
+  s.mfp_912[c] = maxd(sg::mfp_min, s.mfp_912[c]);
+}
+
+// set_Treion (subgrid.cc:652-666): the D'Aloisio et al. 2019 post-front
+// temperature appended to the running average.
+static void set_Treion(State& s, int c, float vIF, float fi_old, float fi_new) {
+  float Cn[5] = {9.5432, -1.6441e-2, -9.8010e-3, 4.1664e-3, -2.2710e-4};
+  float lnTreion = 0.;
+  for (int t = 0; t < 5; t++) lnTreion += Cn[t] * dm_pow(dm_log(vIF / 1e5), t);
+  if (fi_new > 0.) s.temp[c] = (fi_old * s.temp[c] + (fi_new - fi_old) * dm_expf(lnTreion)) / fi_new;
+}
+
+// update_fion (subgrid.cc:689-742).
+static void update_fion(State& s, int c) {
+  const int NT = sg::Ntcross;
+  const float vIF_min = (float)(5e1 * 1e5);
+  const double F = s.ngam3[(size_t)c * 3 + 2] / dm_pow((double)s.dx, 2.) / s.dt;
+  const float vIF = (float)(F / s.nH[c] / (1. + s.nhe_to_nh));
+  const float fion_old = s.fion[c];
+  const float fion_max_old = s.fion_max[c];
+  s.fion[c] = mind(s.fion[c] + vIF * s.dt / s.dx, 1.f);
+  if (s.fion[c] > s.fion_max[c]) s.fion_max[c] = s.fion[c];
+  if ((vIF > vIF_min) || (fion_old == 0.)) set_Treion(s, c, maxd(vIF, vIF_min), fion_old, s.fion[c]);
+  if ((fion_old < 0.5) && (s.fion[c] >= 0.5)) s.treion_out[c] = s.t - s.dt;
+  const float tdiff = (s.fion[c] >= 1.) ? (float)(0.5 * s.dt) : 0.f;
+  s.IF_speed[c] = vIF;
+  s.Inc_Flux[c] = (float)F;
+  s.delta_fion[c] = s.fion[c] - fion_old;
+  s.tcross[c] += s.dt - tdiff;
+  const float tc_frac = (float)(1. / ((float)NT));
+  const float fmax = s.fion_max[c];
+  for (int tt = 0; tt < NT; tt++) {
+    float& tcn = s.tcross_new[(size_t)c * NT + tt];
+    float& chk = s.tcross_check[(size_t)c * NT + tt];
+    if ((fmax < tc_frac * (tt + 1.)) && (chk == 0.f)) {
+      tcn += s.dt;
+    } else if ((fmax >= tc_frac * (tt + 1.)) && (fion_max_old < tc_frac * (tt + 1.)) && (chk == 0.f)) {
+      tcn = (float)(tcn + s.dt * (tc_frac * (tt + 1.) - fion_max_old) / (fmax - fion_max_old));
+      chk = 1.f;
+    }
+  }
+  s.tcross_flag[c] = 1.f;
+}
+
+// Photoheat_HIHeI (rates.cc:240-254), the analytic photoheating for a
+// power-law spectrum of index alpha.
+static double phothtHI(double alpha, double T, double ne) {
+  return 1.360e+1 / h_eV * h * ((alpha + 2.8) / (1.8 + alpha) * (1. - dm_pow(4., -1.8 - alpha)) / (1. - dm_pow(4., -2.8 - alpha)) - 1.) *
+         alphaA_H2(T) * ne;
+}
+static double phothtHeI(double alpha, double T, double ne) {
+  return 2.459e+1 / h_eV * h * ((alpha + 1.7) / (0.7 + alpha) * (1. - dm_pow(2.2, -0.7 - alpha)) / (1. - dm_pow(2.2, -1.7 - alpha)) - 1.) *
+         alphaA_He2(T) * ne;
+}
+
+// update_heat_cool, subgrid branches (gas_funcs.cc:171-241).
+static void update_heat_cool_sg(State& s, int c) {
+  s.heat_rate[c] = 0;
+  s.cool_rate[c] = 0;
+  const float T = maxd(s.temp_min, s.temp[c]);
+  const float nn = s.nH[c] + s.nHe[c];
+  if (s.gamma_H1_tot[c] > 3e-14)
+    s.heat_rate[c] = (float)(phothtHI(1.5, (double)T, (double)nn) * s.nH[c] + phothtHeI(1.5, (double)T, (double)nn) * s.nHe[c]);
+  else
+    s.heat_rate[c] = 0.;
+  if (s.cfg.recomb_cool) {
+    if (s.cfg.rec_case_b == 0) {
+      s.cool_rate[c] += rcrA_H2(T) * nn * s.nH[c];
+      s.cool_rate[c] += rcrA_He2(T) * nn * s.nHe[c];
+    } else {
+      s.cool_rate[c] += rcrB_H2(T) * nn * s.nH[c];
+      s.cool_rate[c] += rcrB_He2(T) * nn * s.nHe[c];
+    }
+  }
+  if (s.cfg.free_free) {
+    float nff = s.fion[c] * nn;
+    s.cool_rate[c] += free_free_rate(T) * dm_pow((double)nff, 2.);
+  }
+  if (s.cfg.compton) s.cool_rate[c] -= compton_rate(s.zz) * nn * (2.726 * (1 + s.zz) - T);
+}
+
+// find_avg_unu with subgrid = ifront = equilibrium = TRUE
+// (ray_tracing.cc:623-1216).
+static void find_avg_unu_subgrid(State& s) {
+  const int F = s.Nfreq;
+  const int itercount = s.cfg.itercount;
+  int64_t dumped = 0, terminated = 0;
+  const double vol = ((double)s.dx) * ((double)s.dx) * ((double)s.dx);
+  const double frac_h1 = 1. - (Y / 4.) / (1. - Y + Y / 4.);  // :766-767
+  const double frac_he1 = (Y / 4.) / (1. - Y + Y / 4.);
+  const double chi = (double)s.nhe_to_nh;
+  const double sig0 = sigmapi_H1(s.nu_phot[0]);
+  for (int iter = 0; iter < itercount; iter++) {
+    const bool last = iter == itercount - 1;
+    // cell reset and iteration-0 setup (:630-697)
+    for (int c = 0; c < s.N3; c++) {
+      s.avg_gamma[c] = 0.;
+      s.avg_gamma_he1[c] = 0.;
+      for (int q = 0; q < 3; q++) s.ngam3[(size_t)c * 3 + q] = 0.;
+      for (int nu = 0; nu < F; nu++) {
+        s.ngam_acc[c * F + nu] = 0.;
+        s.avg_gamma_nu[c * F + nu] = 0.;
+        s.avg_gamma_nu_he1[c * F + nu] = 0.;
+      }
+      s.IF_speed[c] = 0.;
+      s.Inc_Flux[c] = 0.;
+      s.delta_fion[c] = 0.;
+      if (iter == 0) {
+        s.avg_nh1[c] = s.nH1[c];
+        s.avg_nhe1[c] = s.nHe1[c];
+        if (s.cfg.rec_case_b == 0) {
+          s.recomb_H2[c] = (float)alphaA_H2(maxd(s.temp_min, s.temp[c]));
+          s.recomb_He2[c] = (float)alphaA_He2(maxd(s.temp_min, s.temp[c]));
+        } else {
+          s.recomb_H2[c] = (float)alphaB_H2(maxd(s.temp_min, s.temp[c]));
+          s.recomb_He2[c] = (float)alphaB_He2(maxd(s.temp_min, s.temp[c]));
+        }
+        if ((s.temp[c] >= 3e4) && (s.temp[c] <= 1e6)) s.recomb_He2[c] += (float)Dalpha_He2(maxd(s.temp_min, s.temp[c]));
+        s.dne_dt[c] = -(s.nH[c] + s.nHe[c]);
+        if (s.treion_flag[c] != 0.f)
+          get_gamma_interp_fh1(s, c, s.treion[c] + s.tinit, (float)(s.rho[c] / s.avg_rho - 1.), s.t - s.treion[c]);
+        else
+          s.mfp[c] = (float)(1. / s.nH1[c] / sig0);
+      }
+    }
+    // ray loop (:701-938), rays in list order
+    for (size_t x = 0; x < s.ray.size(); x++) {
+      Ray& ray = s.ray[x];
+      if (!(ray.tag && !ray.active)) continue;
+      double nphot[MAXF], tau_h1[MAXF], dtau_h1[MAXF], tau_he1[MAXF], dtau_he1[MAXF];
+      for (int nu = 0; nu < F; nu++) {
+        nphot[nu] = ray.nphot[nu] * phot_norm;
+        tau_h1[nu] = dtau_h1[nu] = tau_he1[nu] = dtau_he1[nu] = 0.;
+      }
+      int p = 0;
+      bool was_dumped = false;
+      while (p < ray.ray_end) {
+        const int ind = ray.path_ind[p];
+        const float dist = ray.path_dist[p];
+        const float fi = s.fion[ind], mf = s.mfp[ind];
+        double nphot_in[MAXF];
+        double nphot_in_total = 0.;
+        for (int nu = 0; nu < F; nu++) {
+          nphot_in[nu] = nphot[nu] * dm_exp(-tau_h1[nu] - tau_he1[nu]);
+          nphot_in_total += nphot_in[nu] / phot_norm;
+        }
+        const bool dump = nphot_in_total < s.nphot_min;
+        // deposits of the ionized part (:786-801, :819-834); the same in the dump case
+        double g_sum = 0., g_sum_he = 0.;
+        for (int nu = 0; nu < F; nu++) {
+          s.ngam_acc[ind * F + nu] += nphot_in[nu] / phot_norm;
+          if (fi > 0.) {
+            float dse;
+            if (iter == 0)
+              dse = (float)((double)(fi * dist) / frac_h1);
+            else
+              dse = (float)(mf * (1. - dm_exp((double)(-fi * dist / mf) / frac_h1)));
+            const double gh = frac_h1 * nphot_in[nu] * sg::sigma_eff_rh * dse / s.dt / vol / fi;
+            const double ghe = frac_he1 * nphot_in[nu] * sg::sigma_eff_rh * dse / s.dt / vol / fi;
+            s.avg_gamma_nu[ind * F + nu] += gh;
+            s.avg_gamma_nu_he1[ind * F + nu] += ghe;
+            g_sum += gh;
+            g_sum_he += ghe;
+          }
+        }
+        s.avg_gamma[ind] += g_sum;
+        s.avg_gamma_he1[ind] += g_sum_he;
+        if (dump) {
+          p = 99;
+          was_dumped = true;
+        } else {
+          p += 1;
+        }
+        // MFP opacity of the ionized part (:850-857); in a neutral cell the
+        // previous segment's dtau is added again, as the reference does
+        if (fi > 0.) {
+          for (int nu = 0; nu < F; nu++) {
+            dtau_h1[nu] = (double)(fi * dist / mf);
+            dtau_he1[nu] = chi * dtau_h1[nu];
+          }
+        }
+        for (int nu = 0; nu < F; nu++) {
+          tau_h1[nu] += dtau_h1[nu];
+          tau_he1[nu] += dtau_he1[nu];
+        }
+        // photons absorbed by the neutral part (:867-919)
+        if (fi < 1.) {
+          double* ng = &s.ngam3[(size_t)ind * 3];
+          const double sum_ngam = ng[0];
+          const double free_ngam = vol * s.nH[ind] * (1. + chi) * (1. - fi);
+          double nphot_left = 0.;
+          for (int nu = 0; nu < F; nu++) ng[2] += nphot_in[nu] * dm_exp(-(dtau_h1[nu] + dtau_he1[nu]));
+          if (sum_ngam < free_ngam) {
+            for (int nu = 0; nu < F; nu++) {
+              const double a = nphot_in[nu] * dm_exp(-(dtau_h1[nu] + dtau_he1[nu]));
+              const double room = free_ngam - sum_ngam;
+              const double ngam_abs = (a >= room) ? room : a;  // mindd
+              ng[0] += ngam_abs;
+              ng[1] += ngam_abs * sg::sigma_eff_rh * dist;
+              nphot_in[nu] = a - ngam_abs;
+              nphot_left += nphot_in[nu];
+            }
+            if (nphot_left == 0.) {
+              p = 99;
+              for (int nu = 0; nu < F; nu++) tau_h1[nu] = 999.;
+            } else {
+              for (int nu = 0; nu < F; nu++)
+                tau_h1[nu] += -dm_log(nphot_in[nu] / nphot[nu] / dm_exp(-(tau_h1[nu] + tau_he1[nu])));
+            }
+          }
+        }
+      }
+      if (last) {
+        for (int nu = 0; nu < F; nu++) ray.nphot[nu] = (float)(nphot[nu] * dm_exp(-tau_h1[nu] - tau_he1[nu]) / phot_norm);
+        if (was_dumped) dumped++;
+        if (sum_nphot(s, ray) > s.nphot_min) {
+          ray.active = true;
+          ray.path_ind[0] = ray.path_ind[ray.ray_end - 1];
+        } else {
+          ray.tag = false;
+          ray.active = false;
+          terminated++;
+        }
+      }
+    }
+    // cell update, subgrid branch (:954-1064) and the last-pass updates (:1139-1210)
+    for (int c = 0; c < s.N3; c++) {
+      const double nelec = (double)(s.nH[c] - s.avg_nh1[c] + s.nHe[c] - s.avg_nhe1[c]);
+      float ag = (float)s.avg_gamma[c];
+      bool first = false;
+      if (last && (s.treion_flag[c] == 0.f) && ((s.ngam3[(size_t)c * 3 + 1] > 0.) || (ag > 0.))) {
+        s.treion[c] = s.t - s.dt;
+        s.treion_flag[c] = 1.f;
+        get_gamma_interp_fh1(s, c, s.treion[c] + s.tinit, (float)(s.rho[c] / s.avg_rho - 1.), s.t - s.treion[c]);
+        first = true;
+      }
+      if ((s.treion_flag[c] != 0.f) && (s.fion[c] > 0.)) mfp_equilibrium(s, c, ag);
+      if ((s.treion_flag[c] != 0.f) && last && (s.fion[c] < 1.)) {
+        const double gamma_fion = s.ngam3[(size_t)c * 3 + 1] / s.dt / vol;
+        update_fion(s, c);
+        if ((s.fion[c] < 1.) && (ag == 0.) && !first)
+          s.fion[c] = maxd(0.f, (float)(s.fion[c] - s.clump[c] * alphaB_H2(maxd(s.temp_min, s.temp[c])) * (1. + s.nhe_to_nh) *
+                                                        s.nH[c] * s.fion[c] * s.dt));
+        if (first) {
+          s.gamma_prev[c] = 0.f;
+          ag = (float)gamma_fion;
+          mfp_equilibrium(s, c, ag);
+        }
+      }
+      if ((s.fion[c] == 1.) && (ag == 0.) && last)  // :1056-1059
+        s.fion[c] = maxd(0.f, (float)(s.fion[c] - s.clump[c] * alphaB_H2(maxd(s.temp_min, s.temp[c])) * (1. + s.nhe_to_nh) *
+                                                      s.nH[c] * s.fion[c] * s.dt));
+      if (!last) continue;
+      float nHe3 = 0.f;
+      if (s.treion_flag[c] != 0.f) {  // :1067-1078
+        interp_nh1(s, c, ag);
+        s.nH2[c] = s.nH[c] * s.fion[c];
+        s.nHe2[c] = s.nHe[c] * s.fion[c];
+        s.gamma_prev[c] = ag;
+      }
+      for (int nu = 0; nu < F; nu++) {
+        s.gamma_H1_nu[c * F + nu] = (float)s.avg_gamma_nu[c * F + nu];
+        s.gamma_He1_nu[c * F + nu] = (float)s.avg_gamma_nu_he1[c * F + nu];
+        s.ngam_nu[c * F + nu] = (float)s.ngam_acc[c * F + nu];
+      }
+      s.gamma_H1_tot[c] = ag;
+      s.gamma_He1_tot[c] = (float)s.avg_gamma_he1[c];
+      s.recrate_H2[c] = (float)(s.clump[c] * s.recomb_H2[c] * nelec * s.nH2[c]);
+      s.recrate_He2[c] = (float)(s.clump[c] * s.recomb_He2[c] * nelec * s.nHe2[c]);
+      s.ne[c] = (float)(s.nH2[c] + s.nHe2[c] + 2. * nHe3);
+      s.dne_dt[c] += s.nH[c] + s.nHe[c];
+      s.n_tot[c] = (float)(2. * (s.nH[c] + s.nHe[c]));
+      s.dne_dt[c] /= s.dt;
+      s.nH1_prev[c] = s.nH1[c];
+      s.f_H2[c] = s.nH2[c] / s.nH[c];
+      s.f_He2[c] = s.nHe2[c] / s.nHe[c];
+      s.f_He3[c] = nHe3 / s.nHe[c];
+      if (s.fion[c] > 0.) {
+        s.f_H2[c] *= s.fion[c];
+        s.f_He2[c] *= s.fion[c];
+        s.f_He3[c] *= s.fion[c];
+      }
+      s.f_H1[c] = (float)((double)1. - s.f_H2[c]);
+      s.f_He1[c] = (float)((double)1. - s.f_He2[c] - s.f_He3[c]);
+    }
+  }
+  s.st.rays_dumped = dumped;
+  s.st.rays_terminated = terminated;
+  s.have_paths = false;
+  s.dirty = true;
+}
+
 // ---- find_avg_unu ----------------------------------------------------------------------------
 static void find_avg_unu(State& s) {
+  if (s.subgrid) {
+    find_avg_unu_subgrid(s);
+    return;
+  }
   const int F = s.Nfreq;
   const int itercount = s.cfg.itercount;
   int64_t dumped = 0, terminated = 0;
@@ -969,13 +1452,18 @@ static void update_thermal(State& s) {
   int64_t substeps = 0;
 #pragma omp parallel for schedule(dynamic, 4096) reduction(+ : substeps)
   for (int c = 0; c < s.N3; c++) {
-    if (s.cfg.temp_ev) {
+    if (s.cfg.temp_ev && s.subgrid && s.treion_flag[c] == 0.f) {
+      s.temp[c] = s.temp_0;  // gas_funcs.cc:380-382: neutral until first ionized
+    } else if (s.cfg.temp_ev) {
       float t_temp = 0.;
-      set_Treion_default(s, c);
+      if (!s.subgrid) set_Treion_default(s, c);
       int nsub = 0;
       while (t_temp < s.dt) {
         float dt_temp = mind(get_thermal_tstep(s, c), s.dt - t_temp);
-        update_heat_cool(s, c);
+        if (s.subgrid)
+          update_heat_cool_sg(s, c);
+        else
+          update_heat_cool(s, c);
         // gas_funcs.cc:398-399; the ddelta_dt term subtracts zero while
         // ddelta_dt is 0 (no reload yet) and is skipped then, as rho may not
         // have been given at all
@@ -987,7 +1475,7 @@ static void update_thermal(State& s) {
       }
       substeps += nsub;
       s.temp[c] = maxd(s.temp[c], s.temp_min);
-      s.nH1_prev[c] = s.nH1[c];
+      if (!s.subgrid) s.nH1_prev[c] = s.nH1[c];
     } else {
       s.temp[c] = (s.nH1[c] / s.nH[c] > 0.5) ? 100.f : 1e4f;
     }
@@ -1153,8 +1641,63 @@ void* orc_create(const frt_config* cfg) {
   s->avg_gamma_nu_he1.assign(nf, 0.);
   s->ngam_acc.assign(nf, 0.);
   get_random_angles(*s);
+  if (cfg->subgrid) {
+    s->subgrid = true;
+    for (auto* v : {&s->fion, &s->fion_max, &s->tcross, &s->tcross_flag, &s->treion_out, &s->delta_fion})
+      v->assign(n3, 0.f);
+    s->tcross_new.assign(n3 * sg::Ntcross, 0.f);
+    s->tcross_check.assign(n3 * sg::Ntcross, 0.f);
+    s->fh1_nodes.assign(n3 * 5, 0.f);
+    s->ngam3.assign(n3 * 3, 0.);
+  }
   return s;
 }
+
+int orc_set_subgrid(void* h, const frt_subgrid* g) {
+  State* s = (State*)h;
+  if (!s->subgrid || !g || g->nt < 2 || !g->time_rh || !g->fh1_rh) return FRT_E_ARG;
+  s->nt = g->nt;
+  s->time_rh.assign(g->time_rh, g->time_rh + (size_t)27 * g->nt);
+  s->fh1_rh.assign(g->fh1_rh, g->fh1_rh + (size_t)27 * g->nt);
+  for (int i = 0; i < 3; i++) {
+    s->tre_dom[i] = g->tre_dom[i];
+    s->deltaNL_dom[i] = g->delta_nl[i];
+  }
+  s->t_z12 = g->t_z12;
+  s->t_z8 = g->t_z8;
+  s->tinit = g->tinit;
+  s->avg_rho = g->avg_rho;
+  s->temp_0 = g->temp_0;
+  s->fh1_0 = g->fh1_0;
+  return 0;
+}
+
+int orc_set_subgrid_delta_nl(void* h, const float* d) {
+  State* s = (State*)h;
+  if (!s->subgrid) return FRT_E_ARG;
+  for (int i = 0; i < 3; i++) s->deltaNL_dom[i] = d[i];
+  return 0;
+}
+
+static int subgrid_state_copy(State* s, const frt_subgrid_state* st, bool down) {
+  if (!s->subgrid || !st) return FRT_E_ARG;
+  const size_t n3 = (size_t)s->N3;
+  std::vector<float>* v[9] = {&s->fion, &s->fion_max, &s->tcross, &s->tcross_flag, &s->treion_out,
+                              &s->delta_fion, &s->tcross_new, &s->tcross_check, &s->fh1_nodes};
+  float* p[9] = {st->fion, st->fion_max, st->tcross, st->tcross_flag, st->treion_out,
+                 st->delta_fion, st->tcross_new, st->tcross_check, st->fh1_nodes};
+  const size_t w[9] = {1, 1, 1, 1, 1, 1, (size_t)sg::Ntcross, (size_t)sg::Ntcross, 5};
+  for (int k = 0; k < 9; k++) {
+    if (!p[k]) continue;
+    if (down)
+      std::memcpy(p[k], v[k]->data(), 4 * n3 * w[k]);
+    else
+      std::memcpy(v[k]->data(), p[k], 4 * n3 * w[k]);
+  }
+  return 0;
+}
+int orc_download_subgrid_state(void* h, const frt_subgrid_state* st) { return subgrid_state_copy((State*)h, st, true); }
+int orc_upload_subgrid_state(void* h, const frt_subgrid_state* st) { return subgrid_state_copy((State*)h, st, false); }
 
 void orc_destroy(void* h) { delete (State*)h; }
 
@@ -1221,6 +1764,11 @@ void orc_init_rt(void* h) {
   s->cast_calls = 0;
   s->nphot_merge = 0.f;
   for (auto& r : s->ray_dt) r = s->dt;
+  if (s->subgrid) {  // rt_funcs.cc:50-68, a fresh start with I-front tracking
+    for (auto* v : {&s->fion, &s->fion_max, &s->tcross, &s->tcross_flag, &s->tcross_new, &s->tcross_check})
+      std::fill(v->begin(), v->end(), 0.f);
+    std::fill(s->treion_flag.begin(), s->treion_flag.end(), 0.f);
+  }
 }
 
 void orc_get_nphot_min_max(void* h, double* nmin, double* nmax) {

@@ -10,8 +10,9 @@ import os
 
 import numpy as np
 
-from flexrt_imaging_reion_amd._abi import (FIELD_ID, PER_BIN, FrtOtf, FrtRayView, FrtStepStats,
-                                           config_struct, make_ray_view)
+from flexrt_imaging_reion_amd._abi import (FIELD_ID, PER_BIN, FrtOtf, FrtRayView, FrtStepStats, FrtSubgrid,
+                                           FrtSubgridState, config_struct, make_ray_view, make_subgrid,
+                                           make_subgrid_state)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -74,6 +75,10 @@ _PROTOS = {
     "orc_get_cell_size": (C.c_float, [C.c_void_p]),
     "orc_itheta_hash": (C.c_uint32, [C.c_uint64, C.c_uint64, C.c_uint64]),
     "orc_derived": (None, [_VP, C.POINTER(C.c_double)]),
+    "orc_set_subgrid": (C.c_int, [_VP, C.POINTER(FrtSubgrid)]),
+    "orc_set_subgrid_delta_nl": (C.c_int, [_VP, C.POINTER(C.c_float)]),
+    "orc_download_subgrid_state": (C.c_int, [_VP, C.POINTER(FrtSubgridState)]),
+    "orc_upload_subgrid_state": (C.c_int, [_VP, C.POINTER(FrtSubgridState)]),
 }
 
 RATES = {"sigmapi_H1": 0, "sigmapi_He1": 1, "alphaB_H2": 2, "alphaB_He2": 3, "alphaA_H2": 4,
@@ -171,6 +176,23 @@ class Oracle:
 
     def set_time(self, t):
         self._lib.orc_set_time(self._h, float(t))
+
+    def set_subgrid(self, sg):
+        st, keep = make_subgrid(sg)
+        assert self._lib.orc_set_subgrid(self._h, C.byref(st)) == 0
+
+    def set_subgrid_delta_nl(self, d):
+        a = np.ascontiguousarray(d, dtype=np.float32)
+        assert self._lib.orc_set_subgrid_delta_nl(self._h, a.ctypes.data_as(C.POINTER(C.c_float))) == 0
+
+    def download_subgrid_state(self):
+        st, arrays = make_subgrid_state(self.n ** 3)
+        assert self._lib.orc_download_subgrid_state(self._h, C.byref(st)) == 0
+        return arrays
+
+    def upload_subgrid_state(self, arrays):
+        st, keep = make_subgrid_state(self.n ** 3, arrays)
+        assert self._lib.orc_upload_subgrid_state(self._h, C.byref(st)) == 0
 
     def get_time(self):
         return self._lib.orc_get_time(self._h)
