@@ -1697,6 +1697,7 @@ static int subgrid_state_copy(State* s, const frt_subgrid_state* st, bool down) 
   return 0;
 }
 int orc_download_subgrid_state(void* h, const frt_subgrid_state* st) { return subgrid_state_copy((State*)h, st, true); }
+float orc_sg_deltasig(float deltaNL, float z) { return deltaNL_to_deltasig(deltaNL, z); }
 int orc_upload_subgrid_state(void* h, const frt_subgrid_state* st) { return subgrid_state_copy((State*)h, st, false); }
 
 void orc_destroy(void* h) { delete (State*)h; }

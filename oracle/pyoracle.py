@@ -79,6 +79,7 @@ _PROTOS = {
     "orc_set_subgrid_delta_nl": (C.c_int, [_VP, C.POINTER(C.c_float)]),
     "orc_download_subgrid_state": (C.c_int, [_VP, C.POINTER(FrtSubgridState)]),
     "orc_upload_subgrid_state": (C.c_int, [_VP, C.POINTER(FrtSubgridState)]),
+    "orc_sg_deltasig": (C.c_float, [C.c_float, C.c_float]),
 }
 
 RATES = {"sigmapi_H1": 0, "sigmapi_He1": 1, "alphaB_H2": 2, "alphaB_He2": 3, "alphaA_H2": 4,
@@ -350,3 +351,8 @@ def hubble(a, hh, om, ol):
 
 def itheta_hash(seed, stream, counter):
     return int(load().orc_itheta_hash(seed, stream, counter))
+
+
+def sg_deltasig(delta_nl, z):
+    """deltaNL_to_deltasig of the checker (subgrid.cc:193-204)."""
+    return float(load().orc_sg_deltasig(float(delta_nl), float(z)))
