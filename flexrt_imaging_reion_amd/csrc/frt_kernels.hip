@@ -720,19 +720,25 @@ __device__ __forceinline__ void bin_deposit_fp64(const GridConst& g, double dth,
   }
 }
 
-// Per-bin deposit of one segment, mixed form: transmissions eh, ehe and the
-// Gamma contributions gh, ghe (in units of phot_norm photons).
-__device__ __forceinline__ void bin_deposit_mixed(const GridConst& g, float dth, float dthe, float nin, bool dump,
-                                                  float rh1, float rhe1, float& gh, float& ghe, float& eh,
-                                                  float& ehe) {
-  // transmissions and absorbed fractions 1 - exp(-dtau) from one reduction
-  // each (the absorbed fraction through expm1: in ionized cells dtau is
-  // ~1e-5 and 1 - exp(-dtau) would keep only ~3 significant digits)
+// Transmissions eh, ehe of one segment and bin, mixed form, with the
+// absorbed fractions omh = 1 - eh, omhe = 1 - ehe from the same reduction
+// (through expm1: in ionized cells dtau is ~1e-5 and 1 - exp(-dtau) would
+// keep only ~3 significant digits).
+__device__ __forceinline__ void bin_atten_mixed(float dth, float dthe, float& eh, float& ehe, float& omh,
+                                                float& omhe) {
   float mh, mhe = 0.f;
   ehe = 1.f;
   frt_exp_expm1_neg(dth, eh, mh);
   if (dthe != 0.f) frt_exp_expm1_neg(dthe, ehe, mhe);
-  const float omh = -mh, omhe = -mhe;
+  omh = -mh;
+  omhe = -mhe;
+}
+
+// Gamma contributions gh, ghe (in units of phot_norm photons) of one segment
+// and bin from its transmissions (bin_atten_mixed), mixed form.
+__device__ __forceinline__ void bin_share_mixed(const GridConst& g, float dth, float dthe, float eh, float ehe,
+                                                float omh, float omhe, float nin, bool dump, float rh1,
+                                                float rhe1, float& gh, float& ghe) {
   float fh = 0.f, fhe = 0.f;
   if ((dth > 0.f) || (dthe > 0.f)) {
     // HI share fh = a/(a+b), a = (1 - eh) ehe, b = (1 - ehe) eh
@@ -784,6 +790,16 @@ __device__ __forceinline__ void bin_deposit_mixed(const GridConst& g, float dth,
   const float eabs = dump ? nin : nin * fmaf(-omh, omhe, omh + omhe);
   gh = fh * eabs * rh1;
   ghe = fhe * eabs * rhe1;
+}
+
+// Per-bin deposit of one segment, mixed form: transmissions eh, ehe and the
+// Gamma contributions gh, ghe (in units of phot_norm photons).
+__device__ __forceinline__ void bin_deposit_mixed(const GridConst& g, float dth, float dthe, float nin, bool dump,
+                                                  float rh1, float rhe1, float& gh, float& ghe, float& eh,
+                                                  float& ehe) {
+  float omh, omhe;
+  bin_atten_mixed(dth, dthe, eh, ehe, omh, omhe);
+  bin_share_mixed(g, dth, dthe, eh, ehe, omh, omhe, nin, dump, rh1, rhe1, gh, ghe);
 }
 
 // Software pipeline over a ray's segments: the path entry (cell, length,

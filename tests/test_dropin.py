@@ -87,7 +87,10 @@ _GLOBALS = r"""
 static const int itercount = 5, path_length = 5;
 static const float Lx0 = 40000.f, cl_factor = 0.3f, const_clump = 1.f, temp_0 = 100.f, tstep_factor = 1.f;
 static const double cl = 2.998e10;
-static const float zinit = 8.f;
+static const float zinit = 8.f, fH1_0 = 1.f - 1.2e-3f;
+static const int Nt = 5000;
+#define yr_to_s 3.154e7
+extern float tre_dom[3], deltaNL_dom[3];
 extern float a, zz, dt, t, avg_rho, Lx, Ly, Lz, gadget_step_time;
 extern double nu_phot[Nfreq];
 extern float spectrum[Nfreq];
