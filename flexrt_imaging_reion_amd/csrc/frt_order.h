@@ -37,6 +37,7 @@ struct FrtCellLayout {
   int n;
   int tiled;  // 1: 4x4x4 tiles, 0: row-major
   int t;      // tiles per side
+  int tshift; // log2(t) when t is a power of two, else -1
 };
 
 FRT_ORDER_HD inline FrtCellLayout frt_cell_layout(int n) {
@@ -44,6 +45,9 @@ FRT_ORDER_HD inline FrtCellLayout frt_cell_layout(int n) {
   L.n = n;
   L.tiled = (n % 4 == 0) ? 1 : 0;
   L.t = n >> 2;
+  L.tshift = -1;
+  for (int s = 0; s < 30; ++s)
+    if ((1 << s) == L.t) L.tshift = s;
   return L;
 }
 
@@ -61,10 +65,18 @@ FRT_ORDER_HD inline void frt_cell_ijk(const FrtCellLayout& L, int c, int& i, int
     return;
   }
   const int tile = c >> 6, loc = c & 63;
-  const int tk = tile % L.t;
-  const int tij = tile / L.t;
-  const int tj = tij % L.t;
-  const int ti = tij / L.t;
+  int ti, tj, tk;
+  if (L.tshift >= 0) {  // the same values by shifts
+    const int m = L.t - 1;
+    tk = tile & m;
+    tj = (tile >> L.tshift) & m;
+    ti = tile >> (2 * L.tshift);
+  } else {
+    tk = tile % L.t;
+    const int tij = tile / L.t;
+    tj = tij % L.t;
+    ti = tij / L.t;
+  }
   i = (ti << 2) | (loc >> 4);
   j = (tj << 2) | ((loc >> 2) & 3);
   k = (tk << 2) | (loc & 3);
