@@ -29,9 +29,10 @@ FIELDS = [
     "gamma_H1_tot", "gamma_He1_tot", "recrate_H2", "recrate_He2", "treion", "treion_flag",
     "heat_rate", "cool_rate", "IF_speed", "Inc_Flux", "gamma_H1_nu", "gamma_He1_nu", "ngam_nu",
     "rho", "f_He3", "mfp", "mfp_912", "avg_gamma_prev", "gadget_gamma", "gadget_heat", "ddelta_dt", "rho_prev",
+    "nHe3", "nH2_prev", "nHe1_prev", "nHe2_prev", "nHe3_prev", "recomb_He3", "u_nu",
 ]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
-PER_BIN = {"gamma_H1_nu", "gamma_He1_nu", "ngam_nu"}
+PER_BIN = {"gamma_H1_nu", "gamma_He1_nu", "ngam_nu", "u_nu"}
 
 
 class FrtConfig(C.Structure):
@@ -40,7 +41,7 @@ class FrtConfig(C.Structure):
         ("precision", C.c_int32), ("hpx_lvl", C.c_int32), ("l_max", C.c_int32),
         ("ray_per_cell_max", C.c_int32), ("rec_case_b", C.c_int32), ("temp_ev", C.c_int32),
         ("recomb_cool", C.c_int32), ("free_free", C.c_int32), ("compton", C.c_int32),
-        ("frac_limit", C.c_int32), ("subgrid", C.c_int32), ("reserved1", C.c_int32),
+        ("frac_limit", C.c_int32), ("subgrid", C.c_int32), ("c2ray_off", C.c_int32),
         ("ray_capacity", C.c_int64), ("seed", C.c_uint64),
         ("box_ckpc", C.c_double), ("a_rt", C.c_double), ("zz", C.c_double), ("dt", C.c_double),
         ("cl_factor", C.c_double), ("nl_const", C.c_double), ("safety_factor", C.c_double),

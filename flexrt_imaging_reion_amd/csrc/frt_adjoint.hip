@@ -295,7 +295,8 @@ extern "C" {
 
 int frt_adj_enable(frt_ctx* ctx, int32_t on) {
   if (!ctx) return FRT_E_ARG;
-  if (on && ctx->subgrid) return set_error(ctx, FRT_E_ARG, "the adjoint is built for the C2Ray mode (subgrid = 0)");
+  if (on && (ctx->subgrid || ctx->c2ray_off))
+    return set_error(ctx, FRT_E_ARG, "the adjoint is built for the C2Ray mode (subgrid = 0, c2ray_off = 0)");
   if (on && ctx->g.P > 127) return set_error(ctx, FRT_E_ARG, "the adjoint tape holds at most 127 segments per ray");
   if (on && !ctx->adj_tape) {
     const size_t nd = (size_t)(4 * ctx->cfg.itercount + 2) * (size_t)ctx->g.n3;

@@ -219,6 +219,8 @@ int do_update_rays(frt_ctx* ctx) {
   if ((rc = launch_add_ray_dt(ctx))) return rc;
   if ((rc = launch_march(ctx))) return rc;
   if ((rc = launch_segments(ctx))) return rc;
+  // c2ray_iter = FALSE: the march deposits into u_nu (ray_tracing.cc:1329-1355,1443-1470)
+  if (ctx->c2ray_off && (rc = launch_deposit_inline(ctx))) return rc;
   ctx->have_segments = true;
   ctx->have_alive = true;
   return FRT_OK;
@@ -227,7 +229,16 @@ int do_update_rays(frt_ctx* ctx) {
 int do_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   if (!ctx->have_segments) return set_error(ctx, FRT_E_STATE, "find_avg_unu needs update_rays first");
   ctx->g.nphot_min = ctx->nphot_min;
-  int rc = launch_find_avg_unu(ctx, fuse_thermal);
+  int rc;
+  if (ctx->c2ray_off) {
+    // set_to_true only (main.cc:142-144); in a fused step the cell updates
+    // of main.cc:186-193 run here, as merge and split touch no cell field
+    if (fuse_thermal && ((rc = launch_gamma_chem(ctx)) || (rc = launch_thermal(ctx)))) return rc;
+    ctx->have_segments = false;
+    ctx->have_alive = true;
+    return FRT_OK;
+  }
+  rc = launch_find_avg_unu(ctx, fuse_thermal);
   if (rc) return rc;
   ctx->have_segments = false;
   ctx->have_alive = true;
@@ -426,12 +437,19 @@ float* field_ptr(frt_ctx* ctx, int field) {
     case FRT_F_GADGET_HEAT: return ctx->c.f[C_GADGET_HEAT];
     case FRT_F_DDELTA_DT: return ctx->c.f[C_DDELTA_DT];
     case FRT_F_RHO_PREV: return ctx->c.f[C_RHO_PREV];
+    case FRT_F_NHE3: return ctx->c.f[C_NHE3];
+    case FRT_F_NH2_PREV: return ctx->c.f[C_NH2_PREV];
+    case FRT_F_NHE1_PREV: return ctx->c.f[C_NHE1_PREV];
+    case FRT_F_NHE2_PREV: return ctx->c.f[C_NHE2_PREV];
+    case FRT_F_NHE3_PREV: return ctx->c.f[C_NHE3_PREV];
+    case FRT_F_RECOMB_HE3: return ctx->c.f[C_RECOMB_HE3];
+    case FRT_F_U_NU: return ctx->c.u_nu;
     default: return nullptr;
   }
 }
 
 static bool per_bin(int field) {
-  return field == FRT_F_GAMMA_H1_NU || field == FRT_F_GAMMA_HE1_NU || field == FRT_F_NGAM_NU;
+  return field == FRT_F_GAMMA_H1_NU || field == FRT_F_GAMMA_HE1_NU || field == FRT_F_NGAM_NU || field == FRT_F_U_NU;
 }
 
 // Whole-file reads and writes of the restart files.
@@ -539,6 +557,9 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   // the sub-grid model is built in the FP64 form with temperature evolution
   // (temp_ev = FALSE would take interp_temp, subgrid.cc:637-649)
   if (c.subgrid != 0 && (c.subgrid != 1 || c.precision != FRT_PREC_FP64 || !c.temp_ev)) return FRT_E_ARG;
+  // the c2ray_iter = FALSE mode is built in the FP64 form, without the
+  // sub-grid model (its march deposit ignores it, ray_tracing.cc:1336-1337)
+  if (c.c2ray_off != 0 && (c.c2ray_off != 1 || c.precision != FRT_PREC_FP64 || c.subgrid != 0)) return FRT_E_ARG;
   const int64_t n3 = (int64_t)c.n * c.n * c.n;
   const int64_t cap = c.ray_capacity > 0 ? c.ray_capacity : 30 * n3;
   // segment indices are int32; split bookkeeping packs (ray << 2 | child) in 32 bits
@@ -684,6 +705,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     const size_t nf = (size_t)(C_NSCALAR + 3 * c.nfreq) * (size_t)n3;
     if (e == hipSuccess) e = dalloc(&ctx->cell_block, nf);
     if (e == hipSuccess) e = dalloc(&ctx->c.avg2, (size_t)n3);
+    if (e == hipSuccess) e = dalloc(&ctx->c.u_nu, (size_t)c.nfreq * (size_t)n3);
     if (e == hipSuccess) e = dalloc(&ctx->rot_fwd, 9 * FRT_NTHETA);
     if (e == hipSuccess) e = dalloc(&ctx->rot_inv, 9 * FRT_NTHETA);
     if (e == hipSuccess) e = dalloc(&ctx->dcnt, FRT_CTR_COPIES);
@@ -700,6 +722,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     ctx->c.gamma_he1_nu = ctx->c.gamma_h1_nu + (size_t)c.nfreq * n3;
     ctx->c.ngam_nu = ctx->c.gamma_he1_nu + (size_t)c.nfreq * n3;
     e = hipMemsetAsync(ctx->cell_block, 0, sizeof(float) * (size_t)(C_NSCALAR + 3 * c.nfreq) * n3, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->c.u_nu, 0, sizeof(float) * (size_t)c.nfreq * n3, ctx->stream);
     // clumping factor defaults to const_clump = 1 (user_inputs.h:87, set_clump subgrid.cc:799)
     if (e == hipSuccess) {
       std::vector<float> ones((size_t)n3, 1.0f);
@@ -718,6 +741,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   if (rc == FRT_OK) rc = reserve_workspace(ctx);
   if (rc == FRT_OK) rc = scratch_address(ctx, &ctx->otf_scratch);
   ctx->subgrid = c.subgrid == 1;
+  ctx->c2ray_off = c.c2ray_off == 1;
   if (rc == FRT_OK && ctx->subgrid) rc = alloc_subgrid(ctx);
   if (rc != FRT_OK) {
     std::string msg = ctx->err;
@@ -746,6 +770,7 @@ int frt_destroy(frt_ctx* ctx) {
   }
   hipFree(ctx->contrib.rec); hipFree(ctx->path_slot); hipFree(ctx->pdf);  // rec is the block base
   hipFree(ctx->cell_start); hipFree(ctx->cell_end); hipFree(ctx->cell_block); hipFree(ctx->c.avg2);
+  hipFree(ctx->c.u_nu);
   hipFree(ctx->otf_buf);
   hipFree(ctx->adj_tape);
   hipFree(ctx->adj_nph);
@@ -774,6 +799,7 @@ int frt_set_spectrum(frt_ctx* ctx, const double* nu_phot, const float* spectrum)
     ctx->g.spectrum[nu] = spectrum[nu];
     ctx->g.sigH[nu] = frt_sigmapi_H1(nu_phot[nu]);
     ctx->g.sigHe[nu] = frt_sigmapi_He1(nu_phot[nu]);
+    ctx->g.sigHe2[nu] = frt_sigmapi_He2(nu_phot[nu]);
     ctx->g.sigHf[nu] = (float)ctx->g.sigH[nu];
     ctx->g.sigHef[nu] = (float)ctx->g.sigHe[nu];
   }
@@ -1315,6 +1341,8 @@ int frt_split_rays(frt_ctx* ctx) {
 int frt_update_thermal(frt_ctx* ctx) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  // c2ray_iter = FALSE: update_gamma and update_chem come first (main.cc:186-191)
+  if (ctx->c2ray_off && (rc = launch_gamma_chem(ctx))) return rc;
   if ((rc = launch_thermal(ctx))) return rc;
   return finish_stats(ctx);
 }

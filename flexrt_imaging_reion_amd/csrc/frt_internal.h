@@ -76,6 +76,7 @@ struct GridConst {
   double sig912;        // sigmapi_H1(13.6/h_eV)
   double sigH[FRT_MAX_FREQ];
   double sigHe[FRT_MAX_FREQ];
+  double sigHe2[FRT_MAX_FREQ];  // sigmapi_He2(nu_phot) (update_gamma of the c2ray_off mode)
   float sigHf[FRT_MAX_FREQ];
   float sigHef[FRT_MAX_FREQ];
   double nu[FRT_MAX_FREQ];
@@ -145,7 +146,8 @@ enum CellField {
   C_TEMP, C_DNE_DT, C_CLUMP, C_NH1_PREV, C_AVG_NH1, C_AVG_NHE1, C_RECOMB_H2, C_RECOMB_HE2,
   C_GAMMA_H1_TOT, C_GAMMA_HE1_TOT, C_RECRATE_H2, C_RECRATE_HE2, C_TREION, C_TREION_FLAG,
   C_HEAT, C_COOL, C_IF_SPEED, C_INC_FLUX, C_RHO, C_FHE3, C_MFP, C_MFP912, C_GAMMA_PREV, C_GADGET_GAMMA,
-  C_GADGET_HEAT, C_DDELTA_DT, C_RHO_PREV, C_NSCALAR
+  C_GADGET_HEAT, C_DDELTA_DT, C_RHO_PREV, C_NHE3, C_NH2_PREV, C_NHE1_PREV, C_NHE2_PREV, C_NHE3_PREV,
+  C_RECOMB_HE3, C_NSCALAR
 };
 
 struct CellPtrs {
@@ -153,6 +155,7 @@ struct CellPtrs {
   float* gamma_h1_nu;   // [nfreq][n3]
   float* gamma_he1_nu;  // [nfreq][n3]
   float* ngam_nu;       // [nfreq][n3]
+  float* u_nu;          // [nfreq][n3] the c2ray_off mode's march deposit (reset every step)
   float2* avg2;         // [n3] {avg_nh1, avg_nhe1} of the next deposit pass, one 8-byte gather
 };
 
@@ -293,6 +296,7 @@ struct frt_ctx {
   int psum_state;
   // sub-grid I-front model
   bool subgrid;
+  bool c2ray_off;         // c2ray_iter = FALSE: deposit in the march, update_gamma + update_chem
   bool subgrid_set;       // frt_set_subgrid has been called
   SgConst sgc;
   SgPtrs sg;
@@ -392,6 +396,8 @@ uint32_t itheta_hash(uint64_t seed, uint64_t stream, uint64_t counter);
 
 int launch_march(frt_ctx* ctx);
 int launch_segments(frt_ctx* ctx);
+int launch_deposit_inline(frt_ctx* ctx);
+int launch_gamma_chem(frt_ctx* ctx);
 int reserve_workspace(frt_ctx* ctx);
 int read_device_counters(frt_ctx* ctx);
 int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal);

@@ -1452,6 +1452,172 @@ __global__ void __launch_bounds__(BLK) k_thermal(CellPtrs cf, GridConst g, DevCo
 }
 
 // ---------------------------------------------------------------------------
+// c2ray_iter = FALSE (FP64 form): the march deposits into u_nu segment by
+// segment and attenuates the ray by the HI opacity alone (ray_tracing.cc:
+// 1329-1355 in the crossing loop, 1443-1470 for the last segment, the same
+// expressions), then re-labels the ray's cell (:1507-1509).  One thread per
+// moved ray writes each segment's per-bin u_nu addition (float, as the
+// reference adds it) to the segment's cell-sorted slot; k_unu_sum adds them
+// per cell in slot (= ray list) order.
+__global__ void __launch_bounds__(BLK) k_deposit_inline(RaySoA rb, const int32_t* __restrict__ moved,
+                                                        const StepDev* __restrict__ sd, GridConst g,
+                                                        const int32_t* __restrict__ path_cell,
+                                                        const float* __restrict__ path_dist,
+                                                        const int32_t* __restrict__ seg_cnt,
+                                                        const int32_t* __restrict__ path_slot,
+                                                        const float* __restrict__ nH1, ContribPtrs cp,
+                                                        float* __restrict__ psum) {
+  const int F = g.nfreq;
+  const double vol = g.vol;  // ((double)dx)^3 as the reference's ddx*ddy*ddz
+  const int64_t nmoved = sd->nmoved;
+  // a slot holds F float additions in the first F floats of its record
+  const int64_t stride = 2 * (int64_t)cp.rec_len;
+  for_rays(seg_cnt, moved, nmoved, nullptr, 0, [&](int64_t i, int c) {
+    float nph[8];
+    phot_load(rb, i, F, g.cap, nph);
+    int last_cell = 0;
+    for (int p = 0; p < c; ++p) {
+      const int cell = path_cell[(int64_t)p * g.cap + i];
+      const float dist = path_dist[(int64_t)p * g.cap + i];
+      const int slot = path_slot[(int64_t)p * g.cap + i];
+      const float n1 = nH1[cell];
+      float* rec = (float*)cp.rec + (int64_t)slot * stride;
+      last_cell = cell;
+      FRT_FOR_BINS(nu) {
+        const double nphot = (double)nph[nu] * FRT_PHOT_NORM;
+        const double dtau = (double)n1 * g.sigH[nu] * (double)dist;
+        const double att = dm_exp(-dtau);
+        const double nphot_add = nphot * (1. - att);
+        const double gamma = nphot_add / (double)g.dt / vol / (double)n1;
+        rec[nu] = (float)((double)g.spectrum[nu] * gamma * FRT_H_PLANCK * g.nu[nu] / (double)g.clight / g.sigH[nu]);
+        nph[nu] *= (float)att;
+      }
+    }
+    phot_store(rb, i, F, g.cap, nph);
+    psum[i] = phot_sum(nph, F);
+    ray_set_cell(rb, i, last_cell);
+  });
+}
+
+// u_nu of every cell: zero (rt_funcs.cc:174-189), plus the additions of the
+// cell's segments in slot order, in float.
+__global__ void __launch_bounds__(BLK) k_unu_sum(CellPtrs cf, GridConst g, ContribPtrs cp,
+                                                 const int32_t* __restrict__ cstart,
+                                                 const int32_t* __restrict__ cend) {
+  const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (k >= g.n3) return;
+  const int F = g.nfreq;
+  const int64_t stride = 2 * (int64_t)cp.rec_len;
+  float u[8];
+  FRT_FOR_BINS(nu) u[nu] = 0.f;
+  const int s0 = cstart[k], s1 = cend[k];
+  for (int sl = s0; sl < s1; ++sl) {
+    const float* rec = (const float*)cp.rec + (int64_t)sl * stride;
+    FRT_FOR_BINS(nu) u[nu] += rec[nu];
+  }
+  FRT_FOR_BINS(nu) cf.u_nu[(int64_t)nu * g.n3 + k] = u[nu];
+}
+
+// update_gamma (gas_funcs.cc:74-99, collisional ionization off) and
+// update_chem (:299-351) with solve_ion (:243-296) iterated five times, one
+// thread per cell.  In solve_ion's first helium branch (He III the most
+// abundant ion at the start of the step) the reference's (nHe[i] - nHe1[i])
+// is a difference of row pointers of two arrays (undefined); the cell values
+// are used.  Photons below 54.4 eV leave gamma_He2_tot = 0, so with the
+// usual initial state (no He III) that branch is not reached.
+__global__ void __launch_bounds__(BLK) k_gamma_chem(CellPtrs cf, GridConst g) {
+  const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (k >= g.n3) return;
+  const int F = g.nfreq;
+  const int64_t n3 = g.n3;
+  // update_gamma: float fields += double expressions
+  float gH1 = 0.f, gHe1 = 0.f, gHe2 = 0.f;
+  FRT_FOR_BINS(nu) {
+    const double u = (double)cf.u_nu[(int64_t)nu * n3 + k];
+    const double cl = (double)g.clight;
+    gH1 = (float)((double)gH1 + g.sigH[nu] * cl * u / FRT_H_PLANCK / g.nu[nu]);
+    gHe1 = (float)((double)gHe1 + g.sigHe[nu] * cl * u / FRT_H_PLANCK / g.nu[nu]);
+    gHe2 = (float)((double)gHe2 + g.sigHe2[nu] * cl * u / FRT_H_PLANCK / g.nu[nu]);
+  }
+  // update_chem
+  const float dt = g.dt;
+  const float C = cf.f[C_CLUMP][k];
+  const float nH = cf.f[C_NH][k], nHe = cf.f[C_NHE][k];
+  const float nH1p = cf.f[C_NH1_PREV][k], nH2p = cf.f[C_NH2_PREV][k];
+  const float nHe1p = cf.f[C_NHE1_PREV][k], nHe2p = cf.f[C_NHE2_PREV][k], nHe3p = cf.f[C_NHE3_PREV][k];
+  float nH1 = cf.f[C_NH1][k], nH2 = cf.f[C_NH2][k];
+  float nHe1 = cf.f[C_NHE1][k], nHe2 = cf.f[C_NHE2][k], nHe3 = cf.f[C_NHE3][k];
+  float ne = cf.f[C_NE][k];
+  const float temp = cf.f[C_TEMP][k];
+  float dne_dt = -ne;
+  const double T = (double)temp;
+  float rH2, rHe2, rHe3;
+  if (!g.rec_case_b) {
+    rH2 = (float)frt_alphaA_H2(T);
+    rHe2 = (float)frt_alphaA_He2(T);
+    rHe3 = (float)frt_alphaA_He3(T);
+  } else {
+    rH2 = (float)frt_alphaB_H2(T);
+    rHe2 = (float)frt_alphaB_He2(T);
+    rHe3 = (float)frt_alphaB_He3(T);
+  }
+  if (((double)temp >= 3e4) && ((double)temp <= 1e6)) rHe2 += (float)frt_Dalpha_He2(T);
+  for (int m = 0; m < 5; ++m) {
+    if (nH1p >= nH2p) {
+      nH2 = (float)((double)(nH2p + gH1 * nH * dt) / (1. + (double)(gH1 * dt) + (double)(C * rH2 * ne * dt)));
+      nH1 = nH - nH2;
+    } else {
+      nH1 = (float)((double)(nH1p + C * rH2 * ne * nH * dt) / (1. + (double)(gH1 * dt) + (double)(C * rH2 * ne * dt)));
+      nH2 = nH - nH1;
+    }
+    if ((nHe1p <= nHe3p) && (nHe2p <= nHe3p)) {
+      nHe1 = (float)((double)(nHe1p + C * rHe2 * ne * (nHe - nHe3) * dt) /
+                     (1. + (double)(gHe1 * dt) + (double)(C * rHe2 * ne * dt)));
+      nHe2 = (float)((double)(nHe2p + gHe1 * (nHe - nHe3) * dt + C * rHe3 * ne * (nHe - nHe1) * dt) /
+                     (1. + (double)((gHe1 + gHe2) * dt) + (double)(C * (rHe2 + rHe3) * ne * dt)));
+      nHe3 = nHe - nHe1 - nHe2;
+    } else if ((nHe1p <= nHe2p) && (nHe3p <= nHe2p)) {
+      nHe1 = (float)((double)(nHe1p + C * rHe2 * ne * (nHe - nHe3) * dt) /
+                     (1. + (double)(gHe1 * dt) + (double)(C * rHe2 * ne * dt)));
+      nHe3 = (float)((double)(nHe3p + gHe2 * (nHe - nHe1) * dt) / (1. + (double)(gHe2 * dt) + (double)(C * rHe3 * ne * dt)));
+      nHe2 = nHe - nHe1 - nHe3;
+    } else {
+      nHe2 = (float)((double)(nHe2p + gHe1 * (nHe - nHe3) * dt + C * rHe3 * ne * (nHe - nHe1) * dt) /
+                     (1. + (double)((gHe1 + gHe2) * dt) + (double)(C * (rHe2 + rHe3) * ne * dt)));
+      nHe3 = (float)((double)(nHe3p + gHe2 * (nHe - nHe1) * dt) / (1. + (double)(gHe2 * dt) + (double)(C * rHe3 * ne * dt)));
+      nHe1 = nHe - nHe2 - nHe3;
+    }
+    ne = (float)((double)(nH2 + nHe2) + 2. * (double)nHe3);
+  }
+  const float ntot = nH + nHe + ne;
+  dne_dt += ne;
+  dne_dt /= dt;
+  cf.f[C_GAMMA_H1_TOT][k] = gH1;
+  cf.f[C_GAMMA_HE1_TOT][k] = gHe1;
+  cf.f[C_RECOMB_H2][k] = rH2;
+  cf.f[C_RECOMB_HE2][k] = rHe2;
+  cf.f[C_RECOMB_HE3][k] = rHe3;
+  cf.f[C_NH1][k] = nH1;
+  cf.f[C_NH2][k] = nH2;
+  cf.f[C_NHE1][k] = nHe1;
+  cf.f[C_NHE2][k] = nHe2;
+  cf.f[C_NHE3][k] = nHe3;
+  cf.f[C_NE][k] = ne;
+  cf.f[C_NTOT][k] = ntot;
+  cf.f[C_DNE_DT][k] = dne_dt;
+  cf.f[C_NH1_PREV][k] = nH1;
+  cf.f[C_NH2_PREV][k] = nH2;
+  cf.f[C_NHE1_PREV][k] = nHe1;
+  cf.f[C_NHE2_PREV][k] = nHe2;
+  cf.f[C_NHE3_PREV][k] = nHe3;
+  cf.f[C_FH1][k] = nH1 / nH;
+  cf.f[C_FH2][k] = nH2 / nH;
+  cf.f[C_FHE1][k] = nHe1 / nHe;
+  cf.f[C_FHE2][k] = nHe2 / nHe;
+  cf.f[C_FHE3][k] = nHe3 / nHe;
+}
+
+// ---------------------------------------------------------------------------
 // Merging (ray_tracing.cc:176-227 calc_nphot_merge, :324-477 merge).
 __global__ void __launch_bounds__(BLK) k_count_alive(const uint8_t* __restrict__ alive, const StepDev* __restrict__ sd,
                                                      DevCounters* cnt) {
@@ -2317,6 +2483,25 @@ int launch_find_avg_unu(frt_ctx* ctx, bool fuse_thermal) {
   // the last pass wrote the photon sums of the moved rays; the march's hold
   // for the others
   if (I > 0 && ctx->psum_state != FRT_PSUM_NONE) ctx->psum_state = FRT_PSUM_VALID;
+  return FRT_OK;
+}
+
+// c2ray_iter = FALSE: the march's deposit and u_nu (after launch_segments).
+int launch_deposit_inline(frt_ctx* ctx) {
+  if (int rc0 = materialize_rays(ctx)) return rc0;
+  GridConst g = ctx->g;
+  FRT_LAUNCH(ctx, "deposit_inline", ray_grid(ctx, bound_rays(ctx)), k_deposit_inline, cur_buf(ctx), ctx->moved_idx,
+             ctx->sd, g, ctx->path_cell, ctx->path_dist, ctx->seg_cnt, ctx->path_slot, ctx->c.f[C_NH1],
+             ctx->contrib, ctx->psum);
+  FRT_LAUNCH(ctx, "unu_sum", grid_for(g.n3), k_unu_sum, ctx->c, g, ctx->contrib, ctx->cell_start, ctx->cell_end);
+  // the deposit wrote the photon sums of the moved rays; the march's hold for the others
+  if (ctx->psum_state != FRT_PSUM_NONE) ctx->psum_state = FRT_PSUM_VALID;
+  return FRT_OK;
+}
+
+// c2ray_iter = FALSE: update_gamma and update_chem (main.cc:186-191).
+int launch_gamma_chem(frt_ctx* ctx) {
+  FRT_LAUNCH(ctx, "gamma_chem", grid_for(ctx->g.n3), k_gamma_chem, ctx->c, ctx->g);
   return FRT_OK;
 }
 

@@ -51,6 +51,9 @@ FRT_HD double frt_sigmapi_H1(double nu) {
 FRT_HD double frt_sigmapi_He1(double nu) {
   return 9.492e-16 * frt_F_of_y(nu, FRT_ETH_HE1, 5e+4, 1.361e+1, 4.434e-1, 2.136, 2.039, 1.469, 3.188);
 }
+FRT_HD double frt_sigmapi_He2(double nu) {  // rates.cc:81-83, the c2ray_off mode's update_gamma
+  return 1.369e-14 * frt_F_of_y(nu, FRT_ETH_HE2, 5e+4, 1.720e0, 0, 0, 0, 3.288e+1, 2.963);
+}
 
 // Recombination coefficients in cm^3/s (rates.cc:128-171).
 FRT_HD double frt_alphaA_H2(double T) {
@@ -68,6 +71,15 @@ FRT_HD double frt_alphaA_He2(double T) {
 FRT_HD double frt_alphaB_He2(double T) {
   double x = frt_lambda(T, FRT_ETH_HE1);
   return 1.26e-14 * dm_pow(x, 0.750);
+}
+// Hui & Gnedin 1997 He III recombination, rates.cc:154-166 (the c2ray_off mode).
+FRT_HD double frt_alphaA_He3(double T) {
+  double x = frt_lambda(T, FRT_ETH_HE2);
+  return 2. * 1.269e-13 * dm_pow(x, 1.503) / dm_pow((1.0 + dm_pow((x / 0.522), 0.470)), 1.923);
+}
+FRT_HD double frt_alphaB_He3(double T) {
+  double x = frt_lambda(T, FRT_ETH_HE2);
+  return 2 * 2.753e-14 * dm_pow(x, 1.500) / dm_pow((1.0 + dm_pow((x / 2.740), 0.407)), 2.2420);
 }
 FRT_HD double frt_Dalpha_He2(double T) {
   double x = frt_lambda(T, FRT_ETH_HE2);

@@ -51,7 +51,7 @@ def reference_defaults(n):
     return dict(
         n=n, nfreq=1, path_length=4, itercount=5, precision=0, hpx_lvl=0, l_max=12,
         ray_per_cell_max=22, rec_case_b=1, temp_ev=1, recomb_cool=1, free_free=1, compton=1,
-        frac_limit=1, subgrid=0, reserved1=0, ray_capacity=0, seed=1,
+        frac_limit=1, subgrid=0, c2ray_off=0, ray_capacity=0, seed=1,
         box_ckpc=40 * 1e3 / 0.68, a_rt=1.0 / (1.0 + 12.0038), zz=12.0038, dt=0.0,
         cl_factor=0.3, nl_const=0.5, safety_factor=2.0, min_dist=1e-4, temp_min=1e2,
         hh=0.68, omega_m=0.305147, omega_l=0.6948,
@@ -201,7 +201,9 @@ def init_gas(rho, temp0, clump):
     f_He3 = np.full(n3, np.float32(1.0 - f64(FHE1_0) - f64(FHE2_0)), dtype=np.float32)
     return dict(nH=nH, nHe=nHe, nH1=nH1, nHe1=nHe1, nH2=nH2, nHe2=nHe2, ne=ne, n_tot=n_tot,
                 f_H1=f_H1, f_H2=f_H2, f_He1=f_He1, f_He2=f_He2, f_He3=f_He3, temp=temp, clump=clump,
-                nH1_prev=nH1.copy(), avg_nh1=nH1.copy(), avg_nhe1=nHe1.copy(), rho=rho.copy())
+                nH1_prev=nH1.copy(), avg_nh1=nH1.copy(), avg_nhe1=nHe1.copy(), rho=rho.copy(),
+                # the backward-difference state of the c2ray_off mode (init_funcs.cc:77,92-95)
+                nH2_prev=nH2.copy(), nHe1_prev=nHe1.copy(), nHe2_prev=nHe2.copy())
 
 
 # ---- workloads ----------------------------------------------------------------------------------
@@ -229,8 +231,9 @@ def path_length_for(cfg):
     return int(math.ceil(math.sqrt(3.0) * cells)) + 3
 
 
-def make_workload(name, n=None, n_src=None, steps=None, precision=0, seed_offset=0):
-    """Build the inputs of workload `name`; n / n_src / steps override its size."""
+def make_workload(name, n=None, n_src=None, steps=None, precision=0, seed_offset=0, c2ray_off=False):
+    """Build the inputs of workload `name`; n / n_src / steps override its size.
+    c2ray_off runs it in the reference's c2ray_iter = FALSE mode (FP64 form)."""
     w = dict(WORKLOADS[name])
     if n is not None:
         w["n"] = int(n)
@@ -247,6 +250,8 @@ def make_workload(name, n=None, n_src=None, steps=None, precision=0, seed_offset
                zz=w["z"], dt=w["dt_myr"] * 1e6 * YR_TO_S, hh=h, omega_m=COSMO["omega_m"],
                omega_l=1.0 - COSMO["omega_m"], seed=1000 + seed)
     cfg["path_length"] = path_length_for(cfg)
+    if c2ray_off:
+        cfg["c2ray_off"] = 1
     # path entries are indexed in int32: capacity x path_length stays below 2^31
     # (at 256^3 the reference's 30 n^3 slots would not)
     limit = (2 ** 31 - 1) // cfg["path_length"] - 1

@@ -68,7 +68,11 @@ typedef struct frt_config {
   int32_t subgrid;          /* 1: the sub-grid I-front model in its production setting,
                                subgrid = ifront = equilibrium = TRUE, relaxed = FALSE
                                (user_inputs.h:11-15); FP64 form; needs frt_set_subgrid */
-  int32_t reserved1;
+  int32_t c2ray_off;        /* 1: c2ray_iter = FALSE (user_inputs.h:10): photons deposited
+                               during the march into u_nu (ray_tracing.cc:1329-1355,
+                               1443-1470), then update_gamma and update_chem's backward-
+                               difference solve (gas_funcs.cc:74-99,243-351) after the
+                               split; FP64 form, subgrid 0, collisional ionization off */
   int64_t ray_capacity;     /* rays held per buffer; 0 selects 30 n^3 (global_variables.cc:40-43) */
   uint64_t seed;            /* seed of the rotation-index stream that replaces rand() */
   double box_ckpc;          /* comoving box side in kpc, Lx0 (user_inputs.h:92) */
@@ -154,6 +158,15 @@ typedef enum {
   FRT_F_GADGET_HEAT,   /* gadget_heat: sum of heat_rate dt / rho since the last reset (then the average) */
   FRT_F_DDELTA_DT,     /* ddelta_dt: rate of change of rho/avg_rho between hydro steps (gas_funcs.cc:733), 1/s */
   FRT_F_RHO_PREV,      /* rho_prev: rho after the last hydro-step reload (gas_funcs.cc:737) */
+  /* state of the c2ray_off mode's backward-difference solve (gas_funcs.cc:243-351): */
+  FRT_F_NHE3,          /* nHe3 */
+  FRT_F_NH2_PREV,      /* nH2_prev */
+  FRT_F_NHE1_PREV,     /* nHe1_prev */
+  FRT_F_NHE2_PREV,     /* nHe2_prev */
+  FRT_F_NHE3_PREV,     /* nHe3_prev */
+  FRT_F_RECOMB_HE3,    /* recomb_He3 */
+  FRT_F_U_NU,          /* u_nu[cell][nfreq], the march's deposit in the c2ray_off mode
+                          (ray_tracing.cc:1350,1465); 0 in the C2Ray mode */
   FRT_F_COUNT
 } frt_field;
 

@@ -17,6 +17,9 @@
 //   rates.cc        the rate fits used on the path (restated below)
 //   gas_funcs.cc    update_thermal 362-419, get_thermal_tstep 353-358,
 //                   update_heat_cool 134-242, set_Treion_default 104-129
+//   gas_funcs.cc    (c2ray_iter = FALSE) update_gamma 74-99, solve_ion 243-296,
+//                   update_chem 299-351; ray_tracing.cc 1329-1355,1443-1470,
+//                   1507-1509 (the march's deposit into u_nu)
 //   cosmo_funcs.cc  H(a) 13-15;  io_funcs.cc sum_nphot 10-16;
 //   general_funcs.cc mind/maxd/absd/mod 53-82,189-197
 //   main.cc         step order 117-198
@@ -40,6 +43,12 @@
 //    ray_tracing.cc:770-771 is 0/0; with frac_limit its limit is used.
 //  * Uniform clight and nc (set_clight) are scalars; static cosmology (no
 //    update_cosmo_expansion); subgrid = ifront = FALSE, so fion = 1.
+//  * c2ray_iter = FALSE: the march's per-cell u_nu sums (under gam_lock in
+//    the reference) are taken in list order; in solve_ion's first helium
+//    branch (He III the most abundant ion, gas_funcs.cc:264-272) the term
+//    (nHe[i] - nHe1[i]) is a difference of two row pointers of different
+//    arrays, which is undefined; the cell values nHe - nHe1 are used (the
+//    branch needs n_HeIII >= n_HeI and >= n_HeII at the start of the step).
 //
 // Parity status of each function is recorded in tests/ (see the oracle pin
 // tests) and in the design notes of the repository.
@@ -131,6 +140,15 @@ double alphaB_H2(double T) {
   double x = lambda_T(T, 13.60);
   return 2.753e-14 * dm_pow(x, 1.500) / dm_pow((1.0 + dm_pow((x / 2.740), 0.407)), 2.2420);
 }
+double sigmapi_He2(double nu) { return 1.369e-14 * F_of_y(nu, 54.42, 5e4, 1.720, 0, 0, 0, 32.88, 2.963); }
+double alphaA_He3(double T) {
+  double x = lambda_T(T, 54.42);
+  return 2. * 1.269e-13 * dm_pow(x, 1.503) / dm_pow((1.0 + dm_pow((x / 0.522), 0.470)), 1.923);
+}
+double alphaB_He3(double T) {
+  double x = lambda_T(T, 54.42);
+  return 2 * 2.753e-14 * dm_pow(x, 1.500) / dm_pow((1.0 + dm_pow((x / 2.740), 0.407)), 2.2420);
+}
 double alphaA_He2(double T) { return 3.0e-14 * dm_pow(lambda_T(T, 24.59), 0.654); }
 double alphaB_He2(double T) { return 1.26e-14 * dm_pow(lambda_T(T, 24.59), 0.750); }
 double Dalpha_He2(double T) {
@@ -180,7 +198,7 @@ struct State {
       clump, nH1_prev, avg_nh1, avg_nhe1, recomb_H2, recomb_He2, gamma_H1_tot, gamma_He1_tot,
       recrate_H2, recrate_He2, treion, treion_flag, heat_rate, cool_rate, IF_speed, Inc_Flux,
       gamma_H1_nu, gamma_He1_nu, ngam_nu, rho, f_He3, mfp, mfp_912, gamma_prev, gadget_gamma, gadget_heat,
-      ddelta_dt, rho_prev;
+      ddelta_dt, rho_prev, nHe3, nH2_prev, nHe1_prev, nHe2_prev, nHe3_prev, recomb_He3, u_nu;
   float gadget_step_time = 0.f;
   float avg_rho = 0.f;  // the driver's avg_rho (global_variables.cc:84), set by update_hydro_step
   uint64_t dither_seed = 0;  // one-ulp opacity dither after every pass (frt_rng.h), 0 = off
@@ -254,6 +272,13 @@ static std::vector<float>* field_vec(State& s, int f) {
     case FRT_F_GADGET_HEAT: return &s.gadget_heat;
     case FRT_F_DDELTA_DT: return &s.ddelta_dt;
     case FRT_F_RHO_PREV: return &s.rho_prev;
+    case FRT_F_NHE3: return &s.nHe3;
+    case FRT_F_NH2_PREV: return &s.nH2_prev;
+    case FRT_F_NHE1_PREV: return &s.nHe1_prev;
+    case FRT_F_NHE2_PREV: return &s.nHe2_prev;
+    case FRT_F_NHE3_PREV: return &s.nHe3_prev;
+    case FRT_F_RECOMB_HE3: return &s.recomb_He3;
+    case FRT_F_U_NU: return &s.u_nu;
     default: return nullptr;
   }
 }
@@ -466,12 +491,40 @@ static void advance_ray(State& s, Ray& ray) {
     ray.x = x0;
     ray.y = y0;
     ray.z = z0;
+    if (s.cfg.c2ray_off) {
+      // c2ray_iter = FALSE: every segment deposits into u_nu as the march
+      // crosses it and attenuates the ray by the HI opacity alone
+      // (ray_tracing.cc:1329-1355 inside the crossing loop, 1443-1470 for the
+      // last segment: the same expressions, subgrid = FALSE); then the first
+      // path index is the ray's new cell (:1507-1509)
+      const double vol = ((double)dxf) * ((double)dxf) * ((double)dxf);
+      const int F = s.Nfreq;
+      for (int q = 0; q < ray.ray_end; q++) {
+        const int c = ray.path_ind[q];
+        const float pd = ray.path_dist[q];
+        double nphot[MAXF];
+        for (int nu = 0; nu < F; nu++) nphot[nu] = (double)ray.nphot[nu] * phot_norm;
+        for (int nu = 0; nu < F; nu++) {
+          const double dtau = (double)s.nH1[c] * sigmapi_H1(s.nu_phot[nu]) * pd;
+          const double nphot_add = nphot[nu] * (1. - dm_exp(-dtau));
+          const double gamma = nphot_add / s.dt / vol / s.nH1[c];
+          const float add = (float)(s.spectrum[nu] * gamma * h * s.nu_phot[nu] / s.clight /
+                                    sigmapi_H1(s.nu_phot[nu]));
+          float& u = s.u_nu[(size_t)c * F + nu];
+#pragma omp atomic
+          u += add;
+          ray.nphot[nu] *= (float)dm_exp(-dtau);
+        }
+      }
+      ray.path_ind[0] = ray.path_ind[ray.ray_end - 1];
+    }
   }
 }
 
 // ---- update_rays ----------------------------------------------------------------------------
 static void update_rays(State& s) {
   for (size_t src = 0; src < s.src_cell.size(); src++) s.ray_dt[src] += s.dt;
+  if (s.cfg.c2ray_off) std::fill(s.u_nu.begin(), s.u_nu.end(), 0.f);  // rt_funcs.cc:174-189
   s.st.rays_start = (int64_t)s.ray.size();
   int64_t moved = 0, segs = 0;
   const int64_t nray = (int64_t)s.ray.size();
@@ -1483,6 +1536,107 @@ static void update_thermal(State& s) {
   s.st.thermal_substeps = substeps;
 }
 
+// ---- c2ray_iter = FALSE: update_gamma, solve_ion, update_chem ---------------------------------
+// update_gamma (gas_funcs.cc:74-99), collisional ionization off: the rates
+// from the march's u_nu; gamma_He2_tot is kept per cell for solve_ion.
+static void update_gamma(State& s, std::vector<float>& gamma_He2_tot) {
+  const int F = s.Nfreq;
+  for (int c = 0; c < s.N3; c++) {
+    s.gamma_H1_tot[c] = 0.;
+    s.gamma_He1_tot[c] = 0.;
+    gamma_He2_tot[c] = 0.;
+    for (int nu = 0; nu < F; nu++) {
+      const float u = s.u_nu[(size_t)c * F + nu];
+      s.gamma_H1_tot[c] += sigmapi_H1(s.nu_phot[nu]) * s.clight * u / h / s.nu_phot[nu];
+      s.gamma_He1_tot[c] += sigmapi_He1(s.nu_phot[nu]) * s.clight * u / h / s.nu_phot[nu];
+      gamma_He2_tot[c] += sigmapi_He2(s.nu_phot[nu]) * s.clight * u / h / s.nu_phot[nu];
+    }
+  }
+}
+
+// solve_ion (gas_funcs.cc:243-296): one backward-difference step of the H and
+// He balance, each equation solved for the less abundant species.
+static void solve_ion(State& s, int c, float g2tot) {
+  const float dt = s.dt;
+  const float C = s.clump[c];
+  const float gH1 = s.gamma_H1_tot[c], gHe1 = s.gamma_He1_tot[c], gHe2 = g2tot;
+  const float rH2 = s.recomb_H2[c], rHe2 = s.recomb_He2[c], rHe3 = s.recomb_He3[c];
+  if (s.nH1_prev[c] >= s.nH2_prev[c]) {
+    s.nH2[c] = (s.nH2_prev[c] + gH1 * s.nH[c] * dt) / (1. + gH1 * dt + C * rH2 * s.ne[c] * dt);
+    s.nH1[c] = s.nH[c] - s.nH2[c];
+  } else {
+    s.nH1[c] = (s.nH1_prev[c] + C * rH2 * s.ne[c] * s.nH[c] * dt) / (1. + gH1 * dt + C * rH2 * s.ne[c] * dt);
+    s.nH2[c] = s.nH[c] - s.nH1[c];
+  }
+  if ((s.nHe1_prev[c] <= s.nHe3_prev[c]) && (s.nHe2_prev[c] <= s.nHe3_prev[c])) {
+    s.nHe1[c] = (s.nHe1_prev[c] + C * rHe2 * s.ne[c] * (s.nHe[c] - s.nHe3[c]) * dt) /
+                (1. + gHe1 * dt + C * rHe2 * s.ne[c] * dt);
+    // gas_funcs.cc:268: (nHe[i] - nHe1[i]) there, the cell values here
+    s.nHe2[c] = (s.nHe2_prev[c] + gHe1 * (s.nHe[c] - s.nHe3[c]) * dt + C * rHe3 * s.ne[c] * (s.nHe[c] - s.nHe1[c]) * dt) /
+                (1. + (gHe1 + gHe2) * dt + C * (rHe2 + rHe3) * s.ne[c] * dt);
+    s.nHe3[c] = s.nHe[c] - s.nHe1[c] - s.nHe2[c];
+  } else if ((s.nHe1_prev[c] <= s.nHe2_prev[c]) && (s.nHe3_prev[c] <= s.nHe2_prev[c])) {
+    s.nHe1[c] = (s.nHe1_prev[c] + C * rHe2 * s.ne[c] * (s.nHe[c] - s.nHe3[c]) * dt) /
+                (1. + gHe1 * dt + C * rHe2 * s.ne[c] * dt);
+    s.nHe3[c] = (s.nHe3_prev[c] + gHe2 * (s.nHe[c] - s.nHe1[c]) * dt) / (1. + gHe2 * dt + C * rHe3 * s.ne[c] * dt);
+    s.nHe2[c] = s.nHe[c] - s.nHe1[c] - s.nHe3[c];
+  } else {
+    s.nHe2[c] = (s.nHe2_prev[c] + gHe1 * (s.nHe[c] - s.nHe3[c]) * dt + C * rHe3 * s.ne[c] * (s.nHe[c] - s.nHe1[c]) * dt) /
+                (1. + (gHe1 + gHe2) * dt + C * (rHe2 + rHe3) * s.ne[c] * dt);
+    s.nHe3[c] = (s.nHe3_prev[c] + gHe2 * (s.nHe[c] - s.nHe1[c]) * dt) / (1. + gHe2 * dt + C * rHe3 * s.ne[c] * dt);
+    s.nHe1[c] = s.nHe[c] - s.nHe2[c] - s.nHe3[c];
+  }
+  s.ne[c] = s.nH2[c] + s.nHe2[c] + 2. * s.nHe3[c];
+}
+
+// update_chem (gas_funcs.cc:299-351).
+static void update_chem(State& s, const std::vector<float>& gamma_He2_tot) {
+  for (int c = 0; c < s.N3; c++) {
+    s.dne_dt[c] = -s.ne[c];
+    const double T = s.temp[c];
+    if (!s.cfg.rec_case_b) {
+      s.recomb_H2[c] = (float)alphaA_H2(T);
+      s.recomb_He2[c] = (float)alphaA_He2(T);
+      s.recomb_He3[c] = (float)alphaA_He3(T);
+    } else {
+      s.recomb_H2[c] = (float)alphaB_H2(T);
+      s.recomb_He2[c] = (float)alphaB_He2(T);
+      s.recomb_He3[c] = (float)alphaB_He3(T);
+    }
+    if ((s.temp[c] >= 3e4) && (s.temp[c] <= 1e6)) s.recomb_He2[c] += (float)Dalpha_He2(T);
+    for (int m = 0; m < 5; m++) solve_ion(s, c, gamma_He2_tot[c]);
+    s.n_tot[c] = s.nH[c] + s.nHe[c] + s.ne[c];
+    s.dne_dt[c] += s.ne[c];
+    s.dne_dt[c] /= s.dt;
+    s.nH1_prev[c] = s.nH1[c];
+    s.nH2_prev[c] = s.nH2[c];
+    s.nHe1_prev[c] = s.nHe1[c];
+    s.nHe2_prev[c] = s.nHe2[c];
+    s.nHe3_prev[c] = s.nHe3[c];
+    s.f_H1[c] = s.nH1[c] / s.nH[c];
+    s.f_H2[c] = s.nH2[c] / s.nH[c];
+    s.f_He1[c] = s.nHe1[c] / s.nHe[c];
+    s.f_He2[c] = s.nHe2[c] / s.nHe[c];
+    s.f_He3[c] = s.nHe3[c] / s.nHe[c];
+  }
+}
+
+// set_to_true (rt_funcs.cc:146-156), the c2ray_off mode's energy phase.
+static void set_to_true(State& s) {
+  for (auto& r : s.ray)
+    if (r.tag && !r.active) r.active = true;
+}
+
+// main.cc:186-193 in the c2ray_off mode: update_gamma, update_chem, update_thermal.
+static void gamma_chem_thermal(State& s) {
+  if (s.cfg.c2ray_off) {
+    std::vector<float> g2((size_t)s.N3, 0.f);
+    update_gamma(s, g2);
+    update_chem(s, g2);
+  }
+  update_thermal(s);
+}
+
 // ---- casting ------------------------------------------------------------------------------------------
 static void init_rays(State& s, int src) {
   unsigned short itheta = (unsigned short)frt_itheta_hash(s.cfg.seed, s.cast_calls, (uint64_t)src);
@@ -1572,10 +1726,13 @@ static void step(State& s) {
   s.t += s.dt;                       // main.cc:123
   drop_dead(s);
   update_rays(s);                    // main.cc:131
-  find_avg_unu(s);                   // main.cc:140
+  if (s.cfg.c2ray_off)
+    set_to_true(s);                  // main.cc:142-144
+  else
+    find_avg_unu(s);                 // main.cc:140
   merge_phase(s);                    // main.cc:153-165
   split_rays(s);                     // main.cc:173
-  update_thermal(s);                 // main.cc:193
+  gamma_chem_thermal(s);             // main.cc:186-193
   new_rays(s);                       // main.cc:198
   s.step += 1;
   s.st.step = s.step;
@@ -1631,7 +1788,7 @@ void* orc_create(const frt_config* cfg) {
   const size_t n3 = (size_t)s->N3, nf = n3 * s->Nfreq;
   for (int f = 0; f < FRT_F_COUNT; f++) {
     std::vector<float>* v = field_vec(*s, f);
-    const bool per_bin = f == FRT_F_GAMMA_H1_NU || f == FRT_F_GAMMA_HE1_NU || f == FRT_F_NGAM_NU;
+    const bool per_bin = f == FRT_F_GAMMA_H1_NU || f == FRT_F_GAMMA_HE1_NU || f == FRT_F_NGAM_NU || f == FRT_F_U_NU;
     v->assign(per_bin ? nf : n3, 0.f);
   }
   s->clump.assign(n3, 1.0f);
@@ -1801,10 +1958,16 @@ int orc_update_rays(void* h) {
   update_rays(*s);
   return s->path_overflow ? FRT_E_PATH_OVERFLOW : 0;
 }
-void orc_find_avg_unu(void* h) { find_avg_unu(*(State*)h); }
+void orc_find_avg_unu(void* h) {
+  State* s = (State*)h;
+  if (s->cfg.c2ray_off)
+    set_to_true(*s);
+  else
+    find_avg_unu(*s);
+}
 void orc_merge_rays(void* h) { merge_phase(*(State*)h); }
 void orc_split_rays(void* h) { split_rays(*(State*)h); }
-void orc_update_thermal(void* h) { update_thermal(*(State*)h); }
+void orc_update_thermal(void* h) { gamma_chem_thermal(*(State*)h); }
 void orc_get_stats(void* h, frt_step_stats* st) {
   // scalars reflect the state now, also after a single phase call
   State* s = (State*)h;
@@ -2166,6 +2329,9 @@ double orc_rate(int which, double x) {
     case 9: return rcrA_H2(x);
     case 10: return free_free_rate(x);
     case 11: return compton_rate(x);
+    case 12: return sigmapi_He2(x);
+    case 13: return alphaB_He3(x);
+    case 14: return alphaA_He3(x);
     default: return NAN;
   }
 }

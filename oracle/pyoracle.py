@@ -84,7 +84,7 @@ _PROTOS = {
 
 RATES = {"sigmapi_H1": 0, "sigmapi_He1": 1, "alphaB_H2": 2, "alphaB_He2": 3, "alphaA_H2": 4,
          "alphaA_He2": 5, "Dalpha_He2": 6, "rcrB_H2": 7, "rcrB_He2": 8, "rcrA_H2": 9,
-         "free_free_rate": 10, "compton_rate": 11}
+         "free_free_rate": 10, "compton_rate": 11, "sigmapi_He2": 12, "alphaB_He3": 13, "alphaA_He3": 14}
 
 
 _omp_lib = None

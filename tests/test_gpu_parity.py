@@ -24,7 +24,7 @@ def test_create_and_defaults():
 
     _print_case("create n=8")
     cfg = default_config(8)
-    assert cfg == {**inputs.reference_defaults(8), **{k: cfg[k] for k in ("reserved1",)}}
+    assert cfg == inputs.reference_defaults(8)
     with FlexRT(cfg) as g:
         assert g.ray_count() == 0
 
