@@ -442,11 +442,12 @@ int launch_hydro_reload(frt_ctx* ctx, double rho_scale, float dt_cosmic);
 // device-count scan and stable radix sort (frt_scan_sort.hip): the count is
 // *dcount + add (dcount may be NULL), bound a host-side upper bound that sizes
 // the grid; dtotal (may be NULL) receives the scan's total; the sort leaves
-// its result in key[*which], val[*which]
+// its result in key[*which], val[*which]; with inv_last, the last pass writes
+// inv_last[value] = sorted position instead of val[*which]
 int dev_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* dcount, int64_t add, int64_t bound,
              int64_t* dtotal, const char* name);
 int dev_sort_pairs(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t add,
-                   int64_t bound, int bits, int* which, const char* name);
+                   int64_t bound, int bits, int* which, const char* name, int32_t* inv_last = nullptr);
 int alloc_scan_sort(frt_ctx* ctx);
 void free_scan_sort(frt_ctx* ctx);
 // host mirror of the step scalars <-> device

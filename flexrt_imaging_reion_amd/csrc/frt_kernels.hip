@@ -593,20 +593,17 @@ __global__ void __launch_bounds__(BLK) k_fill_u8_dev(uint8_t* a, const int64_t* 
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) a[i] = v;
 }
 
-// Per-cell slot ranges of the sorted segments, and the slot of every path
-// entry (the inverse of the sort permutation, laid out like path_cell).
+// Per-cell slot ranges of the sorted segments (the sort's last pass wrote
+// the slot of every path entry, path_slot).
 __global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict__ key,
-                                                     const int32_t* __restrict__ val,
                                                      const StepDev* __restrict__ sd,
                                                      int32_t* __restrict__ start,
-                                                     int32_t* __restrict__ end,
-                                                     int32_t* __restrict__ inv) {
+                                                     int32_t* __restrict__ end) {
   const int64_t nseg = sd->nseg;
   for (int64_t s = xcd_block() * BLK + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * BLK) {
     const uint32_t k = key[s];
     if (s == 0 || key[s - 1] != k) start[k] = (int32_t)s;
     if (s == nseg - 1 || key[s + 1] != k) end[k] = (int32_t)(s + 1);
-    inv[val[s]] = (int32_t)s;
   }
 }
 
@@ -2347,13 +2344,14 @@ int launch_segments(frt_ctx* ctx) {
   FRT_LAUNCH(ctx, "seg_keys", ray_grid(ctx, nb), k_seg_keys, sd, ctx->g.cap, ctx->seg_cnt, ctx->seg_off,
              ctx->path_cell, ctx->seg_key[0], ctx->seg_val[0]);
   int which = 0;
+  // the last pass writes the cell-sorted slot of every path entry
   rc = dev_sort_pairs(ctx, ctx->seg_key, ctx->seg_val, &sd->nseg, 0, bound_segs(ctx), ctx->key_bits, &which,
-                      "sort_segments");
+                      "sort_segments", ctx->path_slot);
   if (rc) return rc;
   // light per-segment work: a capped grid strides over the segments (a
   // grid sized for the bound would mostly run empty blocks)
   FRT_LAUNCH(ctx, "cell_ranges", std::min(ray_grid(ctx, bound_segs(ctx)), 4 * MAX_GRID), k_cell_ranges, ctx->seg_key[which],
-             ctx->seg_val[which], sd, ctx->cell_start, ctx->cell_end, ctx->path_slot);
+             sd, ctx->cell_start, ctx->cell_end);
   return FRT_OK;
 }
 

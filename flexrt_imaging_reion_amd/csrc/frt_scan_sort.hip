@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(SBLK) k_rs_hist(const uint32_t* __restrict__ k
 __global__ void __launch_bounds__(SBLK) k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
                                                      uint32_t* __restrict__ kout, int32_t* __restrict__ vout,
                                                      const int64_t* dcount, int64_t add, int shift, int nbits,
-                                                     const int32_t* __restrict__ hoff) {
+                                                     const int32_t* __restrict__ hoff, int32_t* __restrict__ inv) {
   constexpr int W = SBLK / 64;
   constexpr int RMAX = 1 << RADIX_MAX_BITS;
   constexpr int QUARTER = SORT_TILE / W;  // keys per wavefront per tile
@@ -307,7 +307,10 @@ __global__ void __launch_bounds__(SBLK) k_rs_scatter(const uint32_t* __restrict_
       const int d = (int)((kk >> shift) & mask);
       const int pos = goff[d] + (p - tstart[d]);
       kout[pos] = kk;
-      vout[pos] = sv[p];
+      if (inv)
+        inv[sv[p]] = pos;  // the inverse permutation instead of the sorted values
+      else
+        vout[pos] = sv[p];
     }
     lds_barrier();
     if (t < R) goff[t] += tcnt[t];
@@ -345,7 +348,7 @@ int dev_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* dcoun
 }
 
 int dev_sort_pairs(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t add, int64_t bound,
-                   int bits, int* which, const char* name) {
+                   int bits, int* which, const char* name, int32_t* inv_last) {
   bits = std::max(bits, 1);
   struct {
     int passes;
@@ -369,7 +372,8 @@ int dev_sort_pairs(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_
     int rc = dev_scan(ctx, ctx->ss_hist, ctx->ss_hoff, nullptr, R * G, R * G, nullptr, name);
     if (rc) return rc;
     SS_LAUNCH(ctx, name, G, k_rs_scatter, (const uint32_t*)key[src], (const int32_t*)val[src], key[1 - src],
-              val[1 - src], dcount, add, plan.shift[p], plan.nbits[p], (const int32_t*)ctx->ss_hoff);
+              val[1 - src], dcount, add, plan.shift[p], plan.nbits[p], (const int32_t*)ctx->ss_hoff,
+              p == plan.passes - 1 ? inv_last : (int32_t*)nullptr);
     src = 1 - src;
   }
   *which = src;
