@@ -1804,31 +1804,16 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, const StepDev* 
     float Y = rbase.y * nb;
     float Z = rbase.z * nb;
     float T = rbase.time * nb;
-    // members in list order; the next member's data is loaded while the
-    // current one is summed (the sums stay in the same order)
-    int mn = 0;
-    float nmn[8], rn = 0.f, drn = 0.f;
-    float4 pmn = make_float4(0.f, 0.f, 0.f, 0.f);
-    auto fetch = [&](int64_t j) {
-      mn = val[j];
-      phot_load(rb, mn, F, cap, nmn);
-      pmn = rb.pos[mn];
-      rn = rb.dir[mn].w;
-      drn = __int_as_float(rb.meta[mn].x);
-    };
-    if (s + 1 < e) fetch(s + 1);
     for (int64_t j = s + 1; j < e; ++j) {
-      const int m = mn;
+      const int m = val[j];
       float nmv[8];
-      FRT_FOR_BINS(nu) nmv[nu] = nmn[nu];
-      const float4 pm = pmn;
-      const float rm = rn, drm = drn;
-      if (j + 1 < e) fetch(j + 1);
+      phot_load(rb, m, F, cap, nmv);
       const float nm = phot_sum(nmv, F);
       FRT_FOR_BINS(nu) NP[nu] += nmv[nu];
-      const float rrm = rm / g.dx0;
+      const float4 pm = rb.pos[m];
+      const float rrm = rb.dir[m].w / g.dx0;
       R = (float)((double)R + (double)rrm * (double)rrm * (double)nm);
-      DR += drm * nm;
+      DR += __int_as_float(rb.meta[m].x) * nm;
       X += pm.x * nm;
       Y += pm.y * nm;
       Z += pm.z * nm;
