@@ -54,6 +54,8 @@ _PROTOS = {
     "frt_selftest_scan_sort": (C.c_int, [_CTX, C.c_int64, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int64)]),
+    "frt_selftest_sort_cells": (C.c_int, [_CTX, C.c_int64, C.POINTER(C.c_uint32), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "frt_ray_count": (C.c_int, [_CTX, C.POINTER(C.c_int64)]),
     "frt_download_rays": (C.c_int, [_CTX, C.POINTER(FrtRayView)]),
     "frt_upload_rays": (C.c_int, [_CTX, C.POINTER(FrtRayView)]),

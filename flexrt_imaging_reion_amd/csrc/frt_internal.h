@@ -310,6 +310,7 @@ struct frt_ctx {
   int64_t* ss_part;
   int32_t* ss_hist;
   int32_t* ss_hoff;
+  int32_t* ss_grp;  // [2][ceil(n3 / 64)]: segment run of each group of 64 cells
   // step graph: one executable graph per ray-buffer parity, captured on first use
   int graph_on;         // frt_set_step_graph
   bool capturing;       // launches size their grids for capture_rays rays
@@ -441,13 +442,18 @@ int launch_hydro_reload(frt_ctx* ctx, double rho_scale, float dt_cosmic);
 
 // device-count scan and stable radix sort (frt_scan_sort.hip): the count is
 // *dcount + add (dcount may be NULL), bound a host-side upper bound that sizes
-// the grid; dtotal (may be NULL) receives the scan's total; the sort leaves
-// its result in key[*which], val[*which]; with inv_last, the last pass writes
-// inv_last[value] = sorted position instead of val[*which]
+// the grid; dtotal (may be NULL) receives the scan's total; the sort orders
+// by key bits [lo, bits) and leaves its result in key[*which], val[*which].
+// dev_sort_cells: stable sort of *dcount (cell key, value) pairs by cell
+// (keys < ncells <= 2^key_bits): inv[value] = sorted position and, for every
+// cell, the range [cstart, cend) of its positions (empty: 0, 0); key and val
+// are overwritten
 int dev_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* dcount, int64_t add, int64_t bound,
              int64_t* dtotal, const char* name);
 int dev_sort_pairs(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t add,
-                   int64_t bound, int bits, int* which, const char* name, int32_t* inv_last = nullptr);
+                   int64_t bound, int bits, int* which, const char* name, int lo = 0);
+int dev_sort_cells(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t bound,
+                   int key_bits, int64_t ncells, int32_t* inv, int32_t* cstart, int32_t* cend, const char* name);
 int alloc_scan_sort(frt_ctx* ctx);
 void free_scan_sort(frt_ctx* ctx);
 // host mirror of the step scalars <-> device

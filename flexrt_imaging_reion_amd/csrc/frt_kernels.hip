@@ -583,29 +583,12 @@ __global__ void __launch_bounds__(BLK) k_moved_scatter(const int32_t* __restrict
     if (flag[i]) moved[off[i]] = (int32_t)i;
 }
 
-__global__ void __launch_bounds__(BLK) k_fill_i32(int32_t* a, int64_t n, int32_t v) {
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) a[i] = v;
-}
-
 // Fill of the first *dn + add bytes (device count) with v.
 __global__ void __launch_bounds__(BLK) k_fill_u8_dev(uint8_t* a, const int64_t* __restrict__ dn, int64_t add, uint8_t v) {
   const int64_t n = *dn + add;
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) a[i] = v;
 }
 
-// Per-cell slot ranges of the sorted segments (the sort's last pass wrote
-// the slot of every path entry, path_slot).
-__global__ void __launch_bounds__(BLK) k_cell_ranges(const uint32_t* __restrict__ key,
-                                                     const StepDev* __restrict__ sd,
-                                                     int32_t* __restrict__ start,
-                                                     int32_t* __restrict__ end) {
-  const int64_t nseg = sd->nseg;
-  for (int64_t s = xcd_block() * BLK + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * BLK) {
-    const uint32_t k = key[s];
-    if (s == 0 || key[s - 1] != k) start[k] = (int32_t)s;
-    if (s == nseg - 1 || key[s + 1] != k) end[k] = (int32_t)(s + 1);
-  }
-}
 
 
 // ---------------------------------------------------------------------------
@@ -2324,20 +2307,11 @@ int launch_segments(frt_ctx* ctx) {
   if (rc) return rc;
   FRT_LAUNCH(ctx, "moved_list", ray_grid(ctx, nb), k_moved_scatter, ctx->cand_flag, ctx->cand_off, sd,
              ctx->moved_idx);
-  FRT_LAUNCH(ctx, "fill", grid_for(ctx->g.n3), k_fill_i32, ctx->cell_start, (int64_t)ctx->g.n3, 0);
-  FRT_LAUNCH(ctx, "fill", grid_for(ctx->g.n3), k_fill_i32, ctx->cell_end, (int64_t)ctx->g.n3, 0);
   FRT_LAUNCH(ctx, "seg_keys", ray_grid(ctx, nb), k_seg_keys, sd, ctx->g.cap, ctx->seg_cnt, ctx->seg_off,
              ctx->path_cell, ctx->seg_key[0], ctx->seg_val[0]);
-  int which = 0;
-  // the last pass writes the cell-sorted slot of every path entry
-  rc = dev_sort_pairs(ctx, ctx->seg_key, ctx->seg_val, &sd->nseg, 0, bound_segs(ctx), ctx->key_bits, &which,
-                      "sort_segments", ctx->path_slot);
-  if (rc) return rc;
-  // light per-segment work: a capped grid strides over the segments (a
-  // grid sized for the bound would mostly run empty blocks)
-  FRT_LAUNCH(ctx, "cell_ranges", std::min(ray_grid(ctx, bound_segs(ctx)), 4 * MAX_GRID), k_cell_ranges, ctx->seg_key[which],
-             sd, ctx->cell_start, ctx->cell_end);
-  return FRT_OK;
+  // the cell-sorted slot of every path entry and the cells' slot ranges
+  return dev_sort_cells(ctx, ctx->seg_key, ctx->seg_val, &sd->nseg, bound_segs(ctx), ctx->key_bits, ctx->g.n3,
+                        ctx->path_slot, ctx->cell_start, ctx->cell_end, "sort_segments");
 }
 
 __global__ void __launch_bounds__(BLK) k_tape_nph(RaySoA rb, int64_t n, int F, int64_t cap, float* __restrict__ out) {

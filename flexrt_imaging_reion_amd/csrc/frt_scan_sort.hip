@@ -17,6 +17,14 @@
 // the ranks of a stable in-tile ranking (wavefront ballots per digit bit,
 // then a prefix over the block's wavefronts), so the order of equal keys is
 // their input order and the result equals any stable sort's.
+//
+// Cell sort of the step's segments: the radix passes order the segments by
+// group of 64 cells (key >> 6; a 4x4x4 tile in the tiled cell numbering),
+// then one block per group places the group's segments by cell in LDS-ranked
+// rounds, in their order within the group, and writes each segment's final
+// slot and the cells' slot ranges.  The result equals the stable sort by
+// the whole cell key; the last pass's scattered writes stay within a
+// group's rays instead of spanning the whole list.
 #include <algorithm>
 
 #include "frt_internal.h"
@@ -218,7 +226,7 @@ __global__ void __launch_bounds__(SBLK) k_rs_hist(const uint32_t* __restrict__ k
 __global__ void __launch_bounds__(SBLK) k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
                                                      uint32_t* __restrict__ kout, int32_t* __restrict__ vout,
                                                      const int64_t* dcount, int64_t add, int shift, int nbits,
-                                                     const int32_t* __restrict__ hoff, int32_t* __restrict__ inv) {
+                                                     const int32_t* __restrict__ hoff) {
   constexpr int W = SBLK / 64;
   constexpr int RMAX = 1 << RADIX_MAX_BITS;
   constexpr int QUARTER = SORT_TILE / W;  // keys per wavefront per tile
@@ -307,13 +315,100 @@ __global__ void __launch_bounds__(SBLK) k_rs_scatter(const uint32_t* __restrict_
       const int d = (int)((kk >> shift) & mask);
       const int pos = goff[d] + (p - tstart[d]);
       kout[pos] = kk;
-      if (inv)
-        inv[sv[p]] = pos;  // the inverse permutation instead of the sorted values
-      else
-        vout[pos] = sv[p];
+      vout[pos] = sv[p];
     }
     lds_barrier();
     if (t < R) goff[t] += tcnt[t];
+  }
+}
+
+// ---- cell sort: groups of 64 cells ----------------------------------------------
+
+static constexpr int GROUP_SHIFT = 6;  // cells per group = 64 = one wavefront
+
+__global__ void __launch_bounds__(SBLK) k_zero_i32(int32_t* __restrict__ a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * SBLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * SBLK) a[i] = 0;
+}
+
+// [start, end) of every non-empty group's run in the group-sorted keys
+// (empty groups keep the zeros they were cleared to).
+__global__ void __launch_bounds__(SBLK) k_group_ranges(const uint32_t* __restrict__ key, const int64_t* dcount,
+                                                       int32_t* __restrict__ gstart, int32_t* __restrict__ gend) {
+  const int64_t n = dev_count(dcount, 0);
+  for (int64_t s = (int64_t)blockIdx.x * SBLK + threadIdx.x; s < n; s += (int64_t)gridDim.x * SBLK) {
+    const uint32_t g = key[s] >> GROUP_SHIFT;
+    if (s == 0 || (key[s - 1] >> GROUP_SHIFT) != g) gstart[g] = (int32_t)s;
+    if (s == n - 1 || (key[s + 1] >> GROUP_SHIFT) != g) gend[g] = (int32_t)(s + 1);
+  }
+}
+
+// One block per group of 64 cells: the cells' segment counts (LDS atomics),
+// their slot ranges [cstart, cend) from an exclusive scan, then every
+// segment's slot: its cell's start, plus the segments of that cell earlier
+// in the group run -- counted over the previous rounds (running per-cell
+// position), the earlier wavefronts of this round and the earlier lanes of
+// its own wavefront (ballot match on the 6 cell bits).  inv[value] = slot.
+// Blocks are remapped so each XCD places a contiguous eighth of the groups:
+// neighbouring groups hold neighbouring rays, whose slot entries share
+// cache lines.
+__global__ void __launch_bounds__(SBLK) k_group_local(const uint32_t* __restrict__ key,
+                                                      const int32_t* __restrict__ val,
+                                                      const int32_t* __restrict__ gstart,
+                                                      const int32_t* __restrict__ gend, int32_t* __restrict__ inv,
+                                                      int32_t* __restrict__ cstart, int32_t* __restrict__ cend,
+                                                      int64_t ncells) {
+  constexpr int W = SBLK / 64;
+  __shared__ int32_t next[64];   // next slot of each cell
+  __shared__ int32_t wc[W][64];  // this round: segments of each cell per wavefront
+  const unsigned G = gridDim.x, b = blockIdx.x, G8 = G & ~7u;
+  const int64_t g = b >= G8 ? b : (int64_t)(b & 7u) * (G8 >> 3) + (b >> 3);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int s0 = gstart[g], s1 = gend[g];
+  if (t < 64) next[t] = 0;
+  __syncthreads();
+  for (int s = s0 + t; s < s1; s += SBLK) atomicAdd(&next[key[s] & 63u], 1);
+  __syncthreads();
+  if (w == 0) {
+    const int c = next[lane];
+    int inc = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int u = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += u;
+    }
+    const int first = s0 + inc - c;
+    const int64_t cell = (g << GROUP_SHIFT) + lane;
+    if (cell < ncells) {  // empty cells: (0, 0)
+      cstart[cell] = c ? first : 0;
+      cend[cell] = c ? first + c : 0;
+    }
+    next[lane] = first;
+  }
+  __syncthreads();
+  for (int base = s0; base < s1; base += SBLK) {  // block-uniform trip count
+    const int s = base + t;
+    const bool valid = s < s1;
+    const int d = valid ? (int)(key[s] & 63u) : 0;
+    const int32_t v = valid ? val[s] : 0;
+    const uint64_t m = match_digit(d, GROUP_SHIFT, valid);
+    wc[w][lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && __popcll(m & lt) == 0) wc[w][d] = __popcll(m);
+    __syncthreads();
+    if (valid) {
+      int pos = next[d] + __popcll(m & lt);
+      for (int q = 0; q < w; ++q) pos += wc[q][d];
+      inv[v] = pos;
+    }
+    __syncthreads();
+    if (t < 64) {
+      int tot = 0;
+#pragma unroll
+      for (int q = 0; q < W; ++q) tot += wc[q][t];
+      next[t] += tot;
+    }
+    __syncthreads();
   }
 }
 
@@ -348,18 +443,18 @@ int dev_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* dcoun
 }
 
 int dev_sort_pairs(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t add, int64_t bound,
-                   int bits, int* which, const char* name, int32_t* inv_last) {
-  bits = std::max(bits, 1);
+                   int bits, int* which, const char* name, int lo) {
+  bits = std::max(bits - lo, 1);
   struct {
     int passes;
     int shift[4];
     int nbits[4];
   } plan{};
   plan.passes = (bits + RADIX_MAX_BITS - 1) / RADIX_MAX_BITS;
-  if (plan.passes > 4) return set_error(ctx, FRT_E_ARG, "sort keys wider than 48 bits");
+  if (plan.passes > 4) return set_error(ctx, FRT_E_ARG, "sort keys wider than 32 bits");
   for (int p = 0, sh = 0; p < plan.passes; ++p) {
     plan.nbits[p] = (bits - sh + (plan.passes - p) - 1) / (plan.passes - p);  // spread the bits evenly
-    plan.shift[p] = sh;
+    plan.shift[p] = lo + sh;
     sh += plan.nbits[p];
   }
   const int G = sort_grid(bound);
@@ -372,11 +467,28 @@ int dev_sort_pairs(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_
     int rc = dev_scan(ctx, ctx->ss_hist, ctx->ss_hoff, nullptr, R * G, R * G, nullptr, name);
     if (rc) return rc;
     SS_LAUNCH(ctx, name, G, k_rs_scatter, (const uint32_t*)key[src], (const int32_t*)val[src], key[1 - src],
-              val[1 - src], dcount, add, plan.shift[p], plan.nbits[p], (const int32_t*)ctx->ss_hoff,
-              p == plan.passes - 1 ? inv_last : (int32_t*)nullptr);
+              val[1 - src], dcount, add, plan.shift[p], plan.nbits[p], (const int32_t*)ctx->ss_hoff);
     src = 1 - src;
   }
   *which = src;
+  return FRT_OK;
+}
+
+int dev_sort_cells(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t bound,
+                   int key_bits, int64_t ncells, int32_t* inv, int32_t* cstart, int32_t* cend, const char* name) {
+  int which = 0;
+  if (key_bits > GROUP_SHIFT) {
+    int rc = dev_sort_pairs(ctx, key, val, dcount, 0, bound, key_bits, &which, name, GROUP_SHIFT);
+    if (rc) return rc;
+  }
+  const int64_t G = (ncells + 63) >> GROUP_SHIFT;
+  int32_t* gstart = ctx->ss_grp;
+  int32_t* gend = ctx->ss_grp + G;
+  SS_LAUNCH(ctx, name, (unsigned)std::min<int64_t>(ceil_div(2 * G, SBLK), 4096), k_zero_i32, ctx->ss_grp, 2 * G);
+  SS_LAUNCH(ctx, name, (unsigned)std::min<int64_t>(ceil_div(std::max<int64_t>(bound, 1), SBLK), 4 * 4096),
+            k_group_ranges, (const uint32_t*)key[which], dcount, gstart, gend);
+  SS_LAUNCH(ctx, name, (unsigned)G, k_group_local, (const uint32_t*)key[which], (const int32_t*)val[which],
+            (const int32_t*)gstart, (const int32_t*)gend, inv, cstart, cend, ncells);
   return FRT_OK;
 }
 
@@ -386,6 +498,7 @@ int alloc_scan_sort(frt_ctx* ctx) {
   hipError_t e = hipMalloc((void**)&ctx->ss_part, (FRT_SCAN_MAX_GRID + 1) * sizeof(int64_t));
   if (e == hipSuccess) e = hipMalloc((void**)&ctx->ss_hist, nh * sizeof(int32_t));
   if (e == hipSuccess) e = hipMalloc((void**)&ctx->ss_hoff, nh * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&ctx->ss_grp, 2 * (size_t)((ctx->g.n3 + 63) >> GROUP_SHIFT) * sizeof(int32_t));
   if (e != hipSuccess) return set_error(ctx, FRT_E_NOMEM, "scan/sort workspace");
   return FRT_OK;
 }
@@ -394,6 +507,8 @@ void free_scan_sort(frt_ctx* ctx) {
   hipFree(ctx->ss_part);
   hipFree(ctx->ss_hist);
   hipFree(ctx->ss_hoff);
+  hipFree(ctx->ss_grp);
+  ctx->ss_grp = nullptr;
   ctx->ss_part = nullptr;
   ctx->ss_hist = nullptr;
   ctx->ss_hoff = nullptr;
@@ -449,6 +564,53 @@ int frt_selftest_scan_sort(frt_ctx* ctx, int64_t n, int32_t bits, const uint32_t
   }
   hipStreamSynchronize(ctx->stream);
   hipFree(k[0]); hipFree(k[1]); hipFree(v[0]); hipFree(v[1]); hipFree(ci); hipFree(co); hipFree(dn);
+  return rc;
+}
+
+// Runs the step's cell sort on (cells[i], i), i < n, cells[i] < the
+// context's cell count: slot_out[i] = sorted position of item i, and every
+// cell's [cstart_out, cend_out) (arrays of the cell count; any may be NULL).
+int frt_selftest_sort_cells(frt_ctx* ctx, int64_t n, const uint32_t* cells, int32_t* slot_out, int32_t* cstart_out,
+                            int32_t* cend_out) {
+  if (!ctx || n < 0 || n >= (1ll << 31) || (n > 0 && !cells)) return FRT_E_ARG;
+  const int64_t n3 = ctx->g.n3;
+  for (int64_t i = 0; i < n; ++i)
+    if ((int64_t)cells[i] >= n3) return set_error(ctx, FRT_E_ARG, "selftest_sort_cells: cell out of range");
+  const size_t m = (size_t)std::max<int64_t>(n, 1);
+  uint32_t* k[2] = {nullptr, nullptr};
+  int32_t* v[2] = {nullptr, nullptr};
+  int32_t *slot = nullptr, *cs = nullptr, *ce = nullptr;
+  int64_t* dn = nullptr;
+  hipError_t e = hipMalloc((void**)&k[0], m * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&k[1], m * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&v[0], m * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&v[1], m * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&slot, m * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&cs, (size_t)n3 * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&ce, (size_t)n3 * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&dn, sizeof(int64_t));
+  int rc = FRT_OK;
+  if (e != hipSuccess) rc = set_error(ctx, FRT_E_NOMEM, "selftest buffers");
+  if (rc == FRT_OK) rc = alloc_scan_sort(ctx);
+  std::vector<int32_t> iota(m);
+  for (size_t i = 0; i < m; ++i) iota[i] = (int32_t)i;
+  if (rc == FRT_OK) {
+    e = stream_copy(ctx, dn, &n, sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) e = stream_copy(ctx, k[0], cells, (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = stream_copy(ctx, v[0], iota.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_check(ctx, e, "selftest upload");
+  }
+  if (rc == FRT_OK)
+    rc = dev_sort_cells(ctx, k, v, dn, n, ctx->key_bits, n3, slot, cs, ce, "selftest_sort_cells");
+  if (rc == FRT_OK) {
+    e = hipSuccess;
+    if (slot_out && n) e = stream_copy(ctx, slot_out, slot, (size_t)n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && cstart_out) e = stream_copy(ctx, cstart_out, cs, (size_t)n3 * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && cend_out) e = stream_copy(ctx, cend_out, ce, (size_t)n3 * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_check(ctx, e, "selftest download");
+  }
+  hipStreamSynchronize(ctx->stream);
+  hipFree(k[0]); hipFree(k[1]); hipFree(v[0]); hipFree(v[1]); hipFree(slot); hipFree(cs); hipFree(ce); hipFree(dn);
   return rc;
 }
 

@@ -304,6 +304,19 @@ class FlexRT:
             P32(so) if so is not None else None, C.byref(tot)), "selftest_scan_sort")
         return ko, perm, so, tot.value
 
+    def selftest_sort_cells(self, cells):
+        """The step's cell sort of segments on a host array of cell indices:
+        (slot of every item, start and end of every cell's slot range)."""
+        c = np.ascontiguousarray(cells, dtype=np.uint32)
+        n3 = self.n ** 3
+        slot = np.zeros(len(c), dtype=np.int32)
+        cs = np.zeros(n3, dtype=np.int32)
+        ce = np.zeros(n3, dtype=np.int32)
+        P32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        self._check(self._lib.frt_selftest_sort_cells(self._h, len(c), c.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                      P32(slot), P32(cs), P32(ce)), "selftest_sort_cells")
+        return slot, cs, ce
+
     # -- rays --------------------------------------------------------------------------
     def ray_count(self):
         n = C.c_int64()

@@ -358,6 +358,13 @@ int frt_get_tuning(const frt_ctx* ctx, const char* key, int64_t* value);
  * to host arrays (any may be NULL). */
 int frt_selftest_scan_sort(frt_ctx* ctx, int64_t n, int32_t bits, const uint32_t* keys, uint32_t* keys_out,
                            int32_t* perm_out, const int32_t* counts, int32_t* scan_out, int64_t* total);
+/* Verification hook: the step's cell sort of the segments (the cell-sorted
+ * slots that the deposit passes write and the cell kernel sums) on (cells[i],
+ * i), cells[i] below the context's cell count: slot_out[i] = position of item
+ * i in the stable sort by cell, cstart_out / cend_out = each cell's range of
+ * positions (0, 0 when empty; arrays of N^3). */
+int frt_selftest_sort_cells(frt_ctx* ctx, int64_t n, const uint32_t* cells, int32_t* slot_out, int32_t* cstart_out,
+                            int32_t* cend_out);
 
 /* ---- ray buffer ----------------------------------------------------------- */
 int frt_ray_count(const frt_ctx* ctx, int64_t* count);

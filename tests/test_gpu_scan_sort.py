@@ -77,3 +77,48 @@ def test_sort_of_tile_keys_in_list_order(rt):
     ko, perm, _, _ = rt.selftest_scan_sort(keys=k, bits=12)
     ref = np.argsort(k, kind="stable")
     assert np.array_equal(perm, ref.astype(np.int32))
+
+
+def _cells(kind, count, n3, rng):
+    if kind == "uniform":
+        return rng.integers(0, n3, size=count).astype(np.uint32)
+    if kind == "few":  # long runs per cell: many LDS rounds per group of 64 cells
+        return rng.choice(rng.integers(0, n3, size=7), size=count).astype(np.uint32)
+    # as the segments of a tile-ordered ray list: ascending with local disorder
+    c = np.arange(count) * n3 // max(count, 1) + rng.integers(-70, 71, size=count)
+    return (c % n3).astype(np.uint32)
+
+
+@pytest.mark.parametrize("n,count,kind", [
+    (16, 0, "uniform"), (16, 1, "uniform"), (16, 1000, "uniform"), (16, 300_000, "uniform"),
+    (16, 200_000, "few"), (16, 500_000, "listorder"), (12, 100_000, "uniform"), (5, 5_000, "uniform"),
+    (5, 3_000, "few"), (4, 3_000, "uniform"), (3, 700, "listorder"), (2, 50, "uniform"),
+    (36, 2_000_000, "listorder"),
+])
+def test_cell_sort_matches_stable_argsort(n, count, kind):
+    """The step's cell sort (radix passes over the groups of 64 cells, then
+    one block per group): every item's slot is its position in the stable
+    sort by cell, and each cell's range is [first, last + 1) of its items
+    (0, 0 when it has none).  Grids cover no radix pass (n^3 <= 64), a
+    partial last group (n = 5, 12: n^3 not a multiple of 64), one and two
+    passes."""
+    from flexrt_imaging_reion_amd import inputs
+    from flexrt_imaging_reion_amd.rt import FlexRT
+
+    print("CASE cell sort n=%d count=%d %s" % (n, count, kind), flush=True)
+    n3 = n ** 3
+    rng = np.random.default_rng(n * 7919 + count)
+    cells = _cells(kind, count, n3, rng)
+    wl = inputs.make_workload("uniform32", n=n)
+    with FlexRT(wl["cfg"]) as g:
+        slot, cs, ce = g.selftest_sort_cells(cells)
+    order = np.argsort(cells, kind="stable")
+    ref = np.empty(count, dtype=np.int64)
+    ref[order] = np.arange(count)
+    assert np.array_equal(slot.astype(np.int64), ref)
+    cnt = np.bincount(cells, minlength=n3).astype(np.int64)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    ref_s = np.where(cnt > 0, start, 0)
+    ref_e = np.where(cnt > 0, start + cnt, 0)
+    assert np.array_equal(cs.astype(np.int64), ref_s)
+    assert np.array_equal(ce.astype(np.int64), ref_e)
