@@ -17,7 +17,9 @@ code's own expressions:
     sigma_HeII(4 nu) = sigma_HI(nu) / 4 and alpha_HeIII(T) = 2 alpha_HII(T/4)
     (Z = 2 scaling; the fits' constants carry it to 1e-3), zero below the
     He II edge;
-  * a run with the 13.6-54.4 eV bins never creates He III."""
+  * a run with the 13.6-54.4 eV bins never creates He III;
+  * photoionization equilibrium of H and the three He levels is a fixed
+    point of update_chem in every helium branch of solve_ion."""
 import numpy as np
 
 from flexrt_imaging_reion_amd import inputs
@@ -213,3 +215,84 @@ def test_update_chem_helium_photoionization_limit():
     nhe2 = o.download_field("nHe2").astype(np.float64)
     assert np.allclose(nhe2, expect, rtol=1e-4), np.max(np.abs(nhe2 / expect - 1))
     assert np.allclose(o.download_field("nHe1") + o.download_field("nHe2"), nhe, rtol=1e-6)
+
+
+def _equilibrium(nh, nhe, g_h1, g_he1, g_he2, a_h2, a_he2, a_he3, clump):
+    """Photoionization equilibrium of H and the three He levels with its own
+    electron density (ionization balances n_HI Gamma_HI = C alpha_HII n_e
+    n_HII, n_HeI Gamma_HeI = C alpha_HeII n_e n_HeII, n_HeII Gamma_HeII =
+    C alpha_HeIII n_e n_HeIII), by fixed-point iteration on n_e."""
+    ne = nh.copy()
+    for _ in range(400):
+        nh2 = nh * g_h1 / (g_h1 + clump * a_h2 * ne)
+        r1 = g_he1 / (clump * a_he2 * ne)  # n_HeII / n_HeI
+        r2 = g_he2 / (clump * a_he3 * ne)  # n_HeIII / n_HeII
+        nhe1 = nhe / (1.0 + r1 + r1 * r2)
+        nhe2 = r1 * nhe1
+        nhe3 = r2 * nhe2
+        ne = nh2 + nhe2 + 2.0 * nhe3
+    return nh2, nhe1, nhe2, nhe3, ne
+
+
+def test_update_chem_equilibrium_is_fixed_point():
+    """Every branch of solve_ion's helium solve (gas_funcs.cc:243-296: He III,
+    He II or He I the most abundant at the start of the step) and the
+    hydrogen solve leave a state in photoionization equilibrium unchanged:
+    the backward-difference step x (1 + L dt) = x_prev + S dt has x = S / L as
+    its fixed point whatever dt, and S / L of each branch's equations is the
+    textbook balance only if every rate, clumping factor and source term sits
+    where the reference puts it (a swapped alpha_HeII / alpha_HeIII, a
+    missing C, or n_He - n_HeI in place of n_He - n_HeIII moves it).  Bins
+    at 20-80 eV give Gamma_HeII; Gamma per cell spans the three helium
+    regimes."""
+    o, wl, dx = make_case()
+    cfg = wl["cfg"]
+    n3 = cfg["n"] ** 3
+    F = cfg["nfreq"]
+    nu = np.array([20.0, 30.0, 45.0, 60.0, 80.0]) / H_EV
+    spec = np.full(F, F32(0.2), np.float32)
+    o.set_spectrum(nu, spec)
+    rng = np.random.default_rng(5)
+    # u_nu per cell and bin: below 54.4 eV sets Gamma_HI and Gamma_HeI, above it Gamma_HeII too
+    lo = 10.0 ** rng.uniform(-22, -16, size=(n3, 1))
+    hi = 10.0 ** rng.uniform(-24, -14, size=(n3, 1))
+    u = np.concatenate([np.repeat(lo, 3, axis=1), np.repeat(hi, 2, axis=1)], axis=1).astype(np.float32)
+    o.upload_field("u_nu", u.reshape(-1))
+    T = F32(2e4)  # below the dielectronic range (3e4-1e6 K)
+    o.upload_field("temp", np.full(n3, T, np.float32))
+    clump = np.full(n3, F32(2.5), np.float32)
+    o.upload_field("clump", clump)
+    cl = float(F32(F32(cfg["cl_factor"]) * F32(2.998e10)))
+    u64 = u.astype(np.float64)
+
+    def gam(name):
+        s = np.array([rate(name, v) for v in nu])
+        return (u64 * (s * cl / 6.626068e-27 / nu)[None, :]).sum(axis=1)
+
+    g_h1, g_he1, g_he2 = gam("sigmapi_H1"), gam("sigmapi_He1"), gam("sigmapi_He2")
+    assert np.all(g_he2 > 0)
+    case = "B" if cfg["rec_case_b"] else "A"
+    a_h2 = float(F32(rate("alpha%s_H2" % case, float(T))))
+    a_he2 = float(F32(rate("alpha%s_He2" % case, float(T))))
+    a_he3 = float(F32(rate("alpha%s_He3" % case, float(T))))
+    assert a_he3 > 2.0 * a_he2  # distinct enough for a swap to show
+    nh = o.download_field("nH").astype(np.float64)
+    nhe = o.download_field("nHe").astype(np.float64)
+    nh2, nhe1, nhe2, nhe3, ne = _equilibrium(nh, nhe, g_h1, g_he1, g_he2, a_h2, a_he2, a_he3, 2.5)
+    state = {"nH2": nh2, "nH1": nh - nh2, "nHe1": nhe1, "nHe2": nhe2, "nHe3": nhe3, "ne": ne}
+    for k, v in state.items():
+        o.upload_field(k, v.astype(np.float32))
+        if k != "ne":
+            o.upload_field(k + "_prev", v.astype(np.float32))
+    # each helium branch of the solve is taken by some cells
+    top = np.argmax(np.stack([nhe1, nhe2, nhe3]), axis=0)
+    assert all((top == j).sum() >= 20 for j in range(3)), np.bincount(top)
+    o.update_thermal()
+    assert np.allclose(o.download_field("gamma_H1_tot"), g_h1, rtol=1e-5)
+    for k in ("nH2", "nH1", "nHe1", "nHe2", "nHe3", "ne"):
+        got = o.download_field(k).astype(np.float64)
+        ref = state[k]
+        scale = nhe if k.startswith("nHe") else nh
+        big = ref > 1e-4 * scale  # species too rare to resolve in float are bounded absolutely
+        assert np.all(np.abs(got[big] / ref[big] - 1.0) < 2e-4), (k, np.max(np.abs(got[big] / ref[big] - 1.0)))
+        assert np.all(np.abs(got - ref) < 2e-4 * scale + 1e-4 * ref), k
