@@ -141,3 +141,35 @@ def test_c2ray_off_refusals():
         inputs.setup(g, wl)
         with pytest.raises(RuntimeError):
             g.adj_enable(True)
+
+
+def test_c2ray_off_helium_iii_bit_exact():
+    """Bins at 20-80 eV: photons above 54.4 eV give Gamma_HeII, so every
+    helium branch of solve_ion (gas_funcs.cc:243-296) runs -- He I, He II or
+    He III the most abundant at the start of the step, set per cell -- and
+    He III is created and recombines.  FP64 form against the checker, bit
+    for bit, over six steps."""
+    _print_case("c2ray_off helium III lognormal n=16 src=16 steps=6")
+    wl = inputs.make_workload("lognormal64", n=16, n_src=16, steps=6, c2ray_off=True)
+    wl["nu_phot"] = np.array([20.0, 30.0, 45.0, 60.0, 80.0]) / 4.135667e-15
+    wl["spectrum"] = np.full(5, 0.2, np.float32)
+    g, o = make_pair(wl)
+    try:
+        n3 = 16 ** 3
+        rng = np.random.default_rng(11)
+        nhe = o.download_field("nHe").astype(np.float64)
+        frac = rng.dirichlet([0.3, 0.3, 0.3], size=n3)  # a dominant level per cell
+        for j, k in enumerate(("nHe1", "nHe2", "nHe3")):
+            v = (nhe * frac[:, j]).astype(np.float32)
+            for rt in (g, o):
+                rt.upload_field(k, v)
+                rt.upload_field(k + "_prev", v)
+        top = np.argmax(frac, axis=1)
+        assert all((top == j).sum() > 100 for j in range(3))
+        for k in range(6):
+            sg, so = g.step(), o.step()
+            assert counts(sg) == counts(so), "step %d: %r != %r" % (k, counts(sg), counts(so))
+            _compare(g, o, "step %d" % k)
+        assert float(o.download_field("f_He3").max()) > 0.5
+    finally:
+        g.close()
