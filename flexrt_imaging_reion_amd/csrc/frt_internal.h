@@ -450,6 +450,10 @@ int launch_hydro_reload(frt_ctx* ctx, double rho_scale, float dt_cosmic);
 // are overwritten
 int dev_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* dcount, int64_t add, int64_t bound,
              int64_t* dtotal, const char* name);
+// dev_scan_moved: dev_scan of the per-ray segment counts that also lists, in
+// order, the first *dcount items with a positive count (moved[0 .. *dmoved))
+int dev_scan_moved(frt_ctx* ctx, const int32_t* in, int32_t* out, int32_t* moved, const int64_t* dcount,
+                   int64_t add, int64_t bound, int64_t* dtotal, int64_t* dmoved, const char* name);
 int dev_sort_pairs(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t add,
                    int64_t bound, int bits, int* which, const char* name, int lo = 0);
 int dev_sort_cells(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t bound,

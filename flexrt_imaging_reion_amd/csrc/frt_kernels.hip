@@ -566,23 +566,6 @@ __global__ void __launch_bounds__(BLK) k_tape_pair(const float* __restrict__ a, 
   }
 }
 
-// Moved-ray flags (seg_cnt > 0) and their compaction into the list of moved
-// rays, in list order.
-__global__ void __launch_bounds__(BLK) k_moved_flags(const int32_t* __restrict__ seg_cnt, const StepDev* __restrict__ sd,
-                                                     int32_t* __restrict__ flag) {
-  const int64_t n = sd->nrays;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i <= n; i += (int64_t)gridDim.x * BLK)
-    flag[i] = (i < n && seg_cnt[i] > 0) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(BLK) k_moved_scatter(const int32_t* __restrict__ flag,
-                                                       const int32_t* __restrict__ off, const StepDev* __restrict__ sd,
-                                                       int32_t* __restrict__ moved) {
-  const int64_t n = sd->nrays;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK)
-    if (flag[i]) moved[off[i]] = (int32_t)i;
-}
-
 // Fill of the first *dn + add bytes (device count) with v.
 __global__ void __launch_bounds__(BLK) k_fill_u8_dev(uint8_t* a, const int64_t* __restrict__ dn, int64_t add, uint8_t v) {
   const int64_t n = *dn + add;
@@ -2298,15 +2281,11 @@ int reserve_workspace(frt_ctx* ctx) {
 int launch_segments(frt_ctx* ctx) {
   const int64_t nb = bound_rays(ctx);
   StepDev* sd = ctx->sd;
-  int rc = dev_scan(ctx, ctx->seg_cnt, ctx->seg_off, &sd->nrays, 1, nb + 1, &sd->nseg, "scan_segments");
+  // segment offsets and, in the same scan, the list of moved rays (for the
+  // deposit passes)
+  int rc = dev_scan_moved(ctx, ctx->seg_cnt, ctx->seg_off, ctx->moved_idx, &sd->nrays, 1, nb + 1, &sd->nseg,
+                          &sd->nmoved, "scan_segments");
   if (rc) return rc;
-  // the moved rays, for the deposit passes (cand_flag / cand_off are free
-  // until the merge)
-  FRT_LAUNCH(ctx, "moved_list", ray_grid(ctx, nb + 1), k_moved_flags, ctx->seg_cnt, sd, ctx->cand_flag);
-  rc = dev_scan(ctx, ctx->cand_flag, ctx->cand_off, &sd->nrays, 1, nb + 1, &sd->nmoved, "scan_moved");
-  if (rc) return rc;
-  FRT_LAUNCH(ctx, "moved_list", ray_grid(ctx, nb), k_moved_scatter, ctx->cand_flag, ctx->cand_off, sd,
-             ctx->moved_idx);
   FRT_LAUNCH(ctx, "seg_keys", ray_grid(ctx, nb), k_seg_keys, sd, ctx->g.cap, ctx->seg_cnt, ctx->seg_off,
              ctx->path_cell, ctx->seg_key[0], ctx->seg_val[0]);
   // the cell-sorted slot of every path entry and the cells' slot ranges

@@ -92,13 +92,24 @@ __device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* total) {
   return before + inc - v;
 }
 
+// Scanned value of item i.  With MOVED (the segment scan) the value is the
+// count plus 2^32 for each of the first *dcount items with a positive count,
+// so one scan gives both the count offsets (low 32 bits; their total stays
+// below 2^31) and the rank of every such item among them (high 32 bits).
+template <bool MOVED>
+__device__ __forceinline__ int64_t scan_value(int32_t v, int64_t i, int64_t nflag) {
+  return MOVED ? (int64_t)v + ((i < nflag && v > 0) ? (int64_t(1) << 32) : 0) : (int64_t)v;
+}
+
+template <bool MOVED>
 __global__ void __launch_bounds__(SBLK) k_scan_reduce(const int32_t* __restrict__ in, const int64_t* dcount,
                                                       int64_t add, int64_t* __restrict__ part) {
   const int64_t count = dev_count(dcount, add);
+  const int64_t nflag = dev_count(dcount, 0);
   int64_t lo, hi;
   block_range(count, gridDim.x, SCAN_TILE, blockIdx.x, lo, hi);
   int64_t s = 0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += SBLK) s += in[i];
+  for (int64_t i = lo + threadIdx.x; i < hi; i += SBLK) s += scan_value<MOVED>(in[i], i, nflag);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   __shared__ int64_t ws[SBLK / 64];
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
@@ -111,8 +122,10 @@ __global__ void __launch_bounds__(SBLK) k_scan_reduce(const int32_t* __restrict_
 }
 
 // One block: exclusive scan of the G block sums in place; the grand total to
-// *total64 (if set) and to part[G].
-__global__ void __launch_bounds__(SBLK) k_scan_parts(int64_t* __restrict__ part, int G, int64_t* total64) {
+// *total64 (if set) and to part[G]; with total_hi set, the packed total of
+// the segment scan is split: low 32 bits to *total64, high to *total_hi.
+__global__ void __launch_bounds__(SBLK) k_scan_parts(int64_t* __restrict__ part, int G, int64_t* total64,
+                                                     int64_t* total_hi) {
   int64_t carry = 0;
   for (int base = 0; base < G; base += SBLK) {
     const int i = base + threadIdx.x;
@@ -124,32 +137,44 @@ __global__ void __launch_bounds__(SBLK) k_scan_parts(int64_t* __restrict__ part,
   }
   if (threadIdx.x == 0) {
     part[G] = carry;
-    if (total64) *total64 = carry;
+    if (total_hi) {
+      *total64 = carry & 0xffffffffll;
+      *total_hi = carry >> 32;
+    } else if (total64) {
+      *total64 = carry;
+    }
   }
 }
 
+// With MOVED, out receives the low half of the scan and every flagged item
+// i is written to moved[rank] (the list of items with a positive count).
+template <bool MOVED>
 __global__ void __launch_bounds__(SBLK) k_scan_down(const int32_t* __restrict__ in, int32_t* __restrict__ out,
                                                     const int64_t* dcount, int64_t add,
-                                                    const int64_t* __restrict__ part) {
+                                                    const int64_t* __restrict__ part, int32_t* __restrict__ moved) {
   const int64_t count = dev_count(dcount, add);
+  const int64_t nflag = dev_count(dcount, 0);
   int64_t lo, hi;
   block_range(count, gridDim.x, SCAN_TILE, blockIdx.x, lo, hi);
   int64_t carry = part[blockIdx.x];
   for (int64_t base = lo; base < hi; base += SCAN_TILE) {
     // thread t holds items base + t*ITEMS .. + ITEMS-1 (consecutive)
     const int64_t first = base + (int64_t)threadIdx.x * SCAN_ITEMS;
-    int32_t v[SCAN_ITEMS];
+    int64_t v[SCAN_ITEMS];
     int64_t s = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-      v[k] = first + k < hi ? in[first + k] : 0;
+      v[k] = first + k < hi ? scan_value<MOVED>(in[first + k], first + k, nflag) : 0;
       s += v[k];
     }
     int64_t tot;
     int64_t run = carry + block_excl_scan(s, &tot);
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-      if (first + k < hi) out[first + k] = (int32_t)run;
+      if (first + k < hi) {
+        out[first + k] = (int32_t)(MOVED ? (run & 0xffffffffll) : run);
+        if (MOVED && (v[k] >> 32)) moved[run >> 32] = (int32_t)(first + k);
+      }
       run += v[k];
     }
     carry += tot;
@@ -436,9 +461,19 @@ int dev_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* dcoun
              int64_t* dtotal, const char* name) {
   const int G = scan_grid(bound);
   int64_t* part = ctx->ss_part;
-  SS_LAUNCH(ctx, name, G, k_scan_reduce, in, dcount, add, part);
-  SS_LAUNCH(ctx, name, 1, k_scan_parts, part, G, dtotal);
-  SS_LAUNCH(ctx, name, G, k_scan_down, in, out, dcount, add, (const int64_t*)part);
+  SS_LAUNCH(ctx, name, G, k_scan_reduce<false>, in, dcount, add, part);
+  SS_LAUNCH(ctx, name, 1, k_scan_parts, part, G, dtotal, (int64_t*)nullptr);
+  SS_LAUNCH(ctx, name, G, k_scan_down<false>, in, out, dcount, add, (const int64_t*)part, (int32_t*)nullptr);
+  return FRT_OK;
+}
+
+int dev_scan_moved(frt_ctx* ctx, const int32_t* in, int32_t* out, int32_t* moved, const int64_t* dcount,
+                   int64_t add, int64_t bound, int64_t* dtotal, int64_t* dmoved, const char* name) {
+  const int G = scan_grid(bound);
+  int64_t* part = ctx->ss_part;
+  SS_LAUNCH(ctx, name, G, k_scan_reduce<true>, in, dcount, add, part);
+  SS_LAUNCH(ctx, name, 1, k_scan_parts, part, G, dtotal, dmoved);
+  SS_LAUNCH(ctx, name, G, k_scan_down<true>, in, out, dcount, add, (const int64_t*)part, moved);
   return FRT_OK;
 }
 
