@@ -356,14 +356,45 @@ __global__ void __launch_bounds__(SBLK) k_zero_i32(int32_t* __restrict__ a, int6
 }
 
 // [start, end) of every non-empty group's run in the group-sorted keys
-// (empty groups keep the zeros they were cleared to).
+// (empty groups keep the zeros they were cleared to).  Each thread takes
+// four consecutive keys; the neighbours across threads come from the
+// adjacent lanes, and only a wavefront's edge lanes load one more key.
 __global__ void __launch_bounds__(SBLK) k_group_ranges(const uint32_t* __restrict__ key, const int64_t* dcount,
                                                        int32_t* __restrict__ gstart, int32_t* __restrict__ gend) {
   const int64_t n = dev_count(dcount, 0);
-  for (int64_t s = (int64_t)blockIdx.x * SBLK + threadIdx.x; s < n; s += (int64_t)gridDim.x * SBLK) {
-    const uint32_t g = key[s] >> GROUP_SHIFT;
-    if (s == 0 || (key[s - 1] >> GROUP_SHIFT) != g) gstart[g] = (int32_t)s;
-    if (s == n - 1 || (key[s + 1] >> GROUP_SHIFT) != g) gend[g] = (int32_t)(s + 1);
+  const int lane = threadIdx.x & 63;
+  const int64_t nq = (n + 3) >> 2;  // groups of four keys
+  // block-uniform trip count: the shuffles see every lane
+  for (int64_t qb = (int64_t)blockIdx.x * SBLK; qb < nq; qb += (int64_t)gridDim.x * SBLK) {
+    const int64_t q = qb + threadIdx.x;
+    const int64_t s0 = q << 2;
+    uint32_t g[4];
+    if (s0 + 3 < n) {
+      const uint4 k4 = *reinterpret_cast<const uint4*>(key + s0);  // 16-byte aligned: s0 is a multiple of 4
+      g[0] = k4.x >> GROUP_SHIFT;
+      g[1] = k4.y >> GROUP_SHIFT;
+      g[2] = k4.z >> GROUP_SHIFT;
+      g[3] = k4.w >> GROUP_SHIFT;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) g[j] = s0 + j < n ? key[s0 + j] >> GROUP_SHIFT : 0xffffffffu;
+    }
+    // the group of the key before s0 and after s0 + 3 (none: a value no group has)
+    uint32_t before = __shfl_up(g[3], 1, 64);
+    uint32_t after = __shfl_down(g[0], 1, 64);
+    if (lane == 0) before = s0 > 0 && s0 - 1 < n ? key[s0 - 1] >> GROUP_SHIFT : 0xffffffffu;
+    if (lane == 63) after = s0 + 4 < n ? key[s0 + 4] >> GROUP_SHIFT : 0xffffffffu;
+    if (s0 < n) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t s = s0 + j;
+        if (s >= n) break;
+        const uint32_t prev = j == 0 ? before : g[j - 1];
+        const uint32_t next = j == 3 ? after : g[j + 1];
+        if (s == 0 || prev != g[j]) gstart[g[j]] = (int32_t)s;
+        if (s == n - 1 || next != g[j]) gend[g[j]] = (int32_t)(s + 1);
+      }
+    }
   }
 }
 
@@ -392,7 +423,18 @@ __global__ void __launch_bounds__(SBLK) k_group_local(const uint32_t* __restrict
   const int s0 = gstart[g], s1 = gend[g];
   if (t < 64) next[t] = 0;
   __syncthreads();
-  for (int s = s0 + t; s < s1; s += SBLK) atomicAdd(&next[key[s] & 63u], 1);
+  // the cell bits of the first KEEP rounds stay in registers for the placement
+  constexpr int KEEP = 8;
+  int kd[KEEP];
+#pragma unroll
+  for (int r = 0; r < KEEP; ++r) {
+    const int s = s0 + r * SBLK + t;
+    kd[r] = s < s1 ? (int)(key[s] & 63u) : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < KEEP; ++r)
+    if (s0 + r * SBLK + t < s1) atomicAdd(&next[kd[r]], 1);
+  for (int s = s0 + KEEP * SBLK + t; s < s1; s += SBLK) atomicAdd(&next[key[s] & 63u], 1);
   __syncthreads();
   if (w == 0) {
     const int c = next[lane];
@@ -411,10 +453,15 @@ __global__ void __launch_bounds__(SBLK) k_group_local(const uint32_t* __restrict
     next[lane] = first;
   }
   __syncthreads();
-  for (int base = s0; base < s1; base += SBLK) {  // block-uniform trip count
+  int r = 0;
+  for (int base = s0; base < s1; base += SBLK, ++r) {  // block-uniform trip count
     const int s = base + t;
     const bool valid = s < s1;
-    const int d = valid ? (int)(key[s] & 63u) : 0;
+    int d = 0;
+#pragma unroll
+    for (int j = 0; j < KEEP; ++j)
+      if (r == j) d = kd[j];
+    if (r >= KEEP) d = valid ? (int)(key[s] & 63u) : 0;
     const int32_t v = valid ? val[s] : 0;
     const uint64_t m = match_digit(d, GROUP_SHIFT, valid);
     wc[w][lane] = 0;
