@@ -461,11 +461,15 @@ __global__ void __launch_bounds__(ABLK) k_adjr_cast(int nsrc, int F, int64_t bas
   a_lum[s] += acc;
 }
 
-// Inverse of the compaction's order: output q took pre-sort entry sorted_pos[q].
-__global__ void __launch_bounds__(ABLK) k_adjr_inv(int64_t total, const int32_t* __restrict__ sorted_pos,
-                                                   int32_t* __restrict__ inv) {
+// Inverse of the compaction's order: output q took child ch of ray i
+// (sorted_src[q] = i << 2 | ch), pre-sort entry out_off[i] + ch.
+__global__ void __launch_bounds__(ABLK) k_adjr_inv(int64_t total, const uint32_t* __restrict__ sorted_src,
+                                                   const int32_t* __restrict__ out_off, int32_t* __restrict__ inv) {
   const int64_t q = (int64_t)blockIdx.x * ABLK + threadIdx.x;
-  if (q < total) inv[sorted_pos[q]] = (int32_t)q;
+  if (q < total) {
+    const uint32_t packed = sorted_src[q];
+    inv[out_off[packed >> 2] + (int)(packed & 3u)] = (int32_t)q;
+  }
 }
 
 // split_rays and the compaction (rt_funcs.cc:211-237, ray_tracing.cc:482-619):
@@ -785,7 +789,7 @@ int frt_adj_step_reverse(frt_ctx* ctx) {
                        (const float*)ctx->adj_raydt, (const double*)ctx->adjr_ray, ctx->adjr_lum);
   if (total > 0)
     hipLaunchKernelGGL(k_adjr_inv, dim3(gr(total)), dim3(ABLK), 0, ctx->stream, total,
-                       (const int32_t*)ctx->split_sorted, ctx->adjr_inv);
+                       (const uint32_t*)ctx->split_sorted, (const int32_t*)ctx->out_off, ctx->adjr_inv);
   hipLaunchKernelGGL(k_adjr_split, dim3(gr(n_pre)), dim3(ABLK), 0, ctx->stream, n_pre, (int)F,
                      (const int32_t*)ctx->n_out, (const int32_t*)ctx->out_off, (const int32_t*)ctx->adjr_inv,
                      (const double*)ctx->adjr_ray, ctx->adjr_ray2);

@@ -285,7 +285,7 @@ struct frt_ctx {
   int64_t nseg;
   StepDev* sd;          // device: this context's step scalars
   StepDev* hsd;         // pinned host mirror
-  int32_t* split_sorted;  // compaction order of the last split ([total] positions)
+  int32_t* split_sorted;  // compaction order of the last split: the source (ray << 2 | child) of each output
   bool deferred;          // the last split's compaction is left to the next march (or materialize_rays)
   // psum[i]: the float photon sum of ray i (sum_nphot, in bin order), the
   // value the merge histogram, the merge flag and the split flag compare.

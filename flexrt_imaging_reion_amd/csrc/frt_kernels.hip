@@ -333,13 +333,13 @@ __device__ __forceinline__ ChildGeo child_geometry(const Ray& par, int ch, const
 
 // The ray at position q of the compacted list after a split: ray i copied,
 // or child ch of split ray i (split_ray, ray_tracing.cc:482-619), with its
-// photons in n (a quarter of the parent's per bin).
+// photons in n (a quarter of the parent's per bin); sorted_src[q] = i << 2 | ch
+// (the values the split's tile sort carried).
 __device__ __forceinline__ void output_ray(const RaySoA& src, int64_t q, const GridConst& g,
                                            const int32_t* __restrict__ n_out,
-                                           const int32_t* __restrict__ sorted_pos,
-                                           const uint32_t* __restrict__ src_of,
+                                           const uint32_t* __restrict__ sorted_src,
                                            const float* __restrict__ rot_fwd, Ray& out, float* n) {
-  const uint32_t packed = src_of[sorted_pos[q]];
+  const uint32_t packed = sorted_src[q];
   const int64_t i = (int64_t)(packed >> 2);
   const int ch = (int)(packed & 3u);
   ray_load(src, i, out);
@@ -378,8 +378,7 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst
                                                float* __restrict__ psum,
                                                DevCounters* cnt, int chunk, RaySoA src,
                                                const int32_t* __restrict__ n_out,
-                                               const int32_t* __restrict__ sorted_pos,
-                                               const uint32_t* __restrict__ src_of,
+                                               const uint32_t* __restrict__ sorted_src,
                                                const float* __restrict__ rot_fwd,
                                                const float* __restrict__ fion) {
   unsigned long long nmoved = 0;
@@ -395,7 +394,7 @@ __global__ void __launch_bounds__(BLK) k_march(RaySoA rb, StepDev* sd, GridConst
     float nph[8];
     const bool whole = GATHER && i < ngather;  // write every field of the ray
     if (whole)
-      output_ray(src, i, g, n_out, sorted_pos, src_of, rot_fwd, ry, nph);
+      output_ray(src, i, g, n_out, sorted_src, rot_fwd, ry, nph);
     else
       ray_load(rb, i, ry);
     int ci, cj, ck;
@@ -1892,8 +1891,7 @@ __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, const StepDev* __r
                                                   const int32_t* __restrict__ n_out,
                                                   const int32_t* __restrict__ out_off,
                                                   const float* __restrict__ rot_fwd,
-                                                  uint32_t* __restrict__ key, int32_t* __restrict__ val,
-                                                  uint32_t* __restrict__ src_of) {
+                                                  uint32_t* __restrict__ key, int32_t* __restrict__ val) {
   __shared__ int s_par[BLK];  // split rays of each wavefront, in lane order
   if (sd->ncompact == 0) return;
   const int64_t nrays = sd->nrays;
@@ -1906,8 +1904,7 @@ __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, const StepDev* __r
   if (no == 1) {
     const int o = out_off[i];
     key[o] = frt_tile_key_internal(g.lay, ray_cell(src, i));
-    val[o] = o;
-    src_of[o] = (uint32_t)i << 2;
+    val[o] = (int32_t)((uint32_t)i << 2);  // the source: ray i (ch 0), carried by the sort
   }
   const unsigned long long mask = __ballot(no == 4);
   if (no == 4) s_par[wbase + __popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
@@ -1920,8 +1917,7 @@ __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, const StepDev* __r
     Ray par;
     ray_load(src, ip, par);
     key[o + ch] = frt_tile_key_internal(g.lay, child_geometry(par, ch, g, rot_fwd).cell);
-    val[o + ch] = o + ch;
-    src_of[o + ch] = ((uint32_t)ip << 2) | (uint32_t)ch;
+    val[o + ch] = (int32_t)(((uint32_t)ip << 2) | (uint32_t)ch);
   }
   __syncthreads();  // s_par is rewritten by the next round
   }
@@ -1931,15 +1927,14 @@ __global__ void __launch_bounds__(BLK) k_out_keys(RaySoA src, const StepDev* __r
 // takes the ray (or split child) that sorted to position q.
 __global__ void __launch_bounds__(BLK) k_compact(RaySoA src, RaySoA dst, const StepDev* __restrict__ sd, GridConst g,
                                                  const int32_t* __restrict__ n_out,
-                                                 const int32_t* __restrict__ sorted_pos,
-                                                 const uint32_t* __restrict__ src_of,
+                                                 const uint32_t* __restrict__ sorted_src,
                                                  const float* __restrict__ rot_fwd) {
   const int64_t total = sd->ncompact;
   const int F = g.nfreq;
   const int nplanes = frt_phot_planes(F);
   const int64_t cap = g.cap;
   for (int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x; q < total; q += (int64_t)gridDim.x * BLK) {
-    const uint32_t packed = src_of[sorted_pos[q]];
+    const uint32_t packed = sorted_src[q];
     const int64_t i = (int64_t)(packed >> 2);
     const int ch = (int)(packed & 3u);
     if (n_out[i] == 1) {
@@ -2142,13 +2137,13 @@ int launch_march(frt_ctx* ctx) {
   if (ctx->deferred) {  // the previous split's compaction, gathered by the march
     FRT_LAUNCH(ctx, "march", grid, k_march<true>, cur_buf(ctx), ctx->sd, ctx->g, ctx->path_cell, ctx->path_dist,
                ctx->seg_cnt, ctx->alive, ctx->psum, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)ctx->n_out,
-               (const int32_t*)ctx->split_sorted, (const uint32_t*)ctx->cand_key_tmp, (const float*)ctx->rot_fwd,
+               (const uint32_t*)ctx->split_sorted, (const float*)ctx->rot_fwd,
                (const float*)(ctx->subgrid ? ctx->sg.fion : nullptr));
     ctx->deferred = false;
   } else {
     FRT_LAUNCH(ctx, "march", grid, k_march<false>, cur_buf(ctx), ctx->sd, ctx->g, ctx->path_cell, ctx->path_dist,
                ctx->seg_cnt, ctx->alive, ctx->psum, ctx->dcnt, chunk, ctx->buf[1 - ctx->cur], (const int32_t*)nullptr,
-               (const int32_t*)nullptr, (const uint32_t*)nullptr, (const float*)ctx->rot_fwd,
+               (const uint32_t*)nullptr, (const float*)ctx->rot_fwd,
                (const float*)(ctx->subgrid ? ctx->sg.fion : nullptr));
   }
   ctx->psum_state = FRT_PSUM_UNMOVED;
@@ -2161,7 +2156,7 @@ int materialize_rays(frt_ctx* ctx) {
   if (!ctx->deferred) return FRT_OK;
   const int64_t nb = std::min<int64_t>(ctx->g.cap, 4 * ctx->nrays + 1);
   FRT_LAUNCH(ctx, "compact", ray_grid(ctx, nb), k_compact, ctx->buf[1 - ctx->cur], cur_buf(ctx), ctx->sd, ctx->g,
-             ctx->n_out, ctx->split_sorted, ctx->cand_key_tmp, ctx->rot_fwd);
+             ctx->n_out, (const uint32_t*)ctx->split_sorted, ctx->rot_fwd);
   ctx->deferred = false;
   ctx->psum_state = FRT_PSUM_NONE;
   return FRT_OK;
@@ -2504,7 +2499,7 @@ int launch_split_compact(frt_ctx* ctx, bool nosplit) {
   FRT_LAUNCH(ctx, "split_check", 1, k_split_check, sd, g.cap, (int64_t)ctx->nsrc * 12);
   // stable sort of the output rays by cell tile: the list order of every step
   FRT_LAUNCH(ctx, "out_keys", ray_grid(ctx, nb), k_out_keys, cur_buf(ctx), sd, g, ctx->n_out, ctx->out_off,
-             ctx->rot_fwd, ctx->m_key[0], ctx->m_val[0], ctx->cand_key_tmp);
+             ctx->rot_fwd, ctx->m_key[0], ctx->m_val[0]);
   int which = 0;
   const int64_t nout = std::min<int64_t>(g.cap, 4 * nb);
   rc = dev_sort_pairs(ctx, ctx->m_key, ctx->m_val, &sd->ncompact, 0, nout, ctx->tile_bits, &which, "sort_rays");
@@ -2513,7 +2508,7 @@ int launch_split_compact(frt_ctx* ctx, bool nosplit) {
   if (nosplit) {
     // one thread per input ray; the grid-stride loop takes the split children beyond
     FRT_LAUNCH(ctx, "compact", ray_grid(ctx, nb), k_compact, cur_buf(ctx), ctx->buf[1 - ctx->cur], sd, g, ctx->n_out,
-               ctx->m_val[which], ctx->cand_key_tmp, ctx->rot_fwd);
+               (const uint32_t*)ctx->m_val[which], ctx->rot_fwd);
   } else {
     ctx->deferred = true;  // the next march gathers the list through the permutation
   }
