@@ -1738,20 +1738,40 @@ __device__ __forceinline__ int l_from_r_dev(const GridConst& g, float r, float s
 }
 
 // One thread per occupied (cell, direction) bin: accumulates its members in
-// list order exactly as the serial phase 1, then normalizes (phase 2).
+// list order exactly as the serial phase 1, then normalizes (phase 2).  Each
+// wavefront takes windows of 128 sorted candidates and lists the group heads
+// among them (ballots), then gives one head to each lane: groups hold about
+// two members, so a lane per candidate would leave half the lanes idle.
 __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, const StepDev* __restrict__ sd, GridConst g,
                                                       const uint32_t* __restrict__ key,
                                                       const int32_t* __restrict__ val,
                                                       uint8_t* __restrict__ alive, const float* __restrict__ rot_fwd,
                                                       float safety, DevCounters* cnt, float* __restrict__ psum) {
+  constexpr int WIN = 128;  // candidates per window
+  __shared__ int s_head[BLK / 64][WIN];
   const int64_t ncand = sd->ncand;
   const uint32_t itheta_m = sd->itm;
   const float* fwd = rot_fwd + 9 * (int64_t)itheta_m;
   unsigned long long nheads = 0, merged = 0;
-  for (int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x; s < ncand; s += (int64_t)gridDim.x * BLK) {
-  const bool head = (s == 0) || (key[s - 1] != key[s]);
-  nheads += head ? 1 : 0;
-  if (head) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t nwaves = (int64_t)gridDim.x * (BLK / 64);
+  for (int64_t base = ((int64_t)blockIdx.x * (BLK / 64) + w) * WIN; base < ncand; base += nwaves * WIN) {
+  int nh = 0;
+#pragma unroll
+  for (int j = 0; j < WIN / 64; ++j) {
+    const int64_t c = base + j * 64 + lane;
+    const bool hd = c < ncand && (c == 0 || key[c - 1] != key[c]);
+    const uint64_t m = __ballot(hd);
+    if (hd) s_head[w][nh + __popcll(m & lt)] = (int)(c - base);
+    nh += __popcll(m);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+  if (lane == 0) nheads += (unsigned long long)nh;
+  for (int t = lane; t < nh; t += 64) {
+    const int64_t s = base + s_head[w][t];
     const int F = g.nfreq;
     const int64_t cap = g.cap;
     int64_t e = s + 1;
@@ -1827,6 +1847,7 @@ __global__ void __launch_bounds__(BLK) k_merge_groups(RaySoA rb, const StepDev* 
     rbase.time = T;
     ray_store(rb, b, rbase);
   }
+  __builtin_amdgcn_wave_barrier();  // the window's head list is read before the next is written
   }
   block_add(&cnt->groups, nheads);
   block_add(&cnt->merged_away, merged);
