@@ -692,7 +692,6 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
     if (e == hipSuccess) e = dalloc(&ctx->n_out, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->out_off, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->cand_flag, cap + 1);
-    if (e == hipSuccess) e = dalloc(&ctx->cand_off, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->cand_key_tmp, cap + 1);
     if (e == hipSuccess) e = dalloc(&ctx->moved_idx, cap + 1);
     for (int b = 0; b < 2; ++b) {
@@ -764,7 +763,7 @@ int frt_destroy(frt_ctx* ctx) {
   for (int b = 0; b < 2; ++b) free_rays(ctx->buf[b]);
   hipFree(ctx->path_cell); hipFree(ctx->path_dist); hipFree(ctx->seg_cnt); hipFree(ctx->seg_off);
   hipFree(ctx->alive); hipFree(ctx->psum); hipFree(ctx->n_out); hipFree(ctx->out_off); hipFree(ctx->cand_flag);
-  hipFree(ctx->cand_off); hipFree(ctx->cand_key_tmp); hipFree(ctx->moved_idx);
+  hipFree(ctx->cand_key_tmp); hipFree(ctx->moved_idx);
   for (int b = 0; b < 2; ++b) {
     hipFree(ctx->seg_key[b]); hipFree(ctx->seg_val[b]); hipFree(ctx->m_key[b]); hipFree(ctx->m_val[b]);
   }

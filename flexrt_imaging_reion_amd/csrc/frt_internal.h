@@ -245,7 +245,6 @@ struct frt_ctx {
   int32_t* n_out;      // [cap+1]
   int32_t* out_off;    // [cap+1]
   int32_t* cand_flag;  // [cap+1]
-  int32_t* cand_off;   // [cap+1]
   uint32_t* cand_key_tmp;  // [cap]
   int32_t* moved_idx;  // [cap] rays that moved this step (seg_cnt > 0), in list order
   int64_t nmoved;
@@ -454,6 +453,10 @@ int dev_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* dcoun
 // order, the first *dcount items with a positive count (moved[0 .. *dmoved))
 int dev_scan_moved(frt_ctx* ctx, const int32_t* in, int32_t* out, int32_t* moved, const int64_t* dcount,
                    int64_t add, int64_t bound, int64_t* dtotal, int64_t* dmoved, const char* name);
+// dev_compact: the items i < *dcount + add with flag[i] = 1, in order, to
+// list[0 .. *dtotal) with kdst[r] = ksrc[list[r]]
+int dev_compact(frt_ctx* ctx, const int32_t* flag, const uint32_t* ksrc, int32_t* list, uint32_t* kdst,
+                const int64_t* dcount, int64_t add, int64_t bound, int64_t* dtotal, const char* name);
 int dev_sort_pairs(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t add,
                    int64_t bound, int bits, int* which, const char* name, int lo = 0);
 int dev_sort_cells(frt_ctx* ctx, uint32_t* key[2], int32_t* val[2], const int64_t* dcount, int64_t bound,

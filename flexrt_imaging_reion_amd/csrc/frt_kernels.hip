@@ -1712,20 +1712,6 @@ __global__ void __launch_bounds__(BLK) k_merge_flag(RaySoA rb, const StepDev* __
   }
 }
 
-__global__ void __launch_bounds__(BLK) k_merge_scatter(const StepDev* __restrict__ sd, const int32_t* __restrict__ cand,
-                                                       const int32_t* __restrict__ off,
-                                                       const uint32_t* __restrict__ key_tmp,
-                                                       uint32_t* __restrict__ key,
-                                                       int32_t* __restrict__ val) {
-  if (!sd->merge_on) return;
-  const int64_t nrays = sd->nrays;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nrays; i += (int64_t)gridDim.x * BLK) {
-    if (!cand[i]) continue;
-    key[off[i]] = key_tmp[i];
-    val[off[i]] = (int32_t)i;
-  }
-}
-
 // l_from_r (ray_tracing.cc:87-95).
 __device__ __forceinline__ int l_from_r_dev(const GridConst& g, float r, float safety) {
   const float q = r / g.dx0;
@@ -2489,10 +2475,10 @@ int launch_merge(frt_ctx* ctx) {
   FRT_LAUNCH(ctx, "merge_flag", ray_grid(ctx, nb + 1), k_merge_flag, rb, sd, g, ctx->alive, (const float*)ctx->psum,
              ctx->rot_inv,
              ctx->cand_flag, ctx->cand_key_tmp);
-  rc = dev_scan(ctx, ctx->cand_flag, ctx->cand_off, &sd->mcount, 0, nb + 1, &sd->ncand, "scan_merge");
+  // the candidates in list order with their keys, for the sort
+  rc = dev_compact(ctx, ctx->cand_flag, ctx->cand_key_tmp, ctx->m_val[0], ctx->m_key[0], &sd->mcount, 0, nb + 1,
+                   &sd->ncand, "scan_merge");
   if (rc) return rc;
-  FRT_LAUNCH(ctx, "merge_scatter", ray_grid(ctx, nb), k_merge_scatter, sd, ctx->cand_flag, ctx->cand_off,
-             ctx->cand_key_tmp, ctx->m_key[0], ctx->m_val[0]);
   int bits = 1;
   while ((1ll << bits) < (int64_t)g.n3 * nbins) ++bits;
   int which = 0;

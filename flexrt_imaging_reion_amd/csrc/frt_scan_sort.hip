@@ -96,12 +96,17 @@ __device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* total) {
 // count plus 2^32 for each of the first *dcount items with a positive count,
 // so one scan gives both the count offsets (low 32 bits; their total stays
 // below 2^31) and the rank of every such item among them (high 32 bits).
-template <bool MOVED>
+// Scan modes: plain offsets; the segment scan with the moved-ray list
+// (above); the compaction of flagged items (0/1 flags) into a list with
+// their keys, writing no offsets.
+enum ScanMode { SCAN_PLAIN = 0, SCAN_MOVED = 1, SCAN_COMPACT = 2 };
+
+template <int MODE>
 __device__ __forceinline__ int64_t scan_value(int32_t v, int64_t i, int64_t nflag) {
-  return MOVED ? (int64_t)v + ((i < nflag && v > 0) ? (int64_t(1) << 32) : 0) : (int64_t)v;
+  return MODE == SCAN_MOVED ? (int64_t)v + ((i < nflag && v > 0) ? (int64_t(1) << 32) : 0) : (int64_t)v;
 }
 
-template <bool MOVED>
+template <int MODE>
 __global__ void __launch_bounds__(SBLK) k_scan_reduce(const int32_t* __restrict__ in, const int64_t* dcount,
                                                       int64_t add, int64_t* __restrict__ part) {
   const int64_t count = dev_count(dcount, add);
@@ -109,7 +114,7 @@ __global__ void __launch_bounds__(SBLK) k_scan_reduce(const int32_t* __restrict_
   int64_t lo, hi;
   block_range(count, gridDim.x, SCAN_TILE, blockIdx.x, lo, hi);
   int64_t s = 0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += SBLK) s += scan_value<MOVED>(in[i], i, nflag);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += SBLK) s += scan_value<MODE>(in[i], i, nflag);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   __shared__ int64_t ws[SBLK / 64];
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
@@ -146,12 +151,15 @@ __global__ void __launch_bounds__(SBLK) k_scan_parts(int64_t* __restrict__ part,
   }
 }
 
-// With MOVED, out receives the low half of the scan and every flagged item
-// i is written to moved[rank] (the list of items with a positive count).
-template <bool MOVED>
+// SCAN_MOVED: out receives the low half of the scan and every flagged item
+// i is written to list[rank] (the items with a positive count).
+// SCAN_COMPACT: each item i with flag 1 goes to list[rank] = i with
+// kdst[rank] = ksrc[i]; out is not written.
+template <int MODE>
 __global__ void __launch_bounds__(SBLK) k_scan_down(const int32_t* __restrict__ in, int32_t* __restrict__ out,
                                                     const int64_t* dcount, int64_t add,
-                                                    const int64_t* __restrict__ part, int32_t* __restrict__ moved) {
+                                                    const int64_t* __restrict__ part, int32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ ksrc, uint32_t* __restrict__ kdst) {
   const int64_t count = dev_count(dcount, add);
   const int64_t nflag = dev_count(dcount, 0);
   int64_t lo, hi;
@@ -164,7 +172,7 @@ __global__ void __launch_bounds__(SBLK) k_scan_down(const int32_t* __restrict__ 
     int64_t s = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-      v[k] = first + k < hi ? scan_value<MOVED>(in[first + k], first + k, nflag) : 0;
+      v[k] = first + k < hi ? scan_value<MODE>(in[first + k], first + k, nflag) : 0;
       s += v[k];
     }
     int64_t tot;
@@ -172,8 +180,15 @@ __global__ void __launch_bounds__(SBLK) k_scan_down(const int32_t* __restrict__ 
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
       if (first + k < hi) {
-        out[first + k] = (int32_t)(MOVED ? (run & 0xffffffffll) : run);
-        if (MOVED && (v[k] >> 32)) moved[run >> 32] = (int32_t)(first + k);
+        if (MODE == SCAN_COMPACT) {
+          if (v[k]) {
+            list[run] = (int32_t)(first + k);
+            kdst[run] = ksrc[first + k];
+          }
+        } else {
+          out[first + k] = (int32_t)(MODE == SCAN_MOVED ? (run & 0xffffffffll) : run);
+          if (MODE == SCAN_MOVED && (v[k] >> 32)) list[run >> 32] = (int32_t)(first + k);
+        }
       }
       run += v[k];
     }
@@ -508,9 +523,21 @@ int dev_scan(frt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* dcoun
              int64_t* dtotal, const char* name) {
   const int G = scan_grid(bound);
   int64_t* part = ctx->ss_part;
-  SS_LAUNCH(ctx, name, G, k_scan_reduce<false>, in, dcount, add, part);
+  SS_LAUNCH(ctx, name, G, k_scan_reduce<SCAN_PLAIN>, in, dcount, add, part);
   SS_LAUNCH(ctx, name, 1, k_scan_parts, part, G, dtotal, (int64_t*)nullptr);
-  SS_LAUNCH(ctx, name, G, k_scan_down<false>, in, out, dcount, add, (const int64_t*)part, (int32_t*)nullptr);
+  SS_LAUNCH(ctx, name, G, k_scan_down<SCAN_PLAIN>, in, out, dcount, add, (const int64_t*)part, (int32_t*)nullptr,
+            (const uint32_t*)nullptr, (uint32_t*)nullptr);
+  return FRT_OK;
+}
+
+int dev_compact(frt_ctx* ctx, const int32_t* flag, const uint32_t* ksrc, int32_t* list, uint32_t* kdst,
+                const int64_t* dcount, int64_t add, int64_t bound, int64_t* dtotal, const char* name) {
+  const int G = scan_grid(bound);
+  int64_t* part = ctx->ss_part;
+  SS_LAUNCH(ctx, name, G, k_scan_reduce<SCAN_COMPACT>, flag, dcount, add, part);
+  SS_LAUNCH(ctx, name, 1, k_scan_parts, part, G, dtotal, (int64_t*)nullptr);
+  SS_LAUNCH(ctx, name, G, k_scan_down<SCAN_COMPACT>, flag, (int32_t*)nullptr, dcount, add, (const int64_t*)part,
+            list, ksrc, kdst);
   return FRT_OK;
 }
 
@@ -518,9 +545,10 @@ int dev_scan_moved(frt_ctx* ctx, const int32_t* in, int32_t* out, int32_t* moved
                    int64_t add, int64_t bound, int64_t* dtotal, int64_t* dmoved, const char* name) {
   const int G = scan_grid(bound);
   int64_t* part = ctx->ss_part;
-  SS_LAUNCH(ctx, name, G, k_scan_reduce<true>, in, dcount, add, part);
+  SS_LAUNCH(ctx, name, G, k_scan_reduce<SCAN_MOVED>, in, dcount, add, part);
   SS_LAUNCH(ctx, name, 1, k_scan_parts, part, G, dtotal, dmoved);
-  SS_LAUNCH(ctx, name, G, k_scan_down<true>, in, out, dcount, add, (const int64_t*)part, moved);
+  SS_LAUNCH(ctx, name, G, k_scan_down<SCAN_MOVED>, in, out, dcount, add, (const int64_t*)part, moved,
+            (const uint32_t*)nullptr, (uint32_t*)nullptr);
   return FRT_OK;
 }
 
