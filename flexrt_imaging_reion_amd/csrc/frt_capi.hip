@@ -609,6 +609,7 @@ int frt_create(const frt_config* cfg_in, frt_ctx** out) {
   set_redshift(g, c, (float)c.zz);
   g.nhe_to_nh = (float)((FRT_Y_HE / 4.) / (1. - FRT_Y_HE));
   g.sig912 = frt_sigmapi_H1(13.6 / FRT_H_EV);  // mfp_912 (ray_tracing.cc:1080)
+  g.log4 = dm_log(4.);
   const float safety = (float)c.safety_factor;
   for (int l = 0; l < FRT_MAX_LEVEL; ++l) {
     // rmax_from_l / rmin_from_l (ray_tracing.cc:66-79)

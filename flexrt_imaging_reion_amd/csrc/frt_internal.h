@@ -74,6 +74,7 @@ struct GridConst {
   int pass;             // C2Ray pass of the current cell launch
   float nhe_to_nh;
   double sig912;        // sigmapi_H1(13.6/h_eV)
+  double log4;          // dm_log(4.) (l_from_r's divisor)
   double sigH[FRT_MAX_FREQ];
   double sigHe[FRT_MAX_FREQ];
   double sigHe2[FRT_MAX_FREQ];  // sigmapi_He2(nu_phot) (update_gamma of the c2ray_off mode)

@@ -1691,24 +1691,42 @@ __global__ void __launch_bounds__(BLK) k_merge_flag(RaySoA rb, const StepDev* __
                                                     int32_t* __restrict__ cand,
                                                     uint32_t* __restrict__ key) {
   if (!sd->merge_on) return;
+  constexpr int WIN = 128;  // rays per window of a wavefront
+  __shared__ int s_cand[BLK / 64][WIN];
   const int64_t nrays = sd->nrays;
   const float nphot_merge = sd->nphot_merge;
   const float* inv = rot_inv + 9 * (int64_t)sd->itm;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i <= nrays; i += (int64_t)gridDim.x * BLK) {
-    int c = 0;
-    if (i < nrays && alive[i]) {
-      const float s = psum[i];
-      if (s < nphot_merge) {
-        c = 1;
-        const float4 d = rb.dir[i];
-        float vx, vy, vz;
-        rotate3(inv, d.x, d.y, d.z, &vx, &vy, &vz);  // get_unit_vector(.., 6)
-        const int bin = hpx_vec2pix_nest(g.hpx_lvl, (double)vx, (double)vy, (double)vz);
-        const int nb = 12 << (2 * g.hpx_lvl);
-        key[i] = (uint32_t)ray_cell(rb, i) * (uint32_t)nb + (uint32_t)bin;
-      }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t nwaves = (int64_t)gridDim.x * (BLK / 64);
+  // each wavefront flags a window of 128 rays, lists its candidates (ballots)
+  // and computes their keys one candidate per lane: the HEALPix bin of the
+  // direction is the costly part, and a lane per ray would compute it with
+  // most lanes idle
+  for (int64_t base = ((int64_t)blockIdx.x * (BLK / 64) + w) * WIN; base <= nrays; base += nwaves * WIN) {
+    int nc = 0;
+#pragma unroll
+    for (int j = 0; j < WIN / 64; ++j) {
+      const int64_t i = base + j * 64 + lane;
+      const bool c = i < nrays && alive[i] && psum[i] < nphot_merge;
+      if (i <= nrays) cand[i] = c ? 1 : 0;
+      const uint64_t m = __ballot(c);
+      if (c) s_cand[w][nc + __popcll(m & lt)] = (int)(i - base);
+      nc += __popcll(m);
     }
-    cand[i] = c;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    for (int t = lane; t < nc; t += 64) {
+      const int64_t i = base + s_cand[w][t];
+      const float4 d = rb.dir[i];
+      float vx, vy, vz;
+      rotate3(inv, d.x, d.y, d.z, &vx, &vy, &vz);  // get_unit_vector(.., 6)
+      const int bin = hpx_vec2pix_nest(g.hpx_lvl, (double)vx, (double)vy, (double)vz);
+      const int nb = 12 << (2 * g.hpx_lvl);
+      key[i] = (uint32_t)ray_cell(rb, i) * (uint32_t)nb + (uint32_t)bin;
+    }
+    __builtin_amdgcn_wave_barrier();  // the list is read before the next window writes it
   }
 }
 
@@ -1717,7 +1735,7 @@ __device__ __forceinline__ int l_from_r_dev(const GridConst& g, float r, float s
   const float q = r / g.dx0;
   if ((double)q < 1.) return g.hpx_lvl;
   const double arg = FRT_PI_REF * (double)safety / 3. * ((double)q * (double)q);
-  int l = (int)ceil(dm_log(arg) / dm_log(4.));
+  int l = (int)ceil(dm_log(arg) / g.log4);  // g.log4 = dm_log(4.)
   if (l > g.l_max) l = g.l_max;
   if (l < g.hpx_lvl) l = g.hpx_lvl;
   return l;
@@ -2472,7 +2490,7 @@ int launch_merge(frt_ctx* ctx) {
              del, ctx->pdf);
   FRT_LAUNCH(ctx, "pdf_walk", 1, k_pdf_walk, sd, (const int32_t*)ctx->pdf, num_thresh, lg_min, del, ctx->cfg.seed);
   // merge(): rotation index drawn every pass (ray_tracing.cc:327)
-  FRT_LAUNCH(ctx, "merge_flag", ray_grid(ctx, nb + 1), k_merge_flag, rb, sd, g, ctx->alive, (const float*)ctx->psum,
+  FRT_LAUNCH(ctx, "merge_flag", ray_grid(ctx, nb / 2 + 1), k_merge_flag, rb, sd, g, ctx->alive, (const float*)ctx->psum,
              ctx->rot_inv,
              ctx->cand_flag, ctx->cand_key_tmp);
   // the candidates in list order with their keys, for the sort
