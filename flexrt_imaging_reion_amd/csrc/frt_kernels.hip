@@ -975,9 +975,12 @@ __device__ __forceinline__ bool deposit_segment_mixed(const GridConst& g, Contri
 }
 
 // Mixed form: FP32 attenuation and photon bookkeeping; per-cell sums are
-// compensated (Kahan) in the cell kernel.
+// compensated (Kahan) in the cell kernel.  The first passes are held to 72
+// VGPRs (7 wavefronts per SIMD instead of 6, three registers spilled):
+// the latency-bound pass gains from the extra wavefront; the last pass,
+// which would spill thirty, keeps its registers.
 template <bool LAST, int NFC = 0>
-__global__ void __launch_bounds__(BLK) k_deposit_mixed(RaySoA rb, const int32_t* __restrict__ moved,
+__global__ void __launch_bounds__(BLK, LAST ? 1 : 7) k_deposit_mixed(RaySoA rb, const int32_t* __restrict__ moved,
                                                        StepDev* sd, int64_t* head, int chunk, GridConst g,
                                                        const int32_t* __restrict__ path_cell,
                                                        const float* __restrict__ path_dist,
