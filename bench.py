@@ -37,11 +37,11 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-RT-updates/s at 128^3, 4096 src: mixed vs FP64 GPU at stated x_HI err"
-KERNELS = ["march", "scan_segments", "seg_keys", "sort_segments", "cell_ranges", "pack_avg",
+KERNELS = ["march", "scan_segments", "seg_keys", "sort_segments", "pack_avg",
            "deposit_mixed", "deposit_fp64", "chem", "count_alive", "pdf", "merge_flag",
-           "scan_merge", "merge_scatter", "sort_merge", "merge_groups", "split_flag",
+           "scan_merge", "sort_merge", "merge_groups", "split_flag",
            "scan_split", "out_keys", "sort_rays", "compact", "cast", "zero_ray_dt", "add_ray_dt",
-           "fill", "thermal"]
+           "thermal"]
 
 
 def parse():
