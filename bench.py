@@ -626,6 +626,10 @@ def main():
         "roofline": roofline(wl, st_m, kt_m, True),
         "step_roofline": step_roofline(wl, st_m, wall_m, True),
         "ms_per_step_with_launch_events": round(1e3 * wp_m / steps, 3),
+        # HIP-event time of each launch name (scans and sorts under their
+        # step-phase names), ms per step, from the pass with launch events
+        "kernel_ms_per_step": {k: round(ms / steps, 4) for k, (ms, n) in sorted(kt_m.items(), key=lambda kv: -kv[1][0])
+                               if n} if kt_m else None,
         "rays_end": int(st_m[-1]["rays_end"]),
         "segments_per_step": int(np.mean([s["segments"] for s in st_m])),
         "ray_segments_per_s": round(float(np.sum([s["segments"] for s in st_m])) / wall_m, 1),
