@@ -1200,8 +1200,125 @@ __global__ void __launch_bounds__(BLK) k_chem(CellPtrs cf, GridConst g, ContribP
       // current one is processed
       const float* R = (const float*)cp.rec;
       float qn[FRT_MAX_FREQ], dn = 0.f;
-      if (LAST && s0 < s1) load_photon_record(R + (int64_t)s0 * cp.rec_len, F, cp.rec_len, qn, dn);
-      for (int s = s0; LAST && s < s1; ++s) {
+      // Five bins and whole wavefronts of cells: the wavefront's records
+      // (its 64 cells' contiguous slots) are dealt to its lanes in equal
+      // consecutive shares, each lane keeping compensated partial sums per
+      // cell, and each cell's partials are then added in slot order by the
+      // cell's lane -- a lane per cell would leave about half the lanes idle
+      // (cells hold from none to well over the mean number of records).  The
+      // sums are grouped differently from the one-lane walk below, so the
+      // bits differ from it, and they are the same on every run.
+      const bool balanced = LAST && NFC == 5 && (g.n3 & 63) == 0;
+      if (balanced) {
+        __shared__ float s_part[BLK / 64][128][17];  // per (cell + lane): H, He, H_nu, He_nu, photons_nu
+        __shared__ int s_end[BLK / 64][64];          // prefix maximum of the cells' slot ends
+        __shared__ float s_op[BLK / 64][64][2];      // the cells' opacities
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        int e = s1 > s0 ? s1 : 0;
+        int m = s1 > s0 ? s0 : INT_MAX;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int u = __shfl_up(e, off, 64);
+          if (lane >= off) e = max(e, u);
+          m = min(m, __shfl_xor(m, off, 64));
+        }
+        const int r1 = __shfl(e, 63, 64), r0 = m;
+        s_end[w][lane] = e;
+        s_op[w][lane][0] = ah1;
+        s_op[w][lane][1] = ahe1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        const int nrec = r0 < r1 ? r1 - r0 : 0;
+        const int L = (nrec + 63) >> 6;
+        const int q0 = r0 + lane * L, q1 = min(q0 + L, r1);
+        if (nrec > 0 && q0 < q1) {
+          int c = 0, hi = 63;  // the cell of record q0: the first whose end exceeds it
+          while (c < hi) {
+            const int mid = (c + hi) >> 1;
+            if (s_end[w][mid] > q0) hi = mid;
+            else c = mid + 1;
+          }
+          bool has = false;
+          auto flush = [&](int cell) {
+            float* o = s_part[w][cell + lane];
+            o[0] = a - ca;
+            o[1] = b - cb;
+            FRT_FOR_BINS(nu) {
+              o[2 + nu] = x[nu] - cx[nu];
+              o[7 + nu] = y[nu] - cy[nu];
+              o[12 + nu] = z[nu];
+            }
+            a = ca = b = cb = 0.f;
+            FRT_FOR_BINS(nu) x[nu] = cx[nu] = y[nu] = cy[nu] = z[nu] = 0.f;
+          };
+          load_photon_record(R + (int64_t)q0 * cp.rec_len, F, cp.rec_len, qn, dn);
+          for (int r = q0; r < q1; ++r) {
+            while (s_end[w][c] <= r) {
+              if (has) flush(c);
+              has = false;
+              ++c;
+            }
+            has = true;
+            const float cah1 = s_op[w][c][0], cahe1 = s_op[w][c][1];
+            const float crh1 = inv_dtv / cah1, crhe1 = inv_dtv / cahe1;
+            float q[FRT_MAX_FREQ];
+            FRT_FOR_BINS(nu) q[nu] = qn[nu];
+            const float d = dn;
+            if (r + 1 < q1) load_photon_record(R + (int64_t)(r + 1) * cp.rec_len, F, cp.rec_len, qn, dn);
+            if (isinf(d)) continue;  // behind a dump: no deposit
+            const bool dump = d < 0.f;
+            const float dist = dump ? -d : d;
+            float va = 0.f, vb = 0.f;
+            FRT_FOR_BINS(nu) {
+              const float nin = q[nu];
+              const float dth = cah1 * g.sigHf[nu] * dist;
+              const float dthe = cahe1 * g.sigHef[nu] * dist;
+              float rh, rhe, eh, ehe;
+              bin_deposit_mixed(g, dth, dthe, nin, dump, crh1, crhe1, rh, rhe, eh, ehe);
+              va += rh;
+              vb += rhe;
+              float yx = rh - cx[nu];
+              float tx = x[nu] + yx;
+              cx[nu] = (tx - x[nu]) - yx;
+              x[nu] = tx;
+              float yy = rhe - cy[nu];
+              float ty = y[nu] + yy;
+              cy[nu] = (ty - y[nu]) - yy;
+              y[nu] = ty;
+              z[nu] += nin;
+            }
+            FRT_KAHAN2(va, vb);
+          }
+          if (has) flush(c);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        // this lane's cell: its partials, lanes j0..j1 in order, slot cell + j
+        a = ca = b = cb = 0.f;
+        FRT_FOR_BINS(nu) x[nu] = cx[nu] = y[nu] = cy[nu] = z[nu] = 0.f;
+        if (s1 > s0) {
+          const int j0 = (s0 - r0) / L, j1 = (s1 - 1 - r0) / L;
+          for (int j = j0; j <= j1; ++j) {
+            const float* o = s_part[w][lane + j];
+            FRT_KAHAN2(o[0], o[1]);
+            FRT_FOR_BINS(nu) {
+              float yx = o[2 + nu] - cx[nu];
+              float tx = x[nu] + yx;
+              cx[nu] = (tx - x[nu]) - yx;
+              x[nu] = tx;
+              float yy = o[7 + nu] - cy[nu];
+              float ty = y[nu] + yy;
+              cy[nu] = (ty - y[nu]) - yy;
+              y[nu] = ty;
+              z[nu] += o[12 + nu];
+            }
+          }
+        }
+      }
+      if (LAST && !balanced && s0 < s1) load_photon_record(R + (int64_t)s0 * cp.rec_len, F, cp.rec_len, qn, dn);
+      for (int s = s0; LAST && !balanced && s < s1; ++s) {
         float va, vb;
         {
           float q[FRT_MAX_FREQ];
